@@ -1,0 +1,203 @@
+/*
+ * sc23ad.h - C ABI of the MI355X-native integer AMG-preconditioned FGMRES(m)
+ * solve phase.
+ *
+ * The reference (suzuki-hpc/SC23-AD) is a header-only C++ library whose call
+ * surface is a set of abstract classes.  This ABI replaces them with opaque
+ * handles and plain pointers; the precision that the reference selects by
+ * template argument (solver 0 / 1 / 2 = FP64 / FP32-mixed / integer) is an
+ * argument here.  Every function returns an sc23ad_status; the message of the
+ * last failure on the calling thread is available from sc23ad_last_error().
+ * Nothing in this ABI calls exit(), where the reference does.
+ *
+ * Reference interface replaced by each entry point (file:line, relative to the
+ * reference repository):
+ *   CSRMat, io::ReadMM, CSRMat::LMax1Scaling  -> sc23ad_csrmat_*
+ *       (Library/matrix/csrmat.hpp:13-139, Library/io/mm.hpp:24-98)
+ *   intAMGflex / dfAMG constructors           -> sc23ad_hier_build, sc23ad_amg_create
+ *       (Library/preconditioner/INTAMG.hpp:25-84,149-245)
+ *   Preconditioner<T>::Precondition           -> sc23ad_amg_precondition
+ *       (Library/preconditioner/preconditioner.hpp:12-17; INTAMG.hpp:108-133,269-293)
+ *   SpMat<T>::SpMV, ldtri::SpMat<T>::SpTRSV,  -> sc23ad_amg_level_op
+ *   Smoother<T>::Smooth, direct::LU*::Solve
+ *       (Library/matrix/spmat.hpp:13-19,43-52,128-137;
+ *        Library/matrix/trimat/ldtri.hpp:8-14,172-183,511-525;
+ *        Library/smoother/smoother.hpp:9-16, stationary.hpp:415-430,997-1012;
+ *        Library/solver/lu.hpp:91-94)
+ *   FGMRES<double|float>, FGMRESflex ctors    -> sc23ad_solver_create
+ *       (Library/solver/gmres.hpp:28-29,110-111,194-195)
+ *   Restarted ctor + Restarted::Solve         -> sc23ad_solver_create / sc23ad_solver_solve
+ *       (Library/solver/irsolver.hpp:16,27-43; Solver::Solve solver.hpp:12-30)
+ *   fractional-bit rule of the driver         -> sc23ad_default_frac_bits
+ *       (Work/main.cpp:153-160)
+ *   CSR<Fixed<30>>::SpMV / CSRflex::SpMV       -> sc23ad_spmv_int (raw test hook)
+ *       (Library/matrix/spmat.hpp:77-86,128-137)
+ */
+#ifndef SC23AD_H
+#define SC23AD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  SC23AD_OK = 0,
+  SC23AD_BAD_PARAM = 1,
+  SC23AD_SETUP_FAIL = 2,    /* where the reference's setup calls exit()      */
+  SC23AD_OVERFLOW = 3,      /* fixed-point overflow / division by zero        */
+  SC23AD_STAGNATION = 4,    /* Restarted: |nrm_r - prev| / nrm_r < 1e-8       */
+  SC23AD_HIP_ERROR = 5,
+  SC23AD_NOT_CONVERGED = 6, /* restart budget exhausted                       */
+  SC23AD_IO_ERROR = 7,
+  SC23AD_NO_DEVICE = 8      /* device entry point called without a usable GPU */
+} sc23ad_status;
+
+typedef enum { SC23AD_FP64 = 0, SC23AD_FP32 = 1, SC23AD_INT = 2 } sc23ad_precision;
+typedef enum { SC23AD_SOR = 0, SC23AD_BLOCK_SOR = 1 } sc23ad_smoother;
+
+/* Matrix kinds for sc23ad_csrmat_gen (parameters in brackets). */
+typedef enum {
+  SC23AD_GEN_POISSON2D = 0,   /* [n]                                        */
+  SC23AD_GEN_LAPLACE3D = 1,   /* [n]                                        */
+  SC23AD_GEN_DIFFUSION27 = 2, /* [n, block, ax, ay, az, seed]               */
+  SC23AD_GEN_CONVDIFF2D = 3,  /* [n, peclet, angle]                         */
+  SC23AD_GEN_INCLUSIONS3D = 4 /* [n, nspheres, rmin, rmax, k_inside, seed]  */
+} sc23ad_gen_kind;
+
+typedef struct sc23ad_csrmat_s sc23ad_csrmat; /* host CSR, FP64 values          */
+typedef struct sc23ad_hier_s sc23ad_hier;     /* host AMG hierarchy              */
+typedef struct sc23ad_amg_s sc23ad_amg;       /* hierarchy resident on the GPU   */
+typedef struct sc23ad_solver_s sc23ad_solver; /* Restarted(FGMRES(m)) on the GPU */
+
+/* AMGParam + SORParam / BlockSORParam (Library/parameters.hpp:79-120). */
+typedef struct {
+  int max_levels;  /* AMGParam::level (20 in Work/main.cpp)                  */
+  double theta;    /* strength threshold (0.8)                              */
+  int cycle;       /* 1 = V-cycle (the only cycle the reference implements)  */
+  int smoother;    /* sc23ad_smoother                                       */
+  double omega;    /* relaxation weight (1.0)                               */
+  int iter;        /* sweeps per smoothing step (1)                         */
+  int b_num;       /* BlockSOR block count (40)                             */
+  int coarse_n;    /* stop coarsening at <= coarse_n rows (2000)            */
+} sc23ad_amg_param;
+
+/* Converge{isConverged, iter} (Library/solver/solver.hpp:12-20), extended. */
+typedef struct {
+  int is_converged;
+  int iter;          /* total Arnoldi steps, 0 when not converged (as reference) */
+  int restarts;      /* inner FGMRES cycles run                                   */
+  int status;        /* sc23ad_status of the solve                                */
+  int overflow_flag; /* a fixed-point step divided by zero or saturated          */
+  int fallback_used; /* learned parameters were replaced by the defaults         */
+  int frac_bits;     /* F used by the integer solver                              */
+  double implicit_relres; /* |e_{j+1}| / nrm_b at exit (FP64 view)                 */
+  double solve_ms;   /* device time of the solve (hipEvent)                       */
+} sc23ad_converge;
+
+const char* sc23ad_last_error(void);
+const char* sc23ad_version(void);
+void sc23ad_amg_param_default(sc23ad_amg_param* p, int smoother);
+
+/* ---- host CSR ------------------------------------------------------------ */
+int sc23ad_csrmat_create(int n, int m, const int* rptr, const int* cind,
+                         const double* val, sc23ad_csrmat** out);
+int sc23ad_csrmat_read_mm(const char* path, sc23ad_csrmat** out);
+int sc23ad_csrmat_gen(int kind, const double* params, int nparams,
+                      sc23ad_csrmat** out);
+int sc23ad_csrmat_info(const sc23ad_csrmat* a, int* n, int* m, int* nnz);
+int sc23ad_csrmat_export(const sc23ad_csrmat* a, int* rptr, int* cind,
+                         double* val);
+int sc23ad_csrmat_lmax1_scaling(sc23ad_csrmat* a, double* b);
+int sc23ad_csrmat_matvec(const sc23ad_csrmat* a, const double* x, double* y);
+void sc23ad_csrmat_destroy(sc23ad_csrmat* a);
+/* x* ~ U[-1,1] from mt19937_64(seed) with u = (x >> 11) * 2^-53. */
+int sc23ad_xstar(int n, uint64_t seed, double* xstar);
+/* Draws of the batched convection-diffusion family: n, peclet, angle per
+   system, `count` systems from `seed`. */
+int sc23ad_convdiff_draws(int count, uint64_t seed, int* n, double* pe,
+                          double* angle);
+
+/* ---- host hierarchy (setup; no GPU needed) ------------------------------- */
+int sc23ad_hier_build(const sc23ad_csrmat* a, const sc23ad_amg_param* p,
+                      int precision, sc23ad_hier** out);
+/* depth (number of coarsening steps); level_n / level_nnz receive depth+1
+   entries (operator rows / nnz per level, the last one the coarsest grid). */
+int sc23ad_hier_info(const sc23ad_hier* h, int* depth, int* level_n,
+                     int* level_nnz);
+int sc23ad_hier_export(const sc23ad_hier* h, const char* path);
+void sc23ad_hier_destroy(sc23ad_hier* h);
+
+/* ---- device ---------------------------------------------------------------- */
+int sc23ad_device_count(int* count);
+int sc23ad_dev_alloc(size_t bytes, void** ptr);
+int sc23ad_dev_free(void* ptr);
+int sc23ad_dev_upload(void* dst, const void* src, size_t bytes);
+int sc23ad_dev_download(void* dst, const void* src, size_t bytes);
+int sc23ad_dev_sync(void);
+
+/* Raw fixed-point CSR SpMV on device arrays: y_i = (int32)((sum_j (int64)val_j *
+   (int64)x_{cind_j}) >> bit).  All pointers are device pointers. */
+int sc23ad_spmv_int(int n, const int* rptr, const int* cind, const int32_t* val,
+                    int bit, const int32_t* x, int32_t* y);
+
+/* Upload a hierarchy (kept by the amg; `h` may be destroyed afterwards). */
+int sc23ad_amg_create(const sc23ad_hier* h, sc23ad_amg** out);
+void sc23ad_amg_destroy(sc23ad_amg* m);
+int sc23ad_amg_precision(const sc23ad_amg* m, int* precision);
+/* One V-cycle out = M^-1 in.  Vectors are int32 (INT), double (FP64) or
+   float (FP32) with the fine-level size; on_device selects device pointers. */
+int sc23ad_amg_precondition(sc23ad_amg* m, const void* in, void* out,
+                            int on_device);
+
+/* Level operations (device vectors of the level's element type). */
+typedef enum {
+  SC23AD_OP_SPMV_A = 0,    /* out = A_l in                                   */
+  SC23AD_OP_SPMV_U = 1,    /* out = U_l in                                   */
+  SC23AD_OP_SPMV_R = 2,    /* out = R_l in   (level l -> l+1)                */
+  SC23AD_OP_SPMV_P = 3,    /* out = P_l in   (level l+1 -> l)                */
+  SC23AD_OP_TRSV_LD = 4,   /* out = LD_l^-1 in                               */
+  SC23AD_OP_SMOOTH0 = 5,   /* Smooth(in, out, false)                         */
+  SC23AD_OP_SMOOTH1 = 6,   /* Smooth(in, out, true): out is read and written  */
+  SC23AD_OP_COARSE = 7     /* out = LU^-1 in on the coarsest grid (l = depth) */
+} sc23ad_level_op;
+int sc23ad_amg_level_op(sc23ad_amg* m, int level, int op, const void* in,
+                        void* out);
+
+/* FP64 fine operator for the outer residual (the reference's dA). */
+int sc23ad_solver_create(int precision, const sc23ad_csrmat* a, sc23ad_amg* m,
+                         int restart_m, double eps, int frac_bits,
+                         int max_restarts, sc23ad_solver** out);
+void sc23ad_solver_destroy(sc23ad_solver* s);
+/* Restarted::Solve with x as the initial guess and the result.  on_device
+   selects device pointers for b and x (FP64, fine size). */
+int sc23ad_solver_solve(sc23ad_solver* s, const double* b, double nrm_b,
+                        double* x, int on_device, sc23ad_converge* out);
+/* Record an order-free checksum of every V-cycle input and output (two
+   uint64 per Arnoldi step) during the next solves; cap = capacity. */
+int sc23ad_solver_trace(sc23ad_solver* s, uint64_t* buf, int cap, int* len);
+/* Bracket every fine-level integer SpMV launch of the next solves with hipEvents
+   and report their count and mean duration. */
+int sc23ad_solver_time_spmv(sc23ad_solver* s, int enable);
+int sc23ad_solver_spmv_stats(sc23ad_solver* s, int* launches, double* mean_ms);
+/* Explicit relative residual ||b - A x|| / nrm_b on the device (FP64). */
+int sc23ad_solver_relres(sc23ad_solver* s, const double* b, double nrm_b,
+                         const double* x, int on_device, double* relres);
+
+/* Fractional bits of the integer solver by the driver's rule (Work/main.cpp:
+   153-160): c = M_fp64^-1 b; F = 29 if max|c|/nrm_b < 1 else
+   31 - (ceil(log2(floor(max|c|/nrm_b) + 1)) + 1).  m_fp64 must be an FP64 amg. */
+int sc23ad_default_frac_bits(sc23ad_amg* m_fp64, const double* b, double nrm_b,
+                             int* bits);
+
+/* Order-free checksum used by the traces: sum_i mix64(i * 0x9E3779B97F4A7C15 ^
+   (uint32)v_i) mod 2^64, mix64 = splitmix64's finaliser.  Host arrays. */
+uint64_t sc23ad_checksum_i32(const int32_t* v, int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SC23AD_H */

@@ -1,0 +1,25 @@
+// Definitions of the opaque C-ABI handle types shared by the host and device
+// halves of the library.
+#pragma once
+
+#include <string>
+
+#include "hierarchy.hpp"
+
+struct sc23ad_csrmat_s {
+  sc23ad::HostCsr a;
+};
+
+struct sc23ad_hier_s {
+  int precision = 2;
+  int warnings = 0;
+  int fine_nnz = 0;
+  sc23ad::IntHierarchy ih;
+  sc23ad::FpHierarchy<double> dh;
+  sc23ad::FpHierarchy<float> fh;
+};
+
+namespace sc23ad {
+extern thread_local std::string g_last_error;
+int fail(int code, const std::string& msg);
+}  // namespace sc23ad

@@ -1,0 +1,403 @@
+#include "hierarchy.hpp"
+
+#include <cmath>
+#include <string>
+
+namespace sc23ad {
+
+AmgChain build_chain(const HostCsr& fine, const AmgParams& prm) {
+  if (prm.max_levels < 1 || prm.iter < 1 || !(prm.omega > 0) ||
+      (prm.smoother == SmootherKind::kBlockSOR && prm.b_num < 1))
+    throw SetupError{kBadParam, "invalid AMG parameters"};
+  AmgChain ch;
+  // INTAMG.hpp:162-195 / 38-51: coarsen until max_levels is reached or a
+  // level has at most coarse_n rows.  Each step consumes the unscaled
+  // previous coarse operator.
+  CoarsenStep st = rs_coarsen(fine, prm.theta);
+  ch.warnings += st.warnings;
+  ch.coarse.push_back(std::move(st.coarse));
+  ch.interp.push_back(std::move(st.interp));
+  ch.restrict_.push_back(std::move(st.restrict_));
+  ch.depth = 1;
+  while (prm.max_levels > ch.depth && ch.coarse[ch.depth - 1].n > prm.coarse_n) {
+    ++ch.depth;
+    CoarsenStep nx = rs_coarsen(ch.coarse[ch.depth - 2], prm.theta);
+    ch.warnings += nx.warnings;
+    ch.coarse.push_back(std::move(nx.coarse));
+    ch.interp.push_back(std::move(nx.interp));
+    ch.restrict_.push_back(std::move(nx.restrict_));
+  }
+  return ch;
+}
+
+// SOR: split "LD-U" (helper_matrix.hpp:530-604), then move (1 - 1/omega) * d
+// to the front of each U row and scale the last LD entry by 1/omega
+// (stationary.hpp:385-396).  BlockSOR: b_num contiguous blocks of
+// ceil(n/b_num) rows; LD keeps the diagonal / omega and the lower entries of
+// the row's own block, U everything else (stationary.hpp:934-978).
+void smoother_split(const HostCsr& a, const AmgParams& prm, HostCsr& ld,
+                    HostCsr& u) {
+  const int n = a.n;
+  const double omega = prm.omega;
+  ld.reset(n, a.m);
+  u.reset(n, a.m);
+  if (prm.smoother == SmootherKind::kSOR) {
+    for (int i = 0; i < n; ++i) {
+      const size_t ld_begin = ld.cind.size();
+      for (int p = a.rptr[i]; p < a.rptr[i + 1]; ++p)
+        if (a.cind[p] <= i) {
+          ld.cind.push_back(a.cind[p]);
+          ld.val.push_back(a.val[p]);
+        }
+      if (ld.cind.size() == ld_begin)
+        throw SetupError{kSetupFail,
+                         "SOR split: row " + std::to_string(i) +
+                             " has no diagonal or lower entry"};
+      double& d = ld.val.back();
+      u.cind.push_back(ld.cind.back());
+      u.val.push_back((1 - 1.0 / omega) * d);
+      d = 1.0 / omega * d;
+      for (int p = a.rptr[i]; p < a.rptr[i + 1]; ++p)
+        if (a.cind[p] > i) {
+          u.cind.push_back(a.cind[p]);
+          u.val.push_back(a.val[p]);
+        }
+      ld.rptr[i + 1] = static_cast<int>(ld.cind.size());
+      u.rptr[i + 1] = static_cast<int>(u.cind.size());
+    }
+    return;
+  }
+  const int bn = prm.b_num;
+  const int size = (n + bn - 1) / bn;
+  std::vector<int> bj(bn + 1);
+  bj[0] = 0;
+  for (int k = 0; k < bn; ++k) bj[k + 1] = bj[k] + size;
+  bj[bn] = n;
+  for (int i = 0; i < n; ++i) {
+    const int left = bj[i / size];
+    for (int p = a.rptr[i]; p < a.rptr[i + 1]; ++p) {
+      const int c = a.cind[p];
+      const double v = a.val[p];
+      if (c == i) {
+        ld.cind.push_back(c);
+        ld.val.push_back((1.0 / omega) * v);
+        u.cind.push_back(c);
+        u.val.push_back((1 - 1.0 / omega) * v);
+      } else if (left <= c && c < i) {
+        ld.cind.push_back(c);
+        ld.val.push_back(v);
+      } else {
+        u.cind.push_back(c);
+        u.val.push_back(v);
+      }
+    }
+    ld.rptr[i + 1] = static_cast<int>(ld.cind.size());
+    u.rptr[i + 1] = static_cast<int>(u.cind.size());
+  }
+}
+
+HostCsr extract_ld_inv(const HostCsr& a) {
+  HostCsr o;
+  o.reset(a.n, a.m);
+  for (int i = 0; i < a.n; ++i) {
+    for (int p = a.rptr[i]; p < a.rptr[i + 1]; ++p) {
+      if (a.cind[p] == i) {
+        o.cind.push_back(i);
+        o.val.push_back(1 / a.val[p]);
+      } else if (a.cind[p] < i) {
+        o.cind.push_back(a.cind[p]);
+        o.val.push_back(a.val[p]);
+      }
+    }
+    o.rptr[i + 1] = static_cast<int>(o.cind.size());
+  }
+  return o;
+}
+
+HostCsr extract_l(const HostCsr& a) {
+  HostCsr o;
+  o.reset(a.n, a.m);
+  for (int i = 0; i < a.n; ++i) {
+    for (int p = a.rptr[i]; p < a.rptr[i + 1]; ++p)
+      if (a.cind[p] < i) {
+        o.cind.push_back(a.cind[p]);
+        o.val.push_back(a.val[p]);
+      }
+    o.rptr[i + 1] = static_cast<int>(o.cind.size());
+  }
+  return o;
+}
+
+HostCsr extract_dinv_u(const HostCsr& a) {
+  HostCsr o;
+  o.reset(a.n, a.m);
+  for (int i = 0; i < a.n; ++i) {
+    for (int p = a.rptr[i]; p < a.rptr[i + 1]; ++p) {
+      if (a.cind[p] == i) {
+        o.cind.push_back(i);
+        o.val.push_back(1 / a.val[p]);
+      } else if (a.cind[p] > i) {
+        o.cind.push_back(a.cind[p]);
+        o.val.push_back(a.val[p]);
+      }
+    }
+    o.rptr[i + 1] = static_cast<int>(o.cind.size());
+  }
+  return o;
+}
+
+// Complete sparse LU without pivoting, row by row (IKJ), keeping all fill and
+// explicit zeros.  Row i is eliminated against each earlier row k in
+// ascending column order; the multiplier is a_ik / u_kk and the merged entries
+// are a_ij - m * u_kj (helper_matrix.hpp:978-1060).  A row k without a stored
+// diagonal reuses the previous multiplier, as the reference does.
+HostCsr complete_lu(const HostCsr& a) {
+  const int n = a.n;
+  HostCsr f;
+  f.reset(n, a.m);
+  if (n == 0) return f;
+  for (int p = a.rptr[0]; p < a.rptr[1]; ++p) {
+    f.cind.push_back(a.cind[p]);
+    f.val.push_back(a.val[p]);
+  }
+  f.rptr[1] = static_cast<int>(f.cind.size());
+  std::vector<int> tc, tc2;
+  std::vector<double> tv, tv2;
+  double pivot = 0;
+  for (int i = 1; i < n; ++i) {
+    tc.assign(a.cind.begin() + a.rptr[i], a.cind.begin() + a.rptr[i + 1]);
+    tv.assign(a.val.begin() + a.rptr[i], a.val.begin() + a.rptr[i + 1]);
+    size_t count = 0;
+    while (count < tc.size() && tc[count] < i) {
+      const int k = tc[count];
+      tc2.assign(tc.begin(), tc.begin() + count);
+      tv2.assign(tv.begin(), tv.begin() + count);
+      int j = f.rptr[k];
+      for (; j < f.rptr[k + 1]; ++j)
+        if (f.cind[j] == k) {
+          pivot = tv[count] / f.val[j];
+          break;
+        }
+      ++j;
+      size_t kp = count + 1;
+      tc2.push_back(k);
+      tv2.push_back(pivot);
+      const int kend = f.rptr[k + 1];
+      while (kp < tc.size() || j < kend) {
+        const int c1 = kp < tc.size() ? tc[kp] : n + 1;
+        const int c2 = j < kend ? f.cind[j] : n + 1;
+        if (c1 < c2) {
+          tc2.push_back(tc[kp]);
+          tv2.push_back(tv[kp]);
+          ++kp;
+        } else if (c1 == c2) {
+          tc2.push_back(tc[kp]);
+          tv2.push_back(tv[kp] - pivot * f.val[j]);
+          ++kp;
+          ++j;
+        } else {
+          tc2.push_back(f.cind[j]);
+          tv2.push_back(-pivot * f.val[j]);
+          ++j;
+        }
+      }
+      ++count;
+      tc.swap(tc2);
+      tv.swap(tv2);
+    }
+    f.cind.insert(f.cind.end(), tc.begin(), tc.end());
+    f.val.insert(f.val.end(), tv.begin(), tv.end());
+    f.rptr[i + 1] = static_cast<int>(f.cind.size());
+  }
+  return f;
+}
+
+namespace {
+
+IntCsr int_shell(const HostCsr& a) {
+  IntCsr o;
+  o.n = a.n;
+  o.m = a.m;
+  o.rptr = a.rptr;
+  o.cind = a.cind;
+  o.val.resize(a.val.size());
+  return o;
+}
+
+void check_bits(int bit, const char* what) {
+  if (bit < 0 || bit > 30)
+    throw SetupError{kOverflow, std::string(what) +
+                                    ": values too large for int32 fixed point"};
+}
+
+}  // namespace
+
+IntCsr to_flex(const HostCsr& a) {
+  IntCsr o = int_shell(a);
+  double mx = 0;
+  for (double v : a.val)
+    if (std::fabs(v) > mx) mx = std::fabs(v);
+  if (mx >= 2147483647.0) throw SetupError{kOverflow, "CSRflex: |a| too large"};
+  o.bit = flex_bits(mx);
+  check_bits(o.bit, "CSRflex");
+  for (size_t p = 0; p < a.val.size(); ++p) o.val[p] = to_fixed(a.val[p], o.bit);
+  return o;
+}
+
+IntCsr to_fixed_csr(const HostCsr& a, int bit) {
+  IntCsr o = int_shell(a);
+  o.bit = bit;
+  for (size_t p = 0; p < a.val.size(); ++p) o.val[p] = to_fixed(a.val[p], bit);
+  return o;
+}
+
+// ldtri::CSRflex / BJCSRflex: separate bit counts for the stored inverse
+// diagonal (dbit) and the off-diagonal entries (bit), both by the flex rule.
+IntCsr to_ld_flex(const HostCsr& l) {
+  IntCsr o = int_shell(l);
+  double mx = 0, dmx = 0;
+  for (int i = 0; i < l.n; ++i)
+    for (int p = l.rptr[i]; p < l.rptr[i + 1]; ++p) {
+      const double v = std::fabs(l.val[p]);
+      if (l.cind[p] == i) {
+        if (v > dmx) dmx = v;
+      } else if (v > mx) {
+        mx = v;
+      }
+    }
+  if (mx >= 2147483647.0 || dmx >= 2147483647.0)
+    throw SetupError{kOverflow, "ldtri::CSRflex: |a| too large"};
+  o.dbit = flex_bits(dmx);
+  o.bit = flex_bits(mx);
+  check_bits(o.bit, "ldtri::CSRflex");
+  check_bits(o.dbit, "ldtri::CSRflex");
+  for (int i = 0; i < l.n; ++i)
+    for (int p = l.rptr[i]; p < l.rptr[i + 1]; ++p)
+      o.val[p] = to_fixed(l.val[p], l.cind[p] == i ? o.dbit : o.bit);
+  return o;
+}
+
+IntCsr to_l_flex(const HostCsr& l) {
+  IntCsr o = int_shell(l);
+  double mx = 0;
+  for (double v : l.val)
+    if (std::fabs(v) > mx) mx = std::fabs(v);
+  o.bit = tri_bits(mx);
+  for (size_t p = 0; p < l.val.size(); ++p) o.val[p] = to_fixed(l.val[p], o.bit);
+  return o;
+}
+
+IntCsr to_dinvu_flex(const HostCsr& du) {
+  IntCsr o = int_shell(du);
+  double mx = 0, dmx = 0;
+  for (int i = 0; i < du.n; ++i)
+    for (int p = du.rptr[i]; p < du.rptr[i + 1]; ++p) {
+      const double v = std::fabs(du.val[p]);
+      if (du.cind[p] == i) {
+        if (v > dmx) dmx = v;
+      } else if (v > mx) {
+        mx = v;
+      }
+    }
+  o.dbit = tri_bits(dmx);
+  o.bit = tri_bits(mx);
+  for (int i = 0; i < du.n; ++i)
+    for (int p = du.rptr[i]; p < du.rptr[i + 1]; ++p)
+      o.val[p] = to_fixed(du.val[p], du.cind[p] == i ? o.dbit : o.bit);
+  return o;
+}
+
+IntHierarchy build_int_hierarchy(const HostCsr& fine, const AmgChain& chain,
+                                 const AmgParams& prm) {
+  IntHierarchy h;
+  h.prm = prm;
+  h.depth = chain.depth;
+  const int d = chain.depth;
+  std::vector<HostCsr> ca = chain.coarse;     // scaled in place below
+  std::vector<HostCsr> rt = chain.restrict_;  // scaled in place below
+  // INTAMG.hpp:176-211: divide each coarse operator's rows (and the matching
+  // rows of R) by the row's max |a|, then multiply R's columns by the
+  // previous level's row maxima.  `prev` starts as zeros, so a hierarchy with
+  // one coarse level gets R_0 = 0 exactly as the reference does.
+  std::vector<double> prev(static_cast<size_t>(fine.n), 0.0);
+  for (int l = 0; l < d; ++l) {
+    HostCsr& c = ca[l];
+    HostCsr& r = rt[l];
+    const bool last = (l == d - 1);
+    std::vector<double> mx = row_abs_max(c);
+    for (int i = 0; i < c.n; ++i) {
+      for (int p = c.rptr[i]; p < c.rptr[i + 1]; ++p) c.val[p] /= mx[i];
+      for (int p = r.rptr[i]; p < r.rptr[i + 1]; ++p) r.val[p] /= mx[i];
+      if (!last && l == 0) continue;
+      for (int p = r.rptr[i]; p < r.rptr[i + 1]; ++p) r.val[p] *= prev[r.cind[p]];
+    }
+    if (!last)
+      for (int i = 0; i < c.n; ++i) prev[i] = mx[i];
+  }
+  h.lv.resize(d);
+  for (int l = 0; l < d; ++l) {
+    IntLevel& L = h.lv[l];
+    const HostCsr& src = (l == 0) ? fine : ca[l - 1];
+    L.a = (l == 0) ? to_fixed_csr(fine, 30) : to_flex(src);
+    HostCsr ld, u;
+    smoother_split(src, prm, ld, u);
+    L.ld = to_ld_flex(extract_ld_inv(ld));
+    L.u = to_flex(u);
+    L.r = to_flex(rt[l]);
+    L.p = to_flex(chain.interp[l]);
+  }
+  HostCsr lu = complete_lu(ca[d - 1]);
+  h.lc = to_l_flex(extract_l(lu));
+  h.uc = to_dinvu_flex(extract_dinv_u(lu));
+  h.n_coarse = ca[d - 1].n;
+  return h;
+}
+
+namespace {
+template <typename T>
+FpCsr<T> to_fp(const HostCsr& a) {
+  FpCsr<T> o;
+  o.n = a.n;
+  o.m = a.m;
+  o.rptr = a.rptr;
+  o.cind = a.cind;
+  o.val.resize(a.val.size());
+  for (size_t p = 0; p < a.val.size(); ++p) o.val[p] = static_cast<T>(a.val[p]);
+  return o;
+}
+}  // namespace
+
+template <typename T>
+FpHierarchy<T> build_fp_hierarchy(const HostCsr& fine, const AmgChain& chain,
+                                  const AmgParams& prm) {
+  FpHierarchy<T> h;
+  h.prm = prm;
+  h.depth = chain.depth;
+  const int d = chain.depth;
+  h.lv.resize(d);
+  for (int l = 0; l < d; ++l) {
+    FpLevel<T>& L = h.lv[l];
+    const HostCsr& src = (l == 0) ? fine : chain.coarse[l - 1];
+    L.a = to_fp<T>(src);
+    HostCsr ld, u;
+    smoother_split(src, prm, ld, u);
+    L.ld = to_fp<T>(extract_ld_inv(ld));
+    L.u = to_fp<T>(u);
+    L.r = to_fp<T>(chain.restrict_[l]);
+    L.p = to_fp<T>(chain.interp[l]);
+  }
+  HostCsr lu = complete_lu(chain.coarse[d - 1]);
+  h.lc = to_fp<T>(extract_l(lu));
+  h.uc = to_fp<T>(extract_dinv_u(lu));
+  h.n_coarse = chain.coarse[d - 1].n;
+  return h;
+}
+
+template FpHierarchy<double> build_fp_hierarchy<double>(const HostCsr&,
+                                                        const AmgChain&,
+                                                        const AmgParams&);
+template FpHierarchy<float> build_fp_hierarchy<float>(const HostCsr&,
+                                                      const AmgChain&,
+                                                      const AmgParams&);
+
+}  // namespace sc23ad

@@ -1,0 +1,275 @@
+"""ctypes binding of libsc23ad.so (include/sc23ad.h).
+
+The library is the product: host-side AMG setup in C++ and the solve phase as
+HIP kernels for gfx950.  This module only marshals numpy arrays across the C
+ABI; it computes nothing itself.  Importing it loads the in-tree library and
+fails loudly if it has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libsc23ad.so")
+
+OK, BAD_PARAM, SETUP_FAIL, OVERFLOW, STAGNATION, HIP_ERROR, NOT_CONVERGED, \
+    IO_ERROR, NO_DEVICE = range(9)
+FP64, FP32, INT = 0, 1, 2
+SOR, BLOCK_SOR = 0, 1
+GEN_POISSON2D, GEN_LAPLACE3D, GEN_DIFFUSION27, GEN_CONVDIFF2D, \
+    GEN_INCLUSIONS3D = range(5)
+OP_SPMV_A, OP_SPMV_U, OP_SPMV_R, OP_SPMV_P, OP_TRSV_LD, OP_SMOOTH0, \
+    OP_SMOOTH1, OP_COARSE = range(8)
+
+
+class Sc23adError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"sc23ad error {code}: {msg}")
+        self.code = code
+
+
+class AmgParam(C.Structure):
+    _fields_ = [("max_levels", C.c_int), ("theta", C.c_double),
+                ("cycle", C.c_int), ("smoother", C.c_int),
+                ("omega", C.c_double), ("iter", C.c_int),
+                ("b_num", C.c_int), ("coarse_n", C.c_int)]
+
+
+class Converge(C.Structure):
+    _fields_ = [("is_converged", C.c_int), ("iter", C.c_int),
+                ("restarts", C.c_int), ("status", C.c_int),
+                ("overflow_flag", C.c_int), ("fallback_used", C.c_int),
+                ("frac_bits", C.c_int), ("implicit_relres", C.c_double),
+                ("solve_ms", C.c_double)]
+
+
+def _load() -> C.CDLL:
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it first (python -c "
+            "'import __graft_entry__ as g; g.build()')")
+    lib = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    ip = C.POINTER(C.c_int)
+    dp = C.POINTER(C.c_double)
+    sig = {
+        "sc23ad_last_error": (C.c_char_p, []),
+        "sc23ad_version": (C.c_char_p, []),
+        "sc23ad_amg_param_default": (None, [C.POINTER(AmgParam), C.c_int]),
+        "sc23ad_csrmat_create": (C.c_int, [C.c_int, C.c_int, vp, vp, vp, C.POINTER(vp)]),
+        "sc23ad_csrmat_read_mm": (C.c_int, [C.c_char_p, C.POINTER(vp)]),
+        "sc23ad_csrmat_gen": (C.c_int, [C.c_int, dp, C.c_int, C.POINTER(vp)]),
+        "sc23ad_csrmat_info": (C.c_int, [vp, ip, ip, ip]),
+        "sc23ad_csrmat_export": (C.c_int, [vp, vp, vp, vp]),
+        "sc23ad_csrmat_lmax1_scaling": (C.c_int, [vp, vp]),
+        "sc23ad_csrmat_matvec": (C.c_int, [vp, vp, vp]),
+        "sc23ad_csrmat_destroy": (None, [vp]),
+        "sc23ad_xstar": (C.c_int, [C.c_int, C.c_uint64, vp]),
+        "sc23ad_convdiff_draws": (C.c_int, [C.c_int, C.c_uint64, vp, vp, vp]),
+        "sc23ad_hier_build": (C.c_int, [vp, C.POINTER(AmgParam), C.c_int, C.POINTER(vp)]),
+        "sc23ad_hier_info": (C.c_int, [vp, ip, vp, vp]),
+        "sc23ad_hier_export": (C.c_int, [vp, C.c_char_p]),
+        "sc23ad_hier_destroy": (None, [vp]),
+        "sc23ad_checksum_i32": (C.c_uint64, [vp, C.c_int]),
+    }
+    optional = {
+        "sc23ad_device_count": (C.c_int, [ip]),
+        "sc23ad_dev_alloc": (C.c_int, [C.c_size_t, C.POINTER(vp)]),
+        "sc23ad_dev_free": (C.c_int, [vp]),
+        "sc23ad_dev_upload": (C.c_int, [vp, vp, C.c_size_t]),
+        "sc23ad_dev_download": (C.c_int, [vp, vp, C.c_size_t]),
+        "sc23ad_dev_sync": (C.c_int, []),
+        "sc23ad_spmv_int": (C.c_int, [C.c_int, vp, vp, vp, C.c_int, vp, vp]),
+        "sc23ad_amg_create": (C.c_int, [vp, C.POINTER(vp)]),
+        "sc23ad_amg_destroy": (None, [vp]),
+        "sc23ad_amg_precision": (C.c_int, [vp, ip]),
+        "sc23ad_amg_precondition": (C.c_int, [vp, vp, vp, C.c_int]),
+        "sc23ad_amg_level_op": (C.c_int, [vp, C.c_int, C.c_int, vp, vp]),
+        "sc23ad_solver_create": (C.c_int, [C.c_int, vp, vp, C.c_int, C.c_double,
+                                           C.c_int, C.c_int, C.POINTER(vp)]),
+        "sc23ad_solver_destroy": (None, [vp]),
+        "sc23ad_solver_solve": (C.c_int, [vp, vp, C.c_double, vp, C.c_int,
+                                          C.POINTER(Converge)]),
+        "sc23ad_solver_trace": (C.c_int, [vp, vp, C.c_int, ip]),
+        "sc23ad_solver_time_spmv": (C.c_int, [vp, C.c_int]),
+        "sc23ad_solver_spmv_stats": (C.c_int, [vp, ip, dp]),
+        "sc23ad_solver_relres": (C.c_int, [vp, vp, C.c_double, vp, C.c_int, dp]),
+        "sc23ad_default_frac_bits": (C.c_int, [vp, vp, C.c_double, ip]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    for name, (res, args) in optional.items():
+        f = getattr(lib, name, None)
+        if f is not None:
+            f.restype = res
+            f.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def check(code: int) -> None:
+    if code != OK:
+        raise Sc23adError(code, lib.sc23ad_last_error().decode())
+
+
+def ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def amg_param(smoother: int = SOR, **kw) -> AmgParam:
+    p = AmgParam()
+    lib.sc23ad_amg_param_default(C.byref(p), smoother)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class CsrMat:
+    """Host CSR matrix (the reference's CSRMat)."""
+
+    def __init__(self, handle: int):
+        self.h = C.c_void_p(handle)
+
+    @classmethod
+    def from_arrays(cls, n, rptr, cind, val, m=None):
+        rptr = np.ascontiguousarray(rptr, dtype=np.int32)
+        cind = np.ascontiguousarray(cind, dtype=np.int32)
+        val = np.ascontiguousarray(val, dtype=np.float64)
+        h = C.c_void_p()
+        check(lib.sc23ad_csrmat_create(n, n if m is None else m, ptr(rptr),
+                                       ptr(cind), ptr(val), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def generate(cls, kind: int, *params: float):
+        p = np.asarray(params, dtype=np.float64)
+        h = C.c_void_p()
+        check(lib.sc23ad_csrmat_gen(kind, p.ctypes.data_as(C.POINTER(C.c_double)),
+                                    len(p), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def read_mm(cls, path: str):
+        h = C.c_void_p()
+        check(lib.sc23ad_csrmat_read_mm(path.encode(), C.byref(h)))
+        return cls(h.value)
+
+    def info(self):
+        n, m, nnz = C.c_int(), C.c_int(), C.c_int()
+        check(lib.sc23ad_csrmat_info(self.h, C.byref(n), C.byref(m), C.byref(nnz)))
+        return n.value, m.value, nnz.value
+
+    @property
+    def n(self) -> int:
+        return self.info()[0]
+
+    @property
+    def nnz(self) -> int:
+        return self.info()[2]
+
+    def arrays(self):
+        n, _, nnz = self.info()
+        rptr = np.empty(n + 1, np.int32)
+        cind = np.empty(nnz, np.int32)
+        val = np.empty(nnz, np.float64)
+        check(lib.sc23ad_csrmat_export(self.h, ptr(rptr), ptr(cind), ptr(val)))
+        return rptr, cind, val
+
+    def lmax1_scaling(self, b: np.ndarray) -> None:
+        assert b.dtype == np.float64 and b.flags.c_contiguous
+        check(lib.sc23ad_csrmat_lmax1_scaling(self.h, ptr(b)))
+
+    def matvec(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty(self.n, np.float64)
+        check(lib.sc23ad_csrmat_matvec(self.h, ptr(x), ptr(y)))
+        return y
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib.sc23ad_csrmat_destroy(self.h)
+            self.h = C.c_void_p()
+
+
+def xstar(n: int, seed: int) -> np.ndarray:
+    x = np.empty(n, np.float64)
+    check(lib.sc23ad_xstar(n, seed, ptr(x)))
+    return x
+
+
+def convdiff_draws(count: int, seed: int):
+    n = np.empty(count, np.int32)
+    pe = np.empty(count, np.float64)
+    ang = np.empty(count, np.float64)
+    check(lib.sc23ad_convdiff_draws(count, seed, ptr(n), ptr(pe), ptr(ang)))
+    return n, pe, ang
+
+
+class Hierarchy:
+    """Host AMG hierarchy (setup only; no GPU needed)."""
+
+    def __init__(self, a: CsrMat, precision: int = INT, param: AmgParam | None = None):
+        self.precision = precision
+        h = C.c_void_p()
+        prm = param if param is not None else amg_param()
+        check(lib.sc23ad_hier_build(a.h, C.byref(prm), precision, C.byref(h)))
+        self.h = h
+
+    def info(self):
+        depth = C.c_int()
+        check(lib.sc23ad_hier_info(self.h, C.byref(depth), None, None))
+        n = np.empty(depth.value + 1, np.int32)
+        nnz = np.empty(depth.value + 1, np.int32)
+        check(lib.sc23ad_hier_info(self.h, C.byref(depth), ptr(n), ptr(nnz)))
+        return depth.value, n, nnz
+
+    def export(self, path: str) -> None:
+        check(lib.sc23ad_hier_export(self.h, path.encode()))
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib.sc23ad_hier_destroy(self.h)
+            self.h = C.c_void_p()
+
+
+def checksum_i32(v: np.ndarray) -> int:
+    v = np.ascontiguousarray(v, dtype=np.int32)
+    return int(lib.sc23ad_checksum_i32(ptr(v), len(v)))
+
+
+@dataclass
+class System:
+    """A generated, row-scaled linear system as the reference driver sees it
+    (Work/main.cpp:55-63): A and b = A x* scaled by LMax1Scaling."""
+    a: CsrMat
+    b: np.ndarray
+    xstar: np.ndarray
+    nrm_b: float
+    b_unscaled: np.ndarray
+
+
+def make_system(kind: int, params, seed: int) -> System:
+    a = CsrMat.generate(kind, *params)
+    xs = xstar(a.n, seed)
+    b = a.matvec(xs)
+    b0 = b.copy()
+    a.lmax1_scaling(b)
+    nrm_b = _seq_nrm2(b)
+    return System(a, b, xs, nrm_b, b0)
+
+
+def _seq_nrm2(b: np.ndarray) -> float:
+    """sqrt of the left-to-right sum of squares, as blas1::Nrm2 in a
+    single-threaded build computes nrm_b (Work/main.cpp:63)."""
+    if len(b) == 0:
+        return 0.0
+    # np.add.accumulate adds strictly left to right (unlike np.sum).
+    return float(np.sqrt(np.add.accumulate(b * b)[-1]))
