@@ -175,6 +175,11 @@ void sc23ad_solver_destroy(sc23ad_solver* s);
    selects device pointers for b and x (FP64, fine size). */
 int sc23ad_solver_solve(sc23ad_solver* s, const double* b, double nrm_b,
                         double* x, int on_device, sc23ad_converge* out);
+/* FP reduction order of the solver's norms and inner products:
+   0 = the fixed blocked order (default, 1024 x 256 partial sums, halving
+   trees); 1 = strictly left to right in one thread, the order of the
+   reference built without OpenMP (slow; for bit-exact comparison with it). */
+int sc23ad_solver_set_reduction(sc23ad_solver* s, int sequential);
 /* Record an order-free checksum of every V-cycle input and output (two
    uint64 per Arnoldi step) during the next solves; cap = capacity. */
 int sc23ad_solver_trace(sc23ad_solver* s, uint64_t* buf, int cap, int* len);

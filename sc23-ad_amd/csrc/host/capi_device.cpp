@@ -1,0 +1,317 @@
+// Device half of the C ABI: device buffers, the raw integer SpMV, the
+// device-resident preconditioner and the solver.
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <new>
+
+#include "../../../include/sc23ad.h"
+#include "capi_internal.hpp"
+#include "device.hpp"
+
+using namespace sc23ad;
+
+struct sc23ad_amg_s {
+  int precision = 2;
+  std::unique_ptr<DevAmg> d;
+};
+
+struct sc23ad_solver_s {
+  int precision = 2;
+  std::unique_ptr<DevSolver> s;
+  DevBuf hb, hx;  // staging for host-pointer solves
+};
+
+namespace {
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    return f();
+  } catch (const DevError& e) {
+    return fail(e.code, e.what);
+  } catch (const SetupError& e) {
+    return fail(e.code, e.what);
+  } catch (const std::bad_alloc&) {
+    return fail(SC23AD_HIP_ERROR, "out of host memory");
+  }
+}
+
+dev::Elem elem_of(int precision) {
+  return precision == SC23AD_INT ? dev::Elem::kI32
+         : precision == SC23AD_FP64 ? dev::Elem::kF64
+                                    : dev::Elem::kF32;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sc23ad_device_count(int* count) {
+  if (!count) return fail(SC23AD_BAD_PARAM, "device_count: null");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  *count = n;
+  return SC23AD_OK;
+}
+
+int sc23ad_dev_alloc(size_t bytes, void** ptr) {
+  if (!ptr) return fail(SC23AD_BAD_PARAM, "dev_alloc: null");
+  return guarded([&]() -> int {
+    lib_stream();
+    hip_check(hipMalloc(ptr, bytes ? bytes : 4), "hipMalloc");
+    return SC23AD_OK;
+  });
+}
+
+int sc23ad_dev_free(void* ptr) {
+  if (ptr) hipFree(ptr);
+  return SC23AD_OK;
+}
+
+int sc23ad_dev_upload(void* dst, const void* src, size_t bytes) {
+  return guarded([&]() -> int {
+    hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, lib_stream()), "upload");
+    hip_check(hipStreamSynchronize(lib_stream()), "sync");
+    return SC23AD_OK;
+  });
+}
+
+int sc23ad_dev_download(void* dst, const void* src, size_t bytes) {
+  return guarded([&]() -> int {
+    hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, lib_stream()), "download");
+    hip_check(hipStreamSynchronize(lib_stream()), "sync");
+    return SC23AD_OK;
+  });
+}
+
+int sc23ad_dev_sync(void) {
+  return guarded([&]() -> int {
+    hip_check(hipStreamSynchronize(lib_stream()), "sync");
+    return SC23AD_OK;
+  });
+}
+
+int sc23ad_spmv_int(int n, const int* rptr, const int* cind, const int32_t* val,
+                    int bit, const int32_t* x, int32_t* y) {
+  if (n < 0 || bit < 0 || bit > 62 || (n > 0 && (!rptr || !cind || !val || !x || !y)))
+    return fail(SC23AD_BAD_PARAM, "spmv_int: bad arguments");
+  return guarded([&]() -> int {
+    dev::DCsr a;
+    a.n = n;
+    a.rptr = rptr;
+    a.cind = cind;
+    a.val = val;
+    a.bit = bit;
+    a.group = 8;
+    hip_check(dev::launch_spmv(dev::Elem::kI32, a, dev::SpmvMode::kPlain, x, nullptr, y,
+                               lib_stream()),
+              "spmv_int");
+    hip_check(hipStreamSynchronize(lib_stream()), "sync");
+    return SC23AD_OK;
+  });
+}
+
+int sc23ad_amg_create(const sc23ad_hier* h, sc23ad_amg** out) {
+  if (!h || !out) return fail(SC23AD_BAD_PARAM, "amg_create: bad arguments");
+  return guarded([&]() -> int {
+    auto a = std::make_unique<sc23ad_amg_s>();
+    a->precision = h->precision;
+    LaunchCfg cfg;
+    if (h->precision == SC23AD_INT)
+      a->d = DevAmg::upload(h->ih, cfg);
+    else if (h->precision == SC23AD_FP64)
+      a->d = DevAmg::upload_fp(h->dh, cfg);
+    else
+      a->d = DevAmg::upload_fp(h->fh, cfg);
+    *out = a.release();
+    return SC23AD_OK;
+  });
+}
+
+void sc23ad_amg_destroy(sc23ad_amg* m) { delete m; }
+
+int sc23ad_amg_precision(const sc23ad_amg* m, int* precision) {
+  if (!m || !precision) return fail(SC23AD_BAD_PARAM, "amg_precision: null");
+  *precision = m->precision;
+  return SC23AD_OK;
+}
+
+int sc23ad_amg_precondition(sc23ad_amg* m, const void* in, void* out, int on_device) {
+  if (!m || !in || !out) return fail(SC23AD_BAD_PARAM, "precondition: null");
+  return guarded([&]() -> int {
+    DevAmg& a = *m->d;
+    const size_t bytes = a.esize() * static_cast<size_t>(a.n0());
+    if (on_device) {
+      a.vcycle(in, out);
+    } else {
+      DevBuf di(bytes), dout(bytes);
+      di.upload(in, bytes);
+      a.vcycle(di.get(), dout.get());
+      dout.download(out, bytes);
+    }
+    hip_check(hipStreamSynchronize(lib_stream()), "sync");
+    return SC23AD_OK;
+  });
+}
+
+int sc23ad_amg_level_op(sc23ad_amg* m, int level, int op, const void* in, void* out) {
+  if (!m || !in || !out) return fail(SC23AD_BAD_PARAM, "level_op: null");
+  return guarded([&]() -> int {
+    DevAmg& a = *m->d;
+    if (op == SC23AD_OP_COARSE) {
+      if (level != a.depth) return fail(SC23AD_BAD_PARAM, "level_op: coarse solve is level depth");
+      a.coarse(in, out);
+    } else {
+      if (level < 0 || level >= a.depth) return fail(SC23AD_BAD_PARAM, "level_op: bad level");
+      const DevLevel& L = a.lv[level];
+      switch (op) {
+        case SC23AD_OP_SPMV_A: a.spmv(L.a, dev::SpmvMode::kPlain, in, nullptr, out); break;
+        case SC23AD_OP_SPMV_U: a.spmv(L.u, dev::SpmvMode::kPlain, in, nullptr, out); break;
+        case SC23AD_OP_SPMV_R: a.spmv(L.r, dev::SpmvMode::kPlain, in, nullptr, out); break;
+        case SC23AD_OP_SPMV_P: a.spmv(L.p, dev::SpmvMode::kPlain, in, nullptr, out); break;
+        case SC23AD_OP_TRSV_LD: a.trsv(L.ld_plan, L.ld, in, out); break;
+        case SC23AD_OP_SMOOTH0: a.smooth(level, in, out, false); break;
+        case SC23AD_OP_SMOOTH1: a.smooth(level, in, out, true); break;
+        default: return fail(SC23AD_BAD_PARAM, "level_op: bad op");
+      }
+    }
+    hip_check(hipStreamSynchronize(lib_stream()), "sync");
+    return SC23AD_OK;
+  });
+}
+
+int sc23ad_solver_create(int precision, const sc23ad_csrmat* a, sc23ad_amg* m,
+                         int restart_m, double eps, int frac_bits, int max_restarts,
+                         sc23ad_solver** out) {
+  if (!a || !m || !out || restart_m < 1 || !(eps > 0))
+    return fail(SC23AD_BAD_PARAM, "solver_create: bad arguments");
+  if (precision != m->precision)
+    return fail(SC23AD_BAD_PARAM, "solver_create: precision differs from the amg's");
+  if (precision == SC23AD_INT && (frac_bits < 1 || frac_bits > 30))
+    return fail(SC23AD_BAD_PARAM, "solver_create: frac_bits must be in [1, 30]");
+  if (a->a.n != m->d->n0()) return fail(SC23AD_BAD_PARAM, "solver_create: size mismatch");
+  return guarded([&]() -> int {
+    auto s = std::make_unique<sc23ad_solver_s>();
+    s->precision = precision;
+    const int outer = max_restarts > 0 ? max_restarts : 1000 / restart_m;
+    s->s = std::make_unique<DevSolver>(elem_of(precision), a->a, m->d.get(), restart_m,
+                                       eps, frac_bits, outer);
+    *out = s.release();
+    return SC23AD_OK;
+  });
+}
+
+void sc23ad_solver_destroy(sc23ad_solver* s) { delete s; }
+
+int sc23ad_solver_solve(sc23ad_solver* s, const double* b, double nrm_b, double* x,
+                        int on_device, sc23ad_converge* out) {
+  if (!s || !b || !x) return fail(SC23AD_BAD_PARAM, "solve: null");
+  return guarded([&]() -> int {
+    DevSolver& d = *s->s;
+    const int n = d.amg->n0();
+    const size_t bytes = sizeof(double) * n;
+    DevSolver::Result r;
+    if (on_device) {
+      r = d.solve(b, nrm_b, x);
+    } else {
+      if (s->hb.bytes() < bytes) s->hb.alloc(bytes);
+      if (s->hx.bytes() < bytes) s->hx.alloc(bytes);
+      s->hb.upload(b, bytes);
+      s->hx.upload(x, bytes);
+      r = d.solve(s->hb.as<double>(), nrm_b, s->hx.as<double>());
+      s->hx.download(x, bytes);
+    }
+    if (out) {
+      memset(out, 0, sizeof *out);
+      out->is_converged = r.converged;
+      out->iter = r.iter;
+      out->restarts = r.restarts;
+      out->status = r.status;
+      out->overflow_flag = r.overflow;
+      out->frac_bits = d.bit;
+      out->implicit_relres = r.implicit_relres;
+      out->solve_ms = r.solve_ms;
+    }
+    return SC23AD_OK;
+  });
+}
+
+int sc23ad_solver_trace(sc23ad_solver* s, uint64_t* buf, int cap, int* len) {
+  if (!s) return fail(SC23AD_BAD_PARAM, "trace: null");
+  return guarded([&]() -> int {
+    if (buf == nullptr) {
+      s->s->set_trace(cap);
+    } else {
+      int l = 0;
+      s->s->read_trace(buf, cap, &l);
+      if (len) *len = l;
+    }
+    return SC23AD_OK;
+  });
+}
+
+int sc23ad_solver_set_reduction(sc23ad_solver* s, int sequential) {
+  if (!s) return fail(SC23AD_BAD_PARAM, "set_reduction: null");
+  s->s->opts.seq_reduction = sequential ? 1 : 0;
+  return SC23AD_OK;
+}
+
+int sc23ad_solver_time_spmv(sc23ad_solver* s, int enable) {
+  if (!s) return fail(SC23AD_BAD_PARAM, "time_spmv: null");
+  s->s->time_spmv = enable != 0;
+  s->s->spmv_launches = 0;
+  s->s->spmv_ms_total = 0;
+  return SC23AD_OK;
+}
+
+int sc23ad_solver_spmv_stats(sc23ad_solver* s, int* launches, double* mean_ms) {
+  if (!s) return fail(SC23AD_BAD_PARAM, "spmv_stats: null");
+  if (launches) *launches = s->s->spmv_launches;
+  if (mean_ms)
+    *mean_ms = s->s->spmv_launches ? s->s->spmv_ms_total / s->s->spmv_launches : 0.0;
+  return SC23AD_OK;
+}
+
+int sc23ad_solver_relres(sc23ad_solver* s, const double* b, double nrm_b,
+                         const double* x, int on_device, double* relres) {
+  if (!s || !b || !x || !relres) return fail(SC23AD_BAD_PARAM, "relres: null");
+  return guarded([&]() -> int {
+    DevSolver& d = *s->s;
+    const size_t bytes = sizeof(double) * d.amg->n0();
+    if (on_device) {
+      *relres = d.relres(b, nrm_b, x);
+    } else {
+      DevBuf db(bytes), dx(bytes);
+      db.upload(b, bytes);
+      dx.upload(x, bytes);
+      *relres = d.relres(db.as<double>(), nrm_b, dx.as<double>());
+    }
+    return SC23AD_OK;
+  });
+}
+
+int sc23ad_default_frac_bits(sc23ad_amg* m, const double* b, double nrm_b, int* bits) {
+  if (!m || !b || !bits) return fail(SC23AD_BAD_PARAM, "default_frac_bits: null");
+  if (m->precision != SC23AD_FP64)
+    return fail(SC23AD_BAD_PARAM, "default_frac_bits needs an FP64 amg");
+  return guarded([&]() -> int {
+    DevAmg& a = *m->d;
+    const int n = a.n0();
+    DevBuf db(sizeof(double) * n), dc(sizeof(double) * n);
+    db.upload(b, sizeof(double) * n);
+    a.vcycle(db.get(), dc.get());
+    std::vector<double> c(n);
+    dc.download(c.data(), sizeof(double) * n);
+    double c_max = 0;
+    for (int i = 0; i < n; ++i)
+      if (std::fabs(c[i]) > c_max) c_max = std::fabs(c[i]);
+    // Work/main.cpp:160
+    *bits = (c_max / nrm_b < 1)
+                ? 29
+                : 31 - (static_cast<int>(std::ceil(std::log2(static_cast<int>(c_max / nrm_b) + 1))) + 1);
+    return SC23AD_OK;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,582 @@
+#include "device.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace sc23ad {
+
+using dev::Elem;
+using dev::SpmvMode;
+using dev::TriKind;
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess)
+    throw DevError{5, std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+hipStream_t lib_stream() {
+  static hipStream_t s = nullptr;
+  if (!s) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1)
+      throw DevError{8, "no HIP device available"};
+    hip_check(hipSetDevice(0), "hipSetDevice");
+    hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+  }
+  return s;
+}
+
+void DevBuf::alloc(size_t bytes) {
+  release();
+  lib_stream();
+  if (bytes == 0) bytes = 4;
+  hip_check(hipMalloc(&p_, bytes), "hipMalloc");
+  bytes_ = bytes;
+}
+
+void DevBuf::release() {
+  if (p_) hipFree(p_);
+  p_ = nullptr;
+  bytes_ = 0;
+}
+
+void DevBuf::upload(const void* src, size_t bytes, size_t off) {
+  if (bytes == 0) return;
+  hip_check(hipMemcpyAsync(static_cast<char*>(p_) + off, src, bytes,
+                           hipMemcpyHostToDevice, lib_stream()),
+            "upload");
+  hip_check(hipStreamSynchronize(lib_stream()), "upload sync");
+}
+
+void DevBuf::download(void* dst, size_t bytes, size_t off) const {
+  if (bytes == 0) return;
+  hip_check(hipMemcpyAsync(dst, static_cast<const char*>(p_) + off, bytes,
+                           hipMemcpyDeviceToHost, lib_stream()),
+            "download");
+  hip_check(hipStreamSynchronize(lib_stream()), "download sync");
+}
+
+size_t DevMat::algo_bytes_spmv(size_t elem) const {
+  // values + column indices + row pointers + gathered x (one read per column)
+  // + y written once.
+  return (elem + 4) * static_cast<size_t>(d.nnz) + 4 * static_cast<size_t>(d.n + 1) +
+         elem * static_cast<size_t>(d.m) + elem * static_cast<size_t>(d.n);
+}
+
+namespace {
+
+int pick_group(int n, int nnz, int forced) {
+  if (forced > 0) return forced;
+  const double avg = n > 0 ? static_cast<double>(nnz) / n : 1.0;
+  int g = 1;
+  while (g < 64 && g < avg) g <<= 1;
+  return g;
+}
+
+template <typename V>
+void upload_mat(DevMat& dm, int n, int m, int bit, int dbit,
+                const std::vector<int>& rptr, const std::vector<int>& cind,
+                const std::vector<V>& val, int group) {
+  dm.rptr.alloc(sizeof(int) * rptr.size());
+  dm.rptr.upload(rptr.data(), sizeof(int) * rptr.size());
+  dm.cind.alloc(sizeof(int) * std::max<size_t>(cind.size(), 1));
+  dm.cind.upload(cind.data(), sizeof(int) * cind.size());
+  dm.val.alloc(sizeof(V) * std::max<size_t>(val.size(), 1));
+  dm.val.upload(val.data(), sizeof(V) * val.size());
+  dm.d.n = n;
+  dm.d.m = m;
+  dm.d.nnz = static_cast<int>(cind.size());
+  dm.d.bit = bit;
+  dm.d.dbit = dbit;
+  dm.d.rptr = dm.rptr.as<int>();
+  dm.d.cind = dm.cind.as<int>();
+  dm.d.val = dm.val.get();
+  dm.d.group = pick_group(n, dm.d.nnz, group);
+}
+
+// Level sets of a triangular factor.  Dependencies of row i: every stored
+// column except the stored inverse diagonal (last entry for LD, first for DU;
+// L has none).  `chain_size` > 0 splits rows into independent contiguous
+// chains of that many rows (the BlockSOR partition), each walked by one
+// workgroup in a single launch; otherwise levels are walked in order, small
+// ones inside one workgroup and large ones by a launch of their own.
+void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr,
+                const std::vector<int>& cind, int chain_size, int nchain_req,
+                int single_wg_max) {
+  plan.kind = kind;
+  if (kind != TriKind::kL)
+    for (int i = 0; i < n; ++i)
+      if (rptr[i + 1] == rptr[i])
+        throw DevError{2, "triangular factor row without a stored diagonal"};
+  std::vector<int> lev(n, 0);
+  auto dep_range = [&](int i, int& b, int& e) {
+    b = rptr[i];
+    e = rptr[i + 1];
+    if (kind == TriKind::kLD) e = std::max(b, e - 1);
+    if (kind == TriKind::kDU) b = std::min(e, b + 1);
+  };
+  if (kind == TriKind::kDU) {
+    for (int i = n - 1; i >= 0; --i) {
+      int b, e, L = 0;
+      dep_range(i, b, e);
+      for (int p = b; p < e; ++p) {
+        const int c = cind[p];
+        if (c <= i) throw DevError{2, "triangular factor has an entry on the wrong side"};
+        L = std::max(L, lev[c] + 1);
+      }
+      lev[i] = L;
+    }
+  } else {
+    for (int i = 0; i < n; ++i) {
+      int b, e, L = 0;
+      dep_range(i, b, e);
+      for (int p = b; p < e; ++p) {
+        const int c = cind[p];
+        if (c >= i) throw DevError{2, "triangular factor has an entry on the wrong side"};
+        L = std::max(L, lev[c] + 1);
+      }
+      lev[i] = L;
+    }
+  }
+  int nchain = 1;
+  std::vector<int> chain_begin{0, n};
+  if (chain_size > 0 && nchain_req > 1) {
+    nchain = nchain_req;
+    chain_begin.assign(nchain + 1, 0);
+    for (int c = 0; c < nchain; ++c) chain_begin[c + 1] = std::min(n, chain_begin[c] + chain_size);
+    chain_begin[nchain] = n;
+  }
+  // rows of each chain ordered by (level, row): counting sort per chain
+  std::vector<int> perm(n);
+  std::vector<int> lev_ptr;
+  std::vector<int> chain_lev;  // 2 per chain
+  lev_ptr.push_back(0);
+  int maxlev_all = 0;
+  for (int c = 0; c < nchain; ++c) {
+    const int rb = chain_begin[c], re = chain_begin[c + 1];
+    int maxlev = -1;
+    for (int i = rb; i < re; ++i) maxlev = std::max(maxlev, lev[i]);
+    const int nl = maxlev + 1;
+    std::vector<int> cnt(nl + 1, 0);
+    for (int i = rb; i < re; ++i) ++cnt[lev[i] + 1];
+    for (int L = 0; L < nl; ++L) cnt[L + 1] += cnt[L];
+    const int base = lev_ptr.back();
+    const int first_lev = static_cast<int>(lev_ptr.size()) - 1;
+    for (int L = 1; L <= nl; ++L) lev_ptr.push_back(base + cnt[L]);
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    for (int i = rb; i < re; ++i) perm[base + fill[lev[i]]++] = i;
+    chain_lev.push_back(first_lev);
+    chain_lev.push_back(first_lev + nl);
+    maxlev_all = std::max(maxlev_all, nl);
+  }
+  plan.nlevels = maxlev_all;
+  // segments
+  std::vector<int> seg_chain_lev;  // chain_lev entries of single-chain segments
+  struct PendingSeg {
+    dev::TriSegment s;
+    int chain_off;  // offset into the chain_lev buffer, or -1
+  };
+  std::vector<PendingSeg> pend;
+  std::vector<int> all_chain_lev;
+  if (nchain > 1) {
+    all_chain_lev = chain_lev;
+    PendingSeg ps{};
+    ps.s.chain_kernel = 1;
+    ps.s.nchain = nchain;
+    ps.chain_off = 0;
+    pend.push_back(ps);
+  } else {
+    const int nl = static_cast<int>(lev_ptr.size()) - 1;
+    int L = 0;
+    while (L < nl) {
+      const int sz = lev_ptr[L + 1] - lev_ptr[L];
+      if (sz > single_wg_max) {
+        PendingSeg ps{};
+        ps.s.chain_kernel = 0;
+        ps.s.row_begin = lev_ptr[L];
+        ps.s.row_end = lev_ptr[L + 1];
+        ps.chain_off = -1;
+        pend.push_back(ps);
+        ++L;
+        continue;
+      }
+      int L2 = L;
+      while (L2 < nl && lev_ptr[L2 + 1] - lev_ptr[L2] <= single_wg_max) ++L2;
+      PendingSeg ps{};
+      ps.s.chain_kernel = 1;
+      ps.s.nchain = 1;
+      ps.chain_off = static_cast<int>(all_chain_lev.size());
+      all_chain_lev.push_back(L);
+      all_chain_lev.push_back(L2);
+      pend.push_back(ps);
+      L = L2;
+    }
+  }
+  plan.perm.alloc(sizeof(int) * std::max(n, 1));
+  plan.perm.upload(perm.data(), sizeof(int) * n);
+  plan.lev_ptr.alloc(sizeof(int) * lev_ptr.size());
+  plan.lev_ptr.upload(lev_ptr.data(), sizeof(int) * lev_ptr.size());
+  plan.chain_lev.alloc(sizeof(int) * std::max<size_t>(all_chain_lev.size(), 2));
+  plan.chain_lev.upload(all_chain_lev.data(), sizeof(int) * all_chain_lev.size());
+  plan.segs.clear();
+  for (auto& ps : pend) {
+    if (ps.chain_off >= 0) ps.s.chain_lev = plan.chain_lev.as<int>() + ps.chain_off;
+    plan.segs.push_back(ps.s);
+  }
+  plan.nlaunch = static_cast<int>(plan.segs.size());
+}
+
+}  // namespace
+
+void DevAmg::alloc_work() {
+  const size_t es = esize();
+  ee.resize(depth + 1);
+  xx.resize(depth + 1);
+  rr.resize(depth + 1);
+  tmp.resize(depth + 1);
+  for (int l = 1; l <= depth; ++l) {
+    const size_t n = static_cast<size_t>(std::max(level_n(l), 1));
+    ee[l].alloc(es * n);
+    xx[l].alloc(es * n);
+    rr[l].alloc(es * n);
+    tmp[l].alloc(es * n);
+  }
+  r0.alloc(es * std::max(n0(), 1));
+  tmp[0].alloc(es * std::max(n0(), 1));
+  tmpc.alloc(es * std::max(n_coarse, 1));
+}
+
+std::unique_ptr<DevAmg> DevAmg::upload(const IntHierarchy& h, const LaunchCfg& c) {
+  auto a = std::make_unique<DevAmg>();
+  a->elem = Elem::kI32;
+  a->cfg = c;
+  a->depth = h.depth;
+  a->iter = h.prm.iter;
+  a->n_coarse = h.n_coarse;
+  a->lv.resize(h.depth);
+  const bool bj = h.prm.smoother == SmootherKind::kBlockSOR;
+  for (int l = 0; l < h.depth; ++l) {
+    const IntLevel& L = h.lv[l];
+    DevLevel& D = a->lv[l];
+    D.n = L.a.n;
+    upload_mat(D.a, L.a.n, L.a.m, L.a.bit, -1, L.a.rptr, L.a.cind, L.a.val, c.spmv_group);
+    upload_mat(D.ld, L.ld.n, L.ld.m, L.ld.bit, L.ld.dbit, L.ld.rptr, L.ld.cind, L.ld.val, 0);
+    upload_mat(D.u, L.u.n, L.u.m, L.u.bit, -1, L.u.rptr, L.u.cind, L.u.val, c.spmv_group);
+    upload_mat(D.r, L.r.n, L.r.m, L.r.bit, -1, L.r.rptr, L.r.cind, L.r.val, c.spmv_group);
+    upload_mat(D.p, L.p.n, L.p.m, L.p.bit, -1, L.p.rptr, L.p.cind, L.p.val, c.spmv_group);
+    const int size = bj ? (L.a.n + h.prm.b_num - 1) / h.prm.b_num : 0;
+    build_plan(D.ld_plan, TriKind::kLD, L.ld.n, L.ld.rptr, L.ld.cind, size,
+               bj ? h.prm.b_num : 1, c.single_wg_max_rows);
+  }
+  upload_mat(a->lc, h.lc.n, h.lc.m, h.lc.bit, -1, h.lc.rptr, h.lc.cind, h.lc.val, 0);
+  upload_mat(a->uc, h.uc.n, h.uc.m, h.uc.bit, h.uc.dbit, h.uc.rptr, h.uc.cind, h.uc.val, 0);
+  build_plan(a->lc_plan, TriKind::kL, h.lc.n, h.lc.rptr, h.lc.cind, 0, 1, c.single_wg_max_rows);
+  build_plan(a->uc_plan, TriKind::kDU, h.uc.n, h.uc.rptr, h.uc.cind, 0, 1, c.single_wg_max_rows);
+  a->alloc_work();
+  return a;
+}
+
+template <typename T>
+std::unique_ptr<DevAmg> DevAmg::upload_fp(const FpHierarchy<T>& h, const LaunchCfg& c) {
+  auto a = std::make_unique<DevAmg>();
+  a->elem = sizeof(T) == 8 ? Elem::kF64 : Elem::kF32;
+  a->cfg = c;
+  a->depth = h.depth;
+  a->iter = h.prm.iter;
+  a->n_coarse = h.n_coarse;
+  a->lv.resize(h.depth);
+  const bool bj = h.prm.smoother == SmootherKind::kBlockSOR;
+  for (int l = 0; l < h.depth; ++l) {
+    const FpLevel<T>& L = h.lv[l];
+    DevLevel& D = a->lv[l];
+    D.n = L.a.n;
+    upload_mat(D.a, L.a.n, L.a.m, 0, -1, L.a.rptr, L.a.cind, L.a.val, 1);
+    upload_mat(D.ld, L.ld.n, L.ld.m, 0, -1, L.ld.rptr, L.ld.cind, L.ld.val, 1);
+    upload_mat(D.u, L.u.n, L.u.m, 0, -1, L.u.rptr, L.u.cind, L.u.val, 1);
+    upload_mat(D.r, L.r.n, L.r.m, 0, -1, L.r.rptr, L.r.cind, L.r.val, 1);
+    upload_mat(D.p, L.p.n, L.p.m, 0, -1, L.p.rptr, L.p.cind, L.p.val, 1);
+    const int size = bj ? (L.a.n + h.prm.b_num - 1) / h.prm.b_num : 0;
+    build_plan(D.ld_plan, TriKind::kLD, L.ld.n, L.ld.rptr, L.ld.cind, size,
+               bj ? h.prm.b_num : 1, c.single_wg_max_rows);
+  }
+  upload_mat(a->lc, h.lc.n, h.lc.m, 0, -1, h.lc.rptr, h.lc.cind, h.lc.val, 1);
+  upload_mat(a->uc, h.uc.n, h.uc.m, 0, -1, h.uc.rptr, h.uc.cind, h.uc.val, 1);
+  build_plan(a->lc_plan, TriKind::kL, h.lc.n, h.lc.rptr, h.lc.cind, 0, 1, c.single_wg_max_rows);
+  build_plan(a->uc_plan, TriKind::kDU, h.uc.n, h.uc.rptr, h.uc.cind, 0, 1, c.single_wg_max_rows);
+  a->alloc_work();
+  return a;
+}
+
+template std::unique_ptr<DevAmg> DevAmg::upload_fp<double>(const FpHierarchy<double>&,
+                                                           const LaunchCfg&);
+template std::unique_ptr<DevAmg> DevAmg::upload_fp<float>(const FpHierarchy<float>&,
+                                                          const LaunchCfg&);
+
+void DevAmg::spmv(const DevMat& m, SpmvMode mode, const void* x, const void* z,
+                  void* y) const {
+  hip_check(dev::launch_spmv(elem, m.d, mode, x, z, y, lib_stream()), "spmv");
+}
+
+void DevAmg::trsv(const TriPlan& plan, const DevMat& m, const void* in,
+                  void* out) const {
+  for (const auto& seg : plan.segs)
+    hip_check(dev::launch_trsv_segment(elem, plan.kind, m.d, plan.perm.as<int>(),
+                                       plan.lev_ptr.as<int>(), seg, in, out,
+                                       lib_stream()),
+              "trsv");
+}
+
+// SOR / BlockSOR Smooth (Library/smoother/stationary.hpp:415-430, 997-1012):
+// inited: iter x { t = in - U out; out = LD^-1 t }; otherwise out = LD^-1 in
+// first.
+void DevAmg::smooth(int l, const void* in, void* out, bool inited) const {
+  const DevLevel& L = lv[l];
+  int k0 = 0;
+  if (!inited) {
+    trsv(L.ld_plan, L.ld, in, out);
+    k0 = 1;
+  }
+  void* t = tmp[l].get();
+  for (int k = k0; k < iter; ++k) {
+    spmv(L.u, SpmvMode::kSub, out, in, t);
+    trsv(L.ld_plan, L.ld, t, out);
+  }
+}
+
+// direct::LUflex / LU<T>::Solve (Library/solver/lu.hpp:91-94, 33-36).
+void DevAmg::coarse(const void* in, void* out) const {
+  trsv(lc_plan, lc, in, tmpc.get());
+  trsv(uc_plan, uc, tmpc.get(), out);
+}
+
+// intAMGflex::Precondition / dfAMG::Precondition (INTAMG.hpp:269-293,
+// 108-133).  Coarse work vectors are indexed by level (1..depth).
+void DevAmg::vcycle(const void* in, void* out) const {
+  smooth(0, in, out, false);
+  spmv(lv[0].a, SpmvMode::kSub, out, in, r0.get());  // r = in - A out
+  if (depth == 0) return;
+  spmv(lv[0].r, SpmvMode::kPlain, r0.get(), nullptr, ee[1].get());
+  for (int l = 1; l < depth; ++l) {
+    smooth(l, ee[l].get(), xx[l].get(), false);
+    spmv(lv[l].a, SpmvMode::kSub, xx[l].get(), ee[l].get(), rr[l].get());
+    spmv(lv[l].r, SpmvMode::kPlain, rr[l].get(), nullptr, ee[l + 1].get());
+  }
+  coarse(ee[depth].get(), xx[depth].get());
+  for (int l = depth - 1; l >= 1; --l) {
+    spmv(lv[l].p, SpmvMode::kAdd, xx[l + 1].get(), xx[l].get(), xx[l].get());
+    smooth(l, ee[l].get(), xx[l].get(), true);
+  }
+  spmv(lv[0].p, SpmvMode::kAdd, xx[1].get(), out, out);
+  smooth(0, in, out, true);
+}
+
+double DevAmg::vcycle_bytes() const {
+  const size_t es = esize();
+  double b = 0;
+  auto trsv_bytes = [&](const DevMat& m) {
+    return static_cast<double>((es + 4) * m.d.nnz + 4 * (m.d.n + 1) + 2 * es * m.d.n);
+  };
+  for (int l = 0; l < depth; ++l) {
+    const DevLevel& L = lv[l];
+    const double n = L.n;
+    // pre-smooth (LD solve) + post-smooth (U spmv-sub + LD solve)
+    b += 2 * trsv_bytes(L.ld) + L.u.algo_bytes_spmv(es) + es * n;
+    // residual (A spmv-sub), restriction, prolongation-add
+    b += L.a.algo_bytes_spmv(es) + es * n;
+    b += L.r.algo_bytes_spmv(es);
+    b += L.p.algo_bytes_spmv(es) + es * n;
+  }
+  b += trsv_bytes(lc) + trsv_bytes(uc);
+  return b;
+}
+
+// ---------------------------------------------------------------------------
+
+DevSolver::DevSolver(Elem e, const HostCsr& a, DevAmg* amg_, int m_, double eps_,
+                     int bit_, int max_outer_)
+    : elem(e), m(m_), eps(eps_), bit(bit_), max_outer(max_outer_), amg(amg_) {
+  const int n = a.n;
+  upload_mat(a64, a.n, a.m, 0, -1, a.rptr, a.cind, a.val, 1);
+  const size_t es = amg->esize();
+  V.alloc(es * static_cast<size_t>(n) * (m + 1));
+  Z.alloc(es * static_cast<size_t>(n) * (m + 1));
+  r.alloc(sizeof(double) * std::max(n, 1));
+  // state layout (bytes, 8-aligned)
+  const size_t H = 8 * static_cast<size_t>(m + 1) * m;
+  const size_t sz = H + 8 * m * 2 + 8 * (m + 1) + 8 * m + 4 * m + 8 * dev::kRedBlocks * 2 +
+                    4 * dev::kRedBlocks + 64 + 8 * 8 + 8 * 4 + 4 * 8 + 16 + 128;
+  state.alloc(sz);
+  hip_check(hipMemsetAsync(state.get(), 0, sz, lib_stream()), "memset state");
+  char* p = state.as<char>();
+  auto take = [&](size_t bytes) {
+    char* q = p;
+    p += (bytes + 7) & ~static_cast<size_t>(7);
+    return q;
+  };
+  k.m = m;
+  k.bit = bit;
+  k.eps = eps;
+  k.H = take(H);
+  k.c = take(8 * m);
+  k.s = take(8 * m);
+  k.e = take(8 * (m + 1));
+  k.y = reinterpret_cast<double*>(take(8 * m));
+  k.yf = reinterpret_cast<float*>(take(4 * m));
+  k.ipart = reinterpret_cast<int64_t*>(take(8 * dev::kRedBlocks));
+  k.dpart = reinterpret_cast<double*>(take(8 * dev::kRedBlocks));
+  k.fpart = reinterpret_cast<float*>(take(4 * dev::kRedBlocks));
+  k.counter = reinterpret_cast<unsigned*>(take(64));
+  k.scal = reinterpret_cast<double*>(take(8 * 8));
+  k.iscal = reinterpret_cast<int64_t*>(take(8 * 4));
+  k.flags = reinterpret_cast<int*>(take(4 * 8));
+  k.trace_len = reinterpret_cast<int*>(take(16));
+  k.trace = nullptr;
+  k.trace_cap = 0;
+  hip_check(hipHostMalloc(reinterpret_cast<void**>(&host_flags), 8 * sizeof(int),
+                          hipHostMallocDefault),
+            "hipHostMalloc");
+  hip_check(hipStreamSynchronize(lib_stream()), "solver init");
+}
+
+DevSolver::~DevSolver() {
+  if (host_flags) hipHostFree(host_flags);
+}
+
+void DevSolver::read_flags() {
+  hip_check(hipMemcpyAsync(host_flags, k.flags, 4 * sizeof(int), hipMemcpyDeviceToHost,
+                           lib_stream()),
+            "flags");
+  hip_check(hipStreamSynchronize(lib_stream()), "flags sync");
+}
+
+void DevSolver::set_trace(int cap) {
+  if (cap <= 0) {
+    tracing = false;
+    return;
+  }
+  trace_cap = cap;
+  pinned_dummy.alloc(sizeof(uint64_t) * cap);
+  k.trace = pinned_dummy.as<uint64_t>();
+  k.trace_cap = cap;
+  tracing = true;
+  hip_check(hipMemsetAsync(k.trace_len, 0, sizeof(int), lib_stream()), "trace reset");
+}
+
+void DevSolver::read_trace(uint64_t* out, int cap, int* len) {
+  int l = 0;
+  hip_check(hipMemcpy(&l, k.trace_len, sizeof(int), hipMemcpyDeviceToHost), "trace len");
+  const int cnt = std::min(std::min(l, cap), trace_cap);
+  if (cnt > 0) pinned_dummy.download(out, sizeof(uint64_t) * cnt);
+  *len = cnt;
+}
+
+DevSolver::Result DevSolver::solve(const double* b_dev, double nrm_b, double* x_dev) {
+  Result res;
+  hipStream_t s = lib_stream();
+  const int n = amg->n0();
+  const size_t es = amg->esize();
+  k.nrm_b = nrm_b;
+  hipEvent_t e0, e1, s0 = nullptr, s1 = nullptr;
+  hip_check(hipEventCreate(&e0), "event");
+  hip_check(hipEventCreate(&e1), "event");
+  if (time_spmv) {
+    hip_check(hipEventCreate(&s0), "event");
+    hip_check(hipEventCreate(&s1), "event");
+  }
+  // Restarted: prev = 0 before the first pass
+  hip_check(hipMemsetAsync(k.scal, 0, 8 * sizeof(double), s), "reset scal");
+  hip_check(hipMemsetAsync(k.flags, 0, 8 * sizeof(int), s), "reset flags");
+  hip_check(hipEventRecord(e0, s), "record");
+  const int seq = opts.seq_reduction;
+  char* Vb = V.as<char>();
+  char* Zb = Z.as<char>();
+  int iter = 0;
+  res.status = 6;
+  for (int outer = 0; outer < max_outer; ++outer) {
+    hip_check(dev::launch_outer_residual(a64.d, b_dev, x_dev, r.as<double>(), k, elem, seq, s),
+              "outer residual");
+    read_flags();
+    if (host_flags[2]) {
+      res.status = 4;
+      break;
+    }
+    ++res.restarts;
+    hip_check(dev::launch_v0(elem, n, r.as<double>(), Vb, k, s), "v0");
+    int steps = 0;
+    bool conv = false, ovf = false;
+    for (int j = 0; j < m; ++j) {
+      void* vj = Vb + es * static_cast<size_t>(j) * n;
+      void* zj = Zb + es * static_cast<size_t>(j) * n;
+      void* w = Vb + es * static_cast<size_t>(j + 1) * n;
+      amg->vcycle(vj, zj);
+      if (tracing && elem == Elem::kI32) {
+        hip_check(dev::launch_checksum(n, static_cast<const int32_t*>(vj), k, s), "cs");
+        hip_check(dev::launch_checksum(n, static_cast<const int32_t*>(zj), k, s), "cs");
+      }
+      if (time_spmv) hip_check(hipEventRecord(s0, s), "record");
+      amg->spmv(amg->lv[0].a, SpmvMode::kPlain, zj, nullptr, w);
+      if (time_spmv) {
+        hip_check(hipEventRecord(s1, s), "record");
+        hip_check(hipEventSynchronize(s1), "sync");
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, s0, s1), "elapsed");
+        spmv_ms_total += ms;
+        ++spmv_launches;
+      }
+      hip_check(dev::launch_dot_first(elem, n, Vb, w, k, j, seq, s), "dot");
+      for (int kk = 0; kk <= j; ++kk)
+        hip_check(dev::launch_axpy_dot(elem, n, Vb, w, k, kk, j, seq, s), "axpy_dot");
+      hip_check(dev::launch_scal(elem, n, w, k, s), "scal");
+      read_flags();
+      steps = j + 1;
+      if (host_flags[1]) {
+        ovf = true;
+        break;
+      }
+      if (host_flags[0]) {
+        conv = true;
+        break;
+      }
+    }
+    if (ovf) {
+      res.status = 3;
+      res.overflow = 1;
+      break;
+    }
+    hip_check(dev::launch_update_x(elem, n, Zb, x_dev, k, steps, s), "update x");
+    iter += conv ? steps : m;
+    if (conv) {
+      res.status = 0;
+      res.converged = 1;
+      res.iter = iter;
+      break;
+    }
+  }
+  hip_check(hipEventRecord(e1, s), "record");
+  hip_check(hipEventSynchronize(e1), "sync");
+  float ms = 0;
+  hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+  res.solve_ms = ms;
+  double sc[8];
+  hip_check(hipMemcpy(sc, k.scal, sizeof sc, hipMemcpyDeviceToHost), "scal");
+  res.implicit_relres = sc[5];
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (s0) hipEventDestroy(s0);
+  if (s1) hipEventDestroy(s1);
+  return res;
+}
+
+double DevSolver::relres(const double* b_dev, double nrm_b, const double* x_dev) {
+  hip_check(dev::launch_relres(a64.d, b_dev, x_dev, r.as<double>(), k.dpart,
+                               k.counter + 4, k.scal + 6, lib_stream()),
+            "relres");
+  double v = 0;
+  hip_check(hipMemcpyAsync(&v, k.scal + 6, sizeof(double), hipMemcpyDeviceToHost,
+                           lib_stream()),
+            "relres copy");
+  hip_check(hipStreamSynchronize(lib_stream()), "relres sync");
+  return v / nrm_b;
+}
+
+}  // namespace sc23ad

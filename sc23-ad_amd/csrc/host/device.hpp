@@ -1,0 +1,164 @@
+// Device-resident hierarchy and solver: the host-side sequencing of the HIP
+// kernels for one V-cycle (intAMGflex / dfAMG::Precondition,
+// Library/preconditioner/INTAMG.hpp:108-133,269-293) and for
+// Restarted(FGMRES(m)) (Library/solver/irsolver.hpp:27-43,
+// Library/solver/gmres.hpp:48-88,130-170,225-271).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../kernels/launch.hpp"
+#include "hierarchy.hpp"
+
+namespace sc23ad {
+
+struct DevError {
+  int code;
+  std::string what;
+};
+
+void hip_check(hipError_t e, const char* what);
+hipStream_t lib_stream();  // library stream on device 0 (created on first use)
+
+class DevBuf {
+ public:
+  DevBuf() = default;
+  explicit DevBuf(size_t bytes) { alloc(bytes); }
+  ~DevBuf() { release(); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p_(o.p_), bytes_(o.bytes_) {
+    o.p_ = nullptr;
+    o.bytes_ = 0;
+  }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p_ = o.p_;
+      bytes_ = o.bytes_;
+      o.p_ = nullptr;
+      o.bytes_ = 0;
+    }
+    return *this;
+  }
+  void alloc(size_t bytes);
+  void release();
+  void upload(const void* src, size_t bytes, size_t off = 0);
+  void download(void* dst, size_t bytes, size_t off = 0) const;
+  template <typename T>
+  T* as() const { return static_cast<T*>(p_); }
+  void* get() const { return p_; }
+  size_t bytes() const { return bytes_; }
+
+ private:
+  void* p_ = nullptr;
+  size_t bytes_ = 0;
+};
+
+struct DevMat {
+  dev::DCsr d;
+  DevBuf rptr, cind, val;
+  size_t algo_bytes_spmv(size_t elem) const;
+};
+
+// Level-set schedule of one triangular factor.
+struct TriPlan {
+  dev::TriKind kind = dev::TriKind::kLD;
+  DevBuf perm, lev_ptr, chain_lev;
+  std::vector<dev::TriSegment> segs;
+  int nlevels = 0;    // longest dependency chain (levels of the deepest chain)
+  int nlaunch = 0;
+};
+
+struct DevLevel {
+  DevMat a, ld, u, r, p;
+  TriPlan ld_plan;
+  int n = 0;
+};
+
+// Launch-shape knobs (chosen by the autotuner, or defaults).
+struct LaunchCfg {
+  int single_wg_max_rows = 1024;  // levels up to this size stay in one workgroup
+  int spmv_group = 0;             // 0: choose from nnz/row
+};
+
+class DevAmg {
+ public:
+  dev::Elem elem = dev::Elem::kI32;
+  int depth = 0;
+  int iter = 1;
+  std::vector<DevLevel> lv;
+  DevMat lc, uc;
+  TriPlan lc_plan, uc_plan;
+  int n_coarse = 0;
+  // work vectors: index = level (1..depth); level 0 uses r0 / temp0
+  std::vector<DevBuf> ee, xx, rr, tmp;
+  DevBuf r0, tmpc;
+  LaunchCfg cfg;
+
+  static std::unique_ptr<DevAmg> upload(const IntHierarchy& h, const LaunchCfg& c);
+  template <typename T>
+  static std::unique_ptr<DevAmg> upload_fp(const FpHierarchy<T>& h, const LaunchCfg& c);
+
+  size_t esize() const { return elem == dev::Elem::kF64 ? 8 : 4; }
+  int n0() const { return lv.empty() ? 0 : lv[0].n; }
+  int level_n(int l) const { return l < depth ? lv[l].n : n_coarse; }
+
+  void spmv(const DevMat& m, dev::SpmvMode mode, const void* x, const void* z,
+            void* y) const;
+  void trsv(const TriPlan& plan, const DevMat& m, const void* in, void* out) const;
+  void smooth(int l, const void* in, void* out, bool inited) const;
+  void coarse(const void* in, void* out) const;
+  void vcycle(const void* in, void* out) const;
+  // Algorithmic HBM bytes of one V-cycle (sum over its kernels).
+  double vcycle_bytes() const;
+
+ private:
+  void alloc_work();
+};
+
+struct SolverOpts {
+  int seq_reduction = 0;  // 1: FP reductions strictly left to right
+};
+
+class DevSolver {
+ public:
+  dev::Elem elem;
+  int m = 30;
+  double eps = 1e-10;
+  int bit = 29;
+  int max_outer = 33;
+  DevAmg* amg = nullptr;  // not owned
+  DevMat a64;             // FP64 fine operator (outer residual)
+  SolverOpts opts;
+  DevBuf V, Z, r, b, x, state, pinned_dummy;
+  dev::KState k{};
+  int* host_flags = nullptr;  // pinned
+  // tracing / timing
+  bool tracing = false;
+  int trace_cap = 0;
+  bool time_spmv = false;
+  int spmv_launches = 0;
+  double spmv_ms_total = 0;
+
+  DevSolver(dev::Elem e, const HostCsr& a, DevAmg* amg, int m, double eps,
+            int bit, int max_outer);
+  ~DevSolver();
+  struct Result {
+    int converged = 0, iter = 0, restarts = 0, status = 0, overflow = 0;
+    double implicit_relres = 0, solve_ms = 0;
+  };
+  Result solve(const double* b_dev, double nrm_b, double* x_dev);
+  double relres(const double* b_dev, double nrm_b, const double* x_dev);
+  void set_trace(int cap);
+  void read_trace(uint64_t* out, int cap, int* len);
+
+ private:
+  void read_flags();
+};
+
+}  // namespace sc23ad

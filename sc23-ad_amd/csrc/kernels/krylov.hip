@@ -1,0 +1,634 @@
+// Krylov kernels of Restarted(FGMRES(m)) for the three precisions.
+//
+// Integer solver: FGMRESflex::Solve (Library/solver/gmres.hpp:225-271) with
+// blas1::int32 (Library/blas/blas1.hpp:106-119,230-307) and
+// blas2::int32::Trsv (Library/blas/blas2.hpp:51-61).
+// FP solvers: FGMRES<double|float>::Solve (gmres.hpp:48-88,130-170).
+// Outer loop: Restarted::Solve (Library/solver/irsolver.hpp:27-43).
+//
+// Modified Gram-Schmidt keeps the reference's order: each pass updates w with
+// V_k and then accumulates the next inner product <V_{k+1}, w> (or the norm
+// of w) from the updated values, so one pass per k replaces the reference's
+// separate Axpy and Dot.  Scalar work (lsqrt, H_inv, Givens, e, the
+// convergence test) runs in the last workgroup of the reduction, so the
+// Arnoldi step never waits for the host except for the convergence flag.
+#include "dev_common.hpp"
+#include "launch.hpp"
+
+namespace sc23ad::dev {
+
+namespace {
+
+constexpr int B = kRedThreads;
+
+__device__ __forceinline__ int64_t lsqrt64(int64_t x) {
+  if (x <= 0) return 0;
+  int64_t s = 1, t = x;
+  while (s < t) {
+    s <<= 1;
+    t >>= 1;
+  }
+  do {
+    t = s;
+    s = (x / s + s) >> 1;
+  } while (s < t);
+  return t;
+}
+
+// Tree over the kRedBlocks partials in the last block (256 threads, 4 values
+// each): identical to the halving tree over 1024 values.
+template <typename T>
+__device__ __forceinline__ T final_tree(const T* part, T* sh) {
+  const int t = threadIdx.x;
+  const T a = part[t] + part[t + 512];
+  const T b = part[t + 256] + part[t + 768];
+  return block_tree<B, T>(a + b, sh);
+}
+
+__device__ __forceinline__ void start_scalars(const KState& k, Elem e,
+                                              double nrm_r) {
+  const double prev = k.scal[1];
+  k.flags[2] = (fabs(nrm_r - prev) / nrm_r < 1.0e-8) ? 1 : 0;
+  k.scal[0] = nrm_r;
+  k.scal[1] = nrm_r;
+  k.flags[0] = 0;
+  k.flags[1] = 0;
+  k.flags[3] = 0;
+  if (e == Elem::kI32) {
+    const double e0 = nrm_r;
+    const double e_inv = 1.0 / e0;
+    const double fact = static_cast<double>(static_cast<int64_t>(1) << k.bit);
+    k.scal[2] = e0;
+    k.scal[3] = e_inv;
+    k.iscal[0] = trunc_i64(k.nrm_b * k.eps * e_inv * fact);
+    static_cast<int64_t*>(k.e)[0] = static_cast<int64_t>(1) << k.bit;
+  } else if (e == Elem::kF64) {
+    static_cast<double*>(k.e)[0] = nrm_r;
+    k.scal[2] = nrm_r;
+    k.scal[3] = 1.0 / nrm_r;
+  } else {
+    const float e0 = static_cast<float>(nrm_r);
+    static_cast<float*>(k.e)[0] = e0;
+    k.scal[2] = static_cast<double>(e0);
+    k.scal[3] = 1.0 / static_cast<double>(e0);
+  }
+}
+
+// r = b - A x with each row summed in column order; blocked-order norm.
+__global__ __launch_bounds__(B) void k_outer_residual(
+    int n, const int* __restrict__ rptr, const int* __restrict__ cind,
+    const double* __restrict__ val, const double* __restrict__ b,
+    const double* __restrict__ x, double* __restrict__ r, KState k, Elem e,
+    int with_start, double* out_norm) {
+  __shared__ double sh[B];
+  __shared__ int flag;
+  double acc = 0;
+  for (int i = blockIdx.x * B + threadIdx.x; i < n; i += kRedTotal) {
+    double t = 0;
+    const int re = rptr[i + 1];
+    for (int p = rptr[i]; p < re; ++p) t += val[p] * x[cind[p]];
+    const double ri = b[i] - t;
+    r[i] = ri;
+    acc += ri * ri;
+  }
+  const double part = block_tree<B, double>(acc, sh);
+  if (threadIdx.x == 0) k.dpart[blockIdx.x] = part;
+  if (!last_block(k.counter, &flag)) return;
+  const double tot = final_tree<double>(k.dpart, sh);
+  if (threadIdx.x == 0) {
+    const double nrm = sqrt(tot);
+    if (with_start) start_scalars(k, e, nrm);
+    if (out_norm) *out_norm = nrm;
+  }
+}
+
+// Residual only (no reduction), for the strictly sequential norm mode.
+__global__ __launch_bounds__(B) void k_residual_only(
+    int n, const int* __restrict__ rptr, const int* __restrict__ cind,
+    const double* __restrict__ val, const double* __restrict__ b,
+    const double* __restrict__ x, double* __restrict__ r) {
+  const int i = blockIdx.x * B + threadIdx.x;
+  if (i >= n) return;
+  double t = 0;
+  const int re = rptr[i + 1];
+  for (int p = rptr[i]; p < re; ++p) t += val[p] * x[cind[p]];
+  r[i] = b[i] - t;
+}
+
+// Left-to-right sum of squares by one thread (reference order; small n only).
+__global__ void k_seq_norm(int n, const double* __restrict__ r, KState k, Elem e,
+                           int with_start, double* out_norm) {
+  double s = 0;
+  for (int i = 0; i < n; ++i) s += r[i] * r[i];
+  const double nrm = sqrt(s);
+  if (with_start) start_scalars(k, e, nrm);
+  if (out_norm) *out_norm = nrm;
+}
+
+__global__ __launch_bounds__(B) void k_v0(int n, const double* __restrict__ r,
+                                          void* v0, KState k, Elem e) {
+  const int i = blockIdx.x * B + threadIdx.x;
+  if (i >= n) return;
+  const double e_inv = k.scal[3];
+  if (e == Elem::kI32) {
+    const double fact = static_cast<double>(static_cast<int64_t>(1) << k.bit);
+    static_cast<int32_t*>(v0)[i] = trunc_i32(r[i] * e_inv * fact);
+  } else if (e == Elem::kF64) {
+    static_cast<double*>(v0)[i] = r[i] * e_inv;
+  } else {
+    static_cast<float*>(v0)[i] = static_cast<float>(r[i] * e_inv);
+  }
+}
+
+// ---- integer MGS ---------------------------------------------------------
+
+__global__ __launch_bounds__(B) void k_dot_first_int(int n, const int32_t* __restrict__ v,
+                                                     const int32_t* __restrict__ w,
+                                                     KState k, int j) {
+  __shared__ uint64_t sh[B / 64];
+  __shared__ int flag;
+  uint64_t acc = 0;
+  for (int i = blockIdx.x * B + threadIdx.x; i < n; i += kRedTotal)
+    acc += prod64(v[i], w[i]);
+  const uint64_t part = block_sum_u64<B>(acc, sh);
+  if (threadIdx.x == 0) k.ipart[blockIdx.x] = static_cast<int64_t>(part);
+  if (!last_block(k.counter, &flag)) return;
+  uint64_t s = 0;
+  for (int b = threadIdx.x; b < kRedBlocks; b += B) s += static_cast<uint64_t>(k.ipart[b]);
+  s = block_sum_u64<B>(s, sh);
+  if (threadIdx.x == 0) static_cast<int64_t*>(k.H)[j * (k.m + 1)] = as_i64(s) >> k.bit;
+}
+
+__device__ void givens_int(const KState& k, int j, uint64_t sumsq) {
+  int64_t* H = static_cast<int64_t*>(k.H) + j * (k.m + 1);
+  int64_t* c = static_cast<int64_t*>(k.c);
+  int64_t* s = static_cast<int64_t*>(k.s);
+  int64_t* e = static_cast<int64_t*>(k.e);
+  const int bit = k.bit;
+  const int64_t hn = lsqrt64(as_i64(sumsq));
+  H[j + 1] = hn;
+  k.iscal[2] = hn;
+  k.flags[3] = j + 1;
+  if (hn == 0) {  // the reference divides by zero here
+    k.flags[1] = 1;
+    return;
+  }
+  k.iscal[1] = ((static_cast<int64_t>(1) << 62) / hn) >> (62 - bit - bit);
+  for (int q = 0; q < j; ++q) {  // Grot
+    const int64_t t = H[q];
+    H[q] = as_i64(static_cast<uint64_t>(mul_wrap(c[q], t)) -
+                  static_cast<uint64_t>(mul_wrap(s[q], H[q + 1]))) >> bit;
+    H[q + 1] = as_i64(static_cast<uint64_t>(mul_wrap(s[q], t)) +
+                      static_cast<uint64_t>(mul_wrap(c[q], H[q + 1]))) >> bit;
+  }
+  {  // Ggen
+    const int64_t a = H[j], b = H[j + 1];
+    int64_t r = lsqrt64(as_i64(static_cast<uint64_t>(mul_wrap(a, a)) +
+                               static_cast<uint64_t>(mul_wrap(b, b))));
+    if (r == 0) {
+      k.flags[1] = 1;
+      return;
+    }
+    const int64_t inv = ((static_cast<int64_t>(1) << 62) / r) >> (62 - bit - bit);
+    c[j] = mul_wrap(a, inv) >> bit;
+    s[j] = as_i64(0 - static_cast<uint64_t>(mul_wrap(b, inv))) >> bit;
+    H[j] = r;
+  }
+  H[j + 1] = 0;
+  e[j + 1] = mul_wrap(s[j], e[j]) >> bit;
+  e[j] = mul_wrap(c[j], e[j]) >> bit;
+  const int64_t ej = e[j + 1];
+  const int64_t ae = ej < 0 ? as_i64(0 - static_cast<uint64_t>(ej)) : ej;
+  k.flags[0] = (ae <= k.iscal[0]) ? 1 : 0;
+  const double fact = static_cast<double>(static_cast<int64_t>(1) << bit);
+  k.scal[5] = fabs(k.scal[2] * static_cast<double>(ej) / fact) / k.nrm_b;
+}
+
+__global__ __launch_bounds__(B) void k_axpy_dot_int(int n, const int32_t* __restrict__ vk,
+                                                    const int32_t* __restrict__ vk1,
+                                                    int32_t* __restrict__ w, KState k,
+                                                    int kk, int j) {
+  __shared__ uint64_t sh[B / 64];
+  __shared__ int flag;
+  const int64_t a = as_i64(0 - static_cast<uint64_t>(
+                                   static_cast<int64_t*>(k.H)[j * (k.m + 1) + kk]));
+  const int bit = k.bit;
+  const bool norm = (kk == j);
+  uint64_t acc = 0;
+  for (int i = blockIdx.x * B + threadIdx.x; i < n; i += kRedTotal) {
+    const int32_t wi = add32(w[i], low32(mul_wrap(a, vk[i]) >> bit));
+    w[i] = wi;
+    acc += norm ? prod64(wi, wi) : prod64(vk1[i], wi);
+  }
+  const uint64_t part = block_sum_u64<B>(acc, sh);
+  if (threadIdx.x == 0) k.ipart[blockIdx.x] = static_cast<int64_t>(part);
+  if (!last_block(k.counter, &flag)) return;
+  uint64_t s = 0;
+  for (int b = threadIdx.x; b < kRedBlocks; b += B) s += static_cast<uint64_t>(k.ipart[b]);
+  s = block_sum_u64<B>(s, sh);
+  if (threadIdx.x == 0) {
+    if (!norm)
+      static_cast<int64_t*>(k.H)[j * (k.m + 1) + kk + 1] = as_i64(s) >> bit;
+    else
+      givens_int(k, j, s);
+  }
+}
+
+__global__ __launch_bounds__(B) void k_scal_int(int n, int32_t* __restrict__ w,
+                                                KState k) {
+  const int i = blockIdx.x * B + threadIdx.x;
+  if (i >= n || k.flags[1]) return;
+  w[i] = low32(mul_wrap(k.iscal[1], w[i]) >> k.bit);
+}
+
+// ---- FP MGS (blocked reduction order) ------------------------------------
+
+template <typename T>
+__global__ __launch_bounds__(B) void k_dot_first_fp(int n, const T* __restrict__ v,
+                                                    const T* __restrict__ w,
+                                                    KState k, int j, T* part) {
+  __shared__ T sh[B];
+  __shared__ int flag;
+  T acc = 0;
+  for (int i = blockIdx.x * B + threadIdx.x; i < n; i += kRedTotal) acc += v[i] * w[i];
+  const T p = block_tree<B, T>(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = p;
+  if (!last_block(k.counter, &flag)) return;
+  const T tot = final_tree<T>(part, sh);
+  if (threadIdx.x == 0) static_cast<T*>(k.H)[j * (k.m + 1)] = tot;
+}
+
+template <typename T>
+__device__ __forceinline__ T fp_sqrt(T v);
+template <>
+__device__ __forceinline__ double fp_sqrt<double>(double v) { return sqrt(v); }
+template <>
+__device__ __forceinline__ float fp_sqrt<float>(float v) { return sqrtf(v); }
+
+template <typename T>
+__device__ void givens_fp(const KState& k, int j, T sumsq) {
+  T* H = static_cast<T*>(k.H) + j * (k.m + 1);
+  T* c = static_cast<T*>(k.c);
+  T* s = static_cast<T*>(k.s);
+  T* e = static_cast<T*>(k.e);
+  const T hn = fp_sqrt<T>(sumsq);
+  H[j + 1] = hn;
+  k.flags[3] = j + 1;
+  k.scal[4] = static_cast<double>(static_cast<T>(1) / hn);
+  for (int q = 0; q < j; ++q) {
+    const T t = H[q];
+    H[q] = c[q] * t - s[q] * H[q + 1];
+    H[q + 1] = s[q] * t + c[q] * H[q + 1];
+  }
+  {
+    const T a = H[j], b = H[j + 1];
+    const T r = fp_sqrt<T>(a * a + b * b);
+    c[j] = a / r;
+    s[j] = -b / r;
+    H[j] = r;
+  }
+  H[j + 1] = 0;
+  e[j + 1] = s[j] * e[j];
+  e[j] = c[j] * e[j];
+  const double ae = fabs(static_cast<double>(e[j + 1]));
+  k.flags[0] = (ae <= k.nrm_b * k.eps) ? 1 : 0;
+  k.scal[5] = ae / k.nrm_b;
+}
+
+template <typename T>
+__global__ __launch_bounds__(B) void k_axpy_dot_fp(int n, const T* __restrict__ vk,
+                                                   const T* __restrict__ vk1,
+                                                   T* __restrict__ w, KState k,
+                                                   int kk, int j, T* part) {
+  __shared__ T sh[B];
+  __shared__ int flag;
+  const T a = -static_cast<T*>(k.H)[j * (k.m + 1) + kk];
+  const bool norm = (kk == j);
+  T acc = 0;
+  for (int i = blockIdx.x * B + threadIdx.x; i < n; i += kRedTotal) {
+    const T wi = w[i] + a * vk[i];
+    w[i] = wi;
+    acc += norm ? wi * wi : vk1[i] * wi;
+  }
+  const T p = block_tree<B, T>(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = p;
+  if (!last_block(k.counter, &flag)) return;
+  const T tot = final_tree<T>(part, sh);
+  if (threadIdx.x == 0) {
+    if (!norm)
+      static_cast<T*>(k.H)[j * (k.m + 1) + kk + 1] = tot;
+    else
+      givens_fp<T>(k, j, tot);
+  }
+}
+
+// Sequential-order variants: one thread computes the reduction left to right.
+template <typename T>
+__global__ void k_seq_dot_fp(int n, const T* __restrict__ v, const T* __restrict__ w,
+                             KState k, int slot_j, int slot_k) {
+  T acc = 0;
+  for (int i = 0; i < n; ++i) acc += v[i] * w[i];
+  static_cast<T*>(k.H)[slot_j * (k.m + 1) + slot_k] = acc;
+}
+template <typename T>
+__global__ __launch_bounds__(B) void k_axpy_fp(int n, const T* __restrict__ vk,
+                                               T* __restrict__ w, KState k, int kk,
+                                               int j) {
+  const int i = blockIdx.x * B + threadIdx.x;
+  if (i >= n) return;
+  const T a = -static_cast<T*>(k.H)[j * (k.m + 1) + kk];
+  w[i] += a * vk[i];
+}
+template <typename T>
+__global__ void k_seq_norm_givens_fp(int n, const T* __restrict__ w, KState k, int j) {
+  T acc = 0;
+  for (int i = 0; i < n; ++i) acc += w[i] * w[i];
+  givens_fp<T>(k, j, acc);
+}
+
+template <typename T>
+__global__ __launch_bounds__(B) void k_scal_fp(int n, T* __restrict__ w, KState k) {
+  const int i = blockIdx.x * B + threadIdx.x;
+  if (i >= n) return;
+  w[i] *= static_cast<T>(k.scal[4]);
+}
+
+// ---- solution update -----------------------------------------------------
+
+__global__ void k_trsv_int(KState k, int steps) {
+  const int64_t* H = static_cast<const int64_t*>(k.H);
+  const int64_t* e = static_cast<const int64_t*>(k.e);
+  const int nn = k.m + 1;
+  const double fact = static_cast<double>(static_cast<int64_t>(1) << k.bit);
+  const double e0 = k.scal[2];
+  for (int i = steps - 1; i >= 0; --i) {
+    double temp = e0 * static_cast<double>(e[i]) / fact;
+    for (int q = steps - 1; q > i; --q)
+      temp -= static_cast<double>(H[q * nn + i]) / fact * k.y[q];
+    k.y[i] = temp / (static_cast<double>(H[i * nn + i]) / fact);
+  }
+}
+
+template <typename T>
+__global__ void k_trsv_fp(KState k, int steps, T* y) {
+  const T* H = static_cast<const T*>(k.H);
+  const T* e = static_cast<const T*>(k.e);
+  const int nn = k.m + 1;
+  for (int i = steps - 1; i >= 0; --i) {
+    T temp = e[i];
+    for (int q = steps - 1; q > i; --q) temp -= H[q * nn + i] * y[q];
+    y[i] = temp / H[i * nn + i];
+  }
+}
+
+__global__ __launch_bounds__(B) void k_update_x_int(int n, const int32_t* __restrict__ Z,
+                                                    double* __restrict__ x, KState k,
+                                                    int steps) {
+  const int i = blockIdx.x * B + threadIdx.x;
+  if (i >= n) return;
+  const double fact_inv = 1.0 / static_cast<double>(static_cast<int64_t>(1) << k.bit);
+  double xi = x[i];
+  for (int q = 0; q < steps; ++q)
+    xi += k.y[q] * static_cast<double>(Z[static_cast<size_t>(q) * n + i]) * fact_inv;
+  x[i] = xi;
+}
+
+__global__ __launch_bounds__(B) void k_update_x_f64(int n, const double* __restrict__ Z,
+                                                    double* __restrict__ x, KState k,
+                                                    int steps) {
+  const int i = blockIdx.x * B + threadIdx.x;
+  if (i >= n) return;
+  double xi = x[i];
+  for (int q = 0; q < steps; ++q) xi += k.y[q] * Z[static_cast<size_t>(q) * n + i];
+  x[i] = xi;
+}
+
+__global__ __launch_bounds__(B) void k_update_x_f32(int n, const float* __restrict__ Z,
+                                                    double* __restrict__ x, KState k,
+                                                    int steps) {
+  const int i = blockIdx.x * B + threadIdx.x;
+  if (i >= n) return;
+  double xi = x[i];
+  for (int q = 0; q < steps; ++q)
+    xi += static_cast<double>(k.yf[q] * Z[static_cast<size_t>(q) * n + i]);
+  x[i] = xi;
+}
+
+// ---- checksum --------------------------------------------------------------
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(B) void k_checksum(int n, const int32_t* __restrict__ v,
+                                                KState k) {
+  __shared__ uint64_t sh[B / 64];
+  __shared__ int flag;
+  uint64_t acc = 0;
+  for (int i = blockIdx.x * B + threadIdx.x; i < n; i += kRedTotal)
+    acc += mix64(static_cast<uint64_t>(i) * 0x9E3779B97F4A7C15ULL ^
+                 static_cast<uint64_t>(static_cast<uint32_t>(v[i])));
+  const uint64_t part = block_sum_u64<B>(acc, sh);
+  if (threadIdx.x == 0) k.ipart[blockIdx.x] = static_cast<int64_t>(part);
+  if (!last_block(k.counter, &flag)) return;
+  uint64_t s = 0;
+  for (int b = threadIdx.x; b < kRedBlocks; b += B) s += static_cast<uint64_t>(k.ipart[b]);
+  s = block_sum_u64<B>(s, sh);
+  if (threadIdx.x == 0) {
+    const int len = *k.trace_len;
+    if (len < k.trace_cap) k.trace[len] = s;
+    *k.trace_len = len + 1;
+  }
+}
+
+__global__ __launch_bounds__(B) void k_copy_bytes(size_t n4, const uint32_t* __restrict__ a,
+                                                  uint32_t* __restrict__ b) {
+  for (size_t i = blockIdx.x * static_cast<size_t>(B) + threadIdx.x; i < n4;
+       i += static_cast<size_t>(gridDim.x) * B)
+    b[i] = a[i];
+}
+
+__global__ __launch_bounds__(B) void k_zero_bytes(size_t n4, uint32_t* __restrict__ b) {
+  for (size_t i = blockIdx.x * static_cast<size_t>(B) + threadIdx.x; i < n4;
+       i += static_cast<size_t>(gridDim.x) * B)
+    b[i] = 0;
+}
+
+inline int grid_for(int n) { return (n + B - 1) / B; }
+
+}  // namespace
+
+hipError_t launch_outer_residual(const DCsr& a64, const double* b, const double* x,
+                                 double* r, const KState& k, Elem e, int seq_red,
+                                 hipStream_t s) {
+  if (seq_red) {
+    if (a64.n > 0)
+      hipLaunchKernelGGL(k_residual_only, dim3(grid_for(a64.n)), dim3(B), 0, s, a64.n,
+                         a64.rptr, a64.cind, static_cast<const double*>(a64.val), b,
+                         x, r);
+    hipLaunchKernelGGL(k_seq_norm, dim3(1), dim3(1), 0, s, a64.n, r, k, e, 1,
+                       static_cast<double*>(nullptr));
+  } else {
+    hipLaunchKernelGGL(k_outer_residual, dim3(kRedBlocks), dim3(B), 0, s, a64.n,
+                       a64.rptr, a64.cind, static_cast<const double*>(a64.val), b, x,
+                       r, k, e, 1, static_cast<double*>(nullptr));
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_relres(const DCsr& a64, const double* b, const double* x,
+                         double* r, double* part, unsigned* counter, double* out,
+                         hipStream_t s) {
+  KState k{};
+  k.dpart = part;
+  k.counter = counter;
+  hipLaunchKernelGGL(k_outer_residual, dim3(kRedBlocks), dim3(B), 0, s, a64.n,
+                     a64.rptr, a64.cind, static_cast<const double*>(a64.val), b, x,
+                     r, k, Elem::kF64, 0, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_v0(Elem e, int n, const double* r, void* v0, const KState& k,
+                     hipStream_t s) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_v0, dim3(grid_for(n)), dim3(B), 0, s, n, r, v0, k, e);
+  return hipGetLastError();
+}
+
+hipError_t launch_dot_first(Elem e, int n, const void* v0, const void* w,
+                            const KState& k, int j, int seq_red, hipStream_t s) {
+  if (e == Elem::kI32) {
+    hipLaunchKernelGGL(k_dot_first_int, dim3(kRedBlocks), dim3(B), 0, s, n,
+                       static_cast<const int32_t*>(v0), static_cast<const int32_t*>(w),
+                       k, j);
+  } else if (e == Elem::kF64) {
+    if (seq_red)
+      hipLaunchKernelGGL(k_seq_dot_fp<double>, dim3(1), dim3(1), 0, s, n,
+                         static_cast<const double*>(v0), static_cast<const double*>(w),
+                         k, j, 0);
+    else
+      hipLaunchKernelGGL(k_dot_first_fp<double>, dim3(kRedBlocks), dim3(B), 0, s, n,
+                         static_cast<const double*>(v0), static_cast<const double*>(w),
+                         k, j, k.dpart);
+  } else {
+    if (seq_red)
+      hipLaunchKernelGGL(k_seq_dot_fp<float>, dim3(1), dim3(1), 0, s, n,
+                         static_cast<const float*>(v0), static_cast<const float*>(w), k,
+                         j, 0);
+    else
+      hipLaunchKernelGGL(k_dot_first_fp<float>, dim3(kRedBlocks), dim3(B), 0, s, n,
+                         static_cast<const float*>(v0), static_cast<const float*>(w), k,
+                         j, k.fpart);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_axpy_dot(Elem e, int n, const void* V, void* w, const KState& k,
+                           int kk, int j, int seq_red, hipStream_t s) {
+  const size_t off = static_cast<size_t>(kk) * n;
+  const size_t off1 = static_cast<size_t>(kk + 1) * n;
+  if (e == Elem::kI32) {
+    const int32_t* v = static_cast<const int32_t*>(V);
+    hipLaunchKernelGGL(k_axpy_dot_int, dim3(kRedBlocks), dim3(B), 0, s, n, v + off,
+                       v + off1, static_cast<int32_t*>(w), k, kk, j);
+    return hipGetLastError();
+  }
+  if (e == Elem::kF64) {
+    const double* v = static_cast<const double*>(V);
+    double* ww = static_cast<double*>(w);
+    if (seq_red) {
+      hipLaunchKernelGGL(k_axpy_fp<double>, dim3(grid_for(n)), dim3(B), 0, s, n, v + off,
+                         ww, k, kk, j);
+      if (kk < j)
+        hipLaunchKernelGGL(k_seq_dot_fp<double>, dim3(1), dim3(1), 0, s, n, v + off1, ww,
+                           k, j, kk + 1);
+      else
+        hipLaunchKernelGGL(k_seq_norm_givens_fp<double>, dim3(1), dim3(1), 0, s, n, ww,
+                           k, j);
+    } else {
+      hipLaunchKernelGGL(k_axpy_dot_fp<double>, dim3(kRedBlocks), dim3(B), 0, s, n,
+                         v + off, v + off1, ww, k, kk, j, k.dpart);
+    }
+    return hipGetLastError();
+  }
+  const float* v = static_cast<const float*>(V);
+  float* ww = static_cast<float*>(w);
+  if (seq_red) {
+    hipLaunchKernelGGL(k_axpy_fp<float>, dim3(grid_for(n)), dim3(B), 0, s, n, v + off,
+                       ww, k, kk, j);
+    if (kk < j)
+      hipLaunchKernelGGL(k_seq_dot_fp<float>, dim3(1), dim3(1), 0, s, n, v + off1, ww, k,
+                         j, kk + 1);
+    else
+      hipLaunchKernelGGL(k_seq_norm_givens_fp<float>, dim3(1), dim3(1), 0, s, n, ww, k,
+                         j);
+  } else {
+    hipLaunchKernelGGL(k_axpy_dot_fp<float>, dim3(kRedBlocks), dim3(B), 0, s, n,
+                       v + off, v + off1, ww, k, kk, j, k.fpart);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_scal(Elem e, int n, void* w, const KState& k, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (e == Elem::kI32)
+    hipLaunchKernelGGL(k_scal_int, dim3(grid_for(n)), dim3(B), 0, s, n,
+                       static_cast<int32_t*>(w), k);
+  else if (e == Elem::kF64)
+    hipLaunchKernelGGL(k_scal_fp<double>, dim3(grid_for(n)), dim3(B), 0, s, n,
+                       static_cast<double*>(w), k);
+  else
+    hipLaunchKernelGGL(k_scal_fp<float>, dim3(grid_for(n)), dim3(B), 0, s, n,
+                       static_cast<float*>(w), k);
+  return hipGetLastError();
+}
+
+hipError_t launch_update_x(Elem e, int n, const void* Z, double* x,
+                                 const KState& k, int steps, hipStream_t s) {
+  if (steps <= 0) return hipSuccess;
+  if (e == Elem::kI32) {
+    hipLaunchKernelGGL(k_trsv_int, dim3(1), dim3(1), 0, s, k, steps);
+    if (n > 0)
+      hipLaunchKernelGGL(k_update_x_int, dim3(grid_for(n)), dim3(B), 0, s, n,
+                         static_cast<const int32_t*>(Z), x, k, steps);
+  } else if (e == Elem::kF64) {
+    hipLaunchKernelGGL(k_trsv_fp<double>, dim3(1), dim3(1), 0, s, k, steps, k.y);
+    if (n > 0)
+      hipLaunchKernelGGL(k_update_x_f64, dim3(grid_for(n)), dim3(B), 0, s, n,
+                         static_cast<const double*>(Z), x, k, steps);
+  } else {
+    hipLaunchKernelGGL(k_trsv_fp<float>, dim3(1), dim3(1), 0, s, k, steps, k.yf);
+    if (n > 0)
+      hipLaunchKernelGGL(k_update_x_f32, dim3(grid_for(n)), dim3(B), 0, s, n,
+                         static_cast<const float*>(Z), x, k, steps);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_checksum(int n, const int32_t* v, const KState& k, hipStream_t s) {
+  hipLaunchKernelGGL(k_checksum, dim3(kRedBlocks), dim3(B), 0, s, n, v, k);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy(Elem e, int n, const void* src, void* dst, hipStream_t s) {
+  const size_t bytes = static_cast<size_t>(n) * (e == Elem::kF64 ? 8 : 4);
+  const size_t n4 = bytes / 4;
+  if (n4 == 0) return hipSuccess;
+  const int grid = static_cast<int>(std::min<size_t>((n4 + B - 1) / B, 8192));
+  hipLaunchKernelGGL(k_copy_bytes, dim3(grid), dim3(B), 0, s, n4,
+                     static_cast<const uint32_t*>(src), static_cast<uint32_t*>(dst));
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_zero(size_t bytes, void* dst, hipStream_t s) {
+  const size_t n4 = bytes / 4;
+  if (n4 == 0) return hipSuccess;
+  const int grid = static_cast<int>(std::min<size_t>((n4 + B - 1) / B, 8192));
+  hipLaunchKernelGGL(k_zero_bytes, dim3(grid), dim3(B), 0, s, n4,
+                     static_cast<uint32_t*>(dst));
+  return hipGetLastError();
+}
+
+}  // namespace sc23ad::dev

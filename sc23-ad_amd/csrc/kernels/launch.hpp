@@ -1,0 +1,116 @@
+// Interface between the host-side orchestration (compiled by g++) and the
+// HIP kernels for gfx950 (compiled by hipcc).  Every launcher enqueues on the
+// given stream and returns hipSuccess or the launch error.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+
+namespace sc23ad::dev {
+
+// Fixed blocked order of every FP reduction on the device: thread g of a
+// kRedBlocks x kRedThreads grid sums elements g, g + kRedTotal, ... in
+// increasing order, then a halving tree s[t] += s[t + stride] combines the
+// threads of a block and, in the last block to finish, the block partials.
+constexpr int kRedBlocks = 1024;
+constexpr int kRedThreads = 256;
+constexpr int kRedTotal = kRedBlocks * kRedThreads;
+
+// Device view of a CSR matrix.  val points at int32 / float / double values.
+struct DCsr {
+  int n = 0, m = 0, nnz = 0, bit = 0, dbit = -1;
+  const int* rptr = nullptr;
+  const int* cind = nullptr;
+  const void* val = nullptr;
+  int group = 8;  // lanes per row for the integer SpMV (power of two <= 64)
+};
+
+enum class Elem : int { kF64 = 0, kF32 = 1, kI32 = 2 };
+enum class SpmvMode : int { kPlain = 0, kSub = 1, kAdd = 2 };
+// Triangular factor layouts:
+//   kLD  L + D/omega with the inverse diagonal stored last (ldtri)
+//   kL   strictly lower, unit diagonal (ltri)
+//   kDU  inverse diagonal first, then the upper entries; solved backward (dutri)
+enum class TriKind : int { kLD = 0, kL = 1, kDU = 2 };
+
+// y = spmv(x)  |  y = z - spmv(x)  |  y = z + spmv(x)   (y may alias z)
+hipError_t launch_spmv(Elem e, const DCsr& a, SpmvMode mode, const void* x,
+                       const void* z, void* y, hipStream_t s);
+
+// Level-scheduled triangular solve.  `perm` lists the rows chain by chain and
+// level by level; `lev_ptr` delimits the levels inside perm.
+struct TriSegment {
+  int chain_kernel;  // 1: one workgroup per chain walks levels [lev_begin_c, lev_end_c)
+                     // 0: one level, rows perm[row_begin, row_end), many workgroups
+  int nchain;
+  const int* chain_lev;  // device, 2 * nchain ints: level range of each chain
+  int row_begin, row_end;
+};
+hipError_t launch_trsv_segment(Elem e, TriKind kind, const DCsr& a,
+                               const int* perm, const int* lev_ptr,
+                               const TriSegment& seg, const void* in, void* out,
+                               hipStream_t s);
+
+// ---- Krylov kernels ------------------------------------------------------
+// Scalars and small arrays of one FGMRES(m) solver, resident on the device.
+struct KState {
+  int m;
+  int bit;          // fractional bits F (integer solver)
+  double nrm_b;
+  double eps;
+  // integer Arnoldi (int64) or FP64 / FP32 Arnoldi (stored as double / float)
+  void* H;          // (m+1) * m, column j at offset j*(m+1)
+  void* c;          // m
+  void* s;          // m
+  void* e;          // m+1
+  double* y;        // m (int / f64) ; float solver keeps y as float in yf
+  float* yf;
+  // reductions
+  int64_t* ipart;   // kRedBlocks partial int64 sums
+  double* dpart;    // kRedBlocks partial double sums
+  float* fpart;     // kRedBlocks partial float sums
+  unsigned* counter;  // last-block-done counters (several)
+  // scalars
+  double* scal;     // [0] nrm_r, [1] prev, [2] e0, [3] e_inv, [4] hn (fp), [5] implicit res
+  int64_t* iscal;   // [0] criteria, [1] hinv, [2] hn
+  int* flags;       // [0] converged, [1] overflow, [2] stagnated, [3] steps done
+  uint64_t* trace;  // checksum trace buffer (optional)
+  int trace_cap;
+  int* trace_len;   // device counter
+};
+
+// Outer residual r = b - A x (FP64, row sums in column order), its norm in the
+// fixed blocked order (seq_red: strictly left to right, one thread), the
+// Restarted stagnation test and the FGMRES start scalars for precision e.
+hipError_t launch_outer_residual(const DCsr& a64, const double* b,
+                                 const double* x, double* r, const KState& k,
+                                 Elem e, int seq_red, hipStream_t s);
+// V_0 from r: int: trunc(r * e_inv * 2^F); f64: r * e_inv; f32: (float)(r*e_inv)
+hipError_t launch_v0(Elem e, int n, const double* r, void* v0, const KState& k,
+                     hipStream_t s);
+// H(0,j) = <V_0, w>  (int: >> F)
+hipError_t launch_dot_first(Elem e, int n, const void* v0, const void* w,
+                            const KState& k, int j, int seq_red, hipStream_t s);
+// w += -H(k,j) V_k, then H(k+1,j) = <V_{k+1}, w> (k < j) or, for k == j, the
+// norm of w, H_inv / 1/h, the Givens rotations, e and the convergence flag.
+hipError_t launch_axpy_dot(Elem e, int n, const void* V, void* w, const KState& k,
+                           int kk, int j, int seq_red, hipStream_t s);
+// w = H_inv w (int) or w *= 1/h (fp)
+hipError_t launch_scal(Elem e, int n, void* w, const KState& k, hipStream_t s);
+// y = H^-1 e (FP64 / float as the reference) over the first `steps`
+// columns, then x += sum_k y_k Z_k in order of k.
+hipError_t launch_update_x(Elem e, int n, const void* Z, double* x,
+                           const KState& k, int steps, hipStream_t s);
+// order-free checksum of an int32 vector appended to k.trace
+hipError_t launch_checksum(int n, const int32_t* v, const KState& k,
+                           hipStream_t s);
+// explicit residual ||b - A x|| in the blocked order into *out (device)
+hipError_t launch_relres(const DCsr& a64, const double* b, const double* x,
+                         double* r, double* part, unsigned* counter,
+                         double* out, hipStream_t s);
+// V-cycle vector helpers
+hipError_t launch_copy(Elem e, int n, const void* src, void* dst, hipStream_t s);
+hipError_t launch_fill_zero(size_t bytes, void* dst, hipStream_t s);
+
+}  // namespace sc23ad::dev

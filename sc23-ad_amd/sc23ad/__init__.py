@@ -95,6 +95,7 @@ def _load() -> C.CDLL:
         "sc23ad_solver_solve": (C.c_int, [vp, vp, C.c_double, vp, C.c_int,
                                           C.POINTER(Converge)]),
         "sc23ad_solver_trace": (C.c_int, [vp, vp, C.c_int, ip]),
+        "sc23ad_solver_set_reduction": (C.c_int, [vp, C.c_int]),
         "sc23ad_solver_time_spmv": (C.c_int, [vp, C.c_int]),
         "sc23ad_solver_spmv_stats": (C.c_int, [vp, ip, dp]),
         "sc23ad_solver_relres": (C.c_int, [vp, vp, C.c_double, vp, C.c_int, dp]),
@@ -273,3 +274,156 @@ def _seq_nrm2(b: np.ndarray) -> float:
         return 0.0
     # np.add.accumulate adds strictly left to right (unlike np.sum).
     return float(np.sqrt(np.add.accumulate(b * b)[-1]))
+
+
+# ---- device -----------------------------------------------------------------
+
+def device_count() -> int:
+    n = C.c_int()
+    check(lib.sc23ad_device_count(C.byref(n)))
+    return n.value
+
+
+class DevArray:
+    """A device buffer holding a copy of a numpy array."""
+
+    def __init__(self, a: np.ndarray | None = None, nbytes: int | None = None,
+                 dtype=None, shape=None):
+        if a is not None:
+            a = np.ascontiguousarray(a)
+            nbytes, dtype, shape = a.nbytes, a.dtype, a.shape
+        self.nbytes, self.dtype, self.shape = nbytes, np.dtype(dtype), shape
+        p = C.c_void_p()
+        check(lib.sc23ad_dev_alloc(max(nbytes, 4), C.byref(p)))
+        self.p = p
+        if a is not None and nbytes:
+            check(lib.sc23ad_dev_upload(self.p, ptr(a), nbytes))
+
+    @classmethod
+    def empty(cls, shape, dtype):
+        n = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        return cls(nbytes=n, dtype=dtype, shape=shape)
+
+    def get(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            check(lib.sc23ad_dev_download(ptr(out), self.p, self.nbytes))
+        return out
+
+    def set(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        check(lib.sc23ad_dev_upload(self.p, ptr(a), self.nbytes))
+
+    def __del__(self):
+        if getattr(self, "p", None) is not None and self.p.value:
+            lib.sc23ad_dev_free(self.p)
+            self.p = C.c_void_p()
+
+
+DT = {FP64: np.float64, FP32: np.float32, INT: np.int32}
+
+
+class Amg:
+    """A hierarchy resident on the GPU (the reference's preconditioner)."""
+
+    def __init__(self, hier: Hierarchy):
+        h = C.c_void_p()
+        check(lib.sc23ad_amg_create(hier.h, C.byref(h)))
+        self.h = h
+        self.precision = hier.precision
+        self.dtype = DT[hier.precision]
+        self.depth, self.level_n, _ = hier.info()
+
+    def precondition(self, vin: np.ndarray) -> np.ndarray:
+        vin = np.ascontiguousarray(vin, dtype=self.dtype)
+        out = np.zeros(self.level_n[0], self.dtype)
+        check(lib.sc23ad_amg_precondition(self.h, ptr(vin), ptr(out), 0))
+        return out
+
+    def level_op(self, level: int, op: int, vin: np.ndarray, out_n: int,
+                 out_init: np.ndarray | None = None) -> np.ndarray:
+        din = DevArray(np.ascontiguousarray(vin, dtype=self.dtype))
+        init = (np.zeros(out_n, self.dtype) if out_init is None
+                else np.ascontiguousarray(out_init, dtype=self.dtype))
+        dout = DevArray(init)
+        check(lib.sc23ad_amg_level_op(self.h, level, op, din.p, dout.p))
+        return dout.get()
+
+    def default_frac_bits(self, b: np.ndarray, nrm_b: float) -> int:
+        bits = C.c_int()
+        check(lib.sc23ad_default_frac_bits(self.h, ptr(np.ascontiguousarray(b)),
+                                           nrm_b, C.byref(bits)))
+        return bits.value
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib.sc23ad_amg_destroy(self.h)
+            self.h = C.c_void_p()
+
+
+class Solver:
+    """Restarted(FGMRES(m)) on the GPU for solver 0/1/2."""
+
+    def __init__(self, precision: int, a: CsrMat, amg: Amg, m: int = 30,
+                 eps: float = 1e-10, bit: int = 29, max_restarts: int = 0):
+        h = C.c_void_p()
+        check(lib.sc23ad_solver_create(precision, a.h, amg.h, m, eps, bit,
+                                       max_restarts, C.byref(h)))
+        self.h = h
+        self.amg = amg
+        self.n = a.n
+
+    def set_sequential_reduction(self, on: bool) -> None:
+        check(lib.sc23ad_solver_set_reduction(self.h, 1 if on else 0))
+
+    def trace(self, cap: int) -> None:
+        check(lib.sc23ad_solver_trace(self.h, None, cap, None))
+
+    def read_trace(self, cap: int) -> np.ndarray:
+        buf = np.zeros(max(cap, 1), np.uint64)
+        n = C.c_int()
+        check(lib.sc23ad_solver_trace(self.h, ptr(buf), cap, C.byref(n)))
+        return buf[:n.value]
+
+    def time_spmv(self, on: bool) -> None:
+        check(lib.sc23ad_solver_time_spmv(self.h, 1 if on else 0))
+
+    def spmv_stats(self):
+        n, ms = C.c_int(), C.c_double()
+        check(lib.sc23ad_solver_spmv_stats(self.h, C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+    def solve(self, b: np.ndarray, nrm_b: float, x0: np.ndarray | None = None):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(self.n) if x0 is None else np.array(x0, dtype=np.float64)
+        cv = Converge()
+        check(lib.sc23ad_solver_solve(self.h, ptr(b), nrm_b, ptr(x), 0, C.byref(cv)))
+        return x, cv
+
+    def solve_device(self, b: DevArray, nrm_b: float, x: DevArray) -> Converge:
+        cv = Converge()
+        check(lib.sc23ad_solver_solve(self.h, b.p, nrm_b, x.p, 1, C.byref(cv)))
+        return cv
+
+    def relres(self, b: np.ndarray, nrm_b: float, x: np.ndarray) -> float:
+        out = C.c_double()
+        check(lib.sc23ad_solver_relres(self.h, ptr(np.ascontiguousarray(b)), nrm_b,
+                                       ptr(np.ascontiguousarray(x)), 0, C.byref(out)))
+        return out.value
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib.sc23ad_solver_destroy(self.h)
+            self.h = C.c_void_p()
+
+
+def spmv_int(rptr, cind, val, bit, x) -> np.ndarray:
+    """Raw device integer SpMV on host arrays (copied in and out)."""
+    n = len(rptr) - 1
+    dr, dc, dv = DevArray(np.asarray(rptr, np.int32)), DevArray(np.asarray(cind, np.int32)), \
+        DevArray(np.asarray(val, np.int32))
+    dx = DevArray(np.asarray(x, np.int32))
+    dy = DevArray.empty((n,), np.int32)
+    check(lib.sc23ad_spmv_int(n, dr.p, dc.p, dv.p, bit, dx.p, dy.p))
+    return dy.get()
