@@ -1,0 +1,246 @@
+#!/usr/bin/env python3
+"""Benchmark of the solve phase of int-AMG-FGMRES(m) on one MI355X.
+
+Workload (default): the 3D 7-point Laplacian on a 128^3 grid (2,097,152 rows,
+14,581,760 nonzeros, values 6 / -1, Dirichlet), x* ~ U[-1,1] from seed 1,
+b = A x*, both row-scaled as the reference driver does; classical AMG
+(theta 0.8) with the BlockSOR(40) smoother of the reference's multi.exe
+(--smoother 0 selects seq.exe's SOR ordering), Restarted(FGMRES(30)) in
+integer arithmetic (solver 2) with the driver's fractional-bit rule, stopped at
+a relative residual of 1e-8.
+
+One step = one full solve from x = 0 with b and x already resident in HBM.
+The line printed on stdout is the benchmark record; progress goes to stderr.
+Data are synthetic (generated), never downloaded.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "sc23-ad_amd"))
+
+METRIC = ("solve time to 1e-8 rel. residual; int-SpMV HBM GB/s; "
+          "learned/batched vs plain")
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def spmv_int_bytes(n: int, m: int, nnz: int) -> int:
+    """Algorithmic HBM bytes of one fixed-point CSR SpMV: int32 values and
+    column indices, row pointers, the gathered vector read once, the result
+    written once."""
+    return 8 * nnz + 4 * (n + 1) + 4 * m + 4 * n
+
+
+def read_traffic(path: str):
+    """HBM bytes per SpMV launch from a committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE in KiB per dispatch, scaled by the copy-kernel
+    calibration factor recorded with them)."""
+    if not os.path.exists(path):
+        return None
+    txt = open(path).read()
+    f = re.search(r"^FETCH_SIZE_KiB_per_dispatch:\s*([0-9.eE+-]+)", txt, re.M)
+    w = re.search(r"^WRITE_SIZE_KiB_per_dispatch:\s*([0-9.eE+-]+)", txt, re.M)
+    c = re.search(r"^copy_calibration_factor:\s*([0-9.eE+-]+)", txt, re.M)
+    if not (f and w):
+        return None
+    factor = float(c.group(1)) if c else 1.0
+    return (float(f.group(1)) + float(w.group(1))) * 1024.0 * factor
+
+
+def write_system_file(path, a, b_unscaled):
+    rptr, cind, val = a
+    with open(path, "wb") as fh:
+        fh.write(b"SC23SYS1")
+        fh.write(np.asarray([len(rptr) - 1, len(cind)], np.int32).tobytes())
+        fh.write(np.ascontiguousarray(rptr, np.int32).tobytes())
+        fh.write(np.ascontiguousarray(cind, np.int32).tobytes())
+        fh.write(np.ascontiguousarray(val, np.float64).tobytes())
+        fh.write(np.ascontiguousarray(b_unscaled, np.float64).tobytes())
+
+
+def cpu_baseline(args, unscaled_arrays, b_unscaled, smoother):
+    """The reference (oracle/_ref, built from the reference sources with the
+    reference Makefile's GNU flags) timed on this host: Restarted::Solve only,
+    as Work/main.cpp times it.  Falls back to the oracle's CPU restatement
+    (single thread) when the reference build is absent."""
+    import oracle.oracle as O  # noqa: F401  (checker; cpu_baseline leg only)
+    exe = O.MULTI_REF if smoother == 1 else O.SEQ_REF
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
+    threads = min(threads, 16)
+    if os.path.exists(exe):
+        sysf = os.path.join("/tmp", f"sc23ad_sys_{os.getpid()}.bin")
+        write_system_file(sysf, unscaled_arrays, b_unscaled)
+        env = dict(os.environ)
+        env["OMP_NUM_THREADS"] = str(threads if smoother == 1 else 1)
+        cmd = [exe, sysf, str(smoother), "2", str(args.m), "0", repr(args.rtol)]
+        log("cpu baseline:", " ".join(cmd))
+        t0 = time.time()
+        out = subprocess.run(cmd, capture_output=True, text=True, env=env,
+                             timeout=args.cpu_timeout)
+        wall = time.time() - t0
+        os.unlink(sysf)
+        tm = re.search(r"^\[Time\] : ([0-9.]+)", out.stdout, re.M)
+        it = re.search(r"^\[Iter\] : (\d+)", out.stdout, re.M)
+        rs = re.search(r"^# \[Res\] : ([0-9.eE+-]+)", out.stdout, re.M)
+        if out.returncode != 0 or not tm:
+            log("cpu baseline failed", out.returncode, out.stderr[-500:])
+            return None
+        return {"value": float(tm.group(1)), "unit": "s",
+                "cores": threads if smoother == 1 else 1, "kind": "reference",
+                "sample": (f"full Restarted::Solve of the same system "
+                           f"({'multi.exe, BlockSOR(40)' if smoother == 1 else 'seq.exe, SOR'},"
+                           f" solver 2, m={args.m}, eps={args.rtol}); "
+                           f"iterations {it.group(1) if it else '?'}, explicit relres "
+                           f"{rs.group(1) if rs else '?'}; process wall incl. setup {wall:.1f} s"),
+                "iterations": int(it.group(1)) if it else None}
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__,
+                                 formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=128, help="grid side of the 3D 7-point system")
+    ap.add_argument("--smoother", type=int, default=1, choices=(0, 1),
+                    help="0: SOR (seq.exe ordering), 1: BlockSOR(40) (multi.exe)")
+    ap.add_argument("--m", type=int, default=30)
+    ap.add_argument("--rtol", type=float, default=1e-8)
+    ap.add_argument("--no-fp", action="store_true", help="skip the FP64/FP32 solver runs")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-timeout", type=float, default=240.0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: this build has no multi-GPU path "
+              f"(requested --gpus {args.gpus}).", file=sys.stderr)
+        sys.exit(2)
+
+    import sc23ad as S
+
+    if S.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(3)
+
+    n = args.n
+    t0 = time.time()
+    sy = S.make_system(S.GEN_LAPLACE3D, (n,), 1)
+    N, _, nnz = sy.a.info()
+    log(f"system: 3D 7-point {n}^3, N={N}, nnz={nnz} ({time.time() - t0:.1f} s)")
+    # unscaled copy for the CPU baseline (the reference scales it itself)
+    unscaled = S.CsrMat.generate(S.GEN_LAPLACE3D, n).arrays()
+
+    t0 = time.time()
+    prm = S.amg_param(args.smoother)
+    precs = [S.INT, S.FP64] + ([] if args.no_fp else [S.FP32])
+    hs = S.Hierarchy.build_multi(sy.a, precs, prm)
+    hint, h64 = hs[0], hs[1]
+    depth, level_n, level_nnz = hint.info()
+    log(f"setup: depth {depth}, level rows {list(level_n)} ({time.time() - t0:.1f} s)")
+
+    amg64 = S.Amg(h64)
+    bit = amg64.default_frac_bits(sy.b, sy.nrm_b)
+    amgi = S.Amg(hint)
+    log(f"fractional bits (Work/main.cpp rule): {bit}")
+
+    b_dev = S.DevArray(sy.b)
+    x_dev = S.DevArray(np.zeros(N))
+    zeros = np.zeros(N)
+
+    def run(solver, steps, warmup, label):
+        for _ in range(warmup):
+            x_dev.set(zeros)
+            solver.solve_device(b_dev, sy.nrm_b, x_dev)
+        S.check(S.lib.sc23ad_dev_sync())
+        times, cv = [], None
+        for _ in range(steps):
+            x_dev.set(zeros)
+            S.check(S.lib.sc23ad_dev_sync())
+            t = time.perf_counter()
+            cv = solver.solve_device(b_dev, sy.nrm_b, x_dev)
+            S.check(S.lib.sc23ad_dev_sync())
+            times.append(time.perf_counter() - t)
+        x = x_dev.get()
+        rel = solver.relres(sy.b, sy.nrm_b, x)
+        log(f"{label}: {np.mean(times) * 1e3:.1f} ms/solve (device {cv.solve_ms:.1f} ms), "
+            f"iter {cv.iter}, converged {cv.is_converged}, explicit relres {rel:.3e}")
+        return float(np.mean(times)), cv, rel
+
+    soli = S.Solver(S.INT, sy.a, amgi, m=args.m, eps=args.rtol, bit=bit)
+    soli.time_spmv(True)
+    t_int, cv_int, rel_int = run(soli, args.steps, args.warmup, "solver 2 (int)")
+    launches, spmv_ms = soli.spmv_stats()
+    spmv_bytes = spmv_int_bytes(N, N, nnz)
+    achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9 if spmv_ms > 0 else None
+
+    extra = {}
+    if not args.no_fp:
+        sol0 = S.Solver(S.FP64, sy.a, amg64, m=args.m, eps=args.rtol)
+        t0s, cv0, rel0 = run(sol0, 1, 1, "solver 0 (fp64)")
+        amg32 = S.Amg(hs[2])
+        sol1 = S.Solver(S.FP32, sy.a, amg32, m=args.m, eps=args.rtol)
+        t1s, cv1, rel1 = run(sol1, 1, 1, "solver 1 (fp32)")
+        extra = {"solver0_fp64_s": t0s, "solver0_iter": cv0.iter, "solver0_relres": rel0,
+                 "solver1_fp32_s": t1s, "solver1_iter": cv1.iter, "solver1_relres": rel1}
+
+    traffic = read_traffic(os.path.join(HERE, "profiles",
+                                        f"spmv_int_level0_laplace3d_{n}_pmc.txt"))
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline(args, unscaled, sy.b_unscaled, args.smoother)
+        except Exception as e:  # the baseline is reported, never required
+            log("cpu baseline error:", e)
+
+    rec = {
+        "metric": METRIC,
+        "value": t_int,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_int * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32/int64 (FP64 outer refinement)",
+        "data": "synthetic (generated 3D 7-point Laplacian, x* ~ U[-1,1] seed 1)",
+        "config": {"workload": f"int-AMG-FGMRES({args.m}) solve, 3D 7-point Laplacian "
+                               f"{n}^3 (N={N}, nnz={nnz}), "
+                               f"{'BlockSOR(40)' if args.smoother == 1 else 'SOR'} smoother, "
+                               f"rtol {args.rtol}",
+                   "solver": 2, "restart_m": args.m, "frac_bits": bit,
+                   "smoother": "BlockSOR(40)" if args.smoother == 1 else "SOR",
+                   "amg_levels": [int(v) for v in level_n]},
+        "iterations": cv_int.iter,
+        "converged": bool(cv_int.is_converged),
+        "explicit_relres": rel_int,
+        "roofline": {"bound": "hbm", "kernel": "fixed-point CSR SpMV, level 0 (A)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": (achieved / HBM_PEAK_GBPS) if achieved else None,
+                     "traffic": traffic, "algorithmic_bytes": spmv_bytes,
+                     "launches_timed": launches, "mean_ms": spmv_ms},
+        "cpu_baseline": cpu,
+    }
+    rec.update(extra)
+    if cpu and cpu.get("value"):
+        rec["speedup_vs_cpu"] = cpu["value"] / t_int
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -123,6 +123,11 @@ int sc23ad_convdiff_draws(int count, uint64_t seed, int* n, double* pe,
 /* ---- host hierarchy (setup; no GPU needed) ------------------------------- */
 int sc23ad_hier_build(const sc23ad_csrmat* a, const sc23ad_amg_param* p,
                       int precision, sc23ad_hier** out);
+/* Several precisions from one coarsening chain (the chain is the expensive,
+   precision-independent part of the setup): outs[k] receives the hierarchy
+   for precisions[k]. */
+int sc23ad_hier_build_multi(const sc23ad_csrmat* a, const sc23ad_amg_param* p,
+                            const int* precisions, int count, sc23ad_hier** outs);
 /* depth (number of coarsening steps); level_n / level_nnz receive depth+1
    entries (operator rows / nnz per level, the last one the coarsest grid). */
 int sc23ad_hier_info(const sc23ad_hier* h, int* depth, int* level_n,
@@ -187,6 +192,9 @@ int sc23ad_solver_trace(sc23ad_solver* s, uint64_t* buf, int cap, int* len);
    and report their count and mean duration. */
 int sc23ad_solver_time_spmv(sc23ad_solver* s, int enable);
 int sc23ad_solver_spmv_stats(sc23ad_solver* s, int* launches, double* mean_ms);
+/* |e_{j+1}| / nrm_b after every Arnoldi step of the last solve (the
+   reference's "# [j]" convergence history, gmres.hpp:253-255). */
+int sc23ad_solver_history(sc23ad_solver* s, double* buf, int cap, int* len);
 /* Explicit relative residual ||b - A x|| / nrm_b on the device (FP64). */
 int sc23ad_solver_relres(sc23ad_solver* s, const double* b, double nrm_b,
                          const double* x, int on_device, double* relres);

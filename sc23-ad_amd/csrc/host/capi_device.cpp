@@ -257,6 +257,16 @@ int sc23ad_solver_set_reduction(sc23ad_solver* s, int sequential) {
   return SC23AD_OK;
 }
 
+int sc23ad_solver_history(sc23ad_solver* s, double* buf, int cap, int* len) {
+  if (!s || !len) return fail(SC23AD_BAD_PARAM, "history: null");
+  const auto& h = s->s->history;
+  const int n = static_cast<int>(h.size());
+  *len = n;
+  if (buf)
+    for (int i = 0; i < n && i < cap; ++i) buf[i] = h[i];
+  return SC23AD_OK;
+}
+
 int sc23ad_solver_time_spmv(sc23ad_solver* s, int enable) {
   if (!s) return fail(SC23AD_BAD_PARAM, "time_spmv: null");
   s->s->time_spmv = enable != 0;

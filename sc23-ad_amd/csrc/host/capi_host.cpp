@@ -172,14 +172,7 @@ int sc23ad_convdiff_draws(int count, uint64_t seed, int* n, double* pe,
   return SC23AD_OK;
 }
 
-int sc23ad_hier_build(const sc23ad_csrmat* a, const sc23ad_amg_param* p,
-                      int precision, sc23ad_hier** out) {
-  if (!a || !p || !out) return fail(SC23AD_BAD_PARAM, "hier_build: bad arguments");
-  if (precision < 0 || precision > 2)
-    return fail(SC23AD_BAD_PARAM, "hier_build: precision must be 0, 1 or 2");
-  if (p->smoother != 0 && p->smoother != 1)
-    return fail(SC23AD_BAD_PARAM, "hier_build: smoother must be 0 or 1");
-  if (a->a.n != a->a.m) return fail(SC23AD_BAD_PARAM, "hier_build: matrix must be square");
+static AmgParams to_params(const sc23ad_amg_param* p) {
   AmgParams prm;
   prm.max_levels = p->max_levels;
   prm.theta = p->theta;
@@ -189,25 +182,57 @@ int sc23ad_hier_build(const sc23ad_csrmat* a, const sc23ad_amg_param* p,
   prm.iter = p->iter;
   prm.b_num = p->b_num;
   prm.coarse_n = p->coarse_n;
+  return prm;
+}
+
+int sc23ad_hier_build_multi(const sc23ad_csrmat* a, const sc23ad_amg_param* p,
+                            const int* precisions, int count, sc23ad_hier** outs) {
+  if (!a || !p || !precisions || !outs || count < 1)
+    return fail(SC23AD_BAD_PARAM, "hier_build: bad arguments");
+  for (int k = 0; k < count; ++k) {
+    outs[k] = nullptr;
+    if (precisions[k] < 0 || precisions[k] > 2)
+      return fail(SC23AD_BAD_PARAM, "hier_build: precision must be 0, 1 or 2");
+  }
+  if (p->smoother != 0 && p->smoother != 1)
+    return fail(SC23AD_BAD_PARAM, "hier_build: smoother must be 0 or 1");
+  if (a->a.n != a->a.m) return fail(SC23AD_BAD_PARAM, "hier_build: matrix must be square");
+  const AmgParams prm = to_params(p);
   try {
     AmgChain ch = build_chain(a->a, prm);
-    auto* h = new sc23ad_hier;
-    h->precision = precision;
-    h->warnings = ch.warnings;
-    h->fine_nnz = a->a.nnz();
-    if (precision == SC23AD_INT)
-      h->ih = build_int_hierarchy(a->a, ch, prm);
-    else if (precision == SC23AD_FP64)
-      h->dh = build_fp_hierarchy<double>(a->a, ch, prm);
-    else
-      h->fh = build_fp_hierarchy<float>(a->a, ch, prm);
-    *out = h;
+    for (int k = 0; k < count; ++k) {
+      auto h = std::make_unique<sc23ad_hier>();
+      h->precision = precisions[k];
+      h->warnings = ch.warnings;
+      h->fine_nnz = a->a.nnz();
+      if (precisions[k] == SC23AD_INT)
+        h->ih = build_int_hierarchy(a->a, ch, prm);
+      else if (precisions[k] == SC23AD_FP64)
+        h->dh = build_fp_hierarchy<double>(a->a, ch, prm);
+      else
+        h->fh = build_fp_hierarchy<float>(a->a, ch, prm);
+      outs[k] = h.release();
+    }
   } catch (const SetupError& e) {
+    for (int k = 0; k < count; ++k) {
+      delete outs[k];
+      outs[k] = nullptr;
+    }
     return fail(e.code, "hier_build: " + e.what);
   } catch (const std::bad_alloc&) {
+    for (int k = 0; k < count; ++k) {
+      delete outs[k];
+      outs[k] = nullptr;
+    }
     return fail(SC23AD_SETUP_FAIL, "hier_build: out of host memory");
   }
   return SC23AD_OK;
+}
+
+int sc23ad_hier_build(const sc23ad_csrmat* a, const sc23ad_amg_param* p,
+                      int precision, sc23ad_hier** out) {
+  if (!out) return fail(SC23AD_BAD_PARAM, "hier_build: bad arguments");
+  return sc23ad_hier_build_multi(a, p, &precision, 1, out);
 }
 
 int sc23ad_hier_info(const sc23ad_hier* h, int* depth, int* level_n,

@@ -100,9 +100,10 @@ void upload_mat(DevMat& dm, int n, int m, int bit, int dbit,
 // chains of that many rows (the BlockSOR partition), each walked by one
 // workgroup in a single launch; otherwise levels are walked in order, small
 // ones inside one workgroup and large ones by a launch of their own.
+template <typename V>
 void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr,
-                const std::vector<int>& cind, int chain_size, int nchain_req,
-                int single_wg_max) {
+                const std::vector<int>& cind, const std::vector<V>& val, int chain_size,
+                int nchain_req, int single_wg_max) {
   plan.kind = kind;
   if (kind != TriKind::kL)
     for (int i = 0; i < n; ++i)
@@ -224,6 +225,99 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
     plan.segs.push_back(ps.s);
   }
   plan.nlaunch = static_cast<int>(plan.segs.size());
+
+  // Level-ordered layout: dependency entries in the order the sequential
+  // solve subtracts them (ascending columns; descending for DU).
+  std::vector<int> lbeg(static_cast<size_t>(n) + 1, 0), lcol;
+  std::vector<V> lval, ldiag(static_cast<size_t>(std::max(n, 1)), V(0));
+  lcol.reserve(cind.size());
+  lval.reserve(cind.size());
+  for (int idx = 0; idx < n; ++idx) {
+    const int i = perm[idx];
+    const int rb = rptr[i], re = rptr[i + 1];
+    if (kind == TriKind::kLD) {
+      for (int p = rb; p < re - 1; ++p) {
+        lcol.push_back(cind[p]);
+        lval.push_back(val[p]);
+      }
+      ldiag[idx] = val[re - 1];
+    } else if (kind == TriKind::kL) {
+      for (int p = rb; p < re; ++p) {
+        lcol.push_back(cind[p]);
+        lval.push_back(val[p]);
+      }
+    } else {
+      for (int p = re - 1; p >= rb + 1; --p) {
+        lcol.push_back(cind[p]);
+        lval.push_back(val[p]);
+      }
+      ldiag[idx] = val[rb];
+    }
+    lbeg[idx + 1] = static_cast<int>(lcol.size());
+  }
+  std::vector<int> row0(nchain);
+  int max_rows = 0;
+  for (int c = 0; c < nchain; ++c) {
+    row0[c] = chain_begin[c];
+    max_rows = std::max(max_rows, chain_begin[c + 1] - chain_begin[c]);
+  }
+  plan.lrow.alloc(sizeof(int) * std::max(n, 1));
+  plan.lrow.upload(perm.data(), sizeof(int) * n);
+  plan.lbeg.alloc(sizeof(int) * lbeg.size());
+  plan.lbeg.upload(lbeg.data(), sizeof(int) * lbeg.size());
+  plan.lcol.alloc(sizeof(int) * std::max<size_t>(lcol.size(), 1));
+  plan.lcol.upload(lcol.data(), sizeof(int) * lcol.size());
+  plan.lval.alloc(sizeof(V) * std::max<size_t>(lval.size(), 1));
+  plan.lval.upload(lval.data(), sizeof(V) * lval.size());
+  plan.ldiag.alloc(sizeof(V) * ldiag.size());
+  plan.ldiag.upload(ldiag.data(), sizeof(V) * ldiag.size());
+  plan.chain_row0.alloc(sizeof(int) * std::max(nchain, 1));
+  plan.chain_row0.upload(row0.data(), sizeof(int) * nchain);
+  dev::TriLayout& L = plan.layout;
+  L.lrow = plan.lrow.as<int>();
+  L.lbeg = plan.lbeg.as<int>();
+  L.lcol = plan.lcol.as<int>();
+  L.lval = plan.lval.get();
+  L.ldiag = plan.ldiag.get();
+  L.lev_ptr = plan.lev_ptr.as<int>();
+  L.chain_row0 = plan.chain_row0.as<int>();
+  // Solutions stay in LDS when every chain is walked by one workgroup in a
+  // single launch and the largest chain fits.
+  const bool one_launch_chains =
+      plan.segs.size() == 1 && plan.segs[0].chain_kernel == 1 &&
+      (nchain > 1 || (all_chain_lev.size() == 2 && all_chain_lev[0] == 0 &&
+                      all_chain_lev[1] == static_cast<int>(lev_ptr.size()) - 1));
+  L.lds_rows = (one_launch_chains &&
+                sizeof(V) * static_cast<size_t>(max_rows) <= dev::kTriBlockedMaxLds)
+                   ? max_rows
+                   : 0;
+}
+
+// Per-row block boundaries of the dependency entries for the blocked
+// single-workgroup solve; returns false when the factor does not fit in LDS.
+bool build_blocked(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr,
+                   const std::vector<int>& cind, size_t elem, size_t acc) {
+  if (n <= 0 || dev::tri_blocked_lds(n, elem, acc) > dev::kTriBlockedMaxLds) return false;
+  const int W = dev::kTriBlockWidth;
+  const int nb = (n + W - 1) / W;
+  std::vector<int> bp(static_cast<size_t>(n) * (nb + 1));
+  for (int i = 0; i < n; ++i) {
+    int b0 = rptr[i], e0 = rptr[i + 1];
+    if (kind == TriKind::kLD) e0 = std::max(b0, e0 - 1);
+    if (kind == TriKind::kDU) b0 = std::min(e0, b0 + 1);
+    int p = b0;
+    for (int b = 0; b <= nb; ++b) {
+      const int lim = b * W;
+      while (p < e0 && cind[p] < lim) ++p;
+      bp[static_cast<size_t>(i) * (nb + 1) + b] = p;
+    }
+  }
+  plan.bptr.alloc(sizeof(int) * bp.size());
+  plan.bptr.upload(bp.data(), sizeof(int) * bp.size());
+  plan.nb = nb;
+  plan.blocked = true;
+  plan.nlaunch = 1;
+  return true;
 }
 
 }  // namespace
@@ -265,13 +359,17 @@ std::unique_ptr<DevAmg> DevAmg::upload(const IntHierarchy& h, const LaunchCfg& c
     upload_mat(D.r, L.r.n, L.r.m, L.r.bit, -1, L.r.rptr, L.r.cind, L.r.val, c.spmv_group);
     upload_mat(D.p, L.p.n, L.p.m, L.p.bit, -1, L.p.rptr, L.p.cind, L.p.val, c.spmv_group);
     const int size = bj ? (L.a.n + h.prm.b_num - 1) / h.prm.b_num : 0;
-    build_plan(D.ld_plan, TriKind::kLD, L.ld.n, L.ld.rptr, L.ld.cind, size,
+    build_plan(D.ld_plan, TriKind::kLD, L.ld.n, L.ld.rptr, L.ld.cind, L.ld.val, size,
                bj ? h.prm.b_num : 1, c.single_wg_max_rows);
   }
   upload_mat(a->lc, h.lc.n, h.lc.m, h.lc.bit, -1, h.lc.rptr, h.lc.cind, h.lc.val, 0);
   upload_mat(a->uc, h.uc.n, h.uc.m, h.uc.bit, h.uc.dbit, h.uc.rptr, h.uc.cind, h.uc.val, 0);
-  build_plan(a->lc_plan, TriKind::kL, h.lc.n, h.lc.rptr, h.lc.cind, 0, 1, c.single_wg_max_rows);
-  build_plan(a->uc_plan, TriKind::kDU, h.uc.n, h.uc.rptr, h.uc.cind, 0, 1, c.single_wg_max_rows);
+  build_plan(a->lc_plan, TriKind::kL, h.lc.n, h.lc.rptr, h.lc.cind, h.lc.val, 0, 1,
+             c.single_wg_max_rows);
+  build_plan(a->uc_plan, TriKind::kDU, h.uc.n, h.uc.rptr, h.uc.cind, h.uc.val, 0, 1,
+             c.single_wg_max_rows);
+  build_blocked(a->lc_plan, TriKind::kL, h.lc.n, h.lc.rptr, h.lc.cind, 4, 8);
+  build_blocked(a->uc_plan, TriKind::kDU, h.uc.n, h.uc.rptr, h.uc.cind, 4, 8);
   a->alloc_work();
   return a;
 }
@@ -296,13 +394,17 @@ std::unique_ptr<DevAmg> DevAmg::upload_fp(const FpHierarchy<T>& h, const LaunchC
     upload_mat(D.r, L.r.n, L.r.m, 0, -1, L.r.rptr, L.r.cind, L.r.val, 1);
     upload_mat(D.p, L.p.n, L.p.m, 0, -1, L.p.rptr, L.p.cind, L.p.val, 1);
     const int size = bj ? (L.a.n + h.prm.b_num - 1) / h.prm.b_num : 0;
-    build_plan(D.ld_plan, TriKind::kLD, L.ld.n, L.ld.rptr, L.ld.cind, size,
+    build_plan(D.ld_plan, TriKind::kLD, L.ld.n, L.ld.rptr, L.ld.cind, L.ld.val, size,
                bj ? h.prm.b_num : 1, c.single_wg_max_rows);
   }
   upload_mat(a->lc, h.lc.n, h.lc.m, 0, -1, h.lc.rptr, h.lc.cind, h.lc.val, 1);
   upload_mat(a->uc, h.uc.n, h.uc.m, 0, -1, h.uc.rptr, h.uc.cind, h.uc.val, 1);
-  build_plan(a->lc_plan, TriKind::kL, h.lc.n, h.lc.rptr, h.lc.cind, 0, 1, c.single_wg_max_rows);
-  build_plan(a->uc_plan, TriKind::kDU, h.uc.n, h.uc.rptr, h.uc.cind, 0, 1, c.single_wg_max_rows);
+  build_plan(a->lc_plan, TriKind::kL, h.lc.n, h.lc.rptr, h.lc.cind, h.lc.val, 0, 1,
+             c.single_wg_max_rows);
+  build_plan(a->uc_plan, TriKind::kDU, h.uc.n, h.uc.rptr, h.uc.cind, h.uc.val, 0, 1,
+             c.single_wg_max_rows);
+  build_blocked(a->lc_plan, TriKind::kL, h.lc.n, h.lc.rptr, h.lc.cind, sizeof(T), sizeof(T));
+  build_blocked(a->uc_plan, TriKind::kDU, h.uc.n, h.uc.rptr, h.uc.cind, sizeof(T), sizeof(T));
   a->alloc_work();
   return a;
 }
@@ -319,10 +421,15 @@ void DevAmg::spmv(const DevMat& m, SpmvMode mode, const void* x, const void* z,
 
 void DevAmg::trsv(const TriPlan& plan, const DevMat& m, const void* in,
                   void* out) const {
+  if (plan.blocked) {
+    hip_check(dev::launch_trsv_blocked(elem, plan.kind, m.d, plan.bptr.as<int>(), plan.nb,
+                                       in, out, lib_stream()),
+              "trsv blocked");
+    return;
+  }
   for (const auto& seg : plan.segs)
-    hip_check(dev::launch_trsv_segment(elem, plan.kind, m.d, plan.perm.as<int>(),
-                                       plan.lev_ptr.as<int>(), seg, in, out,
-                                       lib_stream()),
+    hip_check(dev::launch_trsv_layout(elem, plan.kind, plan.layout, seg, m.d.bit, m.d.dbit,
+                                      in, out, lib_stream()),
               "trsv");
 }
 
@@ -403,8 +510,9 @@ DevSolver::DevSolver(Elem e, const HostCsr& a, DevAmg* amg_, int m_, double eps_
   r.alloc(sizeof(double) * std::max(n, 1));
   // state layout (bytes, 8-aligned)
   const size_t H = 8 * static_cast<size_t>(m + 1) * m;
-  const size_t sz = H + 8 * m * 2 + 8 * (m + 1) + 8 * m + 4 * m + 8 * dev::kRedBlocks * 2 +
-                    4 * dev::kRedBlocks + 64 + 8 * 8 + 8 * 4 + 4 * 8 + 16 + 128;
+  const size_t sz = H + 8 * m * 2 + 8 * (m + 1) + 8 * m + 4 * m + 8 * dev::kRedBlocks * 3 +
+                    8 * dev::kRedBlocks * 2 + 4 * dev::kRedBlocks * 2 + 64 + 8 * 8 + 8 * 4 +
+                    4 * 8 + 16 + 128;
   state.alloc(sz);
   hip_check(hipMemsetAsync(state.get(), 0, sz, lib_stream()), "memset state");
   char* p = state.as<char>();
@@ -422,9 +530,9 @@ DevSolver::DevSolver(Elem e, const HostCsr& a, DevAmg* amg_, int m_, double eps_
   k.e = take(8 * (m + 1));
   k.y = reinterpret_cast<double*>(take(8 * m));
   k.yf = reinterpret_cast<float*>(take(4 * m));
-  k.ipart = reinterpret_cast<int64_t*>(take(8 * dev::kRedBlocks));
-  k.dpart = reinterpret_cast<double*>(take(8 * dev::kRedBlocks));
-  k.fpart = reinterpret_cast<float*>(take(4 * dev::kRedBlocks));
+  k.ipart = reinterpret_cast<int64_t*>(take(8 * 3 * dev::kRedBlocks));
+  k.dpart = reinterpret_cast<double*>(take(8 * 2 * dev::kRedBlocks));
+  k.fpart = reinterpret_cast<float*>(take(4 * 2 * dev::kRedBlocks));
   k.counter = reinterpret_cast<unsigned*>(take(64));
   k.scal = reinterpret_cast<double*>(take(8 * 8));
   k.iscal = reinterpret_cast<int64_t*>(take(8 * 4));
@@ -432,7 +540,7 @@ DevSolver::DevSolver(Elem e, const HostCsr& a, DevAmg* amg_, int m_, double eps_
   k.trace_len = reinterpret_cast<int*>(take(16));
   k.trace = nullptr;
   k.trace_cap = 0;
-  hip_check(hipHostMalloc(reinterpret_cast<void**>(&host_flags), 8 * sizeof(int),
+  hip_check(hipHostMalloc(reinterpret_cast<void**>(&host_flags), 8 * sizeof(int) + 16,
                           hipHostMallocDefault),
             "hipHostMalloc");
   hip_check(hipStreamSynchronize(lib_stream()), "solver init");
@@ -440,12 +548,16 @@ DevSolver::DevSolver(Elem e, const HostCsr& a, DevAmg* amg_, int m_, double eps_
 
 DevSolver::~DevSolver() {
   if (host_flags) hipHostFree(host_flags);
+  for (hipEvent_t e : spmv_events) hipEventDestroy(e);
 }
 
 void DevSolver::read_flags() {
   hip_check(hipMemcpyAsync(host_flags, k.flags, 4 * sizeof(int), hipMemcpyDeviceToHost,
                            lib_stream()),
             "flags");
+  hip_check(hipMemcpyAsync(host_flags + 8, k.scal + 5, sizeof(double),
+                           hipMemcpyDeviceToHost, lib_stream()),
+            "residual");
   hip_check(hipStreamSynchronize(lib_stream()), "flags sync");
 }
 
@@ -476,13 +588,19 @@ DevSolver::Result DevSolver::solve(const double* b_dev, double nrm_b, double* x_
   const int n = amg->n0();
   const size_t es = amg->esize();
   k.nrm_b = nrm_b;
-  hipEvent_t e0, e1, s0 = nullptr, s1 = nullptr;
+  hipEvent_t e0, e1;
   hip_check(hipEventCreate(&e0), "event");
   hip_check(hipEventCreate(&e1), "event");
-  if (time_spmv) {
-    hip_check(hipEventCreate(&s0), "event");
-    hip_check(hipEventCreate(&s1), "event");
-  }
+  int spmv_used = 0;  // event pairs recorded in this solve
+  auto spmv_event = [&](int idx) {
+    while (static_cast<int>(spmv_events.size()) <= idx) {
+      hipEvent_t ev;
+      hip_check(hipEventCreate(&ev), "event");
+      spmv_events.push_back(ev);
+    }
+    return spmv_events[idx];
+  };
+  history.clear();
   // Restarted: prev = 0 before the first pass
   hip_check(hipMemsetAsync(k.scal, 0, 8 * sizeof(double), s), "reset scal");
   hip_check(hipMemsetAsync(k.flags, 0, 8 * sizeof(int), s), "reset flags");
@@ -513,21 +631,18 @@ DevSolver::Result DevSolver::solve(const double* b_dev, double nrm_b, double* x_
         hip_check(dev::launch_checksum(n, static_cast<const int32_t*>(vj), k, s), "cs");
         hip_check(dev::launch_checksum(n, static_cast<const int32_t*>(zj), k, s), "cs");
       }
-      if (time_spmv) hip_check(hipEventRecord(s0, s), "record");
+      if (time_spmv) hip_check(hipEventRecord(spmv_event(2 * spmv_used), s), "record");
       amg->spmv(amg->lv[0].a, SpmvMode::kPlain, zj, nullptr, w);
       if (time_spmv) {
-        hip_check(hipEventRecord(s1, s), "record");
-        hip_check(hipEventSynchronize(s1), "sync");
-        float ms = 0;
-        hip_check(hipEventElapsedTime(&ms, s0, s1), "elapsed");
-        spmv_ms_total += ms;
-        ++spmv_launches;
+        hip_check(hipEventRecord(spmv_event(2 * spmv_used + 1), s), "record");
+        ++spmv_used;
       }
       hip_check(dev::launch_dot_first(elem, n, Vb, w, k, j, seq, s), "dot");
       for (int kk = 0; kk <= j; ++kk)
         hip_check(dev::launch_axpy_dot(elem, n, Vb, w, k, kk, j, seq, s), "axpy_dot");
-      hip_check(dev::launch_scal(elem, n, w, k, s), "scal");
+      hip_check(dev::launch_scal(elem, n, w, k, j, seq, s), "scal");
       read_flags();
+      history.push_back(*reinterpret_cast<double*>(host_flags + 8));
       steps = j + 1;
       if (host_flags[1]) {
         ovf = true;
@@ -560,10 +675,14 @@ DevSolver::Result DevSolver::solve(const double* b_dev, double nrm_b, double* x_
   double sc[8];
   hip_check(hipMemcpy(sc, k.scal, sizeof sc, hipMemcpyDeviceToHost), "scal");
   res.implicit_relres = sc[5];
+  for (int q = 0; q < spmv_used; ++q) {
+    float t = 0;
+    hip_check(hipEventElapsedTime(&t, spmv_events[2 * q], spmv_events[2 * q + 1]), "elapsed");
+    spmv_ms_total += t;
+    ++spmv_launches;
+  }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
-  if (s0) hipEventDestroy(s0);
-  if (s1) hipEventDestroy(s1);
   return res;
 }
 

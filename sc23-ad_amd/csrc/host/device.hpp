@@ -72,6 +72,13 @@ struct TriPlan {
   std::vector<dev::TriSegment> segs;
   int nlevels = 0;    // longest dependency chain (levels of the deepest chain)
   int nlaunch = 0;
+  // blocked single-workgroup solve (coarsest grid)
+  bool blocked = false;
+  DevBuf bptr;
+  int nb = 0;
+  // level-ordered layout used by the level-scheduled kernels
+  DevBuf lrow, lbeg, lcol, lval, ldiag, chain_row0;
+  dev::TriLayout layout;
 };
 
 struct DevLevel {
@@ -137,13 +144,15 @@ class DevSolver {
   SolverOpts opts;
   DevBuf V, Z, r, b, x, state, pinned_dummy;
   dev::KState k{};
-  int* host_flags = nullptr;  // pinned
+  int* host_flags = nullptr;  // pinned: flags[0..3], then the implicit residual
   // tracing / timing
   bool tracing = false;
   int trace_cap = 0;
   bool time_spmv = false;
   int spmv_launches = 0;
   double spmv_ms_total = 0;
+  std::vector<hipEvent_t> spmv_events;  // pairs, reused across solves
+  std::vector<double> history;          // |e_{j+1}| / nrm_b per Arnoldi step
 
   DevSolver(dev::Elem e, const HostCsr& a, DevAmg* amg, int m, double eps,
             int bit, int max_outer);

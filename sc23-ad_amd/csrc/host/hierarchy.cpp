@@ -1,14 +1,32 @@
 #include "hierarchy.hpp"
 
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <string>
 
 namespace sc23ad {
+
+namespace {
+struct Lap {
+  bool on = std::getenv("SC23AD_VERBOSE") != nullptr;
+  std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+  void operator()(const char* what) {
+    if (!on) return;
+    auto now = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "[sc23ad setup] %-24s %8.3f s\n", what,
+                 std::chrono::duration<double>(now - t).count());
+    t = now;
+  }
+};
+}  // namespace
 
 AmgChain build_chain(const HostCsr& fine, const AmgParams& prm) {
   if (prm.max_levels < 1 || prm.iter < 1 || !(prm.omega > 0) ||
       (prm.smoother == SmootherKind::kBlockSOR && prm.b_num < 1))
     throw SetupError{kBadParam, "invalid AMG parameters"};
+  Lap lap;
   AmgChain ch;
   // INTAMG.hpp:162-195 / 38-51: coarsen until max_levels is reached or a
   // level has at most coarse_n rows.  Each step consumes the unscaled
@@ -27,6 +45,7 @@ AmgChain build_chain(const HostCsr& fine, const AmgParams& prm) {
     ch.interp.push_back(std::move(nx.interp));
     ch.restrict_.push_back(std::move(nx.restrict_));
   }
+  lap("Galerkin chain");
   return ch;
 }
 
@@ -147,66 +166,54 @@ HostCsr extract_dinv_u(const HostCsr& a) {
 }
 
 // Complete sparse LU without pivoting, row by row (IKJ), keeping all fill and
-// explicit zeros.  Row i is eliminated against each earlier row k in
-// ascending column order; the multiplier is a_ik / u_kk and the merged entries
-// are a_ij - m * u_kj (helper_matrix.hpp:978-1060).  A row k without a stored
-// diagonal reuses the previous multiplier, as the reference does.
+// explicit zeros (helper_matrix.hpp:978-1060).  Row i is eliminated against
+// each earlier row k in ascending column order, including fill created on the
+// way: the multiplier is a_ik / u_kk, stored at column k, and every column c
+// of row k's upper part becomes a_ic - m * u_kc (or -m * u_kc where row i had
+// no entry).  A row k without a stored diagonal reuses the previous
+// multiplier and contributes nothing, as the reference does.  The row is
+// held in a dense accumulator; each entry sees the same operations in the
+// same order as in the reference's sorted merge.
 HostCsr complete_lu(const HostCsr& a) {
   const int n = a.n;
   HostCsr f;
   f.reset(n, a.m);
   if (n == 0) return f;
-  for (int p = a.rptr[0]; p < a.rptr[1]; ++p) {
-    f.cind.push_back(a.cind[p]);
-    f.val.push_back(a.val[p]);
-  }
-  f.rptr[1] = static_cast<int>(f.cind.size());
-  std::vector<int> tc, tc2;
-  std::vector<double> tv, tv2;
+  std::vector<int> diag_pos(n, -1);
+  std::vector<double> acc(static_cast<size_t>(a.m), 0.0);
+  std::vector<char> present(static_cast<size_t>(a.m), 0);
   double pivot = 0;
-  for (int i = 1; i < n; ++i) {
-    tc.assign(a.cind.begin() + a.rptr[i], a.cind.begin() + a.rptr[i + 1]);
-    tv.assign(a.val.begin() + a.rptr[i], a.val.begin() + a.rptr[i + 1]);
-    size_t count = 0;
-    while (count < tc.size() && tc[count] < i) {
-      const int k = tc[count];
-      tc2.assign(tc.begin(), tc.begin() + count);
-      tv2.assign(tv.begin(), tv.begin() + count);
-      int j = f.rptr[k];
-      for (; j < f.rptr[k + 1]; ++j)
-        if (f.cind[j] == k) {
-          pivot = tv[count] / f.val[j];
-          break;
-        }
-      ++j;
-      size_t kp = count + 1;
-      tc2.push_back(k);
-      tv2.push_back(pivot);
-      const int kend = f.rptr[k + 1];
-      while (kp < tc.size() || j < kend) {
-        const int c1 = kp < tc.size() ? tc[kp] : n + 1;
-        const int c2 = j < kend ? f.cind[j] : n + 1;
-        if (c1 < c2) {
-          tc2.push_back(tc[kp]);
-          tv2.push_back(tv[kp]);
-          ++kp;
-        } else if (c1 == c2) {
-          tc2.push_back(tc[kp]);
-          tv2.push_back(tv[kp] - pivot * f.val[j]);
-          ++kp;
-          ++j;
-        } else {
-          tc2.push_back(f.cind[j]);
-          tv2.push_back(-pivot * f.val[j]);
-          ++j;
+  for (int i = 0; i < n; ++i) {
+    for (int p = a.rptr[i]; p < a.rptr[i + 1]; ++p) {
+      acc[a.cind[p]] = a.val[p];
+      present[a.cind[p]] = 1;
+    }
+    if (i > 0) {
+      for (int k = 0; k < i; ++k) {
+        if (!present[k]) continue;
+        const int dp = diag_pos[k];
+        if (dp >= 0) pivot = acc[k] / f.val[dp];
+        acc[k] = pivot;
+        if (dp < 0) continue;
+        const int end = f.rptr[k + 1];
+        for (int j = dp + 1; j < end; ++j) {
+          const int c = f.cind[j];
+          if (present[c]) {
+            acc[c] = acc[c] - pivot * f.val[j];
+          } else {
+            acc[c] = -pivot * f.val[j];
+            present[c] = 1;
+          }
         }
       }
-      ++count;
-      tc.swap(tc2);
-      tv.swap(tv2);
     }
-    f.cind.insert(f.cind.end(), tc.begin(), tc.end());
-    f.val.insert(f.val.end(), tv.begin(), tv.end());
+    for (int c = 0; c < a.m; ++c) {
+      if (!present[c]) continue;
+      if (c == i) diag_pos[i] = static_cast<int>(f.cind.size());
+      f.cind.push_back(c);
+      f.val.push_back(acc[c]);
+      present[c] = 0;
+    }
     f.rptr[i + 1] = static_cast<int>(f.cind.size());
   }
   return f;
@@ -307,8 +314,10 @@ IntCsr to_dinvu_flex(const HostCsr& du) {
   return o;
 }
 
+
 IntHierarchy build_int_hierarchy(const HostCsr& fine, const AmgChain& chain,
                                  const AmgParams& prm) {
+  Lap lap;
   IntHierarchy h;
   h.prm = prm;
   h.depth = chain.depth;
@@ -319,6 +328,7 @@ IntHierarchy build_int_hierarchy(const HostCsr& fine, const AmgChain& chain,
   // rows of R) by the row's max |a|, then multiply R's columns by the
   // previous level's row maxima.  `prev` starts as zeros, so a hierarchy with
   // one coarse level gets R_0 = 0 exactly as the reference does.
+  lap("copies");
   std::vector<double> prev(static_cast<size_t>(fine.n), 0.0);
   for (int l = 0; l < d; ++l) {
     HostCsr& c = ca[l];
@@ -339,14 +349,20 @@ IntHierarchy build_int_hierarchy(const HostCsr& fine, const AmgChain& chain,
     IntLevel& L = h.lv[l];
     const HostCsr& src = (l == 0) ? fine : ca[l - 1];
     L.a = (l == 0) ? to_fixed_csr(fine, 30) : to_flex(src);
+    lap("  A");
     HostCsr ld, u;
     smoother_split(src, prm, ld, u);
+    lap("  split");
     L.ld = to_ld_flex(extract_ld_inv(ld));
+    lap("  LD");
     L.u = to_flex(u);
     L.r = to_flex(rt[l]);
     L.p = to_flex(chain.interp[l]);
+    lap("  U R P");
   }
+  lap("int levels");
   HostCsr lu = complete_lu(ca[d - 1]);
+  lap("coarse LU");
   h.lc = to_l_flex(extract_l(lu));
   h.uc = to_dinvu_flex(extract_dinv_u(lu));
   h.n_coarse = ca[d - 1].n;

@@ -1,8 +1,12 @@
 #include "rs_setup.hpp"
 
+#include <chrono>
 #include <climits>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <utility>
+#include <algorithm>
 
 namespace sc23ad {
 
@@ -314,72 +318,108 @@ HostCsr interpolation(const HostCsr& a, const HostCsr& cs, const HostCsr& ds,
 
 }  // namespace
 
+// Each output row is built with a dense accumulator: for every column c the
+// terms alpha_k * R(k,c) arrive in the order of k exactly as in the sorted
+// merge (a column that is absent takes alpha*x, a present one y + alpha*x),
+// and a value that becomes exactly zero is removed, so a later term restarts
+// it from alpha*x just as the merge would.  Rows are independent and are
+// computed in parallel; the result does not depend on the thread count.
 HostCsr rowmerge_product(const HostCsr& l, const HostCsr& r) {
   HostCsr out;
   out.reset(l.n, r.m);
-  std::vector<int> acc_c, tmp_c;
-  std::vector<double> acc_v, tmp_v;
-  for (int i = 0; i < l.n; ++i) {
-    acc_c.clear();
-    acc_v.clear();
-    for (int q = l.rptr[i]; q < l.rptr[i + 1]; ++q) {
-      const double alpha = l.val[q];
-      const int k = l.cind[q];
-      int xp = r.rptr[k];
-      const int xe = r.rptr[k + 1];
-      size_t yp = 0;
-      tmp_c.clear();
-      tmp_v.clear();
-      while (xp < xe || yp < acc_c.size()) {
-        const int xc = xp < xe ? r.cind[xp] : INT_MAX;
-        const int yc = yp < acc_c.size() ? acc_c[yp] : INT_MAX;
-        double v;
-        int c;
-        if (xc < yc) {
-          c = xc;
-          v = alpha * r.val[xp];
-          ++xp;
-        } else if (xc == yc) {
-          c = xc;
-          v = acc_v[yp] + alpha * r.val[xp];
-          ++xp;
-          ++yp;
-        } else {
-          c = yc;
-          v = acc_v[yp];
-          ++yp;
+  const int n = l.n;
+  std::vector<std::vector<int>> rc(n);
+  std::vector<std::vector<double>> rv(n);
+#pragma omp parallel
+  {
+    std::vector<double> acc(static_cast<size_t>(r.m), 0.0);
+    std::vector<char> present(static_cast<size_t>(r.m), 0);
+    std::vector<int> touched;
+#pragma omp for schedule(dynamic, 256)
+    for (int i = 0; i < n; ++i) {
+      touched.clear();
+      for (int q = l.rptr[i]; q < l.rptr[i + 1]; ++q) {
+        const double alpha = l.val[q];
+        const int k = l.cind[q];
+        for (int p = r.rptr[k]; p < r.rptr[k + 1]; ++p) {
+          const int c = r.cind[p];
+          double v;
+          if (present[c]) {
+            v = acc[c] + alpha * r.val[p];
+          } else {
+            v = alpha * r.val[p];
+            touched.push_back(c);
+          }
+          if (std::fabs(v) == 0.0) {
+            present[c] = 0;
+          } else {
+            present[c] = 1;
+            acc[c] = v;
+          }
         }
-        if (std::fabs(v) == 0.0) continue;
-        tmp_c.push_back(c);
-        tmp_v.push_back(v);
       }
-      acc_c.swap(tmp_c);
-      acc_v.swap(tmp_v);
+      std::sort(touched.begin(), touched.end());
+      touched.erase(std::unique(touched.begin(), touched.end()), touched.end());
+      std::vector<int>& oc = rc[i];
+      std::vector<double>& ov = rv[i];
+      for (int c : touched) {
+        if (present[c]) {
+          oc.push_back(c);
+          ov.push_back(acc[c]);
+        }
+        present[c] = 0;
+      }
     }
-    out.cind.insert(out.cind.end(), acc_c.begin(), acc_c.end());
-    out.val.insert(out.val.end(), acc_v.begin(), acc_v.end());
-    out.rptr[i + 1] = static_cast<int>(out.cind.size());
+  }
+  for (int i = 0; i < n; ++i) out.rptr[i + 1] = out.rptr[i] + static_cast<int>(rc[i].size());
+  out.cind.resize(out.rptr[n]);
+  out.val.resize(out.rptr[n]);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; ++i) {
+    std::copy(rc[i].begin(), rc[i].end(), out.cind.begin() + out.rptr[i]);
+    std::copy(rv[i].begin(), rv[i].end(), out.val.begin() + out.rptr[i]);
   }
   return out;
 }
 
+namespace {
+struct PhaseClock {
+  bool on = std::getenv("SC23AD_VERBOSE") != nullptr;
+  std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+  void lap(const char* what, int n) {
+    if (!on) return;
+    auto now = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "[sc23ad setup] n=%d %-14s %8.3f s\n", n, what,
+                 std::chrono::duration<double>(now - t).count());
+    t = now;
+  }
+};
+}  // namespace
+
 CoarsenStep rs_coarsen(const HostCsr& a, double theta) {
   if (theta <= 0 || 1 < theta)
     throw SetupError{kBadParam, "theta must satisfy 0 < theta <= 1"};
+  PhaseClock clk;
   CoarsenStep step;
   HostCsr strong, weak;
   rs_strength(a, theta, strong, weak);
   HostCsr strong_t = pattern_transpose(strong);
+  clk.lap("strength", a.n);
   step.cf = rs_first_pass(strong, strong_t, &step.warnings);
+  clk.lap("first pass", a.n);
   rs_second_pass(strong, step.cf);
+  clk.lap("second pass", a.n);
   HostCsr cs, ds;
   split_strong(strong, step.cf, cs, ds);
   step.interp = interpolation(a, cs, ds, weak, step.cf, &step.warnings);
+  clk.lap("interpolation", a.n);
   step.restrict_ = csr_transpose(step.interp);
   // get_coarse_matrix (helper_multigrid.hpp:564-586): rows of P^T A first,
   // then (P^T A) P.
   HostCsr rt_a = rowmerge_product(step.restrict_, a);
+  clk.lap("R*A", a.n);
   step.coarse = rowmerge_product(rt_a, step.interp);
+  clk.lap("(RA)*P", a.n);
   return step;
 }
 

@@ -45,6 +45,22 @@ __device__ __forceinline__ T final_tree(const T* part, T* sh) {
   return block_tree<B, T>(a + b, sh);
 }
 
+// Thread t of virtual block vb sums f(i) over i = vb*B + t + q*kRedTotal in
+// order; the block's halving tree gives part[vb].
+template <typename T, typename F>
+__device__ __forceinline__ void dro_partials(int n, T* part, T* sh, F&& f) {
+  for (int vb = blockIdx.x; vb < kRedBlocks; vb += gridDim.x) {
+    if (vb * B >= n) {
+      if (threadIdx.x == 0) part[vb] = 0;
+      continue;
+    }
+    T acc = 0;
+    for (int i = vb * B + threadIdx.x; i < n; i += kRedTotal) acc += f(i);
+    const T p = block_tree<B, T>(acc, sh);
+    if (threadIdx.x == 0) part[vb] = p;
+  }
+}
+
 __device__ __forceinline__ void start_scalars(const KState& k, Elem e,
                                               double nrm_r) {
   const double prev = k.scal[1];
@@ -74,27 +90,30 @@ __device__ __forceinline__ void start_scalars(const KState& k, Elem e,
   }
 }
 
-// r = b - A x with each row summed in column order; blocked-order norm.
+// r = b - A x with each row summed in column order, plus the blocked-order
+// partial sums of r_i^2.
 __global__ __launch_bounds__(B) void k_outer_residual(
     int n, const int* __restrict__ rptr, const int* __restrict__ cind,
     const double* __restrict__ val, const double* __restrict__ b,
-    const double* __restrict__ x, double* __restrict__ r, KState k, Elem e,
-    int with_start, double* out_norm) {
+    const double* __restrict__ x, double* __restrict__ r, double* __restrict__ part) {
   __shared__ double sh[B];
-  __shared__ int flag;
-  double acc = 0;
-  for (int i = blockIdx.x * B + threadIdx.x; i < n; i += kRedTotal) {
+  dro_partials<double>(n, part, sh, [&](int i) {
     double t = 0;
     const int re = rptr[i + 1];
     for (int p = rptr[i]; p < re; ++p) t += val[p] * x[cind[p]];
     const double ri = b[i] - t;
     r[i] = ri;
-    acc += ri * ri;
-  }
-  const double part = block_tree<B, double>(acc, sh);
-  if (threadIdx.x == 0) k.dpart[blockIdx.x] = part;
-  if (!last_block(k.counter, &flag)) return;
-  const double tot = final_tree<double>(k.dpart, sh);
+    return ri * ri;
+  });
+}
+
+// norm = sqrt(tree over the partials); optionally the Restarted / FGMRES
+// start scalars.
+__global__ __launch_bounds__(B) void k_norm_finalize(const double* __restrict__ part,
+                                                     KState k, Elem e, int with_start,
+                                                     double* out_norm) {
+  __shared__ double sh[B];
+  const double tot = final_tree<double>(part, sh);
   if (threadIdx.x == 0) {
     const double nrm = sqrt(tot);
     if (with_start) start_scalars(k, e, nrm);
@@ -140,32 +159,43 @@ __global__ __launch_bounds__(B) void k_v0(int n, const double* __restrict__ r,
   }
 }
 
+// ---- partial sums -----------------------------------------------------------
+//
+// Reductions write one partial per (virtual) block and never use atomics: the
+// next kernel in the stream re-derives the scalar from the partials in every
+// workgroup (the same tree in each, so the same value), which avoids
+// device-wide fences and contended atomics across the XCDs.
+//
+// Integer partials: G physical blocks, grid-stride, order-free sum.
+// FP partials: the fixed 1024 x 256 layout; physical block pb computes the
+// virtual blocks pb, pb + G, ... exactly as that layout defines them.
+
+__device__ __forceinline__ uint64_t sum_parts(const int64_t* part, int G, uint64_t* sh) {
+  uint64_t s = 0;
+  for (int b = threadIdx.x; b < G; b += B) s += static_cast<uint64_t>(part[b]);
+  s = block_sum_u64<B>(s, sh);
+  return s;  // valid in thread 0
+}
+
 // ---- integer MGS ---------------------------------------------------------
 
 __global__ __launch_bounds__(B) void k_dot_first_int(int n, const int32_t* __restrict__ v,
                                                      const int32_t* __restrict__ w,
-                                                     KState k, int j) {
+                                                     int64_t* __restrict__ part) {
   __shared__ uint64_t sh[B / 64];
-  __shared__ int flag;
   uint64_t acc = 0;
-  for (int i = blockIdx.x * B + threadIdx.x; i < n; i += kRedTotal)
-    acc += prod64(v[i], w[i]);
-  const uint64_t part = block_sum_u64<B>(acc, sh);
-  if (threadIdx.x == 0) k.ipart[blockIdx.x] = static_cast<int64_t>(part);
-  if (!last_block(k.counter, &flag)) return;
-  uint64_t s = 0;
-  for (int b = threadIdx.x; b < kRedBlocks; b += B) s += static_cast<uint64_t>(k.ipart[b]);
-  s = block_sum_u64<B>(s, sh);
-  if (threadIdx.x == 0) static_cast<int64_t*>(k.H)[j * (k.m + 1)] = as_i64(s) >> k.bit;
+  const int stride = gridDim.x * B;
+  for (int i = blockIdx.x * B + threadIdx.x; i < n; i += stride) acc += prod64(v[i], w[i]);
+  const uint64_t p = block_sum_u64<B>(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = static_cast<int64_t>(p);
 }
 
-__device__ void givens_int(const KState& k, int j, uint64_t sumsq) {
+__device__ void givens_int(const KState& k, int j, int64_t hn) {
   int64_t* H = static_cast<int64_t*>(k.H) + j * (k.m + 1);
   int64_t* c = static_cast<int64_t*>(k.c);
   int64_t* s = static_cast<int64_t*>(k.s);
   int64_t* e = static_cast<int64_t*>(k.e);
   const int bit = k.bit;
-  const int64_t hn = lsqrt64(as_i64(sumsq));
   H[j + 1] = hn;
   k.iscal[2] = hn;
   k.flags[3] = j + 1;
@@ -173,7 +203,6 @@ __device__ void givens_int(const KState& k, int j, uint64_t sumsq) {
     k.flags[1] = 1;
     return;
   }
-  k.iscal[1] = ((static_cast<int64_t>(1) << 62) / hn) >> (62 - bit - bit);
   for (int q = 0; q < j; ++q) {  // Grot
     const int64_t t = H[q];
     H[q] = as_i64(static_cast<uint64_t>(mul_wrap(c[q], t)) -
@@ -204,58 +233,70 @@ __device__ void givens_int(const KState& k, int j, uint64_t sumsq) {
   k.scal[5] = fabs(k.scal[2] * static_cast<double>(ej) / fact) / k.nrm_b;
 }
 
+// w += (int32)((-H(kk,j) * V_kk) >> F), then the partials of <V_{kk+1}, w>
+// (kk < j) or of <w, w> (kk == j).  H(kk,j) is finished here from the
+// previous pass's partials.
 __global__ __launch_bounds__(B) void k_axpy_dot_int(int n, const int32_t* __restrict__ vk,
                                                     const int32_t* __restrict__ vk1,
                                                     int32_t* __restrict__ w, KState k,
-                                                    int kk, int j) {
+                                                    int kk, int j,
+                                                    const int64_t* __restrict__ pin,
+                                                    int64_t* __restrict__ pout) {
   __shared__ uint64_t sh[B / 64];
-  __shared__ int flag;
-  const int64_t a = as_i64(0 - static_cast<uint64_t>(
-                                   static_cast<int64_t*>(k.H)[j * (k.m + 1) + kk]));
+  __shared__ int64_t hsh;
+  const uint64_t tot = sum_parts(pin, gridDim.x, sh);
+  if (threadIdx.x == 0) {
+    const int64_t h = as_i64(tot) >> k.bit;
+    hsh = h;
+    if (blockIdx.x == 0) static_cast<int64_t*>(k.H)[j * (k.m + 1) + kk] = h;
+  }
+  __syncthreads();
+  const int64_t a = as_i64(0 - static_cast<uint64_t>(hsh));
   const int bit = k.bit;
   const bool norm = (kk == j);
   uint64_t acc = 0;
-  for (int i = blockIdx.x * B + threadIdx.x; i < n; i += kRedTotal) {
+  const int stride = gridDim.x * B;
+  for (int i = blockIdx.x * B + threadIdx.x; i < n; i += stride) {
     const int32_t wi = add32(w[i], low32(mul_wrap(a, vk[i]) >> bit));
     w[i] = wi;
     acc += norm ? prod64(wi, wi) : prod64(vk1[i], wi);
   }
-  const uint64_t part = block_sum_u64<B>(acc, sh);
-  if (threadIdx.x == 0) k.ipart[blockIdx.x] = static_cast<int64_t>(part);
-  if (!last_block(k.counter, &flag)) return;
-  uint64_t s = 0;
-  for (int b = threadIdx.x; b < kRedBlocks; b += B) s += static_cast<uint64_t>(k.ipart[b]);
-  s = block_sum_u64<B>(s, sh);
+  const uint64_t p = block_sum_u64<B>(acc, sh);
+  if (threadIdx.x == 0) pout[blockIdx.x] = static_cast<int64_t>(p);
+}
+
+// w = (int32)((H_inv w) >> F); block 0 also finishes the Arnoldi step:
+// h = lsqrt(<w,w>), Givens rotations, e, convergence flag.
+__global__ __launch_bounds__(B) void k_scal_givens_int(int n, int32_t* __restrict__ w,
+                                                       KState k, int j,
+                                                       const int64_t* __restrict__ pin) {
+  __shared__ uint64_t sh[B / 64];
+  __shared__ int64_t hinv_sh;
+  const uint64_t tot = sum_parts(pin, gridDim.x, sh);
   if (threadIdx.x == 0) {
-    if (!norm)
-      static_cast<int64_t*>(k.H)[j * (k.m + 1) + kk + 1] = as_i64(s) >> bit;
-    else
-      givens_int(k, j, s);
+    const int64_t hn = lsqrt64(as_i64(tot));
+    const int bit = k.bit;
+    hinv_sh = hn == 0 ? 0 : ((static_cast<int64_t>(1) << 62) / hn) >> (62 - bit - bit);
+    if (blockIdx.x == 0) {
+      k.iscal[1] = hinv_sh;
+      givens_int(k, j, hn);
+    }
   }
+  __syncthreads();
+  const int64_t hinv = hinv_sh;
+  const int stride = gridDim.x * B;
+  for (int i = blockIdx.x * B + threadIdx.x; i < n; i += stride)
+    w[i] = low32(mul_wrap(hinv, w[i]) >> k.bit);
 }
 
-__global__ __launch_bounds__(B) void k_scal_int(int n, int32_t* __restrict__ w,
-                                                KState k) {
-  const int i = blockIdx.x * B + threadIdx.x;
-  if (i >= n || k.flags[1]) return;
-  w[i] = low32(mul_wrap(k.iscal[1], w[i]) >> k.bit);
-}
-
-// ---- FP MGS (blocked reduction order) ------------------------------------
+// ---- FP MGS (fixed blocked order) --------------------------------------------
 
 template <typename T>
 __global__ __launch_bounds__(B) void k_dot_first_fp(int n, const T* __restrict__ v,
                                                     const T* __restrict__ w,
-                                                    KState k, int j, T* part) {
+                                                    T* __restrict__ part) {
   __shared__ T sh[B];
-  __shared__ int flag;
-  T acc = 0;
-  for (int i = blockIdx.x * B + threadIdx.x; i < n; i += kRedTotal) acc += v[i] * w[i];
-  const T p = block_tree<B, T>(acc, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = p;
-  if (!last_block(k.counter, &flag)) return;
-  const T tot = final_tree<T>(part, sh);
-  if (threadIdx.x == 0) static_cast<T*>(k.H)[j * (k.m + 1)] = tot;
+  dro_partials<T>(n, part, sh, [&](int i) { return v[i] * w[i]; });
 }
 
 template <typename T>
@@ -266,15 +307,13 @@ template <>
 __device__ __forceinline__ float fp_sqrt<float>(float v) { return sqrtf(v); }
 
 template <typename T>
-__device__ void givens_fp(const KState& k, int j, T sumsq) {
+__device__ void givens_fp(const KState& k, int j, T hn) {
   T* H = static_cast<T*>(k.H) + j * (k.m + 1);
   T* c = static_cast<T*>(k.c);
   T* s = static_cast<T*>(k.s);
   T* e = static_cast<T*>(k.e);
-  const T hn = fp_sqrt<T>(sumsq);
   H[j + 1] = hn;
   k.flags[3] = j + 1;
-  k.scal[4] = static_cast<double>(static_cast<T>(1) / hn);
   for (int q = 0; q < j; ++q) {
     const T t = H[q];
     H[q] = c[q] * t - s[q] * H[q + 1];
@@ -298,31 +337,49 @@ __device__ void givens_fp(const KState& k, int j, T sumsq) {
 template <typename T>
 __global__ __launch_bounds__(B) void k_axpy_dot_fp(int n, const T* __restrict__ vk,
                                                    const T* __restrict__ vk1,
-                                                   T* __restrict__ w, KState k,
-                                                   int kk, int j, T* part) {
+                                                   T* __restrict__ w, KState k, int kk,
+                                                   int j, const T* __restrict__ pin,
+                                                   T* __restrict__ pout) {
   __shared__ T sh[B];
-  __shared__ int flag;
-  const T a = -static_cast<T*>(k.H)[j * (k.m + 1) + kk];
+  __shared__ T hsh;
+  const T h = final_tree<T>(pin, sh);
+  if (threadIdx.x == 0) {
+    hsh = h;
+    if (blockIdx.x == 0) static_cast<T*>(k.H)[j * (k.m + 1) + kk] = h;
+  }
+  __syncthreads();
+  const T a = -hsh;
   const bool norm = (kk == j);
-  T acc = 0;
-  for (int i = blockIdx.x * B + threadIdx.x; i < n; i += kRedTotal) {
+  // every element is updated exactly once, by the thread that also adds its
+  // term to the blocked partial sums
+  dro_partials<T>(n, pout, sh, [&](int i) {
     const T wi = w[i] + a * vk[i];
     w[i] = wi;
-    acc += norm ? wi * wi : vk1[i] * wi;
-  }
-  const T p = block_tree<B, T>(acc, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = p;
-  if (!last_block(k.counter, &flag)) return;
-  const T tot = final_tree<T>(part, sh);
-  if (threadIdx.x == 0) {
-    if (!norm)
-      static_cast<T*>(k.H)[j * (k.m + 1) + kk + 1] = tot;
-    else
-      givens_fp<T>(k, j, tot);
-  }
+    return norm ? wi * wi : vk1[i] * wi;
+  });
 }
 
-// Sequential-order variants: one thread computes the reduction left to right.
+template <typename T>
+__global__ __launch_bounds__(B) void k_scal_givens_fp(int n, T* __restrict__ w, KState k,
+                                                      int j, const T* __restrict__ pin) {
+  __shared__ T sh[B];
+  __shared__ T sc_sh;
+  const T tot = final_tree<T>(pin, sh);
+  if (threadIdx.x == 0) {
+    const T hn = fp_sqrt<T>(tot);
+    sc_sh = static_cast<T>(1) / hn;
+    if (blockIdx.x == 0) {
+      k.scal[4] = static_cast<double>(sc_sh);
+      givens_fp<T>(k, j, hn);
+    }
+  }
+  __syncthreads();
+  const T sc = sc_sh;
+  const int stride = gridDim.x * B;
+  for (int i = blockIdx.x * B + threadIdx.x; i < n; i += stride) w[i] *= sc;
+}
+
+// Sequential-order variants: one thread computes each reduction left to right.
 template <typename T>
 __global__ void k_seq_dot_fp(int n, const T* __restrict__ v, const T* __restrict__ w,
                              KState k, int slot_j, int slot_k) {
@@ -343,9 +400,10 @@ template <typename T>
 __global__ void k_seq_norm_givens_fp(int n, const T* __restrict__ w, KState k, int j) {
   T acc = 0;
   for (int i = 0; i < n; ++i) acc += w[i] * w[i];
-  givens_fp<T>(k, j, acc);
+  const T hn = fp_sqrt<T>(acc);
+  k.scal[4] = static_cast<double>(static_cast<T>(1) / hn);
+  givens_fp<T>(k, j, hn);
 }
-
 template <typename T>
 __global__ __launch_bounds__(B) void k_scal_fp(int n, T* __restrict__ w, KState k) {
   const int i = blockIdx.x * B + threadIdx.x;
@@ -422,20 +480,22 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
   return z ^ (z >> 31);
 }
 
-__global__ __launch_bounds__(B) void k_checksum(int n, const int32_t* __restrict__ v,
-                                                KState k) {
+__global__ __launch_bounds__(B) void k_checksum_part(int n, const int32_t* __restrict__ v,
+                                                     int64_t* __restrict__ part) {
   __shared__ uint64_t sh[B / 64];
-  __shared__ int flag;
   uint64_t acc = 0;
-  for (int i = blockIdx.x * B + threadIdx.x; i < n; i += kRedTotal)
+  const int stride = gridDim.x * B;
+  for (int i = blockIdx.x * B + threadIdx.x; i < n; i += stride)
     acc += mix64(static_cast<uint64_t>(i) * 0x9E3779B97F4A7C15ULL ^
                  static_cast<uint64_t>(static_cast<uint32_t>(v[i])));
-  const uint64_t part = block_sum_u64<B>(acc, sh);
-  if (threadIdx.x == 0) k.ipart[blockIdx.x] = static_cast<int64_t>(part);
-  if (!last_block(k.counter, &flag)) return;
-  uint64_t s = 0;
-  for (int b = threadIdx.x; b < kRedBlocks; b += B) s += static_cast<uint64_t>(k.ipart[b]);
-  s = block_sum_u64<B>(s, sh);
+  const uint64_t p = block_sum_u64<B>(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = static_cast<int64_t>(p);
+}
+
+__global__ __launch_bounds__(B) void k_checksum_finalize(const int64_t* __restrict__ part,
+                                                         int G, KState k) {
+  __shared__ uint64_t sh[B / 64];
+  const uint64_t s = sum_parts(part, G, sh);
   if (threadIdx.x == 0) {
     const int len = *k.trace_len;
     if (len < k.trace_cap) k.trace[len] = s;
@@ -458,6 +518,13 @@ __global__ __launch_bounds__(B) void k_zero_bytes(size_t n4, uint32_t* __restric
 
 inline int grid_for(int n) { return (n + B - 1) / B; }
 
+// Physical blocks for the reductions: about 8 elements per thread, at most
+// kRedBlocks.
+inline int red_grid(int n) {
+  const int g = (n + 8 * B - 1) / (8 * B);
+  return g < 1 ? 1 : (g > kRedBlocks ? kRedBlocks : g);
+}
+
 }  // namespace
 
 hipError_t launch_outer_residual(const DCsr& a64, const double* b, const double* x,
@@ -471,9 +538,11 @@ hipError_t launch_outer_residual(const DCsr& a64, const double* b, const double*
     hipLaunchKernelGGL(k_seq_norm, dim3(1), dim3(1), 0, s, a64.n, r, k, e, 1,
                        static_cast<double*>(nullptr));
   } else {
-    hipLaunchKernelGGL(k_outer_residual, dim3(kRedBlocks), dim3(B), 0, s, a64.n,
+    hipLaunchKernelGGL(k_outer_residual, dim3(red_grid(a64.n)), dim3(B), 0, s, a64.n,
                        a64.rptr, a64.cind, static_cast<const double*>(a64.val), b, x,
-                       r, k, e, 1, static_cast<double*>(nullptr));
+                       r, k.dpart);
+    hipLaunchKernelGGL(k_norm_finalize, dim3(1), dim3(B), 0, s, k.dpart, k, e, 1,
+                       static_cast<double*>(nullptr));
   }
   return hipGetLastError();
 }
@@ -481,12 +550,12 @@ hipError_t launch_outer_residual(const DCsr& a64, const double* b, const double*
 hipError_t launch_relres(const DCsr& a64, const double* b, const double* x,
                          double* r, double* part, unsigned* counter, double* out,
                          hipStream_t s) {
+  (void)counter;
   KState k{};
-  k.dpart = part;
-  k.counter = counter;
-  hipLaunchKernelGGL(k_outer_residual, dim3(kRedBlocks), dim3(B), 0, s, a64.n,
-                     a64.rptr, a64.cind, static_cast<const double*>(a64.val), b, x,
-                     r, k, Elem::kF64, 0, out);
+  hipLaunchKernelGGL(k_outer_residual, dim3(red_grid(a64.n)), dim3(B), 0, s, a64.n,
+                     a64.rptr, a64.cind, static_cast<const double*>(a64.val), b, x, r,
+                     part);
+  hipLaunchKernelGGL(k_norm_finalize, dim3(1), dim3(B), 0, s, part, k, Elem::kF64, 0, out);
   return hipGetLastError();
 }
 
@@ -497,30 +566,38 @@ hipError_t launch_v0(Elem e, int n, const double* r, void* v0, const KState& k,
   return hipGetLastError();
 }
 
+// Partial-sum buffers alternate between the two halves of ipart / dpart /
+// fpart: pass q writes half (q & 1) and the next pass reads it.
+template <typename T>
+static T* half(T* base, int q) {
+  return base + (q & 1) * kRedBlocks;
+}
+
 hipError_t launch_dot_first(Elem e, int n, const void* v0, const void* w,
                             const KState& k, int j, int seq_red, hipStream_t s) {
+  const int G = red_grid(n);
   if (e == Elem::kI32) {
-    hipLaunchKernelGGL(k_dot_first_int, dim3(kRedBlocks), dim3(B), 0, s, n,
+    hipLaunchKernelGGL(k_dot_first_int, dim3(G), dim3(B), 0, s, n,
                        static_cast<const int32_t*>(v0), static_cast<const int32_t*>(w),
-                       k, j);
+                       half(k.ipart, 0));
   } else if (e == Elem::kF64) {
     if (seq_red)
       hipLaunchKernelGGL(k_seq_dot_fp<double>, dim3(1), dim3(1), 0, s, n,
                          static_cast<const double*>(v0), static_cast<const double*>(w),
                          k, j, 0);
     else
-      hipLaunchKernelGGL(k_dot_first_fp<double>, dim3(kRedBlocks), dim3(B), 0, s, n,
+      hipLaunchKernelGGL(k_dot_first_fp<double>, dim3(G), dim3(B), 0, s, n,
                          static_cast<const double*>(v0), static_cast<const double*>(w),
-                         k, j, k.dpart);
+                         half(k.dpart, 0));
   } else {
     if (seq_red)
       hipLaunchKernelGGL(k_seq_dot_fp<float>, dim3(1), dim3(1), 0, s, n,
                          static_cast<const float*>(v0), static_cast<const float*>(w), k,
                          j, 0);
     else
-      hipLaunchKernelGGL(k_dot_first_fp<float>, dim3(kRedBlocks), dim3(B), 0, s, n,
-                         static_cast<const float*>(v0), static_cast<const float*>(w), k,
-                         j, k.fpart);
+      hipLaunchKernelGGL(k_dot_first_fp<float>, dim3(G), dim3(B), 0, s, n,
+                         static_cast<const float*>(v0), static_cast<const float*>(w),
+                         half(k.fpart, 0));
   }
   return hipGetLastError();
 }
@@ -529,10 +606,12 @@ hipError_t launch_axpy_dot(Elem e, int n, const void* V, void* w, const KState& 
                            int kk, int j, int seq_red, hipStream_t s) {
   const size_t off = static_cast<size_t>(kk) * n;
   const size_t off1 = static_cast<size_t>(kk + 1) * n;
+  const int G = red_grid(n);
   if (e == Elem::kI32) {
     const int32_t* v = static_cast<const int32_t*>(V);
-    hipLaunchKernelGGL(k_axpy_dot_int, dim3(kRedBlocks), dim3(B), 0, s, n, v + off,
-                       v + off1, static_cast<int32_t*>(w), k, kk, j);
+    hipLaunchKernelGGL(k_axpy_dot_int, dim3(G), dim3(B), 0, s, n, v + off, v + off1,
+                       static_cast<int32_t*>(w), k, kk, j, half(k.ipart, kk),
+                       half(k.ipart, kk + 1));
     return hipGetLastError();
   }
   if (e == Elem::kF64) {
@@ -548,8 +627,8 @@ hipError_t launch_axpy_dot(Elem e, int n, const void* V, void* w, const KState& 
         hipLaunchKernelGGL(k_seq_norm_givens_fp<double>, dim3(1), dim3(1), 0, s, n, ww,
                            k, j);
     } else {
-      hipLaunchKernelGGL(k_axpy_dot_fp<double>, dim3(kRedBlocks), dim3(B), 0, s, n,
-                         v + off, v + off1, ww, k, kk, j, k.dpart);
+      hipLaunchKernelGGL(k_axpy_dot_fp<double>, dim3(G), dim3(B), 0, s, n, v + off,
+                         v + off1, ww, k, kk, j, half(k.dpart, kk), half(k.dpart, kk + 1));
     }
     return hipGetLastError();
   }
@@ -565,28 +644,38 @@ hipError_t launch_axpy_dot(Elem e, int n, const void* V, void* w, const KState& 
       hipLaunchKernelGGL(k_seq_norm_givens_fp<float>, dim3(1), dim3(1), 0, s, n, ww, k,
                          j);
   } else {
-    hipLaunchKernelGGL(k_axpy_dot_fp<float>, dim3(kRedBlocks), dim3(B), 0, s, n,
-                       v + off, v + off1, ww, k, kk, j, k.fpart);
+    hipLaunchKernelGGL(k_axpy_dot_fp<float>, dim3(G), dim3(B), 0, s, n, v + off,
+                       v + off1, ww, k, kk, j, half(k.fpart, kk), half(k.fpart, kk + 1));
   }
   return hipGetLastError();
 }
 
-hipError_t launch_scal(Elem e, int n, void* w, const KState& k, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  if (e == Elem::kI32)
-    hipLaunchKernelGGL(k_scal_int, dim3(grid_for(n)), dim3(B), 0, s, n,
-                       static_cast<int32_t*>(w), k);
-  else if (e == Elem::kF64)
-    hipLaunchKernelGGL(k_scal_fp<double>, dim3(grid_for(n)), dim3(B), 0, s, n,
-                       static_cast<double*>(w), k);
-  else
-    hipLaunchKernelGGL(k_scal_fp<float>, dim3(grid_for(n)), dim3(B), 0, s, n,
-                       static_cast<float*>(w), k);
+hipError_t launch_scal(Elem e, int n, void* w, const KState& k, int j, int seq_red,
+                       hipStream_t s) {
+  const int G = red_grid(n);
+  if (e == Elem::kI32) {
+    hipLaunchKernelGGL(k_scal_givens_int, dim3(G), dim3(B), 0, s, n,
+                       static_cast<int32_t*>(w), k, j, half(k.ipart, j + 1));
+  } else if (e == Elem::kF64) {
+    if (seq_red)
+      hipLaunchKernelGGL(k_scal_fp<double>, dim3(grid_for(n)), dim3(B), 0, s, n,
+                         static_cast<double*>(w), k);
+    else
+      hipLaunchKernelGGL(k_scal_givens_fp<double>, dim3(G), dim3(B), 0, s, n,
+                         static_cast<double*>(w), k, j, half(k.dpart, j + 1));
+  } else {
+    if (seq_red)
+      hipLaunchKernelGGL(k_scal_fp<float>, dim3(grid_for(n)), dim3(B), 0, s, n,
+                         static_cast<float*>(w), k);
+    else
+      hipLaunchKernelGGL(k_scal_givens_fp<float>, dim3(G), dim3(B), 0, s, n,
+                         static_cast<float*>(w), k, j, half(k.fpart, j + 1));
+  }
   return hipGetLastError();
 }
 
-hipError_t launch_update_x(Elem e, int n, const void* Z, double* x,
-                                 const KState& k, int steps, hipStream_t s) {
+hipError_t launch_update_x(Elem e, int n, const void* Z, double* x, const KState& k,
+                           int steps, hipStream_t s) {
   if (steps <= 0) return hipSuccess;
   if (e == Elem::kI32) {
     hipLaunchKernelGGL(k_trsv_int, dim3(1), dim3(1), 0, s, k, steps);
@@ -608,7 +697,10 @@ hipError_t launch_update_x(Elem e, int n, const void* Z, double* x,
 }
 
 hipError_t launch_checksum(int n, const int32_t* v, const KState& k, hipStream_t s) {
-  hipLaunchKernelGGL(k_checksum, dim3(kRedBlocks), dim3(B), 0, s, n, v, k);
+  const int G = red_grid(n);
+  int64_t* part = k.ipart + 2 * kRedBlocks;  // third half: does not disturb MGS
+  hipLaunchKernelGGL(k_checksum_part, dim3(G), dim3(B), 0, s, n, v, part);
+  hipLaunchKernelGGL(k_checksum_finalize, dim3(1), dim3(B), 0, s, part, G, k);
   return hipGetLastError();
 }
 

@@ -52,6 +52,32 @@ hipError_t launch_trsv_segment(Elem e, TriKind kind, const DCsr& a,
                                const TriSegment& seg, const void* in, void* out,
                                hipStream_t s);
 
+// Level-ordered layout of a triangular factor (see k_tri_chain_pre): row
+// lrow[idx] at position idx, its dependency entries lcol/lval[lbeg[idx] ..
+// lbeg[idx+1]) in subtraction order, its inverse diagonal ldiag[idx].
+struct TriLayout {
+  const int* lrow = nullptr;
+  const int* lbeg = nullptr;
+  const int* lcol = nullptr;
+  const void* lval = nullptr;
+  const void* ldiag = nullptr;
+  const int* lev_ptr = nullptr;
+  const int* chain_row0 = nullptr;  // first row of each chain (LDS mode)
+  int lds_rows = 0;                 // > 0: chain solutions kept in LDS (max rows per chain)
+};
+hipError_t launch_trsv_layout(Elem e, TriKind kind, const TriLayout& L,
+                              const TriSegment& seg, int bit, int dbit, const void* in,
+                              void* out, hipStream_t s);
+
+// Blocked single-workgroup solve of a whole triangular factor with its
+// solution and accumulators in LDS (the coarsest grid).  bptr[i*(nb+1)+b] is
+// the first dependency entry of row i with column >= 64 b; nb = ceil(n/64).
+constexpr int kTriBlockWidth = 64;
+constexpr size_t kTriBlockedMaxLds = 160 * 1024;
+size_t tri_blocked_lds(int n, size_t elem, size_t acc);
+hipError_t launch_trsv_blocked(Elem e, TriKind kind, const DCsr& a, const int* bptr,
+                               int nb, const void* in, void* out, hipStream_t s);
+
 // ---- Krylov kernels ------------------------------------------------------
 // Scalars and small arrays of one FGMRES(m) solver, resident on the device.
 struct KState {
@@ -67,10 +93,10 @@ struct KState {
   double* y;        // m (int / f64) ; float solver keeps y as float in yf
   float* yf;
   // reductions
-  int64_t* ipart;   // kRedBlocks partial int64 sums
-  double* dpart;    // kRedBlocks partial double sums
-  float* fpart;     // kRedBlocks partial float sums
-  unsigned* counter;  // last-block-done counters (several)
+  int64_t* ipart;   // 3 x kRedBlocks partial int64 sums (two alternate, one spare)
+  double* dpart;    // 2 x kRedBlocks partial double sums
+  float* fpart;     // 2 x kRedBlocks partial float sums
+  unsigned* counter;  // unused (kept for layout)
   // scalars
   double* scal;     // [0] nrm_r, [1] prev, [2] e0, [3] e_inv, [4] hn (fp), [5] implicit res
   int64_t* iscal;   // [0] criteria, [1] hinv, [2] hn
@@ -96,8 +122,10 @@ hipError_t launch_dot_first(Elem e, int n, const void* v0, const void* w,
 // norm of w, H_inv / 1/h, the Givens rotations, e and the convergence flag.
 hipError_t launch_axpy_dot(Elem e, int n, const void* V, void* w, const KState& k,
                            int kk, int j, int seq_red, hipStream_t s);
-// w = H_inv w (int) or w *= 1/h (fp)
-hipError_t launch_scal(Elem e, int n, void* w, const KState& k, hipStream_t s);
+// h = norm of w from the last pass's partials; w = H_inv w (int) or
+// w *= 1/h (fp); the Givens rotations, e and the convergence flag of step j.
+hipError_t launch_scal(Elem e, int n, void* w, const KState& k, int j, int seq_red,
+                       hipStream_t s);
 // y = H^-1 e (FP64 / float as the reference) over the first `steps`
 // columns, then x += sum_k y_k Z_k in order of k.
 hipError_t launch_update_x(Elem e, int n, const void* Z, double* x,

@@ -72,6 +72,7 @@ def _load() -> C.CDLL:
         "sc23ad_convdiff_draws": (C.c_int, [C.c_int, C.c_uint64, vp, vp, vp]),
         "sc23ad_hier_build": (C.c_int, [vp, C.POINTER(AmgParam), C.c_int, C.POINTER(vp)]),
         "sc23ad_hier_info": (C.c_int, [vp, ip, vp, vp]),
+        "sc23ad_hier_build_multi": (C.c_int, [vp, C.POINTER(AmgParam), vp, C.c_int, vp]),
         "sc23ad_hier_export": (C.c_int, [vp, C.c_char_p]),
         "sc23ad_hier_destroy": (None, [vp]),
         "sc23ad_checksum_i32": (C.c_uint64, [vp, C.c_int]),
@@ -96,6 +97,7 @@ def _load() -> C.CDLL:
                                           C.POINTER(Converge)]),
         "sc23ad_solver_trace": (C.c_int, [vp, vp, C.c_int, ip]),
         "sc23ad_solver_set_reduction": (C.c_int, [vp, C.c_int]),
+        "sc23ad_solver_history": (C.c_int, [vp, vp, C.c_int, ip]),
         "sc23ad_solver_time_spmv": (C.c_int, [vp, C.c_int]),
         "sc23ad_solver_spmv_stats": (C.c_int, [vp, ip, dp]),
         "sc23ad_solver_relres": (C.c_int, [vp, vp, C.c_double, vp, C.c_int, dp]),
@@ -217,12 +219,25 @@ def convdiff_draws(count: int, seed: int):
 class Hierarchy:
     """Host AMG hierarchy (setup only; no GPU needed)."""
 
-    def __init__(self, a: CsrMat, precision: int = INT, param: AmgParam | None = None):
+    def __init__(self, a: CsrMat | None, precision: int = INT,
+                 param: AmgParam | None = None, _handle=None):
         self.precision = precision
+        if _handle is not None:
+            self.h = _handle
+            return
         h = C.c_void_p()
         prm = param if param is not None else amg_param()
         check(lib.sc23ad_hier_build(a.h, C.byref(prm), precision, C.byref(h)))
         self.h = h
+
+    @classmethod
+    def build_multi(cls, a: CsrMat, precisions, param: AmgParam | None = None):
+        """Hierarchies for several precisions from one coarsening chain."""
+        prm = param if param is not None else amg_param()
+        pr = np.asarray(precisions, dtype=np.int32)
+        outs = (C.c_void_p * len(pr))()
+        check(lib.sc23ad_hier_build_multi(a.h, C.byref(prm), ptr(pr), len(pr), outs))
+        return [cls(None, int(p), _handle=C.c_void_p(outs[k])) for k, p in enumerate(pr)]
 
     def info(self):
         depth = C.c_int()
@@ -388,6 +403,13 @@ class Solver:
 
     def time_spmv(self, on: bool) -> None:
         check(lib.sc23ad_solver_time_spmv(self.h, 1 if on else 0))
+
+    def history(self) -> np.ndarray:
+        n = C.c_int()
+        check(lib.sc23ad_solver_history(self.h, None, 0, C.byref(n)))
+        buf = np.zeros(max(n.value, 1))
+        check(lib.sc23ad_solver_history(self.h, ptr(buf), n.value, C.byref(n)))
+        return buf[:n.value]
 
     def spmv_stats(self):
         n, ms = C.c_int(), C.c_double()
