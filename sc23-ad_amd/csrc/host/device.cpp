@@ -1,6 +1,9 @@
 #include "device.hpp"
 
 #include <algorithm>
+#include <type_traits>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 namespace sc23ad {
@@ -255,12 +258,6 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
     }
     lbeg[idx + 1] = static_cast<int>(lcol.size());
   }
-  std::vector<int> row0(nchain);
-  int max_rows = 0;
-  for (int c = 0; c < nchain; ++c) {
-    row0[c] = chain_begin[c];
-    max_rows = std::max(max_rows, chain_begin[c + 1] - chain_begin[c]);
-  }
   plan.lrow.alloc(sizeof(int) * std::max(n, 1));
   plan.lrow.upload(perm.data(), sizeof(int) * n);
   plan.lbeg.alloc(sizeof(int) * lbeg.size());
@@ -271,26 +268,90 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
   plan.lval.upload(lval.data(), sizeof(V) * lval.size());
   plan.ldiag.alloc(sizeof(V) * ldiag.size());
   plan.ldiag.upload(ldiag.data(), sizeof(V) * ldiag.size());
-  plan.chain_row0.alloc(sizeof(int) * std::max(nchain, 1));
-  plan.chain_row0.upload(row0.data(), sizeof(int) * nchain);
+  // Dependency entries as level-ordered positions, for the LDS-ring mode.
+  std::vector<int> pos_of(static_cast<size_t>(std::max(n, 1)));
+  for (int idx = 0; idx < n; ++idx) pos_of[perm[idx]] = idx;
+  std::vector<int> lpos(lcol.size());
+  for (size_t q = 0; q < lcol.size(); ++q) lpos[q] = pos_of[lcol[q]];
+  // fixed-stride copies of the first 8 entries of each position
+  constexpr int W = 8;
+  std::vector<int> ecol(static_cast<size_t>(std::max(n, 1)) * W, 0),
+      epos(static_cast<size_t>(std::max(n, 1)) * W, 0);
+  std::vector<V> eval(static_cast<size_t>(std::max(n, 1)) * W, V(0));
+  for (int idx = 0; idx < n; ++idx)
+    for (int q = 0; q < W && lbeg[idx] + q < lbeg[idx + 1]; ++q) {
+      ecol[static_cast<size_t>(idx) * W + q] = lcol[lbeg[idx] + q];
+      epos[static_cast<size_t>(idx) * W + q] = lpos[lbeg[idx] + q];
+      eval[static_cast<size_t>(idx) * W + q] = lval[lbeg[idx] + q];
+    }
+  plan.ecol.alloc(sizeof(int) * ecol.size());
+  plan.ecol.upload(ecol.data(), sizeof(int) * ecol.size());
+  plan.epos.alloc(sizeof(int) * epos.size());
+  plan.epos.upload(epos.data(), sizeof(int) * epos.size());
+  plan.eval.alloc(sizeof(V) * eval.size());
+  plan.eval.upload(eval.data(), sizeof(V) * eval.size());
+  plan.lpos.alloc(sizeof(int) * std::max<size_t>(lpos.size(), 1));
+  plan.lpos.upload(lpos.data(), sizeof(int) * lpos.size());
   dev::TriLayout& L = plan.layout;
   L.lrow = plan.lrow.as<int>();
   L.lbeg = plan.lbeg.as<int>();
   L.lcol = plan.lcol.as<int>();
   L.lval = plan.lval.get();
   L.ldiag = plan.ldiag.get();
+  L.ecol = plan.ecol.as<int>();
+  L.eval = plan.eval.get();
   L.lev_ptr = plan.lev_ptr.as<int>();
-  L.chain_row0 = plan.chain_row0.as<int>();
-  // Solutions stay in LDS when every chain is walked by one workgroup in a
-  // single launch and the largest chain fits.
+  L.lpos = plan.lpos.as<int>();
+  L.epos = plan.epos.as<int>();
+  // Ring mode needs every chain walked by one workgroup in a single launch and
+  // every dependency within the ring: a row of a level ending at position e
+  // reads position q only if e - q <= ring.
   const bool one_launch_chains =
       plan.segs.size() == 1 && plan.segs[0].chain_kernel == 1 &&
       (nchain > 1 || (all_chain_lev.size() == 2 && all_chain_lev[0] == 0 &&
                       all_chain_lev[1] == static_cast<int>(lev_ptr.size()) - 1));
-  L.lds_rows = (one_launch_chains &&
-                sizeof(V) * static_cast<size_t>(max_rows) <= dev::kTriBlockedMaxLds)
-                   ? max_rows
-                   : 0;
+  int ring = 1;
+  while (static_cast<size_t>(ring) * 2 * sizeof(V) <= 128 * 1024) ring *= 2;
+  long long span = 0;
+  for (size_t Lv = 0; Lv + 1 < lev_ptr.size(); ++Lv)
+    for (int idx = lev_ptr[Lv]; idx < lev_ptr[Lv + 1]; ++idx)
+      for (int q = lbeg[idx]; q < lbeg[idx + 1]; ++q)
+        span = std::max<long long>(span, static_cast<long long>(lev_ptr[Lv + 1]) - lpos[q]);
+  L.ring = (one_launch_chains && span <= ring) ? ring : 0;
+  L.n = n;
+  // entry-parallel mode for integer factors in ring mode
+  size_t maxlev_rows = 0;
+  for (size_t Lv = 0; Lv + 1 < lev_ptr.size(); ++Lv)
+    maxlev_rows = std::max<size_t>(maxlev_rows, lev_ptr[Lv + 1] - lev_ptr[Lv]);
+  if (std::is_same_v<V, int32_t> && L.ring > 0 && maxlev_rows <= 1024 &&
+      sizeof(uint64_t) * 2 * 1024 + sizeof(int32_t) * L.ring <= dev::kTriBlockedMaxLds) {
+    std::vector<int> lslot(std::max<size_t>(lcol.size(), 1), 0),
+        lev_eptr(lev_ptr.size(), 0);
+    for (size_t Lv = 0; Lv + 1 < lev_ptr.size(); ++Lv) {
+      lev_eptr[Lv] = lbeg[lev_ptr[Lv]];
+      for (int idx = lev_ptr[Lv]; idx < lev_ptr[Lv + 1]; ++idx)
+        for (int q = lbeg[idx]; q < lbeg[idx + 1]; ++q) lslot[q] = idx - lev_ptr[Lv];
+    }
+    lev_eptr.back() = lbeg[n];
+    plan.lslot.alloc(sizeof(int) * lslot.size());
+    plan.lslot.upload(lslot.data(), sizeof(int) * lslot.size());
+    plan.lev_eptr.alloc(sizeof(int) * lev_eptr.size());
+    plan.lev_eptr.upload(lev_eptr.data(), sizeof(int) * lev_eptr.size());
+    L.lslot = plan.lslot.as<int>();
+    L.lev_eptr = plan.lev_eptr.as<int>();
+    int w = 1;
+    while (w < 8 && static_cast<size_t>(w) * 64 * 2 < maxlev_rows) w *= 2;
+    L.ep_waves = w;
+  }
+  int maxdeps = 0;
+  for (int idx = 0; idx < n; ++idx) maxdeps = std::max(maxdeps, lbeg[idx + 1] - lbeg[idx]);
+  if (std::getenv("SC23AD_VERBOSE")) {
+    std::fprintf(stderr,
+                 "[sc23ad plan] kind %d n %d nnz %zu chains %d levels(max chain) %d "
+                 "max level rows %zu launches %d ring %d span %lld max deps %d ep waves %d\n",
+                 static_cast<int>(kind), n, cind.size(), nchain, plan.nlevels, maxlev_rows,
+                 plan.nlaunch, L.ring, span, maxdeps, L.ep_waves);
+  }
 }
 
 // Per-row block boundaries of the dependency entries for the blocked
@@ -336,6 +397,7 @@ void DevAmg::alloc_work() {
     tmp[l].alloc(es * n);
   }
   r0.alloc(es * std::max(n0(), 1));
+  perm_scratch.alloc(4 * std::max(n0(), 1));
   tmp[0].alloc(es * std::max(n0(), 1));
   tmpc.alloc(es * std::max(n_coarse, 1));
 }
@@ -425,6 +487,12 @@ void DevAmg::trsv(const TriPlan& plan, const DevMat& m, const void* in,
     hip_check(dev::launch_trsv_blocked(elem, plan.kind, m.d, plan.bptr.as<int>(), plan.nb,
                                        in, out, lib_stream()),
               "trsv blocked");
+    return;
+  }
+  if (plan.layout.ep_waves > 0 && elem == Elem::kI32 && plan.segs.size() == 1) {
+    hip_check(dev::launch_trsv_ep(plan.kind, plan.layout, plan.segs[0], m.d.bit, m.d.dbit, in,
+                                  out, perm_scratch.get(), lib_stream()),
+              "trsv ep");
     return;
   }
   for (const auto& seg : plan.segs)

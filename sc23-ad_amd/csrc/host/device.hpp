@@ -77,7 +77,7 @@ struct TriPlan {
   DevBuf bptr;
   int nb = 0;
   // level-ordered layout used by the level-scheduled kernels
-  DevBuf lrow, lbeg, lcol, lval, ldiag, chain_row0;
+  DevBuf lrow, lbeg, lcol, lval, ldiag, ecol, eval, lpos, epos, lslot, lev_eptr;
   dev::TriLayout layout;
 };
 
@@ -104,7 +104,7 @@ class DevAmg {
   int n_coarse = 0;
   // work vectors: index = level (1..depth); level 0 uses r0 / temp0
   std::vector<DevBuf> ee, xx, rr, tmp;
-  DevBuf r0, tmpc;
+  DevBuf r0, tmpc, perm_scratch;
   LaunchCfg cfg;
 
   static std::unique_ptr<DevAmg> upload(const IntHierarchy& h, const LaunchCfg& c);

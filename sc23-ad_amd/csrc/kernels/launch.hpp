@@ -61,10 +61,24 @@ struct TriLayout {
   const int* lcol = nullptr;
   const void* lval = nullptr;
   const void* ldiag = nullptr;
+  const int* ecol = nullptr;   // first 8 entries of each position, fixed stride 8
+  const void* eval = nullptr;
   const int* lev_ptr = nullptr;
-  const int* chain_row0 = nullptr;  // first row of each chain (LDS mode)
-  int lds_rows = 0;                 // > 0: chain solutions kept in LDS (max rows per chain)
+  // ring mode: entries as level-ordered positions instead of rows
+  const int* lpos = nullptr;
+  const int* epos = nullptr;
+  int ring = 0;  // > 0: LDS ring of this many values (power of two)
+  // entry-parallel mode (integer factors): slot of each entry's row inside its
+  // level, entry offset of each level, waves per chain
+  const int* lslot = nullptr;
+  const int* lev_eptr = nullptr;
+  int ep_waves = 0;  // > 0: use the entry-parallel kernel
+  int n = 0;
 };
+// Entry-parallel integer solve; in_perm is scratch of n int32 (the input
+// gathered into level order).
+hipError_t launch_trsv_ep(TriKind kind, const TriLayout& L, const TriSegment& seg, int bit,
+                          int dbit, const void* in, void* out, void* in_perm, hipStream_t s);
 hipError_t launch_trsv_layout(Elem e, TriKind kind, const TriLayout& L,
                               const TriSegment& seg, int bit, int dbit, const void* in,
                               void* out, hipStream_t s);

@@ -5,6 +5,8 @@
 // split across lanes.  FP kernels (CSR<T>::SpMV spmat.hpp:43-52 and the
 // triangular solves) keep each row's sum strictly in column order, one thread
 // per row, so that the device rounds exactly like the CPU.
+#include <type_traits>
+
 #include "dev_common.hpp"
 #include "launch.hpp"
 
@@ -404,10 +406,106 @@ __global__ __launch_bounds__(kBlkThreads) void k_tri_blocked(
   }
 }
 
+// Integer version of the blocked solve.  Integer row sums are order-free, so
+// (i) the sweeping wave keeps its row's 64 tile values in registers and the
+// 64-step sweep is fully unrolled, and (ii) the off-block update gives each
+// row a whole wave: lane q takes the row's q-th entry in the block and the
+// wave sums the products.
+template <TriKind K>
+__global__ __launch_bounds__(kBlkThreads) void k_tri_blocked_int(
+    int n, const int* __restrict__ rptr, const int* __restrict__ cind,
+    const int32_t* __restrict__ val, int bit, int dbit, const int* __restrict__ bptr,
+    int nb, const int32_t* __restrict__ in, int32_t* __restrict__ out) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  uint64_t* acc = reinterpret_cast<uint64_t*>(smem);
+  int32_t* xs = reinterpret_cast<int32_t*>(acc + n);
+  int32_t* dinv = xs + n;
+  int32_t* tile = reinterpret_cast<int32_t*>(
+      (reinterpret_cast<uintptr_t>(dinv + n) + 15) & ~static_cast<uintptr_t>(15));
+  constexpr bool fwd = (K != TriKind::kDU);
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  constexpr int nw = kBlkThreads / 64;
+  const int stride_b = nb + 1;
+  for (int i = tid; i < n; i += kBlkThreads) {
+    acc[i] = static_cast<uint64_t>(static_cast<int64_t>(in[i])) << bit;
+    if (K == TriKind::kLD) dinv[i] = val[rptr[i + 1] - 1];
+    if (K == TriKind::kDU) dinv[i] = val[rptr[i]];
+  }
+  __syncthreads();
+  for (int q = 0; q < nb; ++q) {
+    const int b = fwd ? q : nb - 1 - q;
+    const int r0 = b * kTB;
+    const int r1 = min(r0 + kTB, n);
+    const int rows = r1 - r0;
+    // (a) the diagonal tile, column-major, zero where no entry
+    for (int idx = tid; idx < kTB * kTB; idx += kBlkThreads) tile[idx] = 0;
+    __syncthreads();
+    for (int rr = wid; rr < rows; rr += nw) {
+      const int i = r0 + rr;
+      const int pb = bptr[i * stride_b + b], pe = bptr[i * stride_b + b + 1];
+      const int p = pb + lane;
+      if (p < pe) tile[(cind[p] - r0) * kTB + rr] = val[p];
+    }
+    __syncthreads();
+    // (b) one wave sweeps the tile with its operands in registers
+    if (wid == 0) {
+      int32_t reg[kTB];
+#pragma unroll
+      for (int c = 0; c < kTB; ++c) reg[c] = tile[c * kTB + lane];
+      uint64_t t = lane < rows ? acc[r0 + lane] : 0;
+      const int32_t dv = lane < rows ? dinv[r0 + lane] : 0;
+      int32_t xv = 0;
+#pragma unroll
+      for (int st = 0; st < kTB; ++st) {
+        const int c = fwd ? st : kTB - 1 - st;
+        if (c < rows) {
+          const int32_t mine = acc_final<int32_t, K>(t, dv, bit, dbit);
+          if (lane == c) xv = mine;
+          const int32_t xc = __builtin_amdgcn_readlane(mine, c);
+          t -= prod64(reg[c], xc);
+        }
+      }
+      if (lane < rows) {
+        xs[r0 + lane] = xv;
+        out[r0 + lane] = xv;
+      }
+    }
+    __syncthreads();
+    // (c) one wave per remaining row
+    const int lo = fwd ? r1 : 0;
+    const int hi = fwd ? n : r0;
+    for (int i = lo + wid; i < hi; i += nw) {
+      const int pb = bptr[i * stride_b + b], pe = bptr[i * stride_b + b + 1];
+      if (pb == pe) continue;
+      const int p = pb + lane;
+      uint64_t prod = 0;
+      if (p < pe) prod = prod64(val[p], xs[cind[p]]);
+      prod = wave_sum_u64(prod);
+      if (lane == 0) acc[i] -= prod;
+    }
+    __syncthreads();
+  }
+}
+
 template <typename T, TriKind K>
 hipError_t tri_blocked(const DCsr& a, const int* bptr, int nb, const void* in, void* out,
                        hipStream_t s) {
   if (a.n <= 0) return hipSuccess;
+  if constexpr (std::is_same_v<T, int32_t>) {
+    const size_t lds = tri_blocked_lds(a.n, 4, 8);
+    static bool attr_i = false;
+    if (!attr_i) {
+      hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tri_blocked_int<K>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         static_cast<int>(kTriBlockedMaxLds));
+      if (e != hipSuccess) return e;
+      attr_i = true;
+    }
+    hipLaunchKernelGGL((k_tri_blocked_int<K>), dim3(1), dim3(kBlkThreads), lds, s, a.n,
+                       a.rptr, a.cind, static_cast<const int32_t*>(a.val), a.bit, a.dbit,
+                       bptr, nb, static_cast<const int32_t*>(in), static_cast<int32_t*>(out));
+    return hipGetLastError();
+  }
   const size_t lds = tri_blocked_lds(a.n, sizeof(T), sizeof(typename AccOf<T>::type));
   static bool attr_set = false;
   if (!attr_set) {
@@ -436,6 +534,9 @@ hipError_t tri_blocked(const DCsr& a, const int* bptr, int nb, const void* in, v
 
 constexpr int kPre = 8;  // dependency entries held in registers per row
 
+// The first kPre dependency entries of position idx are also stored with a
+// fixed stride (ecol / eval[idx * kPre + q]), so every load of a row's data
+// depends only on idx; entries beyond kPre are read from lcol / lval.
 template <typename T>
 struct PreRow {
   int row = -1, beg = 0, end = 0;
@@ -445,13 +546,12 @@ struct PreRow {
 };
 
 template <typename T, TriKind K>
-__device__ __forceinline__ void pre_load(PreRow<T>& r, int idx, int lim,
+__device__ __forceinline__ void pre_meta(PreRow<T>& r, int idx, int lim,
                                          const int* __restrict__ lrow,
                                          const int* __restrict__ lbeg,
-                                         const int* __restrict__ lcol,
-                                         const T* __restrict__ lval,
-                                         const T* __restrict__ ldiag,
-                                         const T* __restrict__ in) {
+                                         const int* __restrict__ ecol,
+                                         const T* __restrict__ eval,
+                                         const T* __restrict__ ldiag) {
   if (idx >= lim) {
     r.row = -1;
     return;
@@ -459,69 +559,248 @@ __device__ __forceinline__ void pre_load(PreRow<T>& r, int idx, int lim,
   r.row = lrow[idx];
   r.beg = lbeg[idx];
   r.end = lbeg[idx + 1];
-  r.in = in[r.row];
   if (K != TriKind::kL) r.dinv = ldiag[idx];
 #pragma unroll
   for (int q = 0; q < kPre; ++q) {
-    const int p = r.beg + q;
-    if (p < r.end) {
-      r.col[q] = lcol[p];
-      r.val[q] = lval[p];
-    }
+    r.col[q] = ecol[static_cast<size_t>(idx) * kPre + q];
+    r.val[q] = eval[static_cast<size_t>(idx) * kPre + q];
   }
 }
 
-template <typename T, TriKind K, bool kLds>
+template <typename T>
+__device__ __forceinline__ void pre_in(PreRow<T>& r, const T* __restrict__ in) {
+  if (r.row >= 0) r.in = in[r.row];
+}
+
+template <typename T, TriKind K, bool kRing>
 __device__ __forceinline__ T pre_eval(const PreRow<T>& r, int bit, int dbit,
                                       const int* __restrict__ lcol,
                                       const T* __restrict__ lval, const T* out,
-                                      const T* xs, int r0) {
+                                      const T* ring, int rmask) {
   using Acc = typename AccOf<T>::type;
   Acc t = acc_init<T>(r.in, bit);
-  auto X = [&](int c) -> T { return kLds ? xs[c - r0] : out[c]; };
+  // kRing: entries name dependency positions, whose values sit in the LDS ring
+  auto X = [&](int c) -> T { return kRing ? ring[c & rmask] : out[c]; };
+  const int cnt = r.end - r.beg;
 #pragma unroll
   for (int q = 0; q < kPre; ++q)
-    if (r.beg + q < r.end) t = acc_sub<T>(t, r.val[q], X(r.col[q]));
+    if (q < cnt) t = acc_sub<T>(t, r.val[q], X(r.col[q]));
   for (int p = r.beg + kPre; p < r.end; ++p) t = acc_sub<T>(t, lval[p], X(lcol[p]));
   return acc_final<T, K>(t, r.dinv, bit, dbit);
 }
 
-template <typename T, TriKind K, bool kLds>
+// Workgroup barrier that orders LDS only: the solution is exchanged through
+// the LDS ring, so outstanding global loads (the prefetch of the next levels)
+// and stores (the result) may stay in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// kRing: dependency entries hold level-ordered positions; the values of the
+// last rmask+1 positions of the chain live in an LDS ring (the host checks
+// that no dependency reaches further back than that).  Otherwise entries hold
+// row indices and values are read back from `out` after a full barrier.
+template <typename T, TriKind K, bool kRing>
 __global__ __launch_bounds__(kChainBlock) void k_tri_chain_pre(
     const int* __restrict__ lrow, const int* __restrict__ lbeg,
     const int* __restrict__ lcol, const T* __restrict__ lval,
+    const int* __restrict__ ecol, const T* __restrict__ eval,
     const T* __restrict__ ldiag, int bit, int dbit, const int* __restrict__ lev_ptr,
-    const int* __restrict__ chain_lev, const int* __restrict__ chain_row0,
-    const T* __restrict__ in, T* out) {
+    const int* __restrict__ chain_lev, int rmask, const T* __restrict__ in, T* out) {
   extern __shared__ __align__(16) unsigned char smem_c[];
-  T* xs = reinterpret_cast<T*>(smem_c);
+  T* ring = reinterpret_cast<T*>(smem_c);
   const int lb = chain_lev[2 * blockIdx.x], le = chain_lev[2 * blockIdx.x + 1];
-  const int r0 = kLds ? chain_row0[blockIdx.x] : 0;
-  PreRow<T> nxt;
-  if (lb < le)
-    pre_load<T, K>(nxt, lev_ptr[lb] + threadIdx.x, lev_ptr[lb + 1], lrow, lbeg, lcol,
-                   lval, ldiag, in);
+  const int tid = threadIdx.x;
+  // pipeline: nxt = level L+1 (metadata and input), far = level L+2 (metadata)
+  PreRow<T> nxt, far;
+  if (lb < le) {
+    pre_meta<T, K>(nxt, lev_ptr[lb] + tid, lev_ptr[lb + 1], lrow, lbeg, ecol, eval, ldiag);
+    pre_in<T>(nxt, in);
+  }
+  if (lb + 1 < le)
+    pre_meta<T, K>(far, lev_ptr[lb + 1] + tid, lev_ptr[lb + 2], lrow, lbeg, ecol, eval,
+                   ldiag);
   for (int L = lb; L < le; ++L) {
     const PreRow<T> cur = nxt;
     const int s = lev_ptr[L], e = lev_ptr[L + 1];
-    if (L + 1 < le)
-      pre_load<T, K>(nxt, lev_ptr[L + 1] + threadIdx.x, lev_ptr[L + 2], lrow, lbeg, lcol,
-                     lval, ldiag, in);
+    if (L + 1 < le) {
+      nxt = far;
+      pre_in<T>(nxt, in);
+    }
+    if (L + 2 < le)
+      pre_meta<T, K>(far, lev_ptr[L + 2] + tid, lev_ptr[L + 3], lrow, lbeg, ecol, eval,
+                     ldiag);
     if (cur.row >= 0) {
-      const T v = pre_eval<T, K, kLds>(cur, bit, dbit, lcol, lval, out, xs, r0);
+      const T v = pre_eval<T, K, kRing>(cur, bit, dbit, lcol, lval, out, ring, rmask);
       out[cur.row] = v;
-      if (kLds) xs[cur.row - r0] = v;
+      if (kRing) ring[(s + tid) & rmask] = v;
     }
     // rows beyond one per thread (large levels)
-    for (int idx = s + kChainBlock + threadIdx.x; idx < e; idx += kChainBlock) {
+    for (int idx = s + kChainBlock + tid; idx < e; idx += kChainBlock) {
       PreRow<T> r;
-      pre_load<T, K>(r, idx, e, lrow, lbeg, lcol, lval, ldiag, in);
-      const T v = pre_eval<T, K, kLds>(r, bit, dbit, lcol, lval, out, xs, r0);
+      pre_meta<T, K>(r, idx, e, lrow, lbeg, ecol, eval, ldiag);
+      pre_in<T>(r, in);
+      const T v = pre_eval<T, K, kRing>(r, bit, dbit, lcol, lval, out, ring, rmask);
       out[r.row] = v;
-      if (kLds) xs[r.row - r0] = v;
+      if (kRing) ring[idx & rmask] = v;
     }
-    __syncthreads();
+    if (kRing)
+      lds_barrier();
+    else
+      __syncthreads();
   }
+}
+
+// ---- entry-parallel chain solve (integer factors, LDS ring) -------------------
+//
+// For integer factors a row's sum is order-free, so the dependency entries of
+// a whole level (contiguous in the level-ordered layout) are spread over the
+// lanes: each lane multiplies its entries by solution values taken from the
+// LDS ring and adds the products into the row's LDS accumulator with 64-bit
+// LDS atomics; then one lane per row finishes the row.  The input arrives
+// already gathered into level order (in_perm), so every load of the next
+// level, prefetched into registers while the current one is computed,
+// depends only on the level's offsets.  kWaves waves walk one chain; with one
+// wave no barrier is needed, LDS operations of a wave being ordered.
+
+constexpr int kEpE = 8;  // entries per lane held in registers
+constexpr int kEpR = 2;  // rows per lane held in registers
+constexpr int kEpMaxLevel = 1024;  // rows of a level (accumulator slots)
+
+struct EpPre {
+  int pos[kEpE];
+  int32_t val[kEpE];
+  int slot[kEpE];
+  int32_t in[kEpR];
+  int32_t dinv[kEpR];
+};
+
+template <int kThreads>
+__device__ __forceinline__ void ep_load(EpPre& r, int e0, int e1, int s, int e,
+                                        const int* __restrict__ lpos,
+                                        const int32_t* __restrict__ lval,
+                                        const int* __restrict__ lslot,
+                                        const int32_t* __restrict__ in_perm,
+                                        const int32_t* __restrict__ ldiag) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < kEpE; ++q) {
+    const int p = e0 + t + q * kThreads;
+    if (p < e1) {
+      r.pos[q] = lpos[p];
+      r.val[q] = lval[p];
+      r.slot[q] = lslot[p];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kEpR; ++q) {
+    const int idx = s + t + q * kThreads;
+    if (idx < e) {
+      r.in[q] = in_perm[idx];
+      r.dinv[q] = ldiag[idx];
+    }
+  }
+}
+
+template <int kWaves>
+__device__ __forceinline__ void ep_sync() {
+  if (kWaves == 1) {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+  } else {
+    lds_barrier();
+  }
+}
+
+template <TriKind K, int kWaves>
+__global__ __launch_bounds__(64 * kWaves) void k_tri_chain_ep(
+    const int* __restrict__ lrow, const int* __restrict__ lpos,
+    const int32_t* __restrict__ lval, const int* __restrict__ lslot,
+    const int32_t* __restrict__ ldiag, int bit, int dbit, const int* __restrict__ lev_ptr,
+    const int* __restrict__ lev_eptr, const int* __restrict__ chain_lev, int rmask,
+    const int32_t* __restrict__ in_perm, int32_t* __restrict__ out) {
+  constexpr int kThreads = 64 * kWaves;
+  extern __shared__ __align__(16) unsigned char smem_e[];
+  uint64_t* acc = reinterpret_cast<uint64_t*>(smem_e);       // 2 x kEpMaxLevel
+  int32_t* ring = reinterpret_cast<int32_t*>(acc + 2 * kEpMaxLevel);
+  const int t = threadIdx.x;
+  for (int k = t; k < 2 * kEpMaxLevel; k += kThreads) acc[k] = 0;
+  ep_sync<kWaves>();
+  const int lb = chain_lev[2 * blockIdx.x], le = chain_lev[2 * blockIdx.x + 1];
+  EpPre nxt;
+  if (lb < le)
+    ep_load<kThreads>(nxt, lev_eptr[lb], lev_eptr[lb + 1], lev_ptr[lb], lev_ptr[lb + 1],
+                      lpos, lval, lslot, in_perm, ldiag);
+  for (int L = lb; L < le; ++L) {
+    const EpPre cur = nxt;
+    const int s = lev_ptr[L], e = lev_ptr[L + 1];
+    const int e0 = lev_eptr[L], e1 = lev_eptr[L + 1];
+    if (L + 1 < le)
+      ep_load<kThreads>(nxt, lev_eptr[L + 1], lev_eptr[L + 2], lev_ptr[L + 1],
+                        lev_ptr[L + 2], lpos, lval, lslot, in_perm, ldiag);
+    uint64_t* a = acc + (L & 1) * kEpMaxLevel;
+    // products of the level's dependency entries
+#pragma unroll
+    for (int q = 0; q < kEpE; ++q) {
+      const int p = e0 + t + q * kThreads;
+      if (p < e1)
+        atomicAdd(reinterpret_cast<unsigned long long*>(&a[cur.slot[q]]),
+                  static_cast<unsigned long long>(prod64(cur.val[q], ring[cur.pos[q] & rmask])));
+    }
+    for (int p = e0 + t + kEpE * kThreads; p < e1; p += kThreads)
+      atomicAdd(reinterpret_cast<unsigned long long*>(&a[lslot[p]]),
+                static_cast<unsigned long long>(prod64(lval[p], ring[lpos[p] & rmask])));
+    ep_sync<kWaves>();
+    // finish the rows
+#pragma unroll
+    for (int q = 0; q < kEpR; ++q) {
+      const int k = t + q * kThreads;
+      if (s + k < e) {
+        const uint64_t tt = (static_cast<uint64_t>(static_cast<int64_t>(cur.in[q])) << bit) - a[k];
+        a[k] = 0;
+        const int32_t v = acc_final<int32_t, K>(tt, cur.dinv[q], bit, dbit);
+        ring[(s + k) & rmask] = v;
+        out[lrow[s + k]] = v;
+      }
+    }
+    for (int k = t + kEpR * kThreads; s + k < e; k += kThreads) {
+      const uint64_t tt =
+          (static_cast<uint64_t>(static_cast<int64_t>(in_perm[s + k])) << bit) - a[k];
+      a[k] = 0;
+      const int32_t v = acc_final<int32_t, K>(tt, ldiag[s + k], bit, dbit);
+      ring[(s + k) & rmask] = v;
+      out[lrow[s + k]] = v;
+    }
+    ep_sync<kWaves>();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_gather_i32(int n, const int* __restrict__ idx,
+                                                    const int32_t* __restrict__ src,
+                                                    int32_t* __restrict__ dst) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
+}
+
+template <TriKind K, int kWaves>
+hipError_t tri_ep_launch(const TriLayout& L, const TriSegment& seg, int bit, int dbit,
+                         const int32_t* in_perm, int32_t* out, hipStream_t s) {
+  const size_t lds = sizeof(uint64_t) * 2 * kEpMaxLevel + sizeof(int32_t) * L.ring;
+  static bool attr = false;
+  if (!attr) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tri_chain_ep<K, kWaves>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       static_cast<int>(kTriBlockedMaxLds));
+    if (e != hipSuccess) return e;
+    attr = true;
+  }
+  hipLaunchKernelGGL((k_tri_chain_ep<K, kWaves>), dim3(seg.nchain), dim3(64 * kWaves), lds, s,
+                     L.lrow, L.lpos, static_cast<const int32_t*>(L.lval), L.lslot,
+                     static_cast<const int32_t*>(L.ldiag), bit, dbit, L.lev_ptr, L.lev_eptr,
+                     seg.chain_lev, L.ring - 1, in_perm, out);
+  return hipGetLastError();
 }
 
 // One level, many workgroups, level-ordered layout.
@@ -529,12 +808,14 @@ template <typename T, TriKind K>
 __global__ __launch_bounds__(kBlock) void k_tri_level_pre(
     const int* __restrict__ lrow, const int* __restrict__ lbeg,
     const int* __restrict__ lcol, const T* __restrict__ lval,
+    const int* __restrict__ ecol, const T* __restrict__ eval,
     const T* __restrict__ ldiag, int bit, int dbit, int row_begin, int row_end,
     const T* __restrict__ in, T* out) {
   const int idx = row_begin + blockIdx.x * kBlock + threadIdx.x;
   if (idx >= row_end) return;
   PreRow<T> r;
-  pre_load<T, K>(r, idx, row_end, lrow, lbeg, lcol, lval, ldiag, in);
+  pre_meta<T, K>(r, idx, row_end, lrow, lbeg, ecol, eval, ldiag);
+  pre_in<T>(r, in);
   out[r.row] = pre_eval<T, K, false>(r, bit, dbit, lcol, lval, out, nullptr, 0);
 }
 
@@ -544,11 +825,12 @@ hipError_t tri_pre_seg(const TriLayout& L, const TriSegment& seg, int bit, int d
   const T* vin = static_cast<const T*>(in);
   T* vout = static_cast<T*>(out);
   const T* lval = static_cast<const T*>(L.lval);
+  const T* eval = static_cast<const T*>(L.eval);
   const T* ldiag = static_cast<const T*>(L.ldiag);
   if (seg.chain_kernel) {
     if (seg.nchain <= 0) return hipSuccess;
-    if (L.lds_rows > 0) {
-      const size_t lds = sizeof(T) * static_cast<size_t>(L.lds_rows);
+    if (L.ring > 0) {
+      const size_t lds = sizeof(T) * static_cast<size_t>(L.ring);
       static bool attr = false;
       if (!attr) {
         hipError_t e = hipFuncSetAttribute(
@@ -558,19 +840,19 @@ hipError_t tri_pre_seg(const TriLayout& L, const TriSegment& seg, int bit, int d
         attr = true;
       }
       hipLaunchKernelGGL((k_tri_chain_pre<T, K, true>), dim3(seg.nchain), dim3(kChainBlock),
-                         lds, s, L.lrow, L.lbeg, L.lcol, lval, ldiag, bit, dbit, L.lev_ptr,
-                         seg.chain_lev, L.chain_row0, vin, vout);
+                         lds, s, L.lrow, L.lbeg, L.lpos, lval, L.epos, eval, ldiag, bit, dbit,
+                         L.lev_ptr, seg.chain_lev, L.ring - 1, vin, vout);
     } else {
       hipLaunchKernelGGL((k_tri_chain_pre<T, K, false>), dim3(seg.nchain),
-                         dim3(kChainBlock), 0, s, L.lrow, L.lbeg, L.lcol, lval, ldiag, bit,
-                         dbit, L.lev_ptr, seg.chain_lev, L.chain_row0, vin, vout);
+                         dim3(kChainBlock), 0, s, L.lrow, L.lbeg, L.lcol, lval, L.ecol, eval,
+                         ldiag, bit, dbit, L.lev_ptr, seg.chain_lev, 0, vin, vout);
     }
   } else {
     const int rows = seg.row_end - seg.row_begin;
     if (rows <= 0) return hipSuccess;
     hipLaunchKernelGGL((k_tri_level_pre<T, K>), dim3((rows + kBlock - 1) / kBlock),
-                       dim3(kBlock), 0, s, L.lrow, L.lbeg, L.lcol, lval, ldiag, bit, dbit,
-                       seg.row_begin, seg.row_end, vin, vout);
+                       dim3(kBlock), 0, s, L.lrow, L.lbeg, L.lcol, lval, L.ecol, eval, ldiag,
+                       bit, dbit, seg.row_begin, seg.row_end, vin, vout);
   }
   return hipGetLastError();
 }
@@ -645,6 +927,34 @@ hipError_t launch_trsv_layout(Elem e, TriKind kind, const TriLayout& L,
   if (kind == TriKind::kL) return tri_pre_seg<float, TriKind::kL>(L, seg, bit, dbit, in, out, s);
   if (kind == TriKind::kLD) return tri_pre_seg<float, TriKind::kLD>(L, seg, bit, dbit, in, out, s);
   return tri_pre_seg<float, TriKind::kDU>(L, seg, bit, dbit, in, out, s);
+}
+
+hipError_t launch_trsv_ep(TriKind kind, const TriLayout& L, const TriSegment& seg, int bit,
+                          int dbit, const void* in, void* out, void* in_perm, hipStream_t s) {
+  if (L.n > 0)
+    hipLaunchKernelGGL(k_gather_i32, dim3((L.n + 255) / 256), dim3(256), 0, s, L.n, L.lrow,
+                       static_cast<const int32_t*>(in), static_cast<int32_t*>(in_perm));
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int32_t* ip = static_cast<const int32_t*>(in_perm);
+  int32_t* o = static_cast<int32_t*>(out);
+  const int w = L.ep_waves;
+  if (kind == TriKind::kLD) {
+    if (w <= 1) return tri_ep_launch<TriKind::kLD, 1>(L, seg, bit, dbit, ip, o, s);
+    if (w <= 2) return tri_ep_launch<TriKind::kLD, 2>(L, seg, bit, dbit, ip, o, s);
+    if (w <= 4) return tri_ep_launch<TriKind::kLD, 4>(L, seg, bit, dbit, ip, o, s);
+    return tri_ep_launch<TriKind::kLD, 8>(L, seg, bit, dbit, ip, o, s);
+  }
+  if (kind == TriKind::kL) {
+    if (w <= 1) return tri_ep_launch<TriKind::kL, 1>(L, seg, bit, dbit, ip, o, s);
+    if (w <= 2) return tri_ep_launch<TriKind::kL, 2>(L, seg, bit, dbit, ip, o, s);
+    if (w <= 4) return tri_ep_launch<TriKind::kL, 4>(L, seg, bit, dbit, ip, o, s);
+    return tri_ep_launch<TriKind::kL, 8>(L, seg, bit, dbit, ip, o, s);
+  }
+  if (w <= 1) return tri_ep_launch<TriKind::kDU, 1>(L, seg, bit, dbit, ip, o, s);
+  if (w <= 2) return tri_ep_launch<TriKind::kDU, 2>(L, seg, bit, dbit, ip, o, s);
+  if (w <= 4) return tri_ep_launch<TriKind::kDU, 4>(L, seg, bit, dbit, ip, o, s);
+  return tri_ep_launch<TriKind::kDU, 8>(L, seg, bit, dbit, ip, o, s);
 }
 
 hipError_t launch_trsv_segment(Elem e, TriKind kind, const DCsr& a,
