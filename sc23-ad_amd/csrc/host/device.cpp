@@ -319,38 +319,63 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
         span = std::max<long long>(span, static_cast<long long>(lev_ptr[Lv + 1]) - lpos[q]);
   L.ring = (one_launch_chains && span <= ring) ? ring : 0;
   L.n = n;
-  // entry-parallel mode for integer factors in ring mode
+  // Entry-parallel mode (integer factors): a dependency is read from the LDS
+  // ring when the ring holds it, else from global memory, which needs it at
+  // least kEpFarLevels levels back.
   size_t maxlev_rows = 0;
   for (size_t Lv = 0; Lv + 1 < lev_ptr.size(); ++Lv)
     maxlev_rows = std::max<size_t>(maxlev_rows, lev_ptr[Lv + 1] - lev_ptr[Lv]);
-  if (std::is_same_v<V, int32_t> && L.ring > 0 && maxlev_rows <= 1024 &&
-      sizeof(uint64_t) * 2 * 1024 + sizeof(int32_t) * L.ring <= dev::kTriBlockedMaxLds) {
-    std::vector<int> lslot(std::max<size_t>(lcol.size(), 1), 0),
-        lev_eptr(lev_ptr.size(), 0);
+  int maxlev_chain = 0;
+  for (size_t c = 0; c + 1 < all_chain_lev.size(); c += 2)
+    maxlev_chain = std::max(maxlev_chain, all_chain_lev[c + 1] - all_chain_lev[c]);
+  const size_t ep_lds = sizeof(uint64_t) * 1024 + sizeof(int32_t) * ring +
+                        sizeof(int) * 2 * (static_cast<size_t>(maxlev_chain) + 1);
+  if (std::is_same_v<V, int32_t> && one_launch_chains && maxlev_rows <= 1024 &&
+      ep_lds <= dev::kTriBlockedMaxLds) {
+    std::vector<int> lev_of(static_cast<size_t>(std::max(n, 1)), 0);
+    std::vector<int> lev_eptr(lev_ptr.size(), 0);
     for (size_t Lv = 0; Lv + 1 < lev_ptr.size(); ++Lv) {
       lev_eptr[Lv] = lbeg[lev_ptr[Lv]];
-      for (int idx = lev_ptr[Lv]; idx < lev_ptr[Lv + 1]; ++idx)
-        for (int q = lbeg[idx]; q < lbeg[idx + 1]; ++q) lslot[q] = idx - lev_ptr[Lv];
+      for (int idx = lev_ptr[Lv]; idx < lev_ptr[Lv + 1]; ++idx) lev_of[idx] = static_cast<int>(Lv);
     }
     lev_eptr.back() = lbeg[n];
-    plan.lslot.alloc(sizeof(int) * lslot.size());
-    plan.lslot.upload(lslot.data(), sizeof(int) * lslot.size());
-    plan.lev_eptr.alloc(sizeof(int) * lev_eptr.size());
-    plan.lev_eptr.upload(lev_eptr.data(), sizeof(int) * lev_eptr.size());
-    L.lslot = plan.lslot.as<int>();
-    L.lev_eptr = plan.lev_eptr.as<int>();
-    int w = 1;
-    while (w < 8 && static_cast<size_t>(w) * 64 * 2 < maxlev_rows) w *= 2;
-    L.ep_waves = w;
+    std::vector<int> lent(std::max<size_t>(lcol.size(), 1) * 4, 0);
+    bool fits = true, any_far = false;
+    for (size_t Lv = 0; Lv + 1 < lev_ptr.size(); ++Lv)
+      for (int idx = lev_ptr[Lv]; idx < lev_ptr[Lv + 1]; ++idx)
+        for (int q = lbeg[idx]; q < lbeg[idx + 1]; ++q) {
+          const int dpos = lpos[q];
+          const bool in_ring = static_cast<long long>(lev_ptr[Lv + 1]) - dpos <= ring;
+          const bool far = !in_ring && lev_of[idx] - lev_of[dpos] >= dev::kEpFarLevels;
+          if (!in_ring && !far) fits = false;
+          any_far = any_far || far;
+          lent[4 * static_cast<size_t>(q)] = far ? -(lcol[q] + 1) : dpos;
+          lent[4 * static_cast<size_t>(q) + 1] = static_cast<int>(lval[q]);
+          lent[4 * static_cast<size_t>(q) + 2] = idx - lev_ptr[Lv];
+        }
+    if (fits) {
+      plan.lent.alloc(sizeof(int) * lent.size());
+      plan.lent.upload(lent.data(), sizeof(int) * lent.size());
+      plan.lev_eptr.alloc(sizeof(int) * lev_eptr.size());
+      plan.lev_eptr.upload(lev_eptr.data(), sizeof(int) * lev_eptr.size());
+      L.lent = plan.lent.get();
+      L.lev_eptr = plan.lev_eptr.as<int>();
+      L.ring = ring;
+      int w = 1;
+      while (w < 8 && static_cast<size_t>(w) * 64 * 2 < maxlev_rows) w *= 2;
+      L.ep_waves = w;
+      L.ep_far = any_far ? 1 : 0;
+      L.ep_maxlev = maxlev_chain;
+    }
   }
   int maxdeps = 0;
   for (int idx = 0; idx < n; ++idx) maxdeps = std::max(maxdeps, lbeg[idx + 1] - lbeg[idx]);
   if (std::getenv("SC23AD_VERBOSE")) {
     std::fprintf(stderr,
                  "[sc23ad plan] kind %d n %d nnz %zu chains %d levels(max chain) %d "
-                 "max level rows %zu launches %d ring %d span %lld max deps %d ep waves %d\n",
+                 "max level rows %zu launches %d ring %d span %lld max deps %d ep waves %d far %d\n",
                  static_cast<int>(kind), n, cind.size(), nchain, plan.nlevels, maxlev_rows,
-                 plan.nlaunch, L.ring, span, maxdeps, L.ep_waves);
+                 plan.nlaunch, L.ring, span, maxdeps, L.ep_waves, L.ep_far);
   }
 }
 
@@ -381,6 +406,48 @@ bool build_blocked(TriPlan& plan, TriKind kind, int n, const std::vector<int>& r
   return true;
 }
 
+// Dense 64-wide column-block slices of an integer triangular factor.
+void build_dense(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr,
+                 const std::vector<int>& cind, const std::vector<int32_t>& val) {
+  const int W = dev::kTriBlockWidth;
+  const int nb = (n + W - 1) / W;
+  const bool fwd = kind != TriKind::kDU;
+  std::vector<long long> boff(nb + 1, 0);
+  for (int b = 0; b < nb; ++b) {
+    const int r0 = b * W, r1 = std::min(n, r0 + W);
+    const long long rows = fwd ? (n - r0) : r1;
+    boff[b + 1] = boff[b] + rows * W;
+  }
+  if (boff[nb] > (64LL << 20)) return;  // more than 256 MB: keep the sparse form
+  std::vector<int32_t> sl(static_cast<size_t>(boff[nb]), 0);
+  std::vector<int32_t> dv(static_cast<size_t>(std::max(n, 1)), 0);
+  for (int i = 0; i < n; ++i) {
+    int b0 = rptr[i], e0 = rptr[i + 1];
+    if (kind == TriKind::kLD) {
+      dv[i] = val[e0 - 1];
+      e0 = std::max(b0, e0 - 1);
+    }
+    if (kind == TriKind::kDU) {
+      dv[i] = val[b0];
+      b0 = std::min(e0, b0 + 1);
+    }
+    for (int p = b0; p < e0; ++p) {
+      const int c = cind[p];
+      const int b = c / W;
+      const int first = fwd ? b * W : 0;
+      sl[static_cast<size_t>(boff[b]) + static_cast<size_t>(i - first) * W + (c - b * W)] = val[p];
+    }
+  }
+  plan.slice.alloc(sizeof(int32_t) * sl.size());
+  plan.slice.upload(sl.data(), sizeof(int32_t) * sl.size());
+  plan.boff.alloc(sizeof(long long) * boff.size());
+  plan.boff.upload(boff.data(), sizeof(long long) * boff.size());
+  plan.ddinv.alloc(sizeof(int32_t) * dv.size());
+  plan.ddinv.upload(dv.data(), sizeof(int32_t) * dv.size());
+  plan.nb = nb;
+  plan.dense = true;
+}
+
 }  // namespace
 
 void DevAmg::alloc_work() {
@@ -397,7 +464,7 @@ void DevAmg::alloc_work() {
     tmp[l].alloc(es * n);
   }
   r0.alloc(es * std::max(n0(), 1));
-  perm_scratch.alloc(4 * std::max(n0(), 1));
+  perm_scratch.alloc(16 * static_cast<size_t>(std::max(n0(), 1)));
   tmp[0].alloc(es * std::max(n0(), 1));
   tmpc.alloc(es * std::max(n_coarse, 1));
 }
@@ -432,6 +499,10 @@ std::unique_ptr<DevAmg> DevAmg::upload(const IntHierarchy& h, const LaunchCfg& c
              c.single_wg_max_rows);
   build_blocked(a->lc_plan, TriKind::kL, h.lc.n, h.lc.rptr, h.lc.cind, 4, 8);
   build_blocked(a->uc_plan, TriKind::kDU, h.uc.n, h.uc.rptr, h.uc.cind, 4, 8);
+  if (12 * static_cast<size_t>(h.lc.n) + 16 <= dev::kTriBlockedMaxLds) {
+    build_dense(a->lc_plan, TriKind::kL, h.lc.n, h.lc.rptr, h.lc.cind, h.lc.val);
+    build_dense(a->uc_plan, TriKind::kDU, h.uc.n, h.uc.rptr, h.uc.cind, h.uc.val);
+  }
   a->alloc_work();
   return a;
 }
@@ -483,6 +554,13 @@ void DevAmg::spmv(const DevMat& m, SpmvMode mode, const void* x, const void* z,
 
 void DevAmg::trsv(const TriPlan& plan, const DevMat& m, const void* in,
                   void* out) const {
+  if (plan.dense && elem == Elem::kI32) {
+    hip_check(dev::launch_trsv_dense(plan.kind, m.d.n, plan.slice.as<int32_t>(),
+                                     plan.boff.as<long long>(), plan.ddinv.as<int32_t>(),
+                                     m.d.bit, m.d.dbit, plan.nb, in, out, lib_stream()),
+              "trsv dense");
+    return;
+  }
   if (plan.blocked) {
     hip_check(dev::launch_trsv_blocked(elem, plan.kind, m.d, plan.bptr.as<int>(), plan.nb,
                                        in, out, lib_stream()),

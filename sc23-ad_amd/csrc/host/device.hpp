@@ -76,8 +76,11 @@ struct TriPlan {
   bool blocked = false;
   DevBuf bptr;
   int nb = 0;
+  // dense 64-wide slices (integer coarse factors)
+  bool dense = false;
+  DevBuf slice, boff, ddinv;
   // level-ordered layout used by the level-scheduled kernels
-  DevBuf lrow, lbeg, lcol, lval, ldiag, ecol, eval, lpos, epos, lslot, lev_eptr;
+  DevBuf lrow, lbeg, lcol, lval, ldiag, ecol, eval, lpos, epos, lent, lev_eptr;
   dev::TriLayout layout;
 };
 

@@ -68,17 +68,21 @@ struct TriLayout {
   const int* lpos = nullptr;
   const int* epos = nullptr;
   int ring = 0;  // > 0: LDS ring of this many values (power of two)
-  // entry-parallel mode (integer factors): slot of each entry's row inside its
-  // level, entry offset of each level, waves per chain
-  const int* lslot = nullptr;
+  // entry-parallel mode (integer factors): packed entries {dep, val, slot, 0}
+  // (see k_tri_chain_ep), entry offset of each level, waves per chain
+  const void* lent = nullptr;
   const int* lev_eptr = nullptr;
   int ep_waves = 0;  // > 0: use the entry-parallel kernel
+  int ep_far = 0;    // entries with dep < 0 (read from global memory) exist
+  int ep_maxlev = 0; // most levels in one chain
   int n = 0;
 };
-// Entry-parallel integer solve; in_perm is scratch of n int32 (the input
+// Entry-parallel integer solve; rows_scratch holds n int4 (the row data
 // gathered into level order).
+constexpr int kEpFarLevels = 12;  // least level distance of a dependency read from global memory
 hipError_t launch_trsv_ep(TriKind kind, const TriLayout& L, const TriSegment& seg, int bit,
-                          int dbit, const void* in, void* out, void* in_perm, hipStream_t s);
+                          int dbit, const void* in, void* out, void* rows_scratch,
+                          hipStream_t s);
 hipError_t launch_trsv_layout(Elem e, TriKind kind, const TriLayout& L,
                               const TriSegment& seg, int bit, int dbit, const void* in,
                               void* out, hipStream_t s);
@@ -89,6 +93,11 @@ hipError_t launch_trsv_layout(Elem e, TriKind kind, const TriLayout& L,
 constexpr int kTriBlockWidth = 64;
 constexpr size_t kTriBlockedMaxLds = 160 * 1024;
 size_t tri_blocked_lds(int n, size_t elem, size_t acc);
+// Integer factor stored as dense 64-wide slices per column block (see
+// k_tri_dense_int); dinv[i] is row i's stored inverse diagonal (LD / DU).
+hipError_t launch_trsv_dense(TriKind kind, int n, const int32_t* slice, const long long* boff,
+                             const int32_t* dinv, int bit, int dbit, int nb, const void* in,
+                             void* out, hipStream_t s);
 hipError_t launch_trsv_blocked(Elem e, TriKind kind, const DCsr& a, const int* bptr,
                                int nb, const void* in, void* out, hipStream_t s);
 

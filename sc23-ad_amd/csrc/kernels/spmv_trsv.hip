@@ -487,6 +487,82 @@ __global__ __launch_bounds__(kBlkThreads) void k_tri_blocked_int(
   }
 }
 
+// Dense-slice variant for integer coarse factors.  Block b of the factor is
+// stored as dense 64-wide rows (zeros where the factor has no entry):
+// forward (L): rows r0 .. n-1, backward (DU): rows 0 .. r1-1, row-major, at
+// slice + boff[b].  The sweep is the same as above; the update of the other
+// rows is a dense matrix-vector product read with 16-byte loads.
+template <TriKind K>
+__global__ __launch_bounds__(kBlkThreads) void k_tri_dense_int(
+    int n, const int32_t* __restrict__ slice, const long long* __restrict__ boff,
+    const int32_t* __restrict__ dinv_g, int bit, int dbit, int nb,
+    const int32_t* __restrict__ in, int32_t* __restrict__ out) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  uint64_t* acc = reinterpret_cast<uint64_t*>(smem);
+  int32_t* xs = reinterpret_cast<int32_t*>(acc + n);
+  constexpr bool fwd = (K != TriKind::kDU);
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  for (int i = tid; i < n; i += kBlkThreads)
+    acc[i] = static_cast<uint64_t>(static_cast<int64_t>(in[i])) << bit;
+  __syncthreads();
+  for (int q = 0; q < nb; ++q) {
+    const int b = fwd ? q : nb - 1 - q;
+    const int r0 = b * kTB;
+    const int r1 = min(r0 + kTB, n);
+    const int rows = r1 - r0;
+    const int32_t* D = slice + boff[b];
+    const int first = fwd ? r0 : 0;  // first stored row of this slice
+    if (wid == 0) {
+      int32_t reg[kTB];
+      const int4* rowp = reinterpret_cast<const int4*>(D + static_cast<size_t>(r0 + lane - first) * kTB);
+#pragma unroll
+      for (int v = 0; v < kTB / 4; ++v) {
+        int4 w = make_int4(0, 0, 0, 0);
+        if (lane < rows) w = rowp[v];
+        reg[4 * v] = w.x;
+        reg[4 * v + 1] = w.y;
+        reg[4 * v + 2] = w.z;
+        reg[4 * v + 3] = w.w;
+      }
+      uint64_t t = lane < rows ? acc[r0 + lane] : 0;
+      const int32_t dv = (K != TriKind::kL && lane < rows) ? dinv_g[r0 + lane] : 0;
+      int32_t xv = 0;
+#pragma unroll
+      for (int st = 0; st < kTB; ++st) {
+        const int c = fwd ? st : kTB - 1 - st;
+        if (c < rows) {
+          const int32_t mine = acc_final<int32_t, K>(t, dv, bit, dbit);
+          if (lane == c) xv = mine;
+          const int32_t xc = __builtin_amdgcn_readlane(mine, c);
+          t -= prod64(reg[c], xc);
+        }
+      }
+      if (lane < rows) {
+        xs[r0 + lane] = xv;
+        out[r0 + lane] = xv;
+      }
+    }
+    __syncthreads();
+    int32_t xb[kTB];
+#pragma unroll
+    for (int c = 0; c < kTB; ++c) xb[c] = (r0 + c < n) ? xs[r0 + c] : 0;
+    const int lo = fwd ? r1 : 0;
+    const int hi = fwd ? n : r0;
+    for (int i = lo + tid; i < hi; i += kBlkThreads) {
+      const int4* rowp = reinterpret_cast<const int4*>(D + static_cast<size_t>(i - first) * kTB);
+      uint64_t sum = 0;
+#pragma unroll
+      for (int v = 0; v < kTB / 4; ++v) {
+        const int4 w = rowp[v];
+        sum += prod64(w.x, xb[4 * v]) + prod64(w.y, xb[4 * v + 1]) +
+               prod64(w.z, xb[4 * v + 2]) + prod64(w.w, xb[4 * v + 3]);
+      }
+      acc[i] -= sum;
+    }
+    __syncthreads();
+  }
+}
+
 template <typename T, TriKind K>
 hipError_t tri_blocked(const DCsr& a, const int* bptr, int nb, const void* in, void* out,
                        hipStream_t s) {
@@ -657,50 +733,78 @@ __global__ __launch_bounds__(kChainBlock) void k_tri_chain_pre(
 //
 // For integer factors a row's sum is order-free, so the dependency entries of
 // a whole level (contiguous in the level-ordered layout) are spread over the
-// lanes: each lane multiplies its entries by solution values taken from the
-// LDS ring and adds the products into the row's LDS accumulator with 64-bit
-// LDS atomics; then one lane per row finishes the row.  The input arrives
-// already gathered into level order (in_perm), so every load of the next
-// level, prefetched into registers while the current one is computed,
-// depends only on the level's offsets.  kWaves waves walk one chain; with one
-// wave no barrier is needed, LDS operations of a wave being ordered.
+// lanes: each lane multiplies its entries by solution values and adds the
+// products into the row's LDS accumulator with 64-bit LDS atomics; then one
+// lane per row finishes the row.
+//
+// Entries are packed as int4 {dep, val, slot, 0}: dep >= 0 is the position of
+// a dependency whose value sits in the LDS ring; dep < 0 (kFar kernels only)
+// names row -dep-1, a dependency at least kEpFarLevels levels back that the
+// ring cannot hold, read from `out` in global memory.  A full workgroup fence
+// every kEpFence levels makes such values visible before any load of them is
+// issued.  Row data arrive as int4 {input, inverse diagonal, row, 0},
+// gathered into level order before the launch.  The chain's level offsets are
+// staged in LDS, so every load of a level depends only on LDS data: the
+// entries and rows of level L + kEpDepth are loaded while level L is
+// computed, and the far values of level L + kEpFarAhead (whose addresses need
+// the entries loaded two levels earlier) as well.
 
-constexpr int kEpE = 8;  // entries per lane held in registers
-constexpr int kEpR = 2;  // rows per lane held in registers
-constexpr int kEpMaxLevel = 1024;  // rows of a level (accumulator slots)
+constexpr int kEpE = 8;       // entries per lane held in registers
+constexpr int kEpR = 2;       // rows per lane held in registers
+constexpr int kEpMaxLevel = 1024;
+constexpr int kEpDepth = 4;     // prefetch distance in levels
+constexpr int kEpFarAhead = 2;  // far values are loaded this many levels ahead
+constexpr int kEpFence = 8;     // full fence period in levels (kFar kernels)
+static_assert(kEpFarAhead < kEpDepth, "far values need their entries loaded first");
+static_assert(kEpFarLevels >= kEpFarAhead + kEpFence,
+              "far dependencies must be fenced before their values are loaded");
 
 struct EpPre {
-  int pos[kEpE];
+  int dep[kEpE];
   int32_t val[kEpE];
   int slot[kEpE];
+  int32_t xfar[kEpE];
   int32_t in[kEpR];
   int32_t dinv[kEpR];
+  int row[kEpR];
 };
 
+// Loads whose addresses depend only on the level offsets.
 template <int kThreads>
-__device__ __forceinline__ void ep_load(EpPre& r, int e0, int e1, int s, int e,
-                                        const int* __restrict__ lpos,
-                                        const int32_t* __restrict__ lval,
-                                        const int* __restrict__ lslot,
-                                        const int32_t* __restrict__ in_perm,
-                                        const int32_t* __restrict__ ldiag) {
+__device__ __forceinline__ void ep_load(EpPre& r, int2 lo, int2 hi,
+                                        const int4* __restrict__ lent,
+                                        const int4* __restrict__ rows) {
   const int t = threadIdx.x;
 #pragma unroll
   for (int q = 0; q < kEpE; ++q) {
-    const int p = e0 + t + q * kThreads;
-    if (p < e1) {
-      r.pos[q] = lpos[p];
-      r.val[q] = lval[p];
-      r.slot[q] = lslot[p];
+    const int p = lo.y + t + q * kThreads;
+    if (p < hi.y) {
+      const int4 w = lent[p];
+      r.dep[q] = w.x;
+      r.val[q] = w.y;
+      r.slot[q] = w.z;
     }
   }
 #pragma unroll
   for (int q = 0; q < kEpR; ++q) {
-    const int idx = s + t + q * kThreads;
-    if (idx < e) {
-      r.in[q] = in_perm[idx];
-      r.dinv[q] = ldiag[idx];
+    const int idx = lo.x + t + q * kThreads;
+    if (idx < hi.x) {
+      const int4 w = rows[idx];
+      r.in[q] = w.x;
+      r.dinv[q] = w.y;
+      r.row[q] = w.z;
     }
+  }
+}
+
+// Far values; their addresses come from entries loaded by ep_load.
+template <int kThreads>
+__device__ __forceinline__ void ep_far(EpPre& r, int2 lo, int2 hi, const int32_t* out) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < kEpE; ++q) {
+    const int p = lo.y + t + q * kThreads;
+    if (p < hi.y && r.dep[q] < 0) r.xfar[q] = out[-r.dep[q] - 1];
   }
 }
 
@@ -714,93 +818,129 @@ __device__ __forceinline__ void ep_sync() {
   }
 }
 
-template <TriKind K, int kWaves>
+// LDS: kEpMaxLevel uint64 accumulators, the ring, then the chain's level
+// offsets {lev_ptr, lev_eptr} (max_levels + 1 int2).
+template <TriKind K, int kWaves, bool kFar>
 __global__ __launch_bounds__(64 * kWaves) void k_tri_chain_ep(
-    const int* __restrict__ lrow, const int* __restrict__ lpos,
-    const int32_t* __restrict__ lval, const int* __restrict__ lslot,
-    const int32_t* __restrict__ ldiag, int bit, int dbit, const int* __restrict__ lev_ptr,
-    const int* __restrict__ lev_eptr, const int* __restrict__ chain_lev, int rmask,
-    const int32_t* __restrict__ in_perm, int32_t* __restrict__ out) {
+    const int4* __restrict__ lent, const int* __restrict__ lev_ptr,
+    const int* __restrict__ lev_eptr, const int* __restrict__ chain_lev, int rmask, int bit,
+    int dbit, const int4* __restrict__ rows, int32_t* out) {
   constexpr int kThreads = 64 * kWaves;
   extern __shared__ __align__(16) unsigned char smem_e[];
-  uint64_t* acc = reinterpret_cast<uint64_t*>(smem_e);       // 2 x kEpMaxLevel
-  int32_t* ring = reinterpret_cast<int32_t*>(acc + 2 * kEpMaxLevel);
+  uint64_t* acc = reinterpret_cast<uint64_t*>(smem_e);  // kEpMaxLevel
+  int32_t* ring = reinterpret_cast<int32_t*>(acc + kEpMaxLevel);
+  int2* lvo = reinterpret_cast<int2*>(ring + rmask + 1);
   const int t = threadIdx.x;
-  for (int k = t; k < 2 * kEpMaxLevel; k += kThreads) acc[k] = 0;
-  ep_sync<kWaves>();
   const int lb = chain_lev[2 * blockIdx.x], le = chain_lev[2 * blockIdx.x + 1];
-  EpPre nxt;
-  if (lb < le)
-    ep_load<kThreads>(nxt, lev_eptr[lb], lev_eptr[lb + 1], lev_ptr[lb], lev_ptr[lb + 1],
-                      lpos, lval, lslot, in_perm, ldiag);
-  for (int L = lb; L < le; ++L) {
-    const EpPre cur = nxt;
-    const int s = lev_ptr[L], e = lev_ptr[L + 1];
-    const int e0 = lev_eptr[L], e1 = lev_eptr[L + 1];
-    if (L + 1 < le)
-      ep_load<kThreads>(nxt, lev_eptr[L + 1], lev_eptr[L + 2], lev_ptr[L + 1],
-                        lev_ptr[L + 2], lpos, lval, lslot, in_perm, ldiag);
-    uint64_t* a = acc + (L & 1) * kEpMaxLevel;
-    // products of the level's dependency entries
+  const int nl = le - lb;
+  for (int k = t; k < kEpMaxLevel; k += kThreads) acc[k] = 0;
+  for (int k = t; k <= nl; k += kThreads) lvo[k] = make_int2(lev_ptr[lb + k], lev_eptr[lb + k]);
+  __syncthreads();
+  EpPre buf[kEpDepth];
 #pragma unroll
-    for (int q = 0; q < kEpE; ++q) {
-      const int p = e0 + t + q * kThreads;
-      if (p < e1)
-        atomicAdd(reinterpret_cast<unsigned long long*>(&a[cur.slot[q]]),
-                  static_cast<unsigned long long>(prod64(cur.val[q], ring[cur.pos[q] & rmask])));
-    }
-    for (int p = e0 + t + kEpE * kThreads; p < e1; p += kThreads)
-      atomicAdd(reinterpret_cast<unsigned long long*>(&a[lslot[p]]),
-                static_cast<unsigned long long>(prod64(lval[p], ring[lpos[p] & rmask])));
-    ep_sync<kWaves>();
-    // finish the rows
+  for (int d = 0; d < kEpDepth; ++d)
+    if (d < nl) ep_load<kThreads>(buf[d], lvo[d], lvo[d + 1], lent, rows);
+  // Dependencies stay inside a chain, so its first kEpFarLevels levels have
+  // no far entries: the first far loads are issued from the loop.
+  for (int L0 = 0; L0 < nl; L0 += kEpDepth) {
 #pragma unroll
-    for (int q = 0; q < kEpR; ++q) {
-      const int k = t + q * kThreads;
-      if (s + k < e) {
-        const uint64_t tt = (static_cast<uint64_t>(static_cast<int64_t>(cur.in[q])) << bit) - a[k];
-        a[k] = 0;
-        const int32_t v = acc_final<int32_t, K>(tt, cur.dinv[q], bit, dbit);
-        ring[(s + k) & rmask] = v;
-        out[lrow[s + k]] = v;
+    for (int d = 0; d < kEpDepth; ++d) {
+      const int L = L0 + d;
+      if (L < nl) {
+        const EpPre cur = buf[d];
+        const int2 lo = lvo[L], hi = lvo[L + 1];
+        const int s = lo.x, e = hi.x, e0 = lo.y, e1 = hi.y;
+        if (kFar) {
+          const int Lf = L + kEpFarAhead;
+          if (Lf < nl)
+            ep_far<kThreads>(buf[(d + kEpFarAhead) % kEpDepth], lvo[Lf], lvo[Lf + 1], out);
+        }
+        const int Lp = L + kEpDepth;
+        if (Lp < nl) ep_load<kThreads>(buf[d], lvo[Lp], lvo[Lp + 1], lent, rows);
+#pragma unroll
+        for (int q = 0; q < kEpE; ++q) {
+          const int p = e0 + t + q * kThreads;
+          if (p < e1) {
+            const int dp = cur.dep[q];
+            const int32_t x = (kFar && dp < 0) ? cur.xfar[q] : ring[dp & rmask];
+            atomicAdd(reinterpret_cast<unsigned long long*>(&acc[cur.slot[q]]),
+                      static_cast<unsigned long long>(prod64(cur.val[q], x)));
+          }
+        }
+        for (int p = e0 + t + kEpE * kThreads; p < e1; p += kThreads) {
+          const int4 w = lent[p];
+          const int32_t x = (kFar && w.x < 0) ? out[-w.x - 1] : ring[w.x & rmask];
+          atomicAdd(reinterpret_cast<unsigned long long*>(&acc[w.z]),
+                    static_cast<unsigned long long>(prod64(w.y, x)));
+        }
+        ep_sync<kWaves>();
+#pragma unroll
+        for (int q = 0; q < kEpR; ++q) {
+          const int k = t + q * kThreads;
+          if (s + k < e) {
+            const uint64_t tt =
+                (static_cast<uint64_t>(static_cast<int64_t>(cur.in[q])) << bit) - acc[k];
+            acc[k] = 0;
+            const int32_t v = acc_final<int32_t, K>(tt, cur.dinv[q], bit, dbit);
+            ring[(s + k) & rmask] = v;
+            out[cur.row[q]] = v;
+          }
+        }
+        for (int k = t + kEpR * kThreads; s + k < e; k += kThreads) {
+          const int4 w = rows[s + k];
+          const uint64_t tt = (static_cast<uint64_t>(static_cast<int64_t>(w.x)) << bit) - acc[k];
+          acc[k] = 0;
+          const int32_t v = acc_final<int32_t, K>(tt, w.y, bit, dbit);
+          ring[(s + k) & rmask] = v;
+          out[w.z] = v;
+        }
+        if (kFar && (L % kEpFence) == kEpFence - 1)
+          __syncthreads();  // full fence: global results visible to later loads
+        else
+          ep_sync<kWaves>();
       }
     }
-    for (int k = t + kEpR * kThreads; s + k < e; k += kThreads) {
-      const uint64_t tt =
-          (static_cast<uint64_t>(static_cast<int64_t>(in_perm[s + k])) << bit) - a[k];
-      a[k] = 0;
-      const int32_t v = acc_final<int32_t, K>(tt, ldiag[s + k], bit, dbit);
-      ring[(s + k) & rmask] = v;
-      out[lrow[s + k]] = v;
-    }
-    ep_sync<kWaves>();
   }
 }
 
-__global__ __launch_bounds__(256) void k_gather_i32(int n, const int* __restrict__ idx,
-                                                    const int32_t* __restrict__ src,
-                                                    int32_t* __restrict__ dst) {
+// rows[idx] = {in[lrow[idx]], ldiag[idx], lrow[idx], 0}
+__global__ __launch_bounds__(256) void k_gather_rows(int n, const int* __restrict__ lrow,
+                                                     const int32_t* __restrict__ ldiag,
+                                                     const int32_t* __restrict__ in,
+                                                     int4* __restrict__ rows) {
   const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < n) dst[i] = src[idx[i]];
+  if (i < n) {
+    const int r = lrow[i];
+    rows[i] = make_int4(in[r], ldiag[i], r, 0);
+  }
+}
+
+template <TriKind K, int kWaves, bool kFar>
+hipError_t tri_ep_launch2(const TriLayout& L, const TriSegment& seg, int bit, int dbit,
+                          const int4* rows, int32_t* out, hipStream_t s) {
+  const size_t lds = sizeof(uint64_t) * kEpMaxLevel + sizeof(int32_t) * L.ring +
+                     sizeof(int2) * (static_cast<size_t>(L.ep_maxlev) + 1);
+  if (lds > static_cast<size_t>(kTriBlockedMaxLds)) return hipErrorInvalidValue;
+  static bool attr = false;
+  if (!attr) {
+    hipError_t e =
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tri_chain_ep<K, kWaves, kFar>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            static_cast<int>(kTriBlockedMaxLds));
+    if (e != hipSuccess) return e;
+    attr = true;
+  }
+  hipLaunchKernelGGL((k_tri_chain_ep<K, kWaves, kFar>), dim3(seg.nchain), dim3(64 * kWaves), lds,
+                     s, reinterpret_cast<const int4*>(L.lent), L.lev_ptr, L.lev_eptr,
+                     seg.chain_lev, L.ring - 1, bit, dbit, rows, out);
+  return hipGetLastError();
 }
 
 template <TriKind K, int kWaves>
 hipError_t tri_ep_launch(const TriLayout& L, const TriSegment& seg, int bit, int dbit,
-                         const int32_t* in_perm, int32_t* out, hipStream_t s) {
-  const size_t lds = sizeof(uint64_t) * 2 * kEpMaxLevel + sizeof(int32_t) * L.ring;
-  static bool attr = false;
-  if (!attr) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tri_chain_ep<K, kWaves>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       static_cast<int>(kTriBlockedMaxLds));
-    if (e != hipSuccess) return e;
-    attr = true;
-  }
-  hipLaunchKernelGGL((k_tri_chain_ep<K, kWaves>), dim3(seg.nchain), dim3(64 * kWaves), lds, s,
-                     L.lrow, L.lpos, static_cast<const int32_t*>(L.lval), L.lslot,
-                     static_cast<const int32_t*>(L.ldiag), bit, dbit, L.lev_ptr, L.lev_eptr,
-                     seg.chain_lev, L.ring - 1, in_perm, out);
-  return hipGetLastError();
+                         const int4* rows, int32_t* out, hipStream_t s) {
+  return L.ep_far ? tri_ep_launch2<K, kWaves, true>(L, seg, bit, dbit, rows, out, s)
+                  : tri_ep_launch2<K, kWaves, false>(L, seg, bit, dbit, rows, out, s);
 }
 
 // One level, many workgroups, level-ordered layout.
@@ -894,6 +1034,29 @@ size_t tri_blocked_lds(int n, size_t elem, size_t acc) {
   return b + 8 * kTB + elem * kTB * kTB;
 }
 
+hipError_t launch_trsv_dense(TriKind kind, int n, const int32_t* slice, const long long* boff,
+                             const int32_t* dinv, int bit, int dbit, int nb, const void* in,
+                             void* out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const size_t lds = 12 * static_cast<size_t>(n) + 16;
+  auto go = [&](auto kern) -> hipError_t {
+    static bool attr = false;
+    if (!attr) {
+      hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         static_cast<int>(kTriBlockedMaxLds));
+      if (e != hipSuccess) return e;
+      attr = true;
+    }
+    hipLaunchKernelGGL(kern, dim3(1), dim3(kBlkThreads), lds, s, n, slice, boff, dinv, bit,
+                       dbit, nb, static_cast<const int32_t*>(in), static_cast<int32_t*>(out));
+    return hipGetLastError();
+  };
+  if (kind == TriKind::kL) return go(&k_tri_dense_int<TriKind::kL>);
+  if (kind == TriKind::kDU) return go(&k_tri_dense_int<TriKind::kDU>);
+  return go(&k_tri_dense_int<TriKind::kLD>);
+}
+
 hipError_t launch_trsv_blocked(Elem e, TriKind kind, const DCsr& a, const int* bptr,
                                int nb, const void* in, void* out, hipStream_t s) {
   if (e == Elem::kI32) {
@@ -930,31 +1093,33 @@ hipError_t launch_trsv_layout(Elem e, TriKind kind, const TriLayout& L,
 }
 
 hipError_t launch_trsv_ep(TriKind kind, const TriLayout& L, const TriSegment& seg, int bit,
-                          int dbit, const void* in, void* out, void* in_perm, hipStream_t s) {
+                          int dbit, const void* in, void* out, void* rows_scratch,
+                          hipStream_t s) {
+  int4* rows = static_cast<int4*>(rows_scratch);
   if (L.n > 0)
-    hipLaunchKernelGGL(k_gather_i32, dim3((L.n + 255) / 256), dim3(256), 0, s, L.n, L.lrow,
-                       static_cast<const int32_t*>(in), static_cast<int32_t*>(in_perm));
+    hipLaunchKernelGGL(k_gather_rows, dim3((L.n + 255) / 256), dim3(256), 0, s, L.n, L.lrow,
+                       static_cast<const int32_t*>(L.ldiag), static_cast<const int32_t*>(in),
+                       rows);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  const int32_t* ip = static_cast<const int32_t*>(in_perm);
   int32_t* o = static_cast<int32_t*>(out);
   const int w = L.ep_waves;
   if (kind == TriKind::kLD) {
-    if (w <= 1) return tri_ep_launch<TriKind::kLD, 1>(L, seg, bit, dbit, ip, o, s);
-    if (w <= 2) return tri_ep_launch<TriKind::kLD, 2>(L, seg, bit, dbit, ip, o, s);
-    if (w <= 4) return tri_ep_launch<TriKind::kLD, 4>(L, seg, bit, dbit, ip, o, s);
-    return tri_ep_launch<TriKind::kLD, 8>(L, seg, bit, dbit, ip, o, s);
+    if (w <= 1) return tri_ep_launch<TriKind::kLD, 1>(L, seg, bit, dbit, rows, o, s);
+    if (w <= 2) return tri_ep_launch<TriKind::kLD, 2>(L, seg, bit, dbit, rows, o, s);
+    if (w <= 4) return tri_ep_launch<TriKind::kLD, 4>(L, seg, bit, dbit, rows, o, s);
+    return tri_ep_launch<TriKind::kLD, 8>(L, seg, bit, dbit, rows, o, s);
   }
   if (kind == TriKind::kL) {
-    if (w <= 1) return tri_ep_launch<TriKind::kL, 1>(L, seg, bit, dbit, ip, o, s);
-    if (w <= 2) return tri_ep_launch<TriKind::kL, 2>(L, seg, bit, dbit, ip, o, s);
-    if (w <= 4) return tri_ep_launch<TriKind::kL, 4>(L, seg, bit, dbit, ip, o, s);
-    return tri_ep_launch<TriKind::kL, 8>(L, seg, bit, dbit, ip, o, s);
+    if (w <= 1) return tri_ep_launch<TriKind::kL, 1>(L, seg, bit, dbit, rows, o, s);
+    if (w <= 2) return tri_ep_launch<TriKind::kL, 2>(L, seg, bit, dbit, rows, o, s);
+    if (w <= 4) return tri_ep_launch<TriKind::kL, 4>(L, seg, bit, dbit, rows, o, s);
+    return tri_ep_launch<TriKind::kL, 8>(L, seg, bit, dbit, rows, o, s);
   }
-  if (w <= 1) return tri_ep_launch<TriKind::kDU, 1>(L, seg, bit, dbit, ip, o, s);
-  if (w <= 2) return tri_ep_launch<TriKind::kDU, 2>(L, seg, bit, dbit, ip, o, s);
-  if (w <= 4) return tri_ep_launch<TriKind::kDU, 4>(L, seg, bit, dbit, ip, o, s);
-  return tri_ep_launch<TriKind::kDU, 8>(L, seg, bit, dbit, ip, o, s);
+  if (w <= 1) return tri_ep_launch<TriKind::kDU, 1>(L, seg, bit, dbit, rows, o, s);
+  if (w <= 2) return tri_ep_launch<TriKind::kDU, 2>(L, seg, bit, dbit, rows, o, s);
+  if (w <= 4) return tri_ep_launch<TriKind::kDU, 4>(L, seg, bit, dbit, rows, o, s);
+  return tri_ep_launch<TriKind::kDU, 8>(L, seg, bit, dbit, rows, o, s);
 }
 
 hipError_t launch_trsv_segment(Elem e, TriKind kind, const DCsr& a,
