@@ -184,9 +184,21 @@ def ref_solve(a_arrays, b, solver, smoother, m=30, eps=1e-10, bit=-1,
     iters, conv, bit_used, tlen = C.c_int(), C.c_int(), C.c_int(-1), C.c_int()
     rel = C.c_double()
     tr = np.zeros(max(trace_cap, 1), np.uint64)
-    ref().ref_solve(n, _p(rptr), _p(cind), _p(val), _p(np.ascontiguousarray(b)),
-                    solver, smoother, m, eps, bit, omega, b_num, theta, _p(x),
-                    C.byref(iters), C.byref(conv), C.byref(rel),
-                    C.byref(bit_used), _p(tr), trace_cap, C.byref(tlen))
+    # The reference prints every iteration's residual; keep it off stdout.
+    libc = C.CDLL(None)
+    libc.fflush(None)
+    saved = os.dup(1)
+    devnull = os.open(os.devnull, os.O_WRONLY)
+    os.dup2(devnull, 1)
+    try:
+        ref().ref_solve(n, _p(rptr), _p(cind), _p(val), _p(np.ascontiguousarray(b)),
+                        solver, smoother, m, eps, bit, omega, b_num, theta, _p(x),
+                        C.byref(iters), C.byref(conv), C.byref(rel),
+                        C.byref(bit_used), _p(tr), trace_cap, C.byref(tlen))
+    finally:
+        libc.fflush(None)
+        os.dup2(saved, 1)
+        os.close(saved)
+        os.close(devnull)
     return dict(x=x, iters=iters.value, converged=conv.value,
                 relres=rel.value, bit=bit_used.value, trace=tr[:tlen.value])

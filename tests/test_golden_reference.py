@@ -1,0 +1,133 @@
+"""Setup and oracle against the reference's own outputs (CPU).
+
+tests/golden/reference.json holds what the reference, compiled in this
+container (oracle/_ref), computes on seeded systems: hierarchy file hashes
+for FP64/FP32/INT x SOR/BlockSOR(40) (preconditioner/INTAMG.hpp:25-106,
+149-245 over helper/helper_multigrid.hpp:34-586) and the results of
+Restarted(FGMRES) for solvers 0/1/2 (solver/irsolver.hpp:27-43,
+solver/gmres.hpp:18-271), including the checksum of every integer V-cycle
+input and output.  Here the product's host setup must write byte-identical
+hierarchies, and the oracle in the reference's sequential reduction order
+must reproduce every result bit for bit.
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+import sc23ad as S
+from helpers import export, file_sha, sha, system
+
+
+def _cases(golden):
+    return golden["cases"]
+
+
+@pytest.fixture(scope="module")
+def systems(golden):
+    return {c["name"]: system(c) for c in _cases(golden)}
+
+
+def test_generated_systems_match_fixture(golden, systems):
+    for c in _cases(golden):
+        sy = systems[c["name"]]
+        assert sy.a.n == c["n"] and sy.a.nnz == c["nnz"], c["name"]
+        assert sha(*sy.a.arrays()) == c["a_sha"], c["name"]
+        assert sha(sy.b) == c["b_sha"], c["name"]
+        assert sy.nrm_b == c["nrm_b"], c["name"]
+
+
+@pytest.mark.parametrize("prec", [0, 1, 2], ids=["fp64", "fp32", "int"])
+@pytest.mark.parametrize("smoother", [0, 1], ids=["sor", "blocksor"])
+def test_setup_byte_identical_to_reference(golden, systems, prec, smoother):
+    for c in _cases(golden):
+        h = S.Hierarchy(systems[c["name"]].a, prec, S.amg_param(smoother))
+        path = export(h)
+        try:
+            assert file_sha(path) == c["hier"][f"{prec}_{smoother}"], c["name"]
+        finally:
+            os.unlink(path)
+
+
+@pytest.fixture(scope="module")
+def oracle_hiers(golden, systems):
+    out = {}
+    for c in _cases(golden):
+        a = systems[c["name"]].a
+        for smoother in (0, 1):
+            hs = S.Hierarchy.build_multi(a, [S.INT, S.FP64, S.FP32], S.amg_param(smoother))
+            for prec, h in zip((S.INT, S.FP64, S.FP32), hs):
+                path = export(h)
+                out[(c["name"], prec, smoother)] = O.OracleHier(path)
+                os.unlink(path)
+    return out
+
+
+def test_default_bits_match_reference(golden, systems, oracle_hiers):
+    for c in _cases(golden):
+        sy = systems[c["name"]]
+        for smoother in (0, 1):
+            bit = oracle_hiers[(c["name"], S.FP64, smoother)].default_bits(sy.b, sy.nrm_b)
+            assert bit == c["solves"][f"2_{smoother}"]["bit"], c["name"]
+
+
+@pytest.mark.parametrize("smoother", [0, 1], ids=["sor", "blocksor"])
+def test_oracle_int_solve_matches_reference(golden, systems, oracle_hiers, smoother):
+    for c in _cases(golden):
+        sy = systems[c["name"]]
+        g = c["solves"][f"2_{smoother}"]
+        x, res, trace = oracle_hiers[(c["name"], S.INT, smoother)].solve(
+            sy.a.arrays(), sy.b, sy.nrm_b, m=c["m"], eps=golden["eps"], bit=g["bit"],
+            mode=O.RED_SEQ, trace_cap=8192)
+        assert [format(int(v), "016x") for v in trace] == g["trace"], c["name"]
+        assert (res.iters, res.converged) == (g["iters"], g["converged"]), c["name"]
+        assert sha(x) == g["x_sha"], c["name"]
+
+
+@pytest.mark.parametrize("solver", [0, 1], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("smoother", [0, 1], ids=["sor", "blocksor"])
+def test_oracle_fp_solve_matches_reference(golden, systems, oracle_hiers, solver, smoother):
+    for c in _cases(golden):
+        sy = systems[c["name"]]
+        g = c["solves"][f"{solver}_{smoother}"]
+        x, res, _ = oracle_hiers[(c["name"], solver, smoother)].solve(
+            sy.a.arrays(), sy.b, sy.nrm_b, m=c["m"], eps=golden["eps"], mode=O.RED_SEQ)
+        assert (res.iters, res.converged) == (g["iters"], g["converged"]), c["name"]
+        assert sha(x) == g["x_sha"], c["name"]
+        assert O.relres(sy.a.arrays(), sy.b, sy.nrm_b, x, O.RED_SEQ) == g["relres"]
+
+
+# ---- live comparison where the reference is compiled (this container) -------
+
+live = pytest.mark.skipif(not O.ref_available(),
+                          reason="oracle/_ref not built (reference checkout absent)")
+
+
+@live
+@pytest.mark.parametrize("case", [
+    (S.GEN_POISSON2D, (37,), 9), (S.GEN_LAPLACE3D, (13,), 8),
+    (S.GEN_CONVDIFF2D, (45, 80.0, 4.0), 7),
+    (S.GEN_DIFFUSION27, (14, 7, 1.0, 0.5, 1e-2, 12), 6),
+], ids=["poisson37", "laplace13", "convdiff45", "diffusion27_14"])
+def test_live_reference(case, tmp_path):
+    kind, params, seed = case
+    sy = S.make_system(kind, params, seed)
+    arrs = sy.a.arrays()
+    for smoother in (0, 1):
+        for prec in (0, 2):
+            ref_path = str(tmp_path / f"ref_{prec}_{smoother}.bin")
+            O.ref_build_hier(arrs, prec, smoother, ref_path)
+            h = S.Hierarchy(sy.a, prec, S.amg_param(smoother))
+            mine = str(tmp_path / f"mine_{prec}_{smoother}.bin")
+            h.export(mine)
+            assert file_sha(mine) == file_sha(ref_path)
+        R = O.ref_solve(arrs, sy.b, 2, smoother, m=30, eps=1e-10, trace_cap=8192)
+        oh = O.OracleHier(str(tmp_path / f"ref_2_{smoother}.bin"))
+        x, res, trace = oh.solve(arrs, sy.b, sy.nrm_b, m=30, eps=1e-10, bit=R["bit"],
+                                 mode=O.RED_SEQ, trace_cap=8192)
+        assert np.array_equal(trace, R["trace"])
+        assert res.iters == R["iters"]
+        assert np.array_equal(x, R["x"])

@@ -1,0 +1,74 @@
+"""The 128^3 7-point Laplacian (2,097,152 rows), FGMRES(30), BlockSOR(40):
+the benchmark workload at full size (MI355X).
+
+Checked bit for bit against the oracle: the level-0 integer SpMV, one integer
+and one FP64 V-cycle, and the whole integer solve (every V-cycle input and
+output checksum, and x).  Size-independent properties: the explicit relative
+residual reaches 1e-8 and the iteration count equals the oracle's.
+"""
+from __future__ import annotations
+
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+import oracle as O
+import sc23ad as S
+
+pytestmark = pytest.mark.gpu
+
+N = 128
+
+
+@pytest.fixture(scope="module")
+def big():
+    print(f"[setup] laplace3d {N}^3 BlockSOR", flush=True)
+    sy = S.make_system(S.GEN_LAPLACE3D, (N,), 1)
+    hi, hd = S.Hierarchy.build_multi(sy.a, [S.INT, S.FP64], S.amg_param(S.BLOCK_SOR))
+    out = {"sy": sy, "amg_i": S.Amg(hi), "amg_d": S.Amg(hd)}
+    for key, h in (("or_i", hi), ("or_d", hd)):
+        fd, path = tempfile.mkstemp(suffix=".bin")
+        os.close(fd)
+        h.export(path)
+        out[key] = O.OracleHier(path)
+        os.unlink(path)
+    return out
+
+
+def test_spmv_and_vcycles(big):
+    sy = big["sy"]
+    n = sy.a.n
+    assert n == N**3
+    rng = np.random.default_rng(5)
+    v = rng.integers(-(2**29), 2**29, n).astype(np.int32)
+    print("[spmv] level 0", flush=True)
+    assert np.array_equal(big["amg_i"].level_op(0, S.OP_SPMV_A, v, n),
+                          big["or_i"].level_op(0, S.OP_SPMV_A, v, n))
+    print("[vcycle] int", flush=True)
+    assert np.array_equal(big["amg_i"].precondition(v), big["or_i"].vcycle(v))
+    print("[vcycle] fp64", flush=True)
+    d = rng.standard_normal(n)
+    assert np.array_equal(big["amg_d"].precondition(d).view(np.uint64),
+                          big["or_d"].vcycle(d).view(np.uint64))
+
+
+def test_int_solve_to_1e8(big):
+    sy = big["sy"]
+    bit = big["amg_d"].default_frac_bits(sy.b, sy.nrm_b)
+    assert bit == big["or_d"].default_bits(sy.b, sy.nrm_b)
+    sol = S.Solver(S.INT, sy.a, big["amg_i"], m=30, eps=1e-8, bit=bit)
+    sol.trace(4096)
+    print(f"[solve] int bit {bit}", flush=True)
+    x, cv = sol.solve(sy.b, sy.nrm_b)
+    trace = sol.read_trace(4096)
+    assert cv.is_converged
+    rel = sol.relres(sy.b, sy.nrm_b, x)
+    assert rel < 1e-8
+    print(f"[oracle] int solve ({cv.iter} iterations on the GPU)", flush=True)
+    xo, ro, to = big["or_i"].solve(sy.a.arrays(), sy.b, sy.nrm_b, m=30, eps=1e-8, bit=bit,
+                                   mode=O.RED_GPU, trace_cap=4096)
+    assert np.array_equal(trace, to)
+    assert cv.iter == ro.iters
+    assert np.array_equal(x.view(np.uint64), xo.view(np.uint64))

@@ -1,0 +1,236 @@
+"""HIP solve path against the oracle, through the C ABI (MI355X).
+
+Bar: bit-exact for every integer vector, and for the FP64/FP32 solvers too,
+since the device library's FP reductions follow a fixed documented order that
+the oracle restates (1024 x 256 blocked partial sums, halving trees).  In the
+device library's sequential-reduction mode the GPU reproduces the
+reference's own results (tests/golden/reference.json) bit for bit.
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+import sc23ad as S
+from helpers import export, sha, system
+
+pytestmark = pytest.mark.gpu
+
+CASES = ["poisson2d_32", "poisson2d_48", "poisson2d_64", "laplace3d_16",
+         "convdiff2d_60", "diffusion27_16", "inclusions3d_20"]
+
+
+@pytest.fixture(scope="module")
+def built(golden):
+    """case -> (case record, system, {(prec, smoother): (Amg, OracleHier)})."""
+    out = {}
+    for c in golden["cases"]:
+        if c["name"] not in CASES:
+            continue
+        print(f"[setup] {c['name']}", flush=True)
+        sy = system(c)
+        hs = {}
+        for smoother in (0, 1):
+            multi = S.Hierarchy.build_multi(sy.a, [S.INT, S.FP64, S.FP32],
+                                            S.amg_param(smoother))
+            for prec, h in zip((S.INT, S.FP64, S.FP32), multi):
+                path = export(h)
+                hs[(prec, smoother)] = (S.Amg(h), O.OracleHier(path))
+                os.unlink(path)
+        out[c["name"]] = (c, sy, hs)
+    return out
+
+
+def _rand(prec, n, rng):
+    if prec == S.INT:
+        return rng.integers(-(2**29), 2**29, n).astype(np.int32)
+    return rng.standard_normal(n).astype(S.DT[prec])
+
+
+def _same(a, b):
+    return a.dtype == b.dtype and np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("prec", [S.INT, S.FP64, S.FP32], ids=["int", "fp64", "fp32"])
+def test_level_ops(built, name, prec):
+    c, sy, hs = built[name]
+    rng = np.random.default_rng(17)
+    for smoother in (0, 1):
+        amg, oh = hs[(prec, smoother)]
+        depth = amg.depth
+        for lv in range(depth):
+            n = oh.level_n(lv)
+            nxt = oh.level_n(lv + 1)
+            x = _rand(prec, n, rng)
+            xc = _rand(prec, nxt, rng)
+            print(f"[level ops] {name} prec {prec} smoother {smoother} level {lv}", flush=True)
+            for op in (S.OP_SPMV_A, S.OP_SPMV_U, S.OP_TRSV_LD, S.OP_SMOOTH0):
+                assert _same(amg.level_op(lv, op, x, n), oh.level_op(lv, op, x, n)), op
+            assert _same(amg.level_op(lv, S.OP_SPMV_R, x, nxt), oh.level_op(lv, S.OP_SPMV_R, x, nxt))
+            assert _same(amg.level_op(lv, S.OP_SPMV_P, xc, n), oh.level_op(lv, S.OP_SPMV_P, xc, n))
+            init = _rand(prec, n, rng)
+            assert _same(amg.level_op(lv, S.OP_SMOOTH1, x, n, init),
+                         oh.level_op(lv, S.OP_SMOOTH1, x, n, init))
+        nc = oh.level_n(depth)
+        xc = _rand(prec, nc, rng)
+        assert _same(amg.level_op(depth, S.OP_COARSE, xc, nc),
+                     oh.level_op(depth, S.OP_COARSE, xc, nc))
+
+
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("prec", [S.INT, S.FP64, S.FP32], ids=["int", "fp64", "fp32"])
+def test_vcycle(built, name, prec):
+    c, sy, hs = built[name]
+    rng = np.random.default_rng(23)
+    for smoother in (0, 1):
+        amg, oh = hs[(prec, smoother)]
+        for _ in range(2):  # the second call reuses every work buffer
+            v = _rand(prec, sy.a.n, rng)
+            assert _same(amg.precondition(v), oh.vcycle(v)), (name, smoother)
+
+
+def test_default_frac_bits(built):
+    for name in CASES:
+        c, sy, hs = built[name]
+        for smoother in (0, 1):
+            amg, _ = hs[(S.FP64, smoother)]
+            assert amg.default_frac_bits(sy.b, sy.nrm_b) == c["solves"][f"2_{smoother}"]["bit"]
+
+
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("smoother", [0, 1], ids=["sor", "blocksor"])
+def test_int_solve_matches_oracle(built, golden, name, smoother):
+    c, sy, hs = built[name]
+    amg, oh = hs[(S.INT, smoother)]
+    bit = c["solves"][f"2_{smoother}"]["bit"]
+    sol = S.Solver(S.INT, sy.a, amg, m=c["m"], eps=golden["eps"], bit=bit)
+    sol.trace(8192)
+    x, cv = sol.solve(sy.b, sy.nrm_b)
+    trace = sol.read_trace(8192)
+    xo, ro, to = oh.solve(sy.a.arrays(), sy.b, sy.nrm_b, m=c["m"], eps=golden["eps"],
+                          bit=bit, mode=O.RED_GPU, trace_cap=8192)
+    assert np.array_equal(trace, to)
+    assert (cv.iter, cv.is_converged) == (ro.iters, ro.converged)
+    assert _same(x, xo)
+    g = c["solves"][f"2_{smoother}"]
+    assert abs(cv.iter - g["iters"]) <= 1  # vs the reference's own (sequential) order
+    assert sol.relres(sy.b, sy.nrm_b, x) < 1e-9
+
+
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("prec", [S.FP64, S.FP32], ids=["fp64", "fp32"])
+def test_fp_solve_matches_oracle(built, golden, name, prec):
+    c, sy, hs = built[name]
+    for smoother in (0, 1):
+        amg, oh = hs[(prec, smoother)]
+        sol = S.Solver(prec, sy.a, amg, m=c["m"], eps=golden["eps"])
+        x, cv = sol.solve(sy.b, sy.nrm_b)
+        xo, ro, _ = oh.solve(sy.a.arrays(), sy.b, sy.nrm_b, m=c["m"], eps=golden["eps"],
+                             mode=O.RED_GPU)
+        assert (cv.iter, cv.is_converged) == (ro.iters, ro.converged)
+        assert _same(x, xo)
+
+
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("solver", [S.INT, S.FP64, S.FP32], ids=["int", "fp64", "fp32"])
+def test_sequential_mode_reproduces_reference(built, golden, name, solver):
+    """With sequential FP reductions the device solve is the reference's."""
+    c, sy, hs = built[name]
+    for smoother in (0, 1):
+        g = c["solves"][f"{solver}_{smoother}"]
+        amg, _ = hs[(solver, smoother)]
+        kw = {"bit": g["bit"]} if solver == S.INT else {}
+        sol = S.Solver(solver, sy.a, amg, m=c["m"], eps=golden["eps"], **kw)
+        sol.set_sequential_reduction(True)
+        if solver == S.INT:
+            sol.trace(8192)
+        x, cv = sol.solve(sy.b, sy.nrm_b)
+        assert (cv.iter, cv.is_converged) == (g["iters"], g["converged"])
+        if solver == S.INT:
+            assert [format(int(v), "016x") for v in sol.read_trace(8192)] == g["trace"]
+        assert sha(x) == g["x_sha"], (name, solver, smoother)
+
+
+def test_device_pointer_solve_equals_host(built, golden):
+    c, sy, hs = built["laplace3d_16"]
+    amg, _ = hs[(S.INT, 1)]
+    bit = c["solves"]["2_1"]["bit"]
+    sol = S.Solver(S.INT, sy.a, amg, m=30, eps=golden["eps"], bit=bit)
+    xh, cvh = sol.solve(sy.b, sy.nrm_b)
+    db = S.DevArray(sy.b)
+    dx = S.DevArray(np.zeros(sy.a.n))
+    cvd = sol.solve_device(db, sy.nrm_b, dx)
+    assert cvd.iter == cvh.iter and _same(dx.get(), xh)
+
+
+@pytest.mark.parametrize("bit", [8, 12])
+def test_small_bits_status_matches_oracle(built, bit):
+    """Few fractional bits: at 8 the restart budget (1000 / m restarts,
+    solver/irsolver.hpp:27-43) runs out without convergence and the status
+    says so; at 12 the solve converges after more restarts."""
+    c, sy, hs = built["poisson2d_64"]
+    amg, oh = hs[(S.INT, 0)]
+    sol = S.Solver(S.INT, sy.a, amg, m=30, eps=1e-10, bit=bit)
+    x, cv = sol.solve(sy.b, sy.nrm_b)
+    xo, ro, _ = oh.solve(sy.a.arrays(), sy.b, sy.nrm_b, m=30, eps=1e-10, bit=bit,
+                         mode=O.RED_GPU)
+    assert cv.status == ro.status and cv.iter == ro.iters and cv.is_converged == ro.converged
+    assert _same(x, xo)
+    assert cv.is_converged == (bit == 12)
+
+
+def test_overflowing_bits_flagged(built):
+    """30 fractional bits overflow this system's Givens generation (a zero
+    norm where lsqrt wraps): the solve stops with the overflow status, as the
+    oracle does, instead of trapping on a division by zero."""
+    c, sy, hs = built["poisson2d_64"]
+    amg, oh = hs[(S.INT, 0)]
+    sol = S.Solver(S.INT, sy.a, amg, m=30, eps=1e-10, bit=30)
+    x, cv = sol.solve(sy.b, sy.nrm_b)
+    xo, ro, _ = oh.solve(sy.a.arrays(), sy.b, sy.nrm_b, m=30, eps=1e-10, bit=30,
+                         mode=O.RED_GPU)
+    assert ro.status == S.OVERFLOW
+    assert (cv.status, cv.iter, cv.overflow_flag) == (ro.status, ro.iters, 1)
+    assert _same(x, xo)
+
+
+# ---- raw integer SpMV -----------------------------------------------------------
+
+def _spmv_ref(rptr, cind, val, bit, x):
+    y = np.zeros(len(rptr) - 1, np.int64)
+    for i in range(len(rptr) - 1):
+        acc = 0
+        for p in range(rptr[i], rptr[i + 1]):
+            acc += int(val[p]) * int(x[cind[p]])
+        acc &= (1 << 64) - 1
+        acc = acc - (1 << 64) if acc >> 63 else acc
+        v = (acc >> bit) & 0xFFFFFFFF
+        y[i] = v - (1 << 32) if v >> 31 else v
+    return y.astype(np.int32)
+
+
+@pytest.mark.parametrize("shape", ["empty_rows", "long_rows", "one_row", "wrap"])
+def test_spmv_int_brute_force(shape):
+    rng = np.random.default_rng(hash(shape) % 2**32)
+    if shape == "one_row":
+        lens = np.array([37])
+    elif shape == "empty_rows":
+        lens = rng.integers(0, 4, 3000) * (rng.random(3000) < 0.6)
+    elif shape == "long_rows":
+        lens = rng.integers(100, 700, 40)
+    else:
+        lens = rng.integers(1, 30, 2000)
+    n = len(lens)
+    m = 5000
+    rptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    cind = rng.integers(0, m, rptr[-1]).astype(np.int32)
+    lim = 2**31 if shape == "wrap" else 2**20
+    val = rng.integers(-lim, lim, rptr[-1]).astype(np.int32)
+    x = rng.integers(-lim, lim, m).astype(np.int32)
+    for bit in (0, 17, 30):
+        assert np.array_equal(S.spmv_int(rptr, cind, val, bit, x),
+                              _spmv_ref(rptr, cind, val, bit, x))
