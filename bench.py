@@ -44,20 +44,60 @@ def spmv_int_bytes(n: int, m: int, nnz: int) -> int:
     return 8 * nnz + 4 * (n + 1) + 4 * m + 4 * n
 
 
+def sptrsv_int_bytes(n: int, nnz_ld: int) -> int:
+    """Algorithmic HBM bytes of one integer LD solve: int32 values and column
+    indices of LD, row pointers, the right-hand side read once and the
+    solution written once."""
+    return 8 * nnz_ld + 4 * (n + 1) + 8 * n
+
+
+def ld_nnz(rptr, cind, smoother: int, b_num: int = 40) -> int:
+    """Entries of the smoother's LD factor (diagonal included): col <= i for
+    SOR; left <= col <= i inside each of the b_num row blocks of ceil(N / b_num)
+    rows for BlockSOR (smoother/stationary.hpp:934-978)."""
+    n = len(rptr) - 1
+    row = np.repeat(np.arange(n, dtype=np.int64), np.diff(rptr))
+    col = np.asarray(cind, np.int64)
+    if smoother == 0:
+        return int(np.count_nonzero(col <= row))
+    size = -(-n // b_num)
+    left = (row // size) * size
+    return int(np.count_nonzero((col <= row) & (col >= left)))
+
+
 def read_traffic(path: str):
-    """HBM bytes per SpMV launch from a committed rocprofv3 --pmc summary
-    (FETCH_SIZE and WRITE_SIZE in KiB per dispatch, scaled by the copy-kernel
-    calibration factor recorded with them)."""
+    """HBM bytes per launch from a committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE in KiB per dispatch, each scaled by the factor
+    the same counter gives for the library's copy kernel of known size)."""
     if not os.path.exists(path):
         return None
     txt = open(path).read()
-    f = re.search(r"^FETCH_SIZE_KiB_per_dispatch:\s*([0-9.eE+-]+)", txt, re.M)
-    w = re.search(r"^WRITE_SIZE_KiB_per_dispatch:\s*([0-9.eE+-]+)", txt, re.M)
-    c = re.search(r"^copy_calibration_factor:\s*([0-9.eE+-]+)", txt, re.M)
-    if not (f and w):
+
+    def num(key):
+        m = re.search(r"^" + key + r":\s*([0-9.eE+-]+)", txt, re.M)
+        return float(m.group(1)) if m else None
+
+    f, w = num("FETCH_SIZE_KiB_per_dispatch"), num("WRITE_SIZE_KiB_per_dispatch")
+    if f is None or w is None:
         return None
-    factor = float(c.group(1)) if c else 1.0
-    return (float(f.group(1)) + float(w.group(1))) * 1024.0 * factor
+    cf = num("fetch_calibration_factor") or 1.0
+    cw = num("write_calibration_factor") or 1.0
+    return (f * cf + w * cw) * 1024.0
+
+
+def calibrate_copy(mib: int, reps: int) -> None:
+    """Copy kernel of known size for the traffic-counter calibration: each
+    launch reads and writes mib MiB once."""
+    import sc23ad as S
+    nbytes = mib << 20
+    src = S.DevArray(nbytes=nbytes)
+    dst = S.DevArray(nbytes=nbytes)
+    S.check(S.lib.sc23ad_dev_copy(dst.p, src.p, nbytes))
+    for _ in range(reps):
+        S.check(S.lib.sc23ad_dev_copy(dst.p, src.p, nbytes))
+    S.check(S.lib.sc23ad_dev_sync())
+    print(json.dumps({"copy_kernel": "k_copy_bytes", "bytes_read_per_launch": nbytes,
+                      "bytes_written_per_launch": nbytes, "launches": reps + 1}), flush=True)
 
 
 def write_system_file(path, a, b_unscaled):
@@ -88,15 +128,30 @@ def cpu_baseline(args, unscaled_arrays, b_unscaled, smoother):
         cmd = [exe, sysf, str(smoother), "2", str(args.m), "0", repr(args.rtol)]
         log("cpu baseline:", " ".join(cmd))
         t0 = time.time()
-        out = subprocess.run(cmd, capture_output=True, text=True, env=env,
-                             timeout=args.cpu_timeout)
+        # The baseline prints one stderr line per finished stage; relay them.
+        outf = sysf + ".out"
+        with open(outf, "w") as fo:
+            proc = subprocess.Popen(cmd, stdout=fo, stderr=subprocess.PIPE, text=True,
+                                    env=env)
+            for line in proc.stderr:
+                log(line.rstrip())
+            try:
+                rc = proc.wait(timeout=max(1.0, args.cpu_timeout - (time.time() - t0)))
+            except subprocess.TimeoutExpired:
+                proc.kill()
+                log("cpu baseline timed out")
+                return None
+            finally:
+                os.unlink(sysf)
+        with open(outf) as fo:
+            stdout = fo.read()
+        os.unlink(outf)
         wall = time.time() - t0
-        os.unlink(sysf)
-        tm = re.search(r"^\[Time\] : ([0-9.]+)", out.stdout, re.M)
-        it = re.search(r"^\[Iter\] : (\d+)", out.stdout, re.M)
-        rs = re.search(r"^# \[Res\] : ([0-9.eE+-]+)", out.stdout, re.M)
-        if out.returncode != 0 or not tm:
-            log("cpu baseline failed", out.returncode, out.stderr[-500:])
+        tm = re.search(r"^\[Time\] : ([0-9.]+)", stdout, re.M)
+        it = re.search(r"^\[Iter\] : (\d+)", stdout, re.M)
+        rs = re.search(r"^# \[Res\] : ([0-9.eE+-]+)", stdout, re.M)
+        if rc != 0 or not tm:
+            log("cpu baseline failed", rc)
             return None
         return {"value": float(tm.group(1)), "unit": "s",
                 "cores": threads if smoother == 1 else 1, "kind": "reference",
@@ -122,7 +177,9 @@ def main():
     ap.add_argument("--rtol", type=float, default=1e-8)
     ap.add_argument("--no-fp", action="store_true", help="skip the FP64/FP32 solver runs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-timeout", type=float, default=240.0)
+    ap.add_argument("--cpu-timeout", type=float, default=400.0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run the copy kernel (MIB MiB per launch) for counter calibration")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -135,6 +192,9 @@ def main():
     if S.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         sys.exit(3)
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy, 9)
+        return
 
     n = args.n
     t0 = time.time()
@@ -186,6 +246,11 @@ def main():
     launches, spmv_ms = soli.spmv_stats()
     spmv_bytes = spmv_int_bytes(N, N, nnz)
     achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9 if spmv_ms > 0 else None
+    trsv_launches, trsv_ms = soli.trsv_stats()
+    rptr, cind, _ = sy.a.arrays()
+    nnz_ld = ld_nnz(rptr, cind, args.smoother)
+    trsv_bytes = sptrsv_int_bytes(N, nnz_ld)
+    trsv_achieved = trsv_bytes / (trsv_ms * 1e-3) / 1e9 if trsv_ms > 0 else None
 
     extra = {}
     if not args.no_fp:
@@ -199,6 +264,8 @@ def main():
 
     traffic = read_traffic(os.path.join(HERE, "profiles",
                                         f"spmv_int_level0_laplace3d_{n}_pmc.txt"))
+    trsv_traffic = read_traffic(os.path.join(HERE, "profiles",
+                                             f"sptrsv_int_level0_laplace3d_{n}_pmc.txt"))
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -234,6 +301,13 @@ def main():
                      "frac": (achieved / HBM_PEAK_GBPS) if achieved else None,
                      "traffic": traffic, "algorithmic_bytes": spmv_bytes,
                      "launches_timed": launches, "mean_ms": spmv_ms},
+        "roofline_sptrsv": {"bound": "hbm",
+                            "kernel": "integer LD SpTRSV, level 0 (smoother)",
+                            "achieved": trsv_achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                            "frac": (trsv_achieved / HBM_PEAK_GBPS) if trsv_achieved else None,
+                            "traffic": trsv_traffic, "algorithmic_bytes": trsv_bytes,
+                            "nnz_ld": nnz_ld, "launches_timed": trsv_launches,
+                            "mean_ms": trsv_ms},
         "cpu_baseline": cpu,
     }
     rec.update(extra)

@@ -141,6 +141,10 @@ int sc23ad_dev_alloc(size_t bytes, void** ptr);
 int sc23ad_dev_free(void* ptr);
 int sc23ad_dev_upload(void* dst, const void* src, size_t bytes);
 int sc23ad_dev_download(void* dst, const void* src, size_t bytes);
+/* Device-to-device copy with the library's streaming copy kernel (one read
+   and one write of every byte; also the calibration kernel for HBM traffic
+   counters).  bytes must be a multiple of 4 and below 8 GiB. */
+int sc23ad_dev_copy(void* dst, const void* src, size_t bytes);
 int sc23ad_dev_sync(void);
 
 /* Raw fixed-point CSR SpMV on device arrays: y_i = (int32)((sum_j (int64)val_j *
@@ -188,10 +192,13 @@ int sc23ad_solver_set_reduction(sc23ad_solver* s, int sequential);
 /* Record an order-free checksum of every V-cycle input and output (two
    uint64 per Arnoldi step) during the next solves; cap = capacity. */
 int sc23ad_solver_trace(sc23ad_solver* s, uint64_t* buf, int cap, int* len);
-/* Bracket every fine-level integer SpMV launch of the next solves with hipEvents
-   and report their count and mean duration. */
+/* Bracket every fine-level SpMV of the Arnoldi steps (A0 Z_j, gmres.hpp:240)
+   and every fine-level LD solve of the smoother (ldtri.hpp:172-183,
+   511-525) with hipEvents during the next solves; report their count and
+   mean duration. */
 int sc23ad_solver_time_spmv(sc23ad_solver* s, int enable);
 int sc23ad_solver_spmv_stats(sc23ad_solver* s, int* launches, double* mean_ms);
+int sc23ad_solver_trsv_stats(sc23ad_solver* s, int* launches, double* mean_ms);
 /* |e_{j+1}| / nrm_b after every Arnoldi step of the last solve (the
    reference's "# [j]" convergence history, gmres.hpp:253-255). */
 int sc23ad_solver_history(sc23ad_solver* s, double* buf, int cap, int* len);

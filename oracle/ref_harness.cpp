@@ -513,6 +513,13 @@ int ref_solve(int n, const int* rptr, const int* cind, const double* val,
 }  // extern "C"
 
 #ifdef SC23_REF_MAIN
+// Progress of the baseline run on stderr, one line per finished stage.
+static double g_t0 = getTime();
+static void stage(const char* what) {
+  fprintf(stderr, "[ref] %s (%.1f s)\n", what, getTime() - g_t0);
+  fflush(stderr);
+}
+
 // CPU baseline: <system file> <smoother 0|1> <solver 0|1|2> <m> [bit]
 // The system file ("SC23SYS1", int32 n, nnz, rptr, cind, double val, b) holds
 // the unscaled matrix and right-hand side; scaling and nrm_b follow
@@ -534,6 +541,7 @@ int main(int argc, char** argv) {
       fread(val, 8, nnz, f) != (size_t)nnz || fread(b, 8, n, f) != (size_t)n)
     return 3;
   fclose(f);
+  stage("system read");
   CSRMat* Abase = new CSRMat(val, cind, rptr, n, n, MMShape::General, MMType::Real);
   int smoother_type = atoi(argv[2]);
   int solver_type = atoi(argv[3]);
@@ -579,11 +587,14 @@ int main(int argc, char** argv) {
     Preconditioner<double>* M;
     if (smoother_type == 0) {
       M = new DAmgSORd(Abase, dA, new AMGParam(20, 0.8, 1, sp));
+      stage("FP64 AMG setup done");
       iM = new IAmgSOR(Abase, iA, new AMGParam(20, 0.8, 1, sp));
     } else {
       M = new DAmgBJd(Abase, dA, new AMGParam(20, 0.8, 1, sp));
+      stage("FP64 AMG setup done");
       iM = new IAmgBJ(Abase, iA, new AMGParam(20, 0.8, 1, sp));
     }
+    stage("integer AMG setup done");
     int bit;
     if (argc > 5 && atoi(argv[5]) > 0) {
       bit = atoi(argv[5]);
@@ -597,6 +608,7 @@ int main(int argc, char** argv) {
       free(c);
     }
     delete M;
+    stage("fractional bits chosen; solving");
     printf("[Bit] : %d\n", bit);
     auto inner = new FGMRESflex(iA, m, epsilon, iM, (int8_t)bit);
     auto s = new Restarted(dA, inner, outer, epsilon);

@@ -85,6 +85,17 @@ int sc23ad_dev_download(void* dst, const void* src, size_t bytes) {
   });
 }
 
+int sc23ad_dev_copy(void* dst, const void* src, size_t bytes) {
+  if ((bytes & 3) != 0 || (!dst && bytes) || (!src && bytes))
+    return fail(SC23AD_BAD_PARAM, "dev_copy: bytes must be a multiple of 4");
+  return guarded([&]() -> int {
+    hip_check(dev::launch_copy(dev::Elem::kI32, static_cast<int>(bytes / 4), src, dst,
+                               lib_stream()),
+              "copy");
+    return SC23AD_OK;
+  });
+}
+
 int sc23ad_dev_sync(void) {
   return guarded([&]() -> int {
     hip_check(hipStreamSynchronize(lib_stream()), "sync");
@@ -151,6 +162,7 @@ int sc23ad_amg_precondition(sc23ad_amg* m, const void* in, void* out, int on_dev
       dout.download(out, bytes);
     }
     hip_check(hipStreamSynchronize(lib_stream()), "sync");
+    a.check_errors();
     return SC23AD_OK;
   });
 }
@@ -177,6 +189,7 @@ int sc23ad_amg_level_op(sc23ad_amg* m, int level, int op, const void* in, void* 
       }
     }
     hip_check(hipStreamSynchronize(lib_stream()), "sync");
+    a.check_errors();
     return SC23AD_OK;
   });
 }
@@ -269,18 +282,33 @@ int sc23ad_solver_history(sc23ad_solver* s, double* buf, int cap, int* len) {
 
 int sc23ad_solver_time_spmv(sc23ad_solver* s, int enable) {
   if (!s) return fail(SC23AD_BAD_PARAM, "time_spmv: null");
-  s->s->time_spmv = enable != 0;
-  s->s->spmv_launches = 0;
-  s->s->spmv_ms_total = 0;
-  return SC23AD_OK;
+  return guarded([&]() -> int {
+    for (EventTimer* t : {&s->s->spmv_timer, &s->s->amg->trsv0_timer}) {
+      t->collect();
+      t->reset();
+      t->on = enable != 0;
+    }
+    return SC23AD_OK;
+  });
+}
+
+static int timer_stats(EventTimer& t, int* launches, double* mean_ms) {
+  return guarded([&]() -> int {
+    t.collect();
+    if (launches) *launches = t.launches;
+    if (mean_ms) *mean_ms = t.launches ? t.ms_total / t.launches : 0.0;
+    return SC23AD_OK;
+  });
 }
 
 int sc23ad_solver_spmv_stats(sc23ad_solver* s, int* launches, double* mean_ms) {
   if (!s) return fail(SC23AD_BAD_PARAM, "spmv_stats: null");
-  if (launches) *launches = s->s->spmv_launches;
-  if (mean_ms)
-    *mean_ms = s->s->spmv_launches ? s->s->spmv_ms_total / s->s->spmv_launches : 0.0;
-  return SC23AD_OK;
+  return timer_stats(s->s->spmv_timer, launches, mean_ms);
+}
+
+int sc23ad_solver_trsv_stats(sc23ad_solver* s, int* launches, double* mean_ms) {
+  if (!s) return fail(SC23AD_BAD_PARAM, "trsv_stats: null");
+  return timer_stats(s->s->amg->trsv0_timer, launches, mean_ms);
 }
 
 int sc23ad_solver_relres(sc23ad_solver* s, const double* b, double nrm_b,

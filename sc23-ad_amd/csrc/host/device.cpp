@@ -361,8 +361,19 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
       L.lent = plan.lent.get();
       L.lev_eptr = plan.lev_eptr.as<int>();
       L.ring = ring;
+      // Waves: every row of a level held in registers, and at most one level
+      // in 16 with more entries than the registers hold (the rest loop over
+      // global memory, one round trip each).
       int w = 1;
       while (w < 8 && static_cast<size_t>(w) * 64 * 2 < maxlev_rows) w *= 2;
+      auto overflowing = [&](int waves) {
+        size_t c = 0;
+        const size_t cap = static_cast<size_t>(waves) * 64 * dev::kEpEntriesPerLane;
+        for (size_t Lv = 0; Lv + 1 < lev_ptr.size(); ++Lv)
+          if (static_cast<size_t>(lev_eptr[Lv + 1] - lev_eptr[Lv]) > cap) ++c;
+        return c;
+      };
+      while (w < 8 && overflowing(w) * 16 > lev_ptr.size() - 1) w *= 2;
       L.ep_waves = w;
       L.ep_far = any_far ? 1 : 0;
       L.ep_maxlev = maxlev_chain;
@@ -407,20 +418,19 @@ bool build_blocked(TriPlan& plan, TriKind kind, int n, const std::vector<int>& r
 }
 
 // Dense 64-wide column-block slices of an integer triangular factor.
-void build_dense(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr,
+// Tiles of an integer coarse factor for the one-workgroup-per-block solve:
+// for block b, the 64 x 64 tiles of the blocks it depends on in processing
+// order, then its diagonal tile; column-major inside a tile.
+void build_tiles(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr,
                  const std::vector<int>& cind, const std::vector<int32_t>& val) {
   const int W = dev::kTriBlockWidth;
   const int nb = (n + W - 1) / W;
+  if (n <= 0 || nb > dev::kTileMaxBlocks) return;
   const bool fwd = kind != TriKind::kDU;
-  std::vector<long long> boff(nb + 1, 0);
-  for (int b = 0; b < nb; ++b) {
-    const int r0 = b * W, r1 = std::min(n, r0 + W);
-    const long long rows = fwd ? (n - r0) : r1;
-    boff[b + 1] = boff[b] + rows * W;
-  }
-  if (boff[nb] > (64LL << 20)) return;  // more than 256 MB: keep the sparse form
-  std::vector<int32_t> sl(static_cast<size_t>(boff[nb]), 0);
-  std::vector<int32_t> dv(static_cast<size_t>(std::max(n, 1)), 0);
+  std::vector<long long> toff(nb + 1, 0);
+  for (int b = 0; b < nb; ++b) toff[b + 1] = toff[b] + static_cast<long long>(fwd ? b + 1 : nb - b) * W * W;
+  std::vector<int32_t> tl(static_cast<size_t>(toff[nb]), 0);
+  std::vector<int32_t> dv(static_cast<size_t>(n), 0);
   for (int i = 0; i < n; ++i) {
     int b0 = rptr[i], e0 = rptr[i + 1];
     if (kind == TriKind::kLD) {
@@ -431,19 +441,28 @@ void build_dense(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rpt
       dv[i] = val[b0];
       b0 = std::min(e0, b0 + 1);
     }
+    const int b = i / W;
     for (int p = b0; p < e0; ++p) {
       const int c = cind[p];
-      const int b = c / W;
-      const int first = fwd ? b * W : 0;
-      sl[static_cast<size_t>(boff[b]) + static_cast<size_t>(i - first) * W + (c - b * W)] = val[p];
+      const int bc = c / W;
+      if (fwd ? bc > b : bc < b)
+        throw DevError{1, "coarse factor entry on the wrong side of the diagonal"};
+      const int slot = fwd ? bc : nb - 1 - bc;  // tile position in processing order
+      tl[static_cast<size_t>(toff[b]) + static_cast<size_t>(slot) * W * W +
+         static_cast<size_t>(c - bc * W) * W + (i - b * W)] = val[p];
     }
   }
-  plan.slice.alloc(sizeof(int32_t) * sl.size());
-  plan.slice.upload(sl.data(), sizeof(int32_t) * sl.size());
-  plan.boff.alloc(sizeof(long long) * boff.size());
-  plan.boff.upload(boff.data(), sizeof(long long) * boff.size());
+  plan.slice.alloc(sizeof(int32_t) * tl.size());
+  plan.slice.upload(tl.data(), sizeof(int32_t) * tl.size());
+  plan.boff.alloc(sizeof(long long) * toff.size());
+  plan.boff.upload(toff.data(), sizeof(long long) * toff.size());
   plan.ddinv.alloc(sizeof(int32_t) * dv.size());
   plan.ddinv.upload(dv.data(), sizeof(int32_t) * dv.size());
+  std::vector<int> zero(static_cast<size_t>(nb) + 1, 0);
+  plan.flags.alloc(sizeof(int) * zero.size());
+  plan.flags.upload(zero.data(), sizeof(int) * zero.size());
+  plan.err.alloc(sizeof(int));
+  plan.err.upload(zero.data(), sizeof(int));
   plan.nb = nb;
   plan.dense = true;
 }
@@ -500,8 +519,8 @@ std::unique_ptr<DevAmg> DevAmg::upload(const IntHierarchy& h, const LaunchCfg& c
   build_blocked(a->lc_plan, TriKind::kL, h.lc.n, h.lc.rptr, h.lc.cind, 4, 8);
   build_blocked(a->uc_plan, TriKind::kDU, h.uc.n, h.uc.rptr, h.uc.cind, 4, 8);
   if (12 * static_cast<size_t>(h.lc.n) + 16 <= dev::kTriBlockedMaxLds) {
-    build_dense(a->lc_plan, TriKind::kL, h.lc.n, h.lc.rptr, h.lc.cind, h.lc.val);
-    build_dense(a->uc_plan, TriKind::kDU, h.uc.n, h.uc.rptr, h.uc.cind, h.uc.val);
+    build_tiles(a->lc_plan, TriKind::kL, h.lc.n, h.lc.rptr, h.lc.cind, h.lc.val);
+    build_tiles(a->uc_plan, TriKind::kDU, h.uc.n, h.uc.rptr, h.uc.cind, h.uc.val);
   }
   a->alloc_work();
   return a;
@@ -555,10 +574,12 @@ void DevAmg::spmv(const DevMat& m, SpmvMode mode, const void* x, const void* z,
 void DevAmg::trsv(const TriPlan& plan, const DevMat& m, const void* in,
                   void* out) const {
   if (plan.dense && elem == Elem::kI32) {
-    hip_check(dev::launch_trsv_dense(plan.kind, m.d.n, plan.slice.as<int32_t>(),
+    plan.epoch = plan.epoch == INT32_MAX ? 1 : plan.epoch + 1;
+    hip_check(dev::launch_trsv_tiles(plan.kind, m.d.n, plan.nb, plan.slice.as<int32_t>(),
                                      plan.boff.as<long long>(), plan.ddinv.as<int32_t>(),
-                                     m.d.bit, m.d.dbit, plan.nb, in, out, lib_stream()),
-              "trsv dense");
+                                     m.d.bit, m.d.dbit, in, out, plan.flags.as<int>(),
+                                     plan.epoch, plan.err.as<int>(), lib_stream()),
+              "trsv tiles");
     return;
   }
   if (plan.blocked) {
@@ -585,14 +606,30 @@ void DevAmg::trsv(const TriPlan& plan, const DevMat& m, const void* in,
 void DevAmg::smooth(int l, const void* in, void* out, bool inited) const {
   const DevLevel& L = lv[l];
   int k0 = 0;
+  EventTimer* tm = l == 0 ? &trsv0_timer : nullptr;
   if (!inited) {
+    if (tm) tm->begin();
     trsv(L.ld_plan, L.ld, in, out);
+    if (tm) tm->end();
     k0 = 1;
   }
   void* t = tmp[l].get();
   for (int k = k0; k < iter; ++k) {
     spmv(L.u, SpmvMode::kSub, out, in, t);
+    if (tm) tm->begin();
     trsv(L.ld_plan, L.ld, t, out);
+    if (tm) tm->end();
+  }
+}
+
+void DevAmg::check_errors() const {
+  for (const TriPlan* p : {&lc_plan, &uc_plan}) {
+    if (!p->dense) continue;
+    int e = 0;
+    hip_check(hipMemcpyAsync(&e, p->err.get(), sizeof(int), hipMemcpyDeviceToHost, lib_stream()),
+              "error flag");
+    hip_check(hipStreamSynchronize(lib_stream()), "sync");
+    if (e != 0) throw DevError{5, "coarse solve: a block waited too long for its dependencies"};
   }
 }
 
@@ -694,7 +731,43 @@ DevSolver::DevSolver(Elem e, const HostCsr& a, DevAmg* amg_, int m_, double eps_
 
 DevSolver::~DevSolver() {
   if (host_flags) hipHostFree(host_flags);
-  for (hipEvent_t e : spmv_events) hipEventDestroy(e);
+}
+
+void EventTimer::begin() {
+  if (!on) return;
+  while (static_cast<int>(ev.size()) < 2 * used + 2) {
+    hipEvent_t e;
+    hip_check(hipEventCreate(&e), "event");
+    ev.push_back(e);
+  }
+  hip_check(hipEventRecord(ev[2 * used], lib_stream()), "record");
+}
+
+void EventTimer::end() {
+  if (!on) return;
+  hip_check(hipEventRecord(ev[2 * used + 1], lib_stream()), "record");
+  ++used;
+}
+
+void EventTimer::collect() {
+  for (int q = 0; q < used; ++q) {
+    hip_check(hipEventSynchronize(ev[2 * q + 1]), "sync");
+    float t = 0;
+    hip_check(hipEventElapsedTime(&t, ev[2 * q], ev[2 * q + 1]), "elapsed");
+    ms_total += t;
+    ++launches;
+  }
+  used = 0;
+}
+
+void EventTimer::reset() {
+  used = 0;
+  launches = 0;
+  ms_total = 0;
+}
+
+EventTimer::~EventTimer() {
+  for (hipEvent_t e : ev) hipEventDestroy(e);
 }
 
 void DevSolver::read_flags() {
@@ -737,15 +810,6 @@ DevSolver::Result DevSolver::solve(const double* b_dev, double nrm_b, double* x_
   hipEvent_t e0, e1;
   hip_check(hipEventCreate(&e0), "event");
   hip_check(hipEventCreate(&e1), "event");
-  int spmv_used = 0;  // event pairs recorded in this solve
-  auto spmv_event = [&](int idx) {
-    while (static_cast<int>(spmv_events.size()) <= idx) {
-      hipEvent_t ev;
-      hip_check(hipEventCreate(&ev), "event");
-      spmv_events.push_back(ev);
-    }
-    return spmv_events[idx];
-  };
   history.clear();
   // Restarted: prev = 0 before the first pass
   hip_check(hipMemsetAsync(k.scal, 0, 8 * sizeof(double), s), "reset scal");
@@ -777,12 +841,9 @@ DevSolver::Result DevSolver::solve(const double* b_dev, double nrm_b, double* x_
         hip_check(dev::launch_checksum(n, static_cast<const int32_t*>(vj), k, s), "cs");
         hip_check(dev::launch_checksum(n, static_cast<const int32_t*>(zj), k, s), "cs");
       }
-      if (time_spmv) hip_check(hipEventRecord(spmv_event(2 * spmv_used), s), "record");
+      spmv_timer.begin();
       amg->spmv(amg->lv[0].a, SpmvMode::kPlain, zj, nullptr, w);
-      if (time_spmv) {
-        hip_check(hipEventRecord(spmv_event(2 * spmv_used + 1), s), "record");
-        ++spmv_used;
-      }
+      spmv_timer.end();
       hip_check(dev::launch_dot_first(elem, n, Vb, w, k, j, seq, s), "dot");
       for (int kk = 0; kk <= j; ++kk)
         hip_check(dev::launch_axpy_dot(elem, n, Vb, w, k, kk, j, seq, s), "axpy_dot");
@@ -821,12 +882,9 @@ DevSolver::Result DevSolver::solve(const double* b_dev, double nrm_b, double* x_
   double sc[8];
   hip_check(hipMemcpy(sc, k.scal, sizeof sc, hipMemcpyDeviceToHost), "scal");
   res.implicit_relres = sc[5];
-  for (int q = 0; q < spmv_used; ++q) {
-    float t = 0;
-    hip_check(hipEventElapsedTime(&t, spmv_events[2 * q], spmv_events[2 * q + 1]), "elapsed");
-    spmv_ms_total += t;
-    ++spmv_launches;
-  }
+  spmv_timer.collect();
+  amg->trsv0_timer.collect();
+  amg->check_errors();
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   return res;

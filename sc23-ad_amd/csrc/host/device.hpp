@@ -76,12 +76,30 @@ struct TriPlan {
   bool blocked = false;
   DevBuf bptr;
   int nb = 0;
-  // dense 64-wide slices (integer coarse factors)
+  // 64 x 64 tiles, one workgroup per block (integer coarse factors)
   bool dense = false;
-  DevBuf slice, boff, ddinv;
+  DevBuf slice, boff, ddinv, flags, err;
+  mutable int epoch = 0;  // value the blocks raise their flags to in the last launch
   // level-ordered layout used by the level-scheduled kernels
   DevBuf lrow, lbeg, lcol, lval, ldiag, ecol, eval, lpos, epos, lent, lev_eptr;
   dev::TriLayout layout;
+};
+
+// hipEvent pairs bracketing launches on the library stream; elapsed times
+// are read by collect() once the stream has been synchronised.
+struct EventTimer {
+  bool on = false;
+  int launches = 0;
+  double ms_total = 0;
+  void begin();
+  void end();
+  void collect();
+  void reset();
+  ~EventTimer();
+
+ private:
+  std::vector<hipEvent_t> ev;
+  int used = 0;
 };
 
 struct DevLevel {
@@ -109,6 +127,7 @@ class DevAmg {
   std::vector<DevBuf> ee, xx, rr, tmp;
   DevBuf r0, tmpc, perm_scratch;
   LaunchCfg cfg;
+  mutable EventTimer trsv0_timer;  // level-0 LD solves (benchmark hook)
 
   static std::unique_ptr<DevAmg> upload(const IntHierarchy& h, const LaunchCfg& c);
   template <typename T>
@@ -126,6 +145,8 @@ class DevAmg {
   void vcycle(const void* in, void* out) const;
   // Algorithmic HBM bytes of one V-cycle (sum over its kernels).
   double vcycle_bytes() const;
+  // Throws if a kernel with a bounded wait gave up (after a stream sync).
+  void check_errors() const;
 
  private:
   void alloc_work();
@@ -151,10 +172,7 @@ class DevSolver {
   // tracing / timing
   bool tracing = false;
   int trace_cap = 0;
-  bool time_spmv = false;
-  int spmv_launches = 0;
-  double spmv_ms_total = 0;
-  std::vector<hipEvent_t> spmv_events;  // pairs, reused across solves
+  EventTimer spmv_timer;  // fine-level A SpMV of the Arnoldi steps (benchmark hook)
   std::vector<double> history;          // |e_{j+1}| / nrm_b per Arnoldi step
 
   DevSolver(dev::Elem e, const HostCsr& a, DevAmg* amg, int m, double eps,

@@ -79,7 +79,8 @@ struct TriLayout {
 };
 // Entry-parallel integer solve; rows_scratch holds n int4 (the row data
 // gathered into level order).
-constexpr int kEpFarLevels = 12;  // least level distance of a dependency read from global memory
+constexpr int kEpEntriesPerLane = 8;  // entry-parallel kernel: entries held in registers
+constexpr int kEpFarLevels = 40;  // least level distance of a dependency read from global memory
 hipError_t launch_trsv_ep(TriKind kind, const TriLayout& L, const TriSegment& seg, int bit,
                           int dbit, const void* in, void* out, void* rows_scratch,
                           hipStream_t s);
@@ -95,9 +96,14 @@ constexpr size_t kTriBlockedMaxLds = 160 * 1024;
 size_t tri_blocked_lds(int n, size_t elem, size_t acc);
 // Integer factor stored as dense 64-wide slices per column block (see
 // k_tri_dense_int); dinv[i] is row i's stored inverse diagonal (LD / DU).
-hipError_t launch_trsv_dense(TriKind kind, int n, const int32_t* slice, const long long* boff,
-                             const int32_t* dinv, int bit, int dbit, int nb, const void* in,
-                             void* out, hipStream_t s);
+// Tiled integer solve of a coarse factor, one workgroup per 64-row block
+// (see k_tri_tiles_int); flags: nb ints, epoch: a value no flag holds yet,
+// err: incremented when a wait gives up.
+constexpr int kTileMaxBlocks = 64;
+hipError_t launch_trsv_tiles(TriKind kind, int n, int nb, const int32_t* tiles,
+                             const long long* toff, const int32_t* dinv, int bit, int dbit,
+                             const void* in, void* out, int* flags, int epoch, int* err,
+                             hipStream_t s);
 hipError_t launch_trsv_blocked(Elem e, TriKind kind, const DCsr& a, const int* bptr,
                                int nb, const void* in, void* out, hipStream_t s);
 

@@ -487,79 +487,92 @@ __global__ __launch_bounds__(kBlkThreads) void k_tri_blocked_int(
   }
 }
 
-// Dense-slice variant for integer coarse factors.  Block b of the factor is
-// stored as dense 64-wide rows (zeros where the factor has no entry):
-// forward (L): rows r0 .. n-1, backward (DU): rows 0 .. r1-1, row-major, at
-// slice + boff[b].  The sweep is the same as above; the update of the other
-// rows is a dense matrix-vector product read with 16-byte loads.
+// Tiled integer coarse solve, one workgroup per 64-row block.  Workgroup b
+// owns rows 64b .. 64b+63.  Its factor entries are stored as dense 64 x 64
+// tiles (zeros where the factor has no entry), column-major (entry (row r,
+// column c) of a tile at c * 64 + r), in processing order: the off-diagonal
+// tiles of the blocks it depends on (forward: 0 .. b-1; backward: nb-1 ..
+// b+1), then its diagonal tile.  For each dependency block the workgroup
+// waits until that block's solution is published (flags[block] == epoch),
+// subtracts the tile's product (four waves, 16 columns each; integer sums
+// are order-free), then wave 0 sweeps the diagonal tile: at step c lane c
+// finishes its row and the value is broadcast to the other lanes.  The block
+// publishes its 64 values with an agent-scope release before raising its
+// flag.  Dependencies point to lower (forward) or higher (backward) blocks
+// only, and all blocks of a coarse grid (at most 2000 rows: 32 blocks) are
+// resident at once, so every wait ends; a wait longer than kTileMaxSpin
+// polls records an error and gives up instead of hanging.
+constexpr int kTileThreads = 256;
+constexpr int kTileMaxSpin = 1 << 18;
+
 template <TriKind K>
-__global__ __launch_bounds__(kBlkThreads) void k_tri_dense_int(
-    int n, const int32_t* __restrict__ slice, const long long* __restrict__ boff,
-    const int32_t* __restrict__ dinv_g, int bit, int dbit, int nb,
-    const int32_t* __restrict__ in, int32_t* __restrict__ out) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  uint64_t* acc = reinterpret_cast<uint64_t*>(smem);
-  int32_t* xs = reinterpret_cast<int32_t*>(acc + n);
+__global__ __launch_bounds__(kTileThreads) void k_tri_tiles_int(
+    int n, int nb, const int32_t* __restrict__ tiles, const long long* __restrict__ toff,
+    const int32_t* __restrict__ dinv_g, int bit, int dbit, const int32_t* __restrict__ in,
+    int32_t* out, int* flags, int epoch, int* err) {
   constexpr bool fwd = (K != TriKind::kDU);
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  for (int i = tid; i < n; i += kBlkThreads)
-    acc[i] = static_cast<uint64_t>(static_cast<int64_t>(in[i])) << bit;
-  __syncthreads();
-  for (int q = 0; q < nb; ++q) {
-    const int b = fwd ? q : nb - 1 - q;
-    const int r0 = b * kTB;
-    const int r1 = min(r0 + kTB, n);
-    const int rows = r1 - r0;
-    const int32_t* D = slice + boff[b];
-    const int first = fwd ? r0 : 0;  // first stored row of this slice
-    if (wid == 0) {
-      int32_t reg[kTB];
-      const int4* rowp = reinterpret_cast<const int4*>(D + static_cast<size_t>(r0 + lane - first) * kTB);
+  __shared__ uint64_t part[kTileThreads / 64][64];
+  __shared__ int32_t xsh[64];
+  const int b = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int r0 = b * kTB;
+  const int rows = min(kTB, n - r0);
+  const int32_t* T = tiles + toff[b];
+  const int ndep = fwd ? b : nb - 1 - b;
+  uint64_t t = 0;
+  if (w == 0 && lane < rows) t = static_cast<uint64_t>(static_cast<int64_t>(in[r0 + lane])) << bit;
+  for (int k = 0; k < ndep; ++k) {
+    const int bp = fwd ? k : nb - 1 - k;
+    const int32_t* Tk = T + static_cast<size_t>(k) * (kTB * kTB);
+    int32_t tv[16];
 #pragma unroll
-      for (int v = 0; v < kTB / 4; ++v) {
-        int4 w = make_int4(0, 0, 0, 0);
-        if (lane < rows) w = rowp[v];
-        reg[4 * v] = w.x;
-        reg[4 * v + 1] = w.y;
-        reg[4 * v + 2] = w.z;
-        reg[4 * v + 3] = w.w;
-      }
-      uint64_t t = lane < rows ? acc[r0 + lane] : 0;
-      const int32_t dv = (K != TriKind::kL && lane < rows) ? dinv_g[r0 + lane] : 0;
-      int32_t xv = 0;
-#pragma unroll
-      for (int st = 0; st < kTB; ++st) {
-        const int c = fwd ? st : kTB - 1 - st;
-        if (c < rows) {
-          const int32_t mine = acc_final<int32_t, K>(t, dv, bit, dbit);
-          if (lane == c) xv = mine;
-          const int32_t xc = __builtin_amdgcn_readlane(mine, c);
-          t -= prod64(reg[c], xc);
+    for (int j = 0; j < 16; ++j) tv[j] = Tk[(16 * w + j) * kTB + lane];
+    if (tid == 0) {
+      int spins = 0;
+      while (__hip_atomic_load(&flags[bp], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > kTileMaxSpin) {
+          atomicAdd(err, 1);
+          break;
         }
       }
-      if (lane < rows) {
-        xs[r0 + lane] = xv;
-        out[r0 + lane] = xv;
-      }
     }
     __syncthreads();
-    int32_t xb[kTB];
-#pragma unroll
-    for (int c = 0; c < kTB; ++c) xb[c] = (r0 + c < n) ? xs[r0 + c] : 0;
-    const int lo = fwd ? r1 : 0;
-    const int hi = fwd ? n : r0;
-    for (int i = lo + tid; i < hi; i += kBlkThreads) {
-      const int4* rowp = reinterpret_cast<const int4*>(D + static_cast<size_t>(i - first) * kTB);
-      uint64_t sum = 0;
-#pragma unroll
-      for (int v = 0; v < kTB / 4; ++v) {
-        const int4 w = rowp[v];
-        sum += prod64(w.x, xb[4 * v]) + prod64(w.y, xb[4 * v + 1]) +
-               prod64(w.z, xb[4 * v + 2]) + prod64(w.w, xb[4 * v + 3]);
-      }
-      acc[i] -= sum;
+    if (tid < kTB) {
+      const int c = bp * kTB + tid;
+      xsh[tid] = c < n ? __hip_atomic_load(&out[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
     }
     __syncthreads();
+    uint64_t ps = 0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) ps += prod64(tv[j], xsh[16 * w + j]);
+    part[w][lane] = ps;
+    __syncthreads();
+    if (w == 0) {
+#pragma unroll
+      for (int q = 0; q < kTileThreads / 64; ++q) t -= part[q][lane];
+    }
+  }
+  if (w == 0) {
+    const int32_t* Td = T + static_cast<size_t>(ndep) * (kTB * kTB);
+    int32_t reg[kTB];
+#pragma unroll
+    for (int c = 0; c < kTB; ++c) reg[c] = Td[c * kTB + lane];
+    const int32_t dv = (K != TriKind::kL && lane < rows) ? dinv_g[r0 + lane] : 0;
+    int32_t xv = 0;
+#pragma unroll
+    for (int st = 0; st < kTB; ++st) {
+      const int c = fwd ? st : kTB - 1 - st;
+      if (c < rows) {
+        const int32_t mine = acc_final<int32_t, K>(t, dv, bit, dbit);
+        if (lane == c) xv = mine;
+        const int32_t xc = __builtin_amdgcn_readlane(mine, c);
+        t -= prod64(reg[c], xc);
+      }
+    }
+    if (lane < rows) out[r0 + lane] = xv;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (lane == 0) __hip_atomic_store(&flags[b], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -749,12 +762,12 @@ __global__ __launch_bounds__(kChainBlock) void k_tri_chain_pre(
 // computed, and the far values of level L + kEpFarAhead (whose addresses need
 // the entries loaded two levels earlier) as well.
 
-constexpr int kEpE = 8;       // entries per lane held in registers
+constexpr int kEpE = kEpEntriesPerLane;  // entries per lane held in registers
 constexpr int kEpR = 2;       // rows per lane held in registers
 constexpr int kEpMaxLevel = 1024;
 constexpr int kEpDepth = 4;     // prefetch distance in levels
 constexpr int kEpFarAhead = 2;  // far values are loaded this many levels ahead
-constexpr int kEpFence = 8;     // full fence period in levels (kFar kernels)
+constexpr int kEpFence = 32;    // full fence period in levels (kFar kernels)
 static_assert(kEpFarAhead < kEpDepth, "far values need their entries loaded first");
 static_assert(kEpFarLevels >= kEpFarAhead + kEpFence,
               "far dependencies must be fenced before their values are loaded");
@@ -769,7 +782,10 @@ struct EpPre {
   int row[kEpR];
 };
 
-// Loads whose addresses depend only on the level offsets.
+// Loads whose addresses depend only on the level offsets.  They are issued
+// unconditionally (lanes past the level's end read element 0, and their data
+// are never used): loads under divergent branches make the compiler wait for
+// every outstanding load at the join, which would serialise the pipeline.
 template <int kThreads>
 __device__ __forceinline__ void ep_load(EpPre& r, int2 lo, int2 hi,
                                         const int4* __restrict__ lent,
@@ -778,33 +794,31 @@ __device__ __forceinline__ void ep_load(EpPre& r, int2 lo, int2 hi,
 #pragma unroll
   for (int q = 0; q < kEpE; ++q) {
     const int p = lo.y + t + q * kThreads;
-    if (p < hi.y) {
-      const int4 w = lent[p];
-      r.dep[q] = w.x;
-      r.val[q] = w.y;
-      r.slot[q] = w.z;
-    }
+    const int4 w = lent[p < hi.y ? p : 0];
+    r.dep[q] = w.x;
+    r.val[q] = w.y;
+    r.slot[q] = w.z;
   }
 #pragma unroll
   for (int q = 0; q < kEpR; ++q) {
     const int idx = lo.x + t + q * kThreads;
-    if (idx < hi.x) {
-      const int4 w = rows[idx];
-      r.in[q] = w.x;
-      r.dinv[q] = w.y;
-      r.row[q] = w.z;
-    }
+    const int4 w = rows[idx < hi.x ? idx : 0];
+    r.in[q] = w.x;
+    r.dinv[q] = w.y;
+    r.row[q] = w.z;
   }
 }
 
-// Far values; their addresses come from entries loaded by ep_load.
+// Far values; their addresses come from entries loaded by ep_load (issued
+// unconditionally too: lanes without a far entry read out[0]).
 template <int kThreads>
 __device__ __forceinline__ void ep_far(EpPre& r, int2 lo, int2 hi, const int32_t* out) {
   const int t = threadIdx.x;
 #pragma unroll
   for (int q = 0; q < kEpE; ++q) {
     const int p = lo.y + t + q * kThreads;
-    if (p < hi.y && r.dep[q] < 0) r.xfar[q] = out[-r.dep[q] - 1];
+    const int d = r.dep[q];
+    r.xfar[q] = out[(p < hi.y && d < 0) ? -d - 1 : 0];
   }
 }
 
@@ -816,6 +830,82 @@ __device__ __forceinline__ void ep_sync() {
   } else {
     lds_barrier();
   }
+}
+
+// One level of the chain: `cur` holds the level's prefetched entries and
+// rows; afterwards the loads of level L + kEpDepth are issued into the same
+// registers (free by then), and, for kFar kernels, the far values of level
+// L + kEpFarAhead into `ahead`.  Level indices past the chain's end are
+// clamped to empty levels, so every load is issued unconditionally.
+template <TriKind K, int kWaves, bool kFar>
+__device__ __forceinline__ void ep_level(EpPre& cur, EpPre& ahead, int L, int nl,
+                                         const int2* lvo, const int4* __restrict__ lent,
+                                         const int4* __restrict__ rows, int32_t* out,
+                                         uint64_t* acc, int32_t* ring, int rmask, int bit,
+                                         int dbit) {
+  constexpr int kThreads = 64 * kWaves;
+  const int t = threadIdx.x;
+  const int2 lo = lvo[L], hi = lvo[L + 1];
+  const int s = lo.x, e = hi.x, e0 = lo.y, e1 = hi.y;
+  if (kFar) {
+    const int Lf = min(L + kEpFarAhead, nl);
+    ep_far<kThreads>(ahead, lvo[Lf], lvo[min(Lf + 1, nl)], out);
+  }
+  // All ring reads first (lanes past the level's end read a valid, unused
+  // slot), then the atomics: one LDS round trip for the whole level.
+  int32_t xv[kEpE];
+#pragma unroll
+  for (int q = 0; q < kEpE; ++q) xv[q] = ring[cur.dep[q] & rmask];
+  if (kFar) {
+    // Far values are rare; waiting for their loads only where a lane of
+    // the wave has one keeps the other levels off the global-load latency.
+#pragma unroll
+    for (int q = 0; q < kEpE; ++q)
+      if (__any(cur.dep[q] < 0)) xv[q] = cur.dep[q] < 0 ? cur.xfar[q] : xv[q];
+  }
+#pragma unroll
+  for (int q = 0; q < kEpE; ++q) {
+    const int p = e0 + t + q * kThreads;
+    if (p < e1)
+      atomicAdd(reinterpret_cast<unsigned long long*>(&acc[cur.slot[q]]),
+                static_cast<unsigned long long>(prod64(cur.val[q], xv[q])));
+  }
+  for (int p = e0 + t + kEpE * kThreads; p < e1; p += kThreads) {
+    const int4 w = lent[p];
+    const int32_t x = (kFar && w.x < 0) ? out[-w.x - 1] : ring[w.x & rmask];
+    atomicAdd(reinterpret_cast<unsigned long long*>(&acc[w.z]),
+              static_cast<unsigned long long>(prod64(w.y, x)));
+  }
+  ep_sync<kWaves>();
+  uint64_t av[kEpR];
+#pragma unroll
+  for (int q = 0; q < kEpR; ++q) av[q] = acc[t + q * kThreads];
+#pragma unroll
+  for (int q = 0; q < kEpR; ++q) {
+    const int k = t + q * kThreads;
+    if (s + k < e) {
+      const uint64_t tt =
+          (static_cast<uint64_t>(static_cast<int64_t>(cur.in[q])) << bit) - av[q];
+      acc[k] = 0;
+      const int32_t v = acc_final<int32_t, K>(tt, cur.dinv[q], bit, dbit);
+      ring[(s + k) & rmask] = v;
+      out[cur.row[q]] = v;
+    }
+  }
+  for (int k = t + kEpR * kThreads; s + k < e; k += kThreads) {
+    const int4 w = rows[s + k];
+    const uint64_t tt = (static_cast<uint64_t>(static_cast<int64_t>(w.x)) << bit) - acc[k];
+    acc[k] = 0;
+    const int32_t v = acc_final<int32_t, K>(tt, w.y, bit, dbit);
+    ring[(s + k) & rmask] = v;
+    out[w.z] = v;
+  }
+  if (kFar && (L % kEpFence) == kEpFence - 1)
+    __syncthreads();  // full fence: global results visible to later loads
+  else
+    ep_sync<kWaves>();
+  const int Lp = min(L + kEpDepth, nl);
+  ep_load<kThreads>(cur, lvo[Lp], lvo[min(Lp + 1, nl)], lent, rows);
 }
 
 // LDS: kEpMaxLevel uint64 accumulators, the ring, then the chain's level
@@ -838,69 +928,24 @@ __global__ __launch_bounds__(64 * kWaves) void k_tri_chain_ep(
   __syncthreads();
   EpPre buf[kEpDepth];
 #pragma unroll
-  for (int d = 0; d < kEpDepth; ++d)
-    if (d < nl) ep_load<kThreads>(buf[d], lvo[d], lvo[d + 1], lent, rows);
+  for (int d = 0; d < kEpDepth; ++d) {
+    const int Lc = min(d, nl);
+    ep_load<kThreads>(buf[d], lvo[Lc], lvo[min(Lc + 1, nl)], lent, rows);
+  }
   // Dependencies stay inside a chain, so its first kEpFarLevels levels have
   // no far entries: the first far loads are issued from the loop.
-  for (int L0 = 0; L0 < nl; L0 += kEpDepth) {
+  int L0 = 0;
+  for (; L0 + kEpDepth <= nl; L0 += kEpDepth) {
 #pragma unroll
-    for (int d = 0; d < kEpDepth; ++d) {
-      const int L = L0 + d;
-      if (L < nl) {
-        const EpPre cur = buf[d];
-        const int2 lo = lvo[L], hi = lvo[L + 1];
-        const int s = lo.x, e = hi.x, e0 = lo.y, e1 = hi.y;
-        if (kFar) {
-          const int Lf = L + kEpFarAhead;
-          if (Lf < nl)
-            ep_far<kThreads>(buf[(d + kEpFarAhead) % kEpDepth], lvo[Lf], lvo[Lf + 1], out);
-        }
-        const int Lp = L + kEpDepth;
-        if (Lp < nl) ep_load<kThreads>(buf[d], lvo[Lp], lvo[Lp + 1], lent, rows);
-#pragma unroll
-        for (int q = 0; q < kEpE; ++q) {
-          const int p = e0 + t + q * kThreads;
-          if (p < e1) {
-            const int dp = cur.dep[q];
-            const int32_t x = (kFar && dp < 0) ? cur.xfar[q] : ring[dp & rmask];
-            atomicAdd(reinterpret_cast<unsigned long long*>(&acc[cur.slot[q]]),
-                      static_cast<unsigned long long>(prod64(cur.val[q], x)));
-          }
-        }
-        for (int p = e0 + t + kEpE * kThreads; p < e1; p += kThreads) {
-          const int4 w = lent[p];
-          const int32_t x = (kFar && w.x < 0) ? out[-w.x - 1] : ring[w.x & rmask];
-          atomicAdd(reinterpret_cast<unsigned long long*>(&acc[w.z]),
-                    static_cast<unsigned long long>(prod64(w.y, x)));
-        }
-        ep_sync<kWaves>();
-#pragma unroll
-        for (int q = 0; q < kEpR; ++q) {
-          const int k = t + q * kThreads;
-          if (s + k < e) {
-            const uint64_t tt =
-                (static_cast<uint64_t>(static_cast<int64_t>(cur.in[q])) << bit) - acc[k];
-            acc[k] = 0;
-            const int32_t v = acc_final<int32_t, K>(tt, cur.dinv[q], bit, dbit);
-            ring[(s + k) & rmask] = v;
-            out[cur.row[q]] = v;
-          }
-        }
-        for (int k = t + kEpR * kThreads; s + k < e; k += kThreads) {
-          const int4 w = rows[s + k];
-          const uint64_t tt = (static_cast<uint64_t>(static_cast<int64_t>(w.x)) << bit) - acc[k];
-          acc[k] = 0;
-          const int32_t v = acc_final<int32_t, K>(tt, w.y, bit, dbit);
-          ring[(s + k) & rmask] = v;
-          out[w.z] = v;
-        }
-        if (kFar && (L % kEpFence) == kEpFence - 1)
-          __syncthreads();  // full fence: global results visible to later loads
-        else
-          ep_sync<kWaves>();
-      }
-    }
+    for (int d = 0; d < kEpDepth; ++d)
+      ep_level<K, kWaves, kFar>(buf[d], buf[(d + kEpFarAhead) % kEpDepth], L0 + d, nl, lvo,
+                                lent, rows, out, acc, ring, rmask, bit, dbit);
   }
+#pragma unroll
+  for (int d = 0; d < kEpDepth; ++d)
+    if (L0 + d < nl)
+      ep_level<K, kWaves, kFar>(buf[d], buf[(d + kEpFarAhead) % kEpDepth], L0 + d, nl, lvo,
+                                lent, rows, out, acc, ring, rmask, bit, dbit);
 }
 
 // rows[idx] = {in[lrow[idx]], ldiag[idx], lrow[idx], 0}
@@ -1034,27 +1079,20 @@ size_t tri_blocked_lds(int n, size_t elem, size_t acc) {
   return b + 8 * kTB + elem * kTB * kTB;
 }
 
-hipError_t launch_trsv_dense(TriKind kind, int n, const int32_t* slice, const long long* boff,
-                             const int32_t* dinv, int bit, int dbit, int nb, const void* in,
-                             void* out, hipStream_t s) {
+hipError_t launch_trsv_tiles(TriKind kind, int n, int nb, const int32_t* tiles,
+                             const long long* toff, const int32_t* dinv, int bit, int dbit,
+                             const void* in, void* out, int* flags, int epoch, int* err,
+                             hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  const size_t lds = 12 * static_cast<size_t>(n) + 16;
   auto go = [&](auto kern) -> hipError_t {
-    static bool attr = false;
-    if (!attr) {
-      hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                         hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         static_cast<int>(kTriBlockedMaxLds));
-      if (e != hipSuccess) return e;
-      attr = true;
-    }
-    hipLaunchKernelGGL(kern, dim3(1), dim3(kBlkThreads), lds, s, n, slice, boff, dinv, bit,
-                       dbit, nb, static_cast<const int32_t*>(in), static_cast<int32_t*>(out));
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(kTileThreads), 0, s, n, nb, tiles, toff, dinv, bit,
+                       dbit, static_cast<const int32_t*>(in), static_cast<int32_t*>(out), flags,
+                       epoch, err);
     return hipGetLastError();
   };
-  if (kind == TriKind::kL) return go(&k_tri_dense_int<TriKind::kL>);
-  if (kind == TriKind::kDU) return go(&k_tri_dense_int<TriKind::kDU>);
-  return go(&k_tri_dense_int<TriKind::kLD>);
+  if (kind == TriKind::kL) return go(&k_tri_tiles_int<TriKind::kL>);
+  if (kind == TriKind::kDU) return go(&k_tri_tiles_int<TriKind::kDU>);
+  return go(&k_tri_tiles_int<TriKind::kLD>);
 }
 
 hipError_t launch_trsv_blocked(Elem e, TriKind kind, const DCsr& a, const int* bptr,

@@ -84,6 +84,7 @@ def _load() -> C.CDLL:
         "sc23ad_dev_upload": (C.c_int, [vp, vp, C.c_size_t]),
         "sc23ad_dev_download": (C.c_int, [vp, vp, C.c_size_t]),
         "sc23ad_dev_sync": (C.c_int, []),
+        "sc23ad_dev_copy": (C.c_int, [vp, vp, C.c_size_t]),
         "sc23ad_spmv_int": (C.c_int, [C.c_int, vp, vp, vp, C.c_int, vp, vp]),
         "sc23ad_amg_create": (C.c_int, [vp, C.POINTER(vp)]),
         "sc23ad_amg_destroy": (None, [vp]),
@@ -100,6 +101,7 @@ def _load() -> C.CDLL:
         "sc23ad_solver_history": (C.c_int, [vp, vp, C.c_int, ip]),
         "sc23ad_solver_time_spmv": (C.c_int, [vp, C.c_int]),
         "sc23ad_solver_spmv_stats": (C.c_int, [vp, ip, dp]),
+        "sc23ad_solver_trsv_stats": (C.c_int, [vp, ip, dp]),
         "sc23ad_solver_relres": (C.c_int, [vp, vp, C.c_double, vp, C.c_int, dp]),
         "sc23ad_default_frac_bits": (C.c_int, [vp, vp, C.c_double, ip]),
     }
@@ -414,6 +416,13 @@ class Solver:
     def spmv_stats(self):
         n, ms = C.c_int(), C.c_double()
         check(lib.sc23ad_solver_spmv_stats(self.h, C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+    def trsv_stats(self):
+        """(launches, mean ms) of the fine-level LD solves timed since
+        time_spmv(True)."""
+        n, ms = C.c_int(), C.c_double()
+        check(lib.sc23ad_solver_trsv_stats(self.h, C.byref(n), C.byref(ms)))
         return n.value, ms.value
 
     def solve(self, b: np.ndarray, nrm_b: float, x0: np.ndarray | None = None):
