@@ -321,17 +321,21 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
   L.n = n;
   // Entry-parallel mode (integer factors): a dependency is read from the LDS
   // ring when the ring holds it, else from global memory, which needs it at
-  // least kEpFarLevels levels back.
+  // least kEpFarLevels levels back.  The ring is the smallest power of two
+  // that holds every dependency, at most 32768 values.
   size_t maxlev_rows = 0;
   for (size_t Lv = 0; Lv + 1 < lev_ptr.size(); ++Lv)
     maxlev_rows = std::max<size_t>(maxlev_rows, lev_ptr[Lv + 1] - lev_ptr[Lv]);
-  int maxlev_chain = 0;
-  for (size_t c = 0; c + 1 < all_chain_lev.size(); c += 2)
+  int maxlev_chain = 0, maxrows_chain = 0;
+  for (size_t c = 0; c + 1 < all_chain_lev.size(); c += 2) {
     maxlev_chain = std::max(maxlev_chain, all_chain_lev[c + 1] - all_chain_lev[c]);
-  const size_t ep_lds = sizeof(uint64_t) * 1024 + sizeof(int32_t) * ring +
-                        sizeof(int) * 2 * (static_cast<size_t>(maxlev_chain) + 1);
+    maxrows_chain = std::max(maxrows_chain,
+                             lev_ptr[all_chain_lev[c + 1]] - lev_ptr[all_chain_lev[c]]);
+  }
+  int ep_ring = 2;
+  while (ep_ring < 32768 && ep_ring < span) ep_ring *= 2;
   if (std::is_same_v<V, int32_t> && one_launch_chains && maxlev_rows <= 1024 &&
-      ep_lds <= dev::kTriBlockedMaxLds) {
+      dev::ep_lds_bytes(ep_ring, maxlev_chain, 0) <= dev::kTriBlockedMaxLds) {
     std::vector<int> lev_of(static_cast<size_t>(std::max(n, 1)), 0);
     std::vector<int> lev_eptr(lev_ptr.size(), 0);
     for (size_t Lv = 0; Lv + 1 < lev_ptr.size(); ++Lv) {
@@ -345,7 +349,7 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
       for (int idx = lev_ptr[Lv]; idx < lev_ptr[Lv + 1]; ++idx)
         for (int q = lbeg[idx]; q < lbeg[idx + 1]; ++q) {
           const int dpos = lpos[q];
-          const bool in_ring = static_cast<long long>(lev_ptr[Lv + 1]) - dpos <= ring;
+          const bool in_ring = static_cast<long long>(lev_ptr[Lv + 1]) - dpos <= ep_ring;
           const bool far = !in_ring && lev_of[idx] - lev_of[dpos] >= dev::kEpFarLevels;
           if (!in_ring && !far) fits = false;
           any_far = any_far || far;
@@ -360,23 +364,45 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
       plan.lev_eptr.upload(lev_eptr.data(), sizeof(int) * lev_eptr.size());
       L.lent = plan.lent.get();
       L.lev_eptr = plan.lev_eptr.as<int>();
-      L.ring = ring;
+      L.ring = ep_ring;
       // Waves: every row of a level held in registers, and at most one level
       // in 16 with more entries than the registers hold (the rest loop over
       // global memory, one round trip each).
-      int w = 1;
-      while (w < 8 && static_cast<size_t>(w) * 64 * 2 < maxlev_rows) w *= 2;
-      auto overflowing = [&](int waves) {
+      auto overflowing = [&](int waves, int per_lane) {
         size_t c = 0;
-        const size_t cap = static_cast<size_t>(waves) * 64 * dev::kEpEntriesPerLane;
+        const size_t cap = static_cast<size_t>(waves) * 64 * per_lane;
         for (size_t Lv = 0; Lv + 1 < lev_ptr.size(); ++Lv)
           if (static_cast<size_t>(lev_eptr[Lv + 1] - lev_eptr[Lv]) > cap) ++c;
         return c;
       };
-      while (w < 8 && overflowing(w) * 16 > lev_ptr.size() - 1) w *= 2;
+      const size_t nlev = lev_ptr.size() - 1;
+      int w = 1;
+      while (w < 8 && static_cast<size_t>(w) * 64 * 2 < maxlev_rows) w *= 2;
+      while (w < 8 && overflowing(w, dev::kEpEntriesPerLane) * 16 > nlev) w *= 2;
       L.ep_waves = w;
       L.ep_far = any_far ? 1 : 0;
       L.ep_maxlev = maxlev_chain;
+      L.ep_maxrows = maxrows_chain;
+      // Right-hand sides staged in LDS per chain when the chains are short
+      // (two gather rounds at most) and fit, else gathered by a separate
+      // kernel into level-ordered row data.
+      L.ep_staged = (!any_far && maxrows_chain <= 16 * 64 * w &&
+                     dev::ep_lds_bytes(ep_ring, maxlev_chain, maxrows_chain) <=
+                         dev::kTriBlockedMaxLds)
+                        ? 1 : 0;
+      // Narrow register layout (4 entries, 1 row per lane; staged kernels)
+      // when it holds every row of a level and about as many entries.
+      L.ep_narrow = (L.ep_staged && maxlev_rows <= static_cast<size_t>(w) * 64 &&
+                     overflowing(w, 4) * 16 <= nlev)
+                        ? 1 : 0;
+      std::vector<int> lrd(2 * static_cast<size_t>(std::max(n, 1)), 0);
+      for (int idx = 0; idx < n; ++idx) {
+        lrd[2 * static_cast<size_t>(idx)] = perm[idx];
+        lrd[2 * static_cast<size_t>(idx) + 1] = static_cast<int>(ldiag[idx]);
+      }
+      plan.lrd.alloc(sizeof(int) * lrd.size());
+      plan.lrd.upload(lrd.data(), sizeof(int) * lrd.size());
+      L.lrd = plan.lrd.get();
     }
   }
   int maxdeps = 0;
@@ -384,9 +410,11 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
   if (std::getenv("SC23AD_VERBOSE")) {
     std::fprintf(stderr,
                  "[sc23ad plan] kind %d n %d nnz %zu chains %d levels(max chain) %d "
-                 "max level rows %zu launches %d ring %d span %lld max deps %d ep waves %d far %d\n",
+                 "max level rows %zu launches %d ring %d span %lld max deps %d ep waves %d far %d "
+                 "narrow %d staged %d\n",
                  static_cast<int>(kind), n, cind.size(), nchain, plan.nlevels, maxlev_rows,
-                 plan.nlaunch, L.ring, span, maxdeps, L.ep_waves, L.ep_far);
+                 plan.nlaunch, L.ring, span, maxdeps, L.ep_waves, L.ep_far, L.ep_narrow,
+                 L.ep_staged);
   }
 }
 
@@ -483,7 +511,7 @@ void DevAmg::alloc_work() {
     tmp[l].alloc(es * n);
   }
   r0.alloc(es * std::max(n0(), 1));
-  perm_scratch.alloc(16 * static_cast<size_t>(std::max(n0(), 1)));
+  rows_scratch.alloc(16 * static_cast<size_t>(std::max(n0(), 1)));
   tmp[0].alloc(es * std::max(n0(), 1));
   tmpc.alloc(es * std::max(n_coarse, 1));
 }
@@ -590,7 +618,7 @@ void DevAmg::trsv(const TriPlan& plan, const DevMat& m, const void* in,
   }
   if (plan.layout.ep_waves > 0 && elem == Elem::kI32 && plan.segs.size() == 1) {
     hip_check(dev::launch_trsv_ep(plan.kind, plan.layout, plan.segs[0], m.d.bit, m.d.dbit, in,
-                                  out, perm_scratch.get(), lib_stream()),
+                                  out, rows_scratch.get(), lib_stream()),
               "trsv ep");
     return;
   }

@@ -81,7 +81,7 @@ struct TriPlan {
   DevBuf slice, boff, ddinv, flags, err;
   mutable int epoch = 0;  // value the blocks raise their flags to in the last launch
   // level-ordered layout used by the level-scheduled kernels
-  DevBuf lrow, lbeg, lcol, lval, ldiag, ecol, eval, lpos, epos, lent, lev_eptr;
+  DevBuf lrow, lbeg, lcol, lval, ldiag, ecol, eval, lpos, epos, lent, lrd, lev_eptr;
   dev::TriLayout layout;
 };
 
@@ -125,7 +125,7 @@ class DevAmg {
   int n_coarse = 0;
   // work vectors: index = level (1..depth); level 0 uses r0 / temp0
   std::vector<DevBuf> ee, xx, rr, tmp;
-  DevBuf r0, tmpc, perm_scratch;
+  DevBuf r0, tmpc, rows_scratch;  // rows_scratch: level-ordered row data (int4 per row)
   LaunchCfg cfg;
   mutable EventTimer trsv0_timer;  // level-0 LD solves (benchmark hook)
 

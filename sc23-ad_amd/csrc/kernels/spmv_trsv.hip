@@ -755,44 +755,45 @@ __global__ __launch_bounds__(kChainBlock) void k_tri_chain_pre(
 // names row -dep-1, a dependency at least kEpFarLevels levels back that the
 // ring cannot hold, read from `out` in global memory.  A full workgroup fence
 // every kEpFence levels makes such values visible before any load of them is
-// issued.  Row data arrive as int4 {input, inverse diagonal, row, 0},
-// gathered into level order before the launch.  The chain's level offsets are
-// staged in LDS, so every load of a level depends only on LDS data: the
-// entries and rows of level L + kEpDepth are loaded while level L is
-// computed, and the far values of level L + kEpFarAhead (whose addresses need
-// the entries loaded two levels earlier) as well.
+// issued.  Right-hand sides reach the kernel in one of two ways:
+//   staged (kStaged): row data are int2 {row, inverse diagonal} per position
+//     and the chain's right-hand side values are gathered into LDS when the
+//     kernel starts;
+//   gathered: row data are int4 {rhs, inverse diagonal, row, 0} per
+//     position, gathered into level order by k_gather_rows before the launch.
+// The chain's level offsets are staged in LDS too, so the loads of a level
+// depend only on LDS data: the entries and row data of level L + kEpDepth are
+// loaded once level L is done (into the registers it has just freed), and
+// far values of level L + kEpFarAhead (whose addresses need the entries) at
+// the start of level L.  Every load is issued unconditionally (lanes past a
+// level's end read element 0 and ignore it): loads under divergent branches
+// make the compiler wait for all outstanding loads at the join.
 
-constexpr int kEpE = kEpEntriesPerLane;  // entries per lane held in registers
-constexpr int kEpR = 2;       // rows per lane held in registers
 constexpr int kEpMaxLevel = 1024;
 constexpr int kEpDepth = 4;     // prefetch distance in levels
-constexpr int kEpFarAhead = 2;  // far values are loaded this many levels ahead
+constexpr int kEpFarAhead = 2;  // far-value distance in levels
 constexpr int kEpFence = 32;    // full fence period in levels (kFar kernels)
 static_assert(kEpFarAhead < kEpDepth, "far values need their entries loaded first");
 static_assert(kEpFarLevels >= kEpFarAhead + kEpFence,
               "far dependencies must be fenced before their values are loaded");
 
+template <int kE, int kR>
 struct EpPre {
-  int dep[kEpE];
-  int32_t val[kEpE];
-  int slot[kEpE];
-  int32_t xfar[kEpE];
-  int32_t in[kEpR];
-  int32_t dinv[kEpR];
-  int row[kEpR];
+  int dep[kE];
+  int32_t val[kE];
+  int slot[kE];
+  int32_t xfar[kE];
+  int row[kR];
+  int32_t dinv[kR];
+  int32_t in[kR];  // gathered mode
 };
 
-// Loads whose addresses depend only on the level offsets.  They are issued
-// unconditionally (lanes past the level's end read element 0, and their data
-// are never used): loads under divergent branches make the compiler wait for
-// every outstanding load at the join, which would serialise the pipeline.
-template <int kThreads>
-__device__ __forceinline__ void ep_load(EpPre& r, int2 lo, int2 hi,
-                                        const int4* __restrict__ lent,
-                                        const int4* __restrict__ rows) {
+template <int kThreads, bool kStaged, int kE, int kR>
+__device__ __forceinline__ void ep_load(EpPre<kE, kR>& r, int2 lo, int2 hi,
+                                        const int4* __restrict__ lent, const void* rowdata) {
   const int t = threadIdx.x;
 #pragma unroll
-  for (int q = 0; q < kEpE; ++q) {
+  for (int q = 0; q < kE; ++q) {
     const int p = lo.y + t + q * kThreads;
     const int4 w = lent[p < hi.y ? p : 0];
     r.dep[q] = w.x;
@@ -800,22 +801,27 @@ __device__ __forceinline__ void ep_load(EpPre& r, int2 lo, int2 hi,
     r.slot[q] = w.z;
   }
 #pragma unroll
-  for (int q = 0; q < kEpR; ++q) {
+  for (int q = 0; q < kR; ++q) {
     const int idx = lo.x + t + q * kThreads;
-    const int4 w = rows[idx < hi.x ? idx : 0];
-    r.in[q] = w.x;
-    r.dinv[q] = w.y;
-    r.row[q] = w.z;
+    const int id = idx < hi.x ? idx : 0;
+    if (kStaged) {
+      const int2 w = static_cast<const int2*>(rowdata)[id];
+      r.row[q] = w.x;
+      r.dinv[q] = w.y;
+    } else {
+      const int4 w = static_cast<const int4*>(rowdata)[id];
+      r.in[q] = w.x;
+      r.dinv[q] = w.y;
+      r.row[q] = w.z;
+    }
   }
 }
 
-// Far values; their addresses come from entries loaded by ep_load (issued
-// unconditionally too: lanes without a far entry read out[0]).
-template <int kThreads>
-__device__ __forceinline__ void ep_far(EpPre& r, int2 lo, int2 hi, const int32_t* out) {
+template <int kThreads, int kE, int kR>
+__device__ __forceinline__ void ep_far(EpPre<kE, kR>& r, int2 lo, int2 hi, const int32_t* out) {
   const int t = threadIdx.x;
 #pragma unroll
-  for (int q = 0; q < kEpE; ++q) {
+  for (int q = 0; q < kE; ++q) {
     const int p = lo.y + t + q * kThreads;
     const int d = r.dep[q];
     r.xfar[q] = out[(p < hi.y && d < 0) ? -d - 1 : 0];
@@ -832,105 +838,162 @@ __device__ __forceinline__ void ep_sync() {
   }
 }
 
-// One level of the chain: `cur` holds the level's prefetched entries and
-// rows; afterwards the loads of level L + kEpDepth are issued into the same
-// registers (free by then), and, for kFar kernels, the far values of level
-// L + kEpFarAhead into `ahead`.  Level indices past the chain's end are
-// clamped to empty levels, so every load is issued unconditionally.
-template <TriKind K, int kWaves, bool kFar>
-__device__ __forceinline__ void ep_level(EpPre& cur, EpPre& ahead, int L, int nl,
-                                         const int2* lvo, const int4* __restrict__ lent,
-                                         const int4* __restrict__ rows, int32_t* out,
-                                         uint64_t* acc, int32_t* ring, int rmask, int bit,
+struct EpShared {
+  const int2* lvo;     // level offsets {row position, entry position}, LDS
+  const int32_t* inl;  // staged right-hand sides by position - chain start, LDS
+  int pos0;            // first position of the chain
+  uint64_t* acc;
+  int32_t* ring;
+  int rmask;
+};
+
+// One level of the chain; `cur` holds the level's prefetched data, `ahead`
+// the buffer of level L + kEpFarAhead.  Level indices past the chain's end
+// are clamped to an empty level.
+template <TriKind K, int kWaves, bool kFar, bool kStaged, int kE, int kR>
+__device__ __forceinline__ void ep_level(EpPre<kE, kR>& cur, EpPre<kE, kR>& ahead, int L,
+                                         int nl, const EpShared& sh,
+                                         const int4* __restrict__ lent, const void* rowdata,
+                                         const int32_t* __restrict__ in, int32_t* out, int bit,
                                          int dbit) {
   constexpr int kThreads = 64 * kWaves;
   const int t = threadIdx.x;
-  const int2 lo = lvo[L], hi = lvo[L + 1];
+  const int2 lo = sh.lvo[L], hi = sh.lvo[L + 1];
   const int s = lo.x, e = hi.x, e0 = lo.y, e1 = hi.y;
+  uint64_t* acc = sh.acc;
+  int32_t* ring = sh.ring;
+  const int rmask = sh.rmask;
   if (kFar) {
     const int Lf = min(L + kEpFarAhead, nl);
-    ep_far<kThreads>(ahead, lvo[Lf], lvo[min(Lf + 1, nl)], out);
+    ep_far<kThreads>(ahead, sh.lvo[Lf], sh.lvo[min(Lf + 1, nl)], out);
   }
   // All ring reads first (lanes past the level's end read a valid, unused
   // slot), then the atomics: one LDS round trip for the whole level.
-  int32_t xv[kEpE];
+  int32_t xv[kE];
 #pragma unroll
-  for (int q = 0; q < kEpE; ++q) xv[q] = ring[cur.dep[q] & rmask];
+  for (int q = 0; q < kE; ++q) xv[q] = ring[cur.dep[q] & rmask];
   if (kFar) {
     // Far values are rare; waiting for their loads only where a lane of
     // the wave has one keeps the other levels off the global-load latency.
 #pragma unroll
-    for (int q = 0; q < kEpE; ++q)
+    for (int q = 0; q < kE; ++q)
       if (__any(cur.dep[q] < 0)) xv[q] = cur.dep[q] < 0 ? cur.xfar[q] : xv[q];
   }
 #pragma unroll
-  for (int q = 0; q < kEpE; ++q) {
+  for (int q = 0; q < kE; ++q) {
     const int p = e0 + t + q * kThreads;
     if (p < e1)
       atomicAdd(reinterpret_cast<unsigned long long*>(&acc[cur.slot[q]]),
                 static_cast<unsigned long long>(prod64(cur.val[q], xv[q])));
   }
-  for (int p = e0 + t + kEpE * kThreads; p < e1; p += kThreads) {
+  for (int p = e0 + t + kE * kThreads; p < e1; p += kThreads) {
     const int4 w = lent[p];
     const int32_t x = (kFar && w.x < 0) ? out[-w.x - 1] : ring[w.x & rmask];
     atomicAdd(reinterpret_cast<unsigned long long*>(&acc[w.z]),
               static_cast<unsigned long long>(prod64(w.y, x)));
   }
   ep_sync<kWaves>();
-  uint64_t av[kEpR];
+  uint64_t av[kR];
+  int32_t rin[kR];
 #pragma unroll
-  for (int q = 0; q < kEpR; ++q) av[q] = acc[t + q * kThreads];
+  for (int q = 0; q < kR; ++q) {
+    av[q] = acc[t + q * kThreads];
+    if (kStaged) {
+      const int k = s + t + q * kThreads - sh.pos0;
+      rin[q] = sh.inl[k < e - sh.pos0 ? k : 0];
+    } else {
+      rin[q] = cur.in[q];
+    }
+  }
 #pragma unroll
-  for (int q = 0; q < kEpR; ++q) {
+  for (int q = 0; q < kR; ++q) {
     const int k = t + q * kThreads;
     if (s + k < e) {
-      const uint64_t tt =
-          (static_cast<uint64_t>(static_cast<int64_t>(cur.in[q])) << bit) - av[q];
+      const uint64_t tt = (static_cast<uint64_t>(static_cast<int64_t>(rin[q])) << bit) - av[q];
       acc[k] = 0;
       const int32_t v = acc_final<int32_t, K>(tt, cur.dinv[q], bit, dbit);
       ring[(s + k) & rmask] = v;
       out[cur.row[q]] = v;
     }
   }
-  for (int k = t + kEpR * kThreads; s + k < e; k += kThreads) {
-    const int4 w = rows[s + k];
-    const uint64_t tt = (static_cast<uint64_t>(static_cast<int64_t>(w.x)) << bit) - acc[k];
+  for (int k = t + kR * kThreads; s + k < e; k += kThreads) {
+    int row, dv;
+    int32_t rv;
+    if (kStaged) {
+      const int2 w = static_cast<const int2*>(rowdata)[s + k];
+      row = w.x;
+      dv = w.y;
+      rv = sh.inl[s + k - sh.pos0];
+    } else {
+      const int4 w = static_cast<const int4*>(rowdata)[s + k];
+      rv = w.x;
+      dv = w.y;
+      row = w.z;
+    }
+    const uint64_t tt = (static_cast<uint64_t>(static_cast<int64_t>(rv)) << bit) - acc[k];
     acc[k] = 0;
-    const int32_t v = acc_final<int32_t, K>(tt, w.y, bit, dbit);
+    const int32_t v = acc_final<int32_t, K>(tt, dv, bit, dbit);
     ring[(s + k) & rmask] = v;
-    out[w.z] = v;
+    out[row] = v;
   }
   if (kFar && (L % kEpFence) == kEpFence - 1)
     __syncthreads();  // full fence: global results visible to later loads
   else
     ep_sync<kWaves>();
   const int Lp = min(L + kEpDepth, nl);
-  ep_load<kThreads>(cur, lvo[Lp], lvo[min(Lp + 1, nl)], lent, rows);
+  ep_load<kThreads, kStaged>(cur, sh.lvo[Lp], sh.lvo[min(Lp + 1, nl)], lent, rowdata);
 }
 
-// LDS: kEpMaxLevel uint64 accumulators, the ring, then the chain's level
-// offsets {lev_ptr, lev_eptr} (max_levels + 1 int2).
-template <TriKind K, int kWaves, bool kFar>
+// LDS: kEpMaxLevel uint64 accumulators, the ring, the chain's level offsets
+// (max_levels + 1 int2), then (staged) the chain's right-hand sides.
+template <TriKind K, int kWaves, bool kFar, bool kStaged, int kE, int kR>
 __global__ __launch_bounds__(64 * kWaves) void k_tri_chain_ep(
-    const int4* __restrict__ lent, const int* __restrict__ lev_ptr,
-    const int* __restrict__ lev_eptr, const int* __restrict__ chain_lev, int rmask, int bit,
-    int dbit, const int4* __restrict__ rows, int32_t* out) {
+    const int4* __restrict__ lent, const void* __restrict__ rowdata,
+    const int* __restrict__ lev_ptr, const int* __restrict__ lev_eptr,
+    const int* __restrict__ chain_lev, int rmask, int maxlev, int bit, int dbit,
+    const int32_t* __restrict__ in, int32_t* out) {
   constexpr int kThreads = 64 * kWaves;
   extern __shared__ __align__(16) unsigned char smem_e[];
-  uint64_t* acc = reinterpret_cast<uint64_t*>(smem_e);  // kEpMaxLevel
-  int32_t* ring = reinterpret_cast<int32_t*>(acc + kEpMaxLevel);
-  int2* lvo = reinterpret_cast<int2*>(ring + rmask + 1);
+  EpShared sh;
+  sh.acc = reinterpret_cast<uint64_t*>(smem_e);  // kEpMaxLevel
+  sh.ring = reinterpret_cast<int32_t*>(sh.acc + kEpMaxLevel);
+  sh.rmask = rmask;
+  int2* lvo = reinterpret_cast<int2*>(sh.ring + rmask + 1);
+  int32_t* inl = reinterpret_cast<int32_t*>(lvo + maxlev + 1);
+  sh.lvo = lvo;
+  sh.inl = inl;
   const int t = threadIdx.x;
   const int lb = chain_lev[2 * blockIdx.x], le = chain_lev[2 * blockIdx.x + 1];
   const int nl = le - lb;
-  for (int k = t; k < kEpMaxLevel; k += kThreads) acc[k] = 0;
+  for (int k = t; k < kEpMaxLevel; k += kThreads) sh.acc[k] = 0;
   for (int k = t; k <= nl; k += kThreads) lvo[k] = make_int2(lev_ptr[lb + k], lev_eptr[lb + k]);
+  sh.pos0 = lev_ptr[lb];
+  if (kStaged) {
+    // Gather in batches of 8 per thread, so that the latency of the two
+    // dependent loads is paid once per batch.
+    const int n_c = lev_ptr[le] - sh.pos0;
+    const int2* rd = static_cast<const int2*>(rowdata) + sh.pos0;
+    for (int k0 = t; k0 < n_c; k0 += 8 * kThreads) {
+      int rw[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int k = k0 + j * kThreads;
+        rw[j] = rd[k < n_c ? k : 0].x;
+      }
+      int32_t v[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) v[j] = in[rw[j]];
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (k0 + j * kThreads < n_c) inl[k0 + j * kThreads] = v[j];
+    }
+  }
   __syncthreads();
-  EpPre buf[kEpDepth];
+  EpPre<kE, kR> buf[kEpDepth];
 #pragma unroll
   for (int d = 0; d < kEpDepth; ++d) {
     const int Lc = min(d, nl);
-    ep_load<kThreads>(buf[d], lvo[Lc], lvo[min(Lc + 1, nl)], lent, rows);
+    ep_load<kThreads, kStaged>(buf[d], lvo[Lc], lvo[min(Lc + 1, nl)], lent, rowdata);
   }
   // Dependencies stay inside a chain, so its first kEpFarLevels levels have
   // no far entries: the first far loads are issued from the loop.
@@ -938,14 +1001,14 @@ __global__ __launch_bounds__(64 * kWaves) void k_tri_chain_ep(
   for (; L0 + kEpDepth <= nl; L0 += kEpDepth) {
 #pragma unroll
     for (int d = 0; d < kEpDepth; ++d)
-      ep_level<K, kWaves, kFar>(buf[d], buf[(d + kEpFarAhead) % kEpDepth], L0 + d, nl, lvo,
-                                lent, rows, out, acc, ring, rmask, bit, dbit);
+      ep_level<K, kWaves, kFar, kStaged>(buf[d], buf[(d + kEpFarAhead) % kEpDepth], L0 + d, nl,
+                                         sh, lent, rowdata, in, out, bit, dbit);
   }
 #pragma unroll
   for (int d = 0; d < kEpDepth; ++d)
     if (L0 + d < nl)
-      ep_level<K, kWaves, kFar>(buf[d], buf[(d + kEpFarAhead) % kEpDepth], L0 + d, nl, lvo,
-                                lent, rows, out, acc, ring, rmask, bit, dbit);
+      ep_level<K, kWaves, kFar, kStaged>(buf[d], buf[(d + kEpFarAhead) % kEpDepth], L0 + d, nl,
+                                         sh, lent, rowdata, in, out, bit, dbit);
 }
 
 // rows[idx] = {in[lrow[idx]], ldiag[idx], lrow[idx], 0}
@@ -960,32 +1023,41 @@ __global__ __launch_bounds__(256) void k_gather_rows(int n, const int* __restric
   }
 }
 
-template <TriKind K, int kWaves, bool kFar>
-hipError_t tri_ep_launch2(const TriLayout& L, const TriSegment& seg, int bit, int dbit,
-                          const int4* rows, int32_t* out, hipStream_t s) {
-  const size_t lds = sizeof(uint64_t) * kEpMaxLevel + sizeof(int32_t) * L.ring +
-                     sizeof(int2) * (static_cast<size_t>(L.ep_maxlev) + 1);
+template <TriKind K, int kWaves, bool kFar, bool kStaged, int kE, int kR>
+hipError_t tri_ep_go(const TriLayout& L, const TriSegment& seg, int bit, int dbit,
+                     const void* rowdata, const int32_t* in, int32_t* out, hipStream_t s) {
+  const size_t lds = ep_lds_bytes(L.ring, L.ep_maxlev, kStaged ? L.ep_maxrows : 0);
   if (lds > static_cast<size_t>(kTriBlockedMaxLds)) return hipErrorInvalidValue;
   static bool attr = false;
   if (!attr) {
-    hipError_t e =
-        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tri_chain_ep<K, kWaves, kFar>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize,
-                            static_cast<int>(kTriBlockedMaxLds));
+    hipError_t e = hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&k_tri_chain_ep<K, kWaves, kFar, kStaged, kE, kR>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kTriBlockedMaxLds));
     if (e != hipSuccess) return e;
     attr = true;
   }
-  hipLaunchKernelGGL((k_tri_chain_ep<K, kWaves, kFar>), dim3(seg.nchain), dim3(64 * kWaves), lds,
-                     s, reinterpret_cast<const int4*>(L.lent), L.lev_ptr, L.lev_eptr,
-                     seg.chain_lev, L.ring - 1, bit, dbit, rows, out);
+  hipLaunchKernelGGL((k_tri_chain_ep<K, kWaves, kFar, kStaged, kE, kR>), dim3(seg.nchain),
+                     dim3(64 * kWaves), lds, s, reinterpret_cast<const int4*>(L.lent), rowdata,
+                     L.lev_ptr, L.lev_eptr, seg.chain_lev, L.ring - 1, L.ep_maxlev, bit, dbit,
+                     in, out);
   return hipGetLastError();
 }
 
 template <TriKind K, int kWaves>
 hipError_t tri_ep_launch(const TriLayout& L, const TriSegment& seg, int bit, int dbit,
-                         const int4* rows, int32_t* out, hipStream_t s) {
-  return L.ep_far ? tri_ep_launch2<K, kWaves, true>(L, seg, bit, dbit, rows, out, s)
-                  : tri_ep_launch2<K, kWaves, false>(L, seg, bit, dbit, rows, out, s);
+                         const int32_t* in, int32_t* out, void* rows_scratch, hipStream_t s) {
+  if (L.ep_staged) {
+    if (L.ep_narrow) return tri_ep_go<K, kWaves, false, true, 4, 1>(L, seg, bit, dbit, L.lrd, in, out, s);
+    return tri_ep_go<K, kWaves, false, true, 8, 2>(L, seg, bit, dbit, L.lrd, in, out, s);
+  }
+  int4* rows = static_cast<int4*>(rows_scratch);
+  if (L.n > 0)
+    hipLaunchKernelGGL(k_gather_rows, dim3((L.n + 255) / 256), dim3(256), 0, s, L.n, L.lrow,
+                       static_cast<const int32_t*>(L.ldiag), in, rows);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (L.ep_far) return tri_ep_go<K, kWaves, true, false, 8, 2>(L, seg, bit, dbit, rows, in, out, s);
+  return tri_ep_go<K, kWaves, false, false, 8, 2>(L, seg, bit, dbit, rows, in, out, s);
 }
 
 // One level, many workgroups, level-ordered layout.
@@ -1131,33 +1203,28 @@ hipError_t launch_trsv_layout(Elem e, TriKind kind, const TriLayout& L,
 }
 
 hipError_t launch_trsv_ep(TriKind kind, const TriLayout& L, const TriSegment& seg, int bit,
-                          int dbit, const void* in, void* out, void* rows_scratch,
+                          int dbit, const void* in_v, void* out_v, void* rows_scratch,
                           hipStream_t s) {
-  int4* rows = static_cast<int4*>(rows_scratch);
-  if (L.n > 0)
-    hipLaunchKernelGGL(k_gather_rows, dim3((L.n + 255) / 256), dim3(256), 0, s, L.n, L.lrow,
-                       static_cast<const int32_t*>(L.ldiag), static_cast<const int32_t*>(in),
-                       rows);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  int32_t* o = static_cast<int32_t*>(out);
+  const int32_t* in = static_cast<const int32_t*>(in_v);
+  int32_t* o = static_cast<int32_t*>(out_v);
+  void* rs = rows_scratch;
   const int w = L.ep_waves;
   if (kind == TriKind::kLD) {
-    if (w <= 1) return tri_ep_launch<TriKind::kLD, 1>(L, seg, bit, dbit, rows, o, s);
-    if (w <= 2) return tri_ep_launch<TriKind::kLD, 2>(L, seg, bit, dbit, rows, o, s);
-    if (w <= 4) return tri_ep_launch<TriKind::kLD, 4>(L, seg, bit, dbit, rows, o, s);
-    return tri_ep_launch<TriKind::kLD, 8>(L, seg, bit, dbit, rows, o, s);
+    if (w <= 1) return tri_ep_launch<TriKind::kLD, 1>(L, seg, bit, dbit, in, o, rs, s);
+    if (w <= 2) return tri_ep_launch<TriKind::kLD, 2>(L, seg, bit, dbit, in, o, rs, s);
+    if (w <= 4) return tri_ep_launch<TriKind::kLD, 4>(L, seg, bit, dbit, in, o, rs, s);
+    return tri_ep_launch<TriKind::kLD, 8>(L, seg, bit, dbit, in, o, rs, s);
   }
   if (kind == TriKind::kL) {
-    if (w <= 1) return tri_ep_launch<TriKind::kL, 1>(L, seg, bit, dbit, rows, o, s);
-    if (w <= 2) return tri_ep_launch<TriKind::kL, 2>(L, seg, bit, dbit, rows, o, s);
-    if (w <= 4) return tri_ep_launch<TriKind::kL, 4>(L, seg, bit, dbit, rows, o, s);
-    return tri_ep_launch<TriKind::kL, 8>(L, seg, bit, dbit, rows, o, s);
+    if (w <= 1) return tri_ep_launch<TriKind::kL, 1>(L, seg, bit, dbit, in, o, rs, s);
+    if (w <= 2) return tri_ep_launch<TriKind::kL, 2>(L, seg, bit, dbit, in, o, rs, s);
+    if (w <= 4) return tri_ep_launch<TriKind::kL, 4>(L, seg, bit, dbit, in, o, rs, s);
+    return tri_ep_launch<TriKind::kL, 8>(L, seg, bit, dbit, in, o, rs, s);
   }
-  if (w <= 1) return tri_ep_launch<TriKind::kDU, 1>(L, seg, bit, dbit, rows, o, s);
-  if (w <= 2) return tri_ep_launch<TriKind::kDU, 2>(L, seg, bit, dbit, rows, o, s);
-  if (w <= 4) return tri_ep_launch<TriKind::kDU, 4>(L, seg, bit, dbit, rows, o, s);
-  return tri_ep_launch<TriKind::kDU, 8>(L, seg, bit, dbit, rows, o, s);
+  if (w <= 1) return tri_ep_launch<TriKind::kDU, 1>(L, seg, bit, dbit, in, o, rs, s);
+  if (w <= 2) return tri_ep_launch<TriKind::kDU, 2>(L, seg, bit, dbit, in, o, rs, s);
+  if (w <= 4) return tri_ep_launch<TriKind::kDU, 4>(L, seg, bit, dbit, in, o, rs, s);
+  return tri_ep_launch<TriKind::kDU, 8>(L, seg, bit, dbit, in, o, rs, s);
 }
 
 hipError_t launch_trsv_segment(Elem e, TriKind kind, const DCsr& a,
