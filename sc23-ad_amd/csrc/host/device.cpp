@@ -95,6 +95,38 @@ void upload_mat(DevMat& dm, int n, int m, int bit, int dbit,
   dm.d.cind = dm.cind.as<int>();
   dm.d.val = dm.val.get();
   dm.d.group = pick_group(n, dm.d.nnz, group);
+  // SELL-64 copy when the rows of each 64-row slice have near-equal lengths
+  // (padding at most 1/8 of the entries): one thread per row, coalesced.
+  // Whether it is used is decided by tune_spmv.
+  if (n >= 4096) {
+    const int ns = (n + 63) / 64;
+    std::vector<long long> soff(static_cast<size_t>(ns) + 1, 0);
+    for (int sl = 0; sl < ns; ++sl) {
+      int w = 0;
+      for (int r = sl * 64; r < std::min(n, sl * 64 + 64); ++r) w = std::max(w, rptr[r + 1] - rptr[r]);
+      soff[sl + 1] = soff[sl] + 64LL * w;
+    }
+    if (soff[ns] * 8 <= static_cast<long long>(cind.size()) * 9) {
+      std::vector<int> scol(static_cast<size_t>(std::max<long long>(soff[ns], 1)), -1);
+      std::vector<V> sval(scol.size(), V(0));
+      for (int i = 0; i < n; ++i) {
+        const long long b = soff[i / 64] + (i % 64);
+        for (int p = rptr[i]; p < rptr[i + 1]; ++p) {
+          scol[static_cast<size_t>(b + 64LL * (p - rptr[i]))] = cind[p];
+          sval[static_cast<size_t>(b + 64LL * (p - rptr[i]))] = val[p];
+        }
+      }
+      dm.soff.alloc(sizeof(long long) * soff.size());
+      dm.soff.upload(soff.data(), sizeof(long long) * soff.size());
+      dm.scol.alloc(sizeof(int) * scol.size());
+      dm.scol.upload(scol.data(), sizeof(int) * scol.size());
+      dm.sval.alloc(sizeof(V) * sval.size());
+      dm.sval.upload(sval.data(), sizeof(V) * sval.size());
+      dm.d.soff = dm.soff.as<long long>();
+      dm.d.scol = dm.scol.as<int>();
+      dm.d.sval = dm.sval.get();
+    }
+  }
 }
 
 // Level sets of a triangular factor.  Dependencies of row i: every stored
@@ -495,7 +527,72 @@ void build_tiles(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rpt
   plan.dense = true;
 }
 
+// SpMV launch-shape autotuner: times every candidate kernel for this
+// matrix (integer CSR with 1..64 lanes per row, SELL-64 when built; FP: CSR
+// thread per row, SELL-64) on the library stream and keeps the fastest.
+// Every candidate computes the same result (integer sums are order-free; the
+// FP kernels add a row's products in stored order), so the choice is free.
+// SC23AD_SPMV_TUNE=0 keeps the defaults (lanes from nnz/row, SELL if built).
+void tune_spmv(DevMat& m, Elem e) {
+  const char* env = std::getenv("SC23AD_SPMV_TUNE");
+  if ((env && env[0] == '0') || m.d.n <= 0) return;
+  const size_t es = e == Elem::kF64 ? 8 : 4;
+  DevBuf x(es * std::max(m.d.m, 1)), y(es * std::max(m.d.n, 1));
+  hip_check(hipMemsetAsync(x.get(), 0, es * std::max(m.d.m, 1), lib_stream()), "memset");
+  std::vector<dev::DCsr> cands;
+  dev::DCsr csr = m.d;
+  csr.soff = nullptr;
+  if (e == Elem::kI32) {
+    const double avg = static_cast<double>(m.d.nnz) / m.d.n;
+    for (int g = 1; g <= 64; g *= 2) {
+      if (g > 2 * avg + 2) break;
+      csr.group = g;
+      cands.push_back(csr);
+    }
+  } else {
+    cands.push_back(csr);
+  }
+  if (m.d.soff) cands.push_back(m.d);
+  hipEvent_t e0, e1;
+  hip_check(hipEventCreate(&e0), "event");
+  hip_check(hipEventCreate(&e1), "event");
+  float best = 1e30f;
+  size_t best_i = 0;
+  for (size_t i = 0; i < cands.size(); ++i) {
+    for (int r = 0; r < 2; ++r)
+      hip_check(dev::launch_spmv(e, cands[i], SpmvMode::kPlain, x.get(), nullptr, y.get(),
+                                 lib_stream()),
+                "tune spmv");
+    hip_check(hipEventRecord(e0, lib_stream()), "record");
+    for (int r = 0; r < 5; ++r)
+      hip_check(dev::launch_spmv(e, cands[i], SpmvMode::kPlain, x.get(), nullptr, y.get(),
+                                 lib_stream()),
+                "tune spmv");
+    hip_check(hipEventRecord(e1, lib_stream()), "record");
+    hip_check(hipEventSynchronize(e1), "sync");
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+    if (ms < best) {
+      best = ms;
+      best_i = i;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  m.d = cands[best_i];
+  if (!m.d.soff && m.soff.get()) {
+    m.soff = DevBuf();
+    m.scol = DevBuf();
+    m.sval = DevBuf();
+  }
+}
+
 }  // namespace
+
+void DevAmg::tune() {
+  for (DevLevel& L : lv)
+    for (DevMat* m : {&L.a, &L.u, &L.r, &L.p}) tune_spmv(*m, elem);
+}
 
 void DevAmg::alloc_work() {
   const size_t es = esize();
@@ -551,6 +648,7 @@ std::unique_ptr<DevAmg> DevAmg::upload(const IntHierarchy& h, const LaunchCfg& c
     build_tiles(a->uc_plan, TriKind::kDU, h.uc.n, h.uc.rptr, h.uc.cind, h.uc.val);
   }
   a->alloc_work();
+  a->tune();
   return a;
 }
 
@@ -586,6 +684,7 @@ std::unique_ptr<DevAmg> DevAmg::upload_fp(const FpHierarchy<T>& h, const LaunchC
   build_blocked(a->lc_plan, TriKind::kL, h.lc.n, h.lc.rptr, h.lc.cind, sizeof(T), sizeof(T));
   build_blocked(a->uc_plan, TriKind::kDU, h.uc.n, h.uc.rptr, h.uc.cind, sizeof(T), sizeof(T));
   a->alloc_work();
+  a->tune();
   return a;
 }
 

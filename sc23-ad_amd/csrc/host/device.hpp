@@ -62,6 +62,7 @@ class DevBuf {
 struct DevMat {
   dev::DCsr d;
   DevBuf rptr, cind, val;
+  DevBuf soff, scol, sval;  // SELL-64 copy (optional)
   size_t algo_bytes_spmv(size_t elem) const;
 };
 
@@ -150,6 +151,7 @@ class DevAmg {
 
  private:
   void alloc_work();
+  void tune();  // SpMV launch shapes per matrix
 };
 
 struct SolverOpts {

@@ -24,6 +24,11 @@ struct DCsr {
   const int* cind = nullptr;
   const void* val = nullptr;
   int group = 8;  // lanes per row for the integer SpMV (power of two <= 64)
+  // SELL-64 copy (rows with near-uniform lengths): slices of 64 rows, entry k
+  // of row r of slice s at soff[s] + 64 k + r, padded with column -1 / value 0.
+  const long long* soff = nullptr;
+  const int* scol = nullptr;
+  const void* sval = nullptr;
 };
 
 enum class Elem : int { kF64 = 0, kF32 = 1, kI32 = 2 };

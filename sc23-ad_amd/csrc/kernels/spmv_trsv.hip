@@ -39,6 +39,74 @@ __global__ __launch_bounds__(kBlock) void k_spmv_int(
   }
 }
 
+// SELL-64 SpMV, one thread per row.  Entries are read eight at a time with
+// clamped indices (every load unconditional), padding contributes nothing.
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_spmv_sell_int(
+    int n, const long long* __restrict__ soff, const int* __restrict__ scol,
+    const int32_t* __restrict__ sval, int bit, const int32_t* __restrict__ x,
+    const int32_t* z, int32_t* y) {
+  const int row = blockIdx.x * kBlock + threadIdx.x;
+  if (row >= n) return;
+  const long long b = soff[row >> 6];
+  const int w = static_cast<int>((soff[(row >> 6) + 1] - b) >> 6);
+  const int* c = scol + b + (row & 63);
+  const int32_t* v = sval + b + (row & 63);
+  uint64_t acc = 0;
+  for (int k0 = 0; k0 < w; k0 += 8) {
+    int cc[8];
+    int32_t vv[8], xx[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int k = k0 + j < w ? k0 + j : k0;
+      cc[j] = c[64 * k];
+      vv[j] = v[64 * k];
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) xx[j] = x[cc[j] < 0 ? 0 : cc[j]];
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (k0 + j < w) acc += prod64(vv[j], xx[j]);
+  }
+  const int32_t s = low32(as_i64(acc) >> bit);
+  if (MODE == 0) y[row] = s;
+  else if (MODE == 1) y[row] = sub32(z[row], s);
+  else y[row] = add32(z[row], s);
+}
+
+// FP SELL-64 SpMV: the row's products are added in stored (column) order,
+// as the CSR kernel adds them; padding is skipped, not added.
+template <typename T, int MODE>
+__global__ __launch_bounds__(kBlock) void k_spmv_sell_fp(
+    int n, const long long* __restrict__ soff, const int* __restrict__ scol,
+    const T* __restrict__ sval, const T* __restrict__ x, const T* z, T* y) {
+  const int row = blockIdx.x * kBlock + threadIdx.x;
+  if (row >= n) return;
+  const long long b = soff[row >> 6];
+  const int w = static_cast<int>((soff[(row >> 6) + 1] - b) >> 6);
+  const int* c = scol + b + (row & 63);
+  const T* v = sval + b + (row & 63);
+  T t = 0;
+  for (int k0 = 0; k0 < w; k0 += 8) {
+    int cc[8];
+    T vv[8], xx[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int k = k0 + j < w ? k0 + j : k0;
+      cc[j] = c[64 * k];
+      vv[j] = v[64 * k];
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) xx[j] = x[cc[j] < 0 ? 0 : cc[j]];
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (k0 + j < w && cc[j] >= 0) t += vv[j] * xx[j];
+  }
+  if (MODE == 0) y[row] = t;
+  else if (MODE == 1) y[row] = z[row] - t;
+  else y[row] = z[row] + t;
+}
+
 template <typename T, int MODE>
 __global__ __launch_bounds__(kBlock) void k_spmv_fp(
     int n, const int* __restrict__ rptr, const int* __restrict__ cind,
@@ -46,8 +114,24 @@ __global__ __launch_bounds__(kBlock) void k_spmv_fp(
   const int row = blockIdx.x * kBlock + threadIdx.x;
   if (row >= n) return;
   T t = 0;
-  const int re = rptr[row + 1];
-  for (int p = rptr[row]; p < re; ++p) t += val[p] * x[cind[p]];
+  const int rb = rptr[row], re = rptr[row + 1];
+  // Eight entries at a time, every load unconditional (clamped), so that a
+  // chunk costs two memory latencies; the sum keeps the stored order.
+  for (int p0 = rb; p0 < re; p0 += 8) {
+    int cc[8];
+    T vv[8], xx[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int p = p0 + j < re ? p0 + j : p0;
+      cc[j] = cind[p];
+      vv[j] = val[p];
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) xx[j] = x[cc[j]];
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (p0 + j < re) t += vv[j] * xx[j];
+  }
   if (MODE == 0) y[row] = t;
   else if (MODE == 1) y[row] = z[row] - t;
   else y[row] = z[row] + t;
@@ -1127,9 +1211,39 @@ hipError_t tri_kind(TriKind kind, const DCsr& a, const int* perm,
 
 }  // namespace
 
+template <typename T, int MODE>
+hipError_t spmv_sell(const DCsr& a, const void* x, const void* z, void* y, hipStream_t s) {
+  const int grid = (a.n + kBlock - 1) / kBlock;
+  if (grid == 0) return hipSuccess;
+  if constexpr (std::is_same_v<T, int32_t>) {
+    hipLaunchKernelGGL((k_spmv_sell_int<MODE>), dim3(grid), dim3(kBlock), 0, s, a.n, a.soff,
+                       a.scol, static_cast<const int32_t*>(a.sval), a.bit,
+                       static_cast<const int32_t*>(x), static_cast<const int32_t*>(z),
+                       static_cast<int32_t*>(y));
+  } else {
+    hipLaunchKernelGGL((k_spmv_sell_fp<T, MODE>), dim3(grid), dim3(kBlock), 0, s, a.n, a.soff,
+                       a.scol, static_cast<const T*>(a.sval), static_cast<const T*>(x),
+                       static_cast<const T*>(z), static_cast<T*>(y));
+  }
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t spmv_sell_mode(const DCsr& a, int md, const void* x, const void* z, void* y,
+                          hipStream_t s) {
+  if (md == 0) return spmv_sell<T, 0>(a, x, z, y, s);
+  if (md == 1) return spmv_sell<T, 1>(a, x, z, y, s);
+  return spmv_sell<T, 2>(a, x, z, y, s);
+}
+
 hipError_t launch_spmv(Elem e, const DCsr& a, SpmvMode mode, const void* x,
                        const void* z, void* y, hipStream_t s) {
   const int md = static_cast<int>(mode);
+  if (a.soff) {
+    if (e == Elem::kI32) return spmv_sell_mode<int32_t>(a, md, x, z, y, s);
+    if (e == Elem::kF64) return spmv_sell_mode<double>(a, md, x, z, y, s);
+    return spmv_sell_mode<float>(a, md, x, z, y, s);
+  }
   if (e == Elem::kI32) {
     if (md == 0) return spmv_int_mode<0>(a, x, z, y, s);
     if (md == 1) return spmv_int_mode<1>(a, x, z, y, s);
