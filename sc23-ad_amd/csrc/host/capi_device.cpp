@@ -173,21 +173,12 @@ int sc23ad_amg_level_op(sc23ad_amg* m, int level, int op, const void* in, void* 
     DevAmg& a = *m->d;
     if (op == SC23AD_OP_COARSE) {
       if (level != a.depth) return fail(SC23AD_BAD_PARAM, "level_op: coarse solve is level depth");
-      a.coarse(in, out);
     } else {
       if (level < 0 || level >= a.depth) return fail(SC23AD_BAD_PARAM, "level_op: bad level");
-      const DevLevel& L = a.lv[level];
-      switch (op) {
-        case SC23AD_OP_SPMV_A: a.spmv(L.a, dev::SpmvMode::kPlain, in, nullptr, out); break;
-        case SC23AD_OP_SPMV_U: a.spmv(L.u, dev::SpmvMode::kPlain, in, nullptr, out); break;
-        case SC23AD_OP_SPMV_R: a.spmv(L.r, dev::SpmvMode::kPlain, in, nullptr, out); break;
-        case SC23AD_OP_SPMV_P: a.spmv(L.p, dev::SpmvMode::kPlain, in, nullptr, out); break;
-        case SC23AD_OP_TRSV_LD: a.trsv(L.ld_plan, L.ld, in, out); break;
-        case SC23AD_OP_SMOOTH0: a.smooth(level, in, out, false); break;
-        case SC23AD_OP_SMOOTH1: a.smooth(level, in, out, true); break;
-        default: return fail(SC23AD_BAD_PARAM, "level_op: bad op");
-      }
+      if (op < SC23AD_OP_SPMV_A || op > SC23AD_OP_SMOOTH1)
+        return fail(SC23AD_BAD_PARAM, "level_op: bad op");
     }
+    a.level_op(level, op, in, out);
     hip_check(hipStreamSynchronize(lib_stream()), "sync");
     a.check_errors();
     return SC23AD_OK;

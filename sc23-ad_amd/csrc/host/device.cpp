@@ -129,17 +129,11 @@ void upload_mat(DevMat& dm, int n, int m, int bit, int dbit,
   }
 }
 
-// Level sets of a triangular factor.  Dependencies of row i: every stored
-// column except the stored inverse diagonal (last entry for LD, first for DU;
-// L has none).  `chain_size` > 0 splits rows into independent contiguous
-// chains of that many rows (the BlockSOR partition), each walked by one
-// workgroup in a single launch; otherwise levels are walked in order, small
-// ones inside one workgroup and large ones by a launch of their own.
-template <typename V>
-void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr,
-                const std::vector<int>& cind, const std::vector<V>& val, int chain_size,
-                int nchain_req, int single_wg_max) {
-  plan.kind = kind;
+// Level of every row of a triangular factor.  Dependencies of row i: every
+// stored column except the stored inverse diagonal (last entry for LD, first
+// for DU; L has none).
+std::vector<int> tri_levels(TriKind kind, int n, const std::vector<int>& rptr,
+                            const std::vector<int>& cind) {
   if (kind != TriKind::kL)
     for (int i = 0; i < n; ++i)
       if (rptr[i + 1] == rptr[i])
@@ -174,19 +168,19 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
       lev[i] = L;
     }
   }
-  int nchain = 1;
-  std::vector<int> chain_begin{0, n};
-  if (chain_size > 0 && nchain_req > 1) {
-    nchain = nchain_req;
-    chain_begin.assign(nchain + 1, 0);
-    for (int c = 0; c < nchain; ++c) chain_begin[c + 1] = std::min(n, chain_begin[c] + chain_size);
-    chain_begin[nchain] = n;
-  }
-  // rows of each chain ordered by (level, row): counting sort per chain
-  std::vector<int> perm(n);
-  std::vector<int> lev_ptr;
-  std::vector<int> chain_lev;  // 2 per chain
-  lev_ptr.push_back(0);
+  return lev;
+}
+
+// Rows of each chain [chain_begin[c], chain_begin[c+1]) ordered by (level,
+// row) (counting sort): perm[position] = row; lev_ptr = level boundaries
+// over all chains; chain_lev = 2 level indices per chain.
+int order_by_level(const std::vector<int>& lev, const std::vector<int>& chain_begin,
+                   std::vector<int>& perm, std::vector<int>& lev_ptr,
+                   std::vector<int>& chain_lev) {
+  const int nchain = static_cast<int>(chain_begin.size()) - 1;
+  perm.assign(lev.size(), 0);
+  lev_ptr.assign(1, 0);
+  chain_lev.clear();
   int maxlev_all = 0;
   for (int c = 0; c < nchain; ++c) {
     const int rb = chain_begin[c], re = chain_begin[c + 1];
@@ -205,6 +199,26 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
     chain_lev.push_back(first_lev + nl);
     maxlev_all = std::max(maxlev_all, nl);
   }
+  return maxlev_all;
+}
+
+// Level schedule of a triangular factor.  `chain_begin` (nchain + 1 row
+// boundaries) splits rows into independent contiguous chains (the BlockSOR
+// partition), each walked by one workgroup in a single launch; with one
+// chain, levels are walked in order, small ones inside one workgroup and
+// large ones by a launch of their own.
+template <typename V>
+void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr,
+                const std::vector<int>& cind, const std::vector<V>& val,
+                const std::vector<int>& chain_begin, int single_wg_max) {
+  plan.kind = kind;
+  const std::vector<int> lev = tri_levels(kind, n, rptr, cind);
+  const int nchain = static_cast<int>(chain_begin.size()) - 1;
+  std::vector<int> perm, lev_ptr, chain_lev;
+  const int maxlev_all = order_by_level(lev, chain_begin, perm, lev_ptr, chain_lev);
+  plan.identity = true;
+  for (int i = 0; i < n; ++i)
+    if (perm[i] != i) plan.identity = false;
   plan.nlevels = maxlev_all;
   // segments
   std::vector<int> seg_chain_lev;  // chain_lev entries of single-chain segments
@@ -358,16 +372,13 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
   size_t maxlev_rows = 0;
   for (size_t Lv = 0; Lv + 1 < lev_ptr.size(); ++Lv)
     maxlev_rows = std::max<size_t>(maxlev_rows, lev_ptr[Lv + 1] - lev_ptr[Lv]);
-  int maxlev_chain = 0, maxrows_chain = 0;
-  for (size_t c = 0; c + 1 < all_chain_lev.size(); c += 2) {
+  int maxlev_chain = 0;
+  for (size_t c = 0; c + 1 < all_chain_lev.size(); c += 2)
     maxlev_chain = std::max(maxlev_chain, all_chain_lev[c + 1] - all_chain_lev[c]);
-    maxrows_chain = std::max(maxrows_chain,
-                             lev_ptr[all_chain_lev[c + 1]] - lev_ptr[all_chain_lev[c]]);
-  }
   int ep_ring = 2;
   while (ep_ring < 32768 && ep_ring < span) ep_ring *= 2;
-  if (std::is_same_v<V, int32_t> && one_launch_chains && maxlev_rows <= 1024 &&
-      dev::ep_lds_bytes(ep_ring, maxlev_chain, 0) <= dev::kTriBlockedMaxLds) {
+  if (std::is_same_v<V, int32_t> && plan.identity && one_launch_chains && maxlev_rows <= 1024 &&
+      dev::ep_lds_bytes(ep_ring, maxlev_chain) <= dev::kTriBlockedMaxLds) {
     std::vector<int> lev_of(static_cast<size_t>(std::max(n, 1)), 0);
     std::vector<int> lev_eptr(lev_ptr.size(), 0);
     for (size_t Lv = 0; Lv + 1 < lev_ptr.size(); ++Lv) {
@@ -414,27 +425,10 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
       L.ep_waves = w;
       L.ep_far = any_far ? 1 : 0;
       L.ep_maxlev = maxlev_chain;
-      L.ep_maxrows = maxrows_chain;
-      // Right-hand sides staged in LDS per chain when the chains are short
-      // (two gather rounds at most) and fit, else gathered by a separate
-      // kernel into level-ordered row data.
-      L.ep_staged = (!any_far && maxrows_chain <= 16 * 64 * w &&
-                     dev::ep_lds_bytes(ep_ring, maxlev_chain, maxrows_chain) <=
-                         dev::kTriBlockedMaxLds)
+      // Narrow register layout (4 entries, 1 row per lane) when it holds
+      // every row of a level and about as many entries.
+      L.ep_narrow = (maxlev_rows <= static_cast<size_t>(w) * 64 && overflowing(w, 4) * 16 <= nlev)
                         ? 1 : 0;
-      // Narrow register layout (4 entries, 1 row per lane; staged kernels)
-      // when it holds every row of a level and about as many entries.
-      L.ep_narrow = (L.ep_staged && maxlev_rows <= static_cast<size_t>(w) * 64 &&
-                     overflowing(w, 4) * 16 <= nlev)
-                        ? 1 : 0;
-      std::vector<int> lrd(2 * static_cast<size_t>(std::max(n, 1)), 0);
-      for (int idx = 0; idx < n; ++idx) {
-        lrd[2 * static_cast<size_t>(idx)] = perm[idx];
-        lrd[2 * static_cast<size_t>(idx) + 1] = static_cast<int>(ldiag[idx]);
-      }
-      plan.lrd.alloc(sizeof(int) * lrd.size());
-      plan.lrd.upload(lrd.data(), sizeof(int) * lrd.size());
-      L.lrd = plan.lrd.get();
     }
   }
   int maxdeps = 0;
@@ -443,10 +437,10 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
     std::fprintf(stderr,
                  "[sc23ad plan] kind %d n %d nnz %zu chains %d levels(max chain) %d "
                  "max level rows %zu launches %d ring %d span %lld max deps %d ep waves %d far %d "
-                 "narrow %d staged %d\n",
+                 "narrow %d identity %d\n",
                  static_cast<int>(kind), n, cind.size(), nchain, plan.nlevels, maxlev_rows,
                  plan.nlaunch, L.ring, span, maxdeps, L.ep_waves, L.ep_far, L.ep_narrow,
-                 L.ep_staged);
+                 plan.identity ? 1 : 0);
   }
 }
 
@@ -525,6 +519,99 @@ void build_tiles(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rpt
   plan.err.upload(zero.data(), sizeof(int));
   plan.nb = nb;
   plan.dense = true;
+}
+
+// Rows and columns of a CSR matrix in a new numbering: new row r is old row
+// row_perm[r] (identity when null), columns map through col_ipos (old -> new,
+// identity when null).  The entries of a row keep their stored order, so FP
+// row sums add in the same order as before.
+template <typename V>
+void renumber(const std::vector<int>& rptr, const std::vector<int>& cind,
+              const std::vector<V>& val, const std::vector<int>* row_perm,
+              const std::vector<int>* col_ipos, std::vector<int>& r2, std::vector<int>& c2,
+              std::vector<V>& v2) {
+  const int n = static_cast<int>(rptr.size()) - 1;
+  r2.assign(static_cast<size_t>(n) + 1, 0);
+  c2.resize(cind.size());
+  v2.resize(val.size());
+  for (int r = 0; r < n; ++r) {
+    const int i = row_perm ? (*row_perm)[r] : r;
+    r2[r + 1] = r2[r] + (rptr[i + 1] - rptr[i]);
+  }
+  for (int r = 0; r < n; ++r) {
+    const int i = row_perm ? (*row_perm)[r] : r;
+    int q = r2[r];
+    for (int p = rptr[i]; p < rptr[i + 1]; ++p, ++q) {
+      c2[q] = col_ipos ? (*col_ipos)[cind[p]] : cind[p];
+      v2[q] = val[p];
+    }
+  }
+}
+
+// Row boundaries of the smoother's chains: BlockSOR's b_num blocks of
+// ceil(n / b_num) rows (smoother/stationary.hpp:934-939), or one chain.
+std::vector<int> chain_rows(int n, bool blocks, int b_num) {
+  if (!blocks || b_num <= 1) return {0, n};
+  const int size = (n + b_num - 1) / b_num;
+  std::vector<int> cb(static_cast<size_t>(b_num) + 1, 0);
+  for (int c = 0; c < b_num; ++c) cb[c + 1] = std::min(n, cb[c] + size);
+  cb[b_num] = n;
+  return cb;
+}
+
+// Uploads the levels of a hierarchy (IntLevel or FpLevel<T>) renumbered into
+// the LD smoother's level order: level l's unknowns take their position in
+// the level schedule of LD_l, so every level-scheduled kernel reads and
+// writes contiguous ranges.  R and P map between the numberings of adjacent
+// levels; the coarsest grid keeps its own order.
+template <typename Level, typename BitsOf>
+void upload_levels(DevAmg& a, const std::vector<Level>& levels, bool blocks, int b_num,
+                   const LaunchCfg& c, BitsOf bits) {
+  const int depth = static_cast<int>(levels.size());
+  std::vector<std::vector<int>> perm(depth), ipos(depth), cbeg(depth);
+  for (int l = 0; l < depth; ++l) {
+    const auto& ld = levels[l].ld;
+    const std::vector<int> lev = tri_levels(TriKind::kLD, ld.n, ld.rptr, ld.cind);
+    std::vector<int> lp, cl;
+    order_by_level(lev, chain_rows(ld.n, blocks, b_num), perm[l], lp, cl);
+    ipos[l].assign(static_cast<size_t>(ld.n), 0);
+    for (int k = 0; k < ld.n; ++k) ipos[l][perm[l][k]] = k;
+    for (size_t q = 0; q + 1 < cl.size(); q += 2) cbeg[l].push_back(lp[cl[q]]);
+    cbeg[l].push_back(ld.n);
+  }
+  for (int l = 0; l < depth; ++l) {
+    const Level& L = levels[l];
+    DevLevel& D = a.lv[l];
+    D.n = L.a.n;
+    const std::vector<int>* rp = &perm[l];
+    const std::vector<int>* ci = &ipos[l];
+    const std::vector<int>* nrp = l + 1 < depth ? &perm[l + 1] : nullptr;
+    const std::vector<int>* nci = l + 1 < depth ? &ipos[l + 1] : nullptr;
+    using V = typename std::decay_t<decltype(L.a.val)>::value_type;
+    std::vector<int> r2, c2;
+    std::vector<V> v2;
+    auto put = [&](DevMat& dm, const auto& m, const std::vector<int>* rows,
+                   const std::vector<int>* cols, int group) {
+      renumber(m.rptr, m.cind, m.val, rows, cols, r2, c2, v2);
+      int b, db;
+      bits(m, b, db);
+      upload_mat(dm, m.n, m.m, b, db, r2, c2, v2, group);
+    };
+    put(D.a, L.a, rp, ci, c.spmv_group);
+    put(D.u, L.u, rp, ci, c.spmv_group);
+    put(D.r, L.r, nrp, ci, c.spmv_group);
+    put(D.p, L.p, rp, nci, c.spmv_group);
+    put(D.ld, L.ld, rp, ci, 0);  // last: r2/c2/v2 keep LD for the plan
+    build_plan(D.ld_plan, TriKind::kLD, L.ld.n, r2, c2, v2, cbeg[l], c.single_wg_max_rows);
+    D.perm.alloc(sizeof(int) * std::max(L.a.n, 1));
+    D.perm.upload(perm[l].data(), sizeof(int) * L.a.n);
+  }
+  if (depth > 0) {
+    const auto& a0 = levels[0].a;
+    int b, db;
+    bits(a0, b, db);
+    upload_mat(a.a0, a0.n, a0.m, b, db, a0.rptr, a0.cind, a0.val, c.spmv_group);
+  }
 }
 
 // SpMV launch-shape autotuner: times every candidate kernel for this
@@ -608,7 +695,8 @@ void DevAmg::alloc_work() {
     tmp[l].alloc(es * n);
   }
   r0.alloc(es * std::max(n0(), 1));
-  rows_scratch.alloc(16 * static_cast<size_t>(std::max(n0(), 1)));
+  vin0.alloc(es * std::max(n0(), 1));
+  vout0.alloc(es * std::max(n0(), 1));
   tmp[0].alloc(es * std::max(n0(), 1));
   tmpc.alloc(es * std::max(n_coarse, 1));
 }
@@ -622,24 +710,15 @@ std::unique_ptr<DevAmg> DevAmg::upload(const IntHierarchy& h, const LaunchCfg& c
   a->n_coarse = h.n_coarse;
   a->lv.resize(h.depth);
   const bool bj = h.prm.smoother == SmootherKind::kBlockSOR;
-  for (int l = 0; l < h.depth; ++l) {
-    const IntLevel& L = h.lv[l];
-    DevLevel& D = a->lv[l];
-    D.n = L.a.n;
-    upload_mat(D.a, L.a.n, L.a.m, L.a.bit, -1, L.a.rptr, L.a.cind, L.a.val, c.spmv_group);
-    upload_mat(D.ld, L.ld.n, L.ld.m, L.ld.bit, L.ld.dbit, L.ld.rptr, L.ld.cind, L.ld.val, 0);
-    upload_mat(D.u, L.u.n, L.u.m, L.u.bit, -1, L.u.rptr, L.u.cind, L.u.val, c.spmv_group);
-    upload_mat(D.r, L.r.n, L.r.m, L.r.bit, -1, L.r.rptr, L.r.cind, L.r.val, c.spmv_group);
-    upload_mat(D.p, L.p.n, L.p.m, L.p.bit, -1, L.p.rptr, L.p.cind, L.p.val, c.spmv_group);
-    const int size = bj ? (L.a.n + h.prm.b_num - 1) / h.prm.b_num : 0;
-    build_plan(D.ld_plan, TriKind::kLD, L.ld.n, L.ld.rptr, L.ld.cind, L.ld.val, size,
-               bj ? h.prm.b_num : 1, c.single_wg_max_rows);
-  }
+  upload_levels(*a, h.lv, bj, h.prm.b_num, c, [](const IntCsr& m, int& b, int& db) {
+    b = m.bit;
+    db = m.dbit;
+  });
   upload_mat(a->lc, h.lc.n, h.lc.m, h.lc.bit, -1, h.lc.rptr, h.lc.cind, h.lc.val, 0);
   upload_mat(a->uc, h.uc.n, h.uc.m, h.uc.bit, h.uc.dbit, h.uc.rptr, h.uc.cind, h.uc.val, 0);
-  build_plan(a->lc_plan, TriKind::kL, h.lc.n, h.lc.rptr, h.lc.cind, h.lc.val, 0, 1,
+  build_plan(a->lc_plan, TriKind::kL, h.lc.n, h.lc.rptr, h.lc.cind, h.lc.val, {0, h.lc.n},
              c.single_wg_max_rows);
-  build_plan(a->uc_plan, TriKind::kDU, h.uc.n, h.uc.rptr, h.uc.cind, h.uc.val, 0, 1,
+  build_plan(a->uc_plan, TriKind::kDU, h.uc.n, h.uc.rptr, h.uc.cind, h.uc.val, {0, h.uc.n},
              c.single_wg_max_rows);
   build_blocked(a->lc_plan, TriKind::kL, h.lc.n, h.lc.rptr, h.lc.cind, 4, 8);
   build_blocked(a->uc_plan, TriKind::kDU, h.uc.n, h.uc.rptr, h.uc.cind, 4, 8);
@@ -662,24 +741,17 @@ std::unique_ptr<DevAmg> DevAmg::upload_fp(const FpHierarchy<T>& h, const LaunchC
   a->n_coarse = h.n_coarse;
   a->lv.resize(h.depth);
   const bool bj = h.prm.smoother == SmootherKind::kBlockSOR;
-  for (int l = 0; l < h.depth; ++l) {
-    const FpLevel<T>& L = h.lv[l];
-    DevLevel& D = a->lv[l];
-    D.n = L.a.n;
-    upload_mat(D.a, L.a.n, L.a.m, 0, -1, L.a.rptr, L.a.cind, L.a.val, 1);
-    upload_mat(D.ld, L.ld.n, L.ld.m, 0, -1, L.ld.rptr, L.ld.cind, L.ld.val, 1);
-    upload_mat(D.u, L.u.n, L.u.m, 0, -1, L.u.rptr, L.u.cind, L.u.val, 1);
-    upload_mat(D.r, L.r.n, L.r.m, 0, -1, L.r.rptr, L.r.cind, L.r.val, 1);
-    upload_mat(D.p, L.p.n, L.p.m, 0, -1, L.p.rptr, L.p.cind, L.p.val, 1);
-    const int size = bj ? (L.a.n + h.prm.b_num - 1) / h.prm.b_num : 0;
-    build_plan(D.ld_plan, TriKind::kLD, L.ld.n, L.ld.rptr, L.ld.cind, L.ld.val, size,
-               bj ? h.prm.b_num : 1, c.single_wg_max_rows);
-  }
+  LaunchCfg cf = c;
+  cf.spmv_group = 1;
+  upload_levels(*a, h.lv, bj, h.prm.b_num, cf, [](const FpCsr<T>&, int& b, int& db) {
+    b = 0;
+    db = -1;
+  });
   upload_mat(a->lc, h.lc.n, h.lc.m, 0, -1, h.lc.rptr, h.lc.cind, h.lc.val, 1);
   upload_mat(a->uc, h.uc.n, h.uc.m, 0, -1, h.uc.rptr, h.uc.cind, h.uc.val, 1);
-  build_plan(a->lc_plan, TriKind::kL, h.lc.n, h.lc.rptr, h.lc.cind, h.lc.val, 0, 1,
+  build_plan(a->lc_plan, TriKind::kL, h.lc.n, h.lc.rptr, h.lc.cind, h.lc.val, {0, h.lc.n},
              c.single_wg_max_rows);
-  build_plan(a->uc_plan, TriKind::kDU, h.uc.n, h.uc.rptr, h.uc.cind, h.uc.val, 0, 1,
+  build_plan(a->uc_plan, TriKind::kDU, h.uc.n, h.uc.rptr, h.uc.cind, h.uc.val, {0, h.uc.n},
              c.single_wg_max_rows);
   build_blocked(a->lc_plan, TriKind::kL, h.lc.n, h.lc.rptr, h.lc.cind, sizeof(T), sizeof(T));
   build_blocked(a->uc_plan, TriKind::kDU, h.uc.n, h.uc.rptr, h.uc.cind, sizeof(T), sizeof(T));
@@ -717,7 +789,7 @@ void DevAmg::trsv(const TriPlan& plan, const DevMat& m, const void* in,
   }
   if (plan.layout.ep_waves > 0 && elem == Elem::kI32 && plan.segs.size() == 1) {
     hip_check(dev::launch_trsv_ep(plan.kind, plan.layout, plan.segs[0], m.d.bit, m.d.dbit, in,
-                                  out, rows_scratch.get(), lib_stream()),
+                                  out, lib_stream()),
               "trsv ep");
     return;
   }
@@ -769,6 +841,49 @@ void DevAmg::coarse(const void* in, void* out) const {
 // intAMGflex::Precondition / dfAMG::Precondition (INTAMG.hpp:269-293,
 // 108-133).  Coarse work vectors are indexed by level (1..depth).
 void DevAmg::vcycle(const void* in, void* out) const {
+  const int n = n0();
+  hip_check(dev::launch_permute(elem, n, perm_of(0), in, vin0.get(), true, lib_stream()),
+            "permute in");
+  vcycle_lo(vin0.get(), vout0.get());
+  hip_check(dev::launch_permute(elem, n, perm_of(0), vout0.get(), out, false, lib_stream()),
+            "permute out");
+}
+
+void DevAmg::level_op(int level, int op, const void* in, void* out) const {
+  // Numbering of the operand and of the result (nullptr: original order).
+  const int* pin = op == 7 ? nullptr : op == 3 ? perm_of(level + 1) : perm_of(level);
+  const int* pout = op == 7 ? nullptr : op == 2 ? perm_of(level + 1) : perm_of(level);
+  const int nin = op == 3 ? level_n(level + 1) : level_n(level);
+  const int nout = op == 2 ? level_n(level + 1) : level_n(level);
+  const size_t es = esize();
+  DevBuf ti(es * std::max(nin, 1)), to(es * std::max(nout, 1));
+  auto to_level = [&](const int* p, int n, const void* src, void* dst) {
+    if (p) hip_check(dev::launch_permute(elem, n, p, src, dst, true, lib_stream()), "permute");
+    else hip_check(hipMemcpyAsync(dst, src, es * n, hipMemcpyDeviceToDevice, lib_stream()), "copy");
+  };
+  to_level(pin, nin, in, ti.get());
+  if (op == 6) to_level(pout, nout, out, to.get());  // Smooth(true) reads out too
+  if (op == 7) {
+    coarse(ti.get(), to.get());
+  } else {
+    const DevLevel& L = lv[level];
+    switch (op) {
+      case 0: spmv(L.a, SpmvMode::kPlain, ti.get(), nullptr, to.get()); break;
+      case 1: spmv(L.u, SpmvMode::kPlain, ti.get(), nullptr, to.get()); break;
+      case 2: spmv(L.r, SpmvMode::kPlain, ti.get(), nullptr, to.get()); break;
+      case 3: spmv(L.p, SpmvMode::kPlain, ti.get(), nullptr, to.get()); break;
+      case 4: trsv(L.ld_plan, L.ld, ti.get(), to.get()); break;
+      case 5: smooth(level, ti.get(), to.get(), false); break;
+      case 6: smooth(level, ti.get(), to.get(), true); break;
+      default: throw DevError{1, "level_op: bad op"};
+    }
+  }
+  if (pout) hip_check(dev::launch_permute(elem, nout, pout, to.get(), out, false, lib_stream()), "permute");
+  else hip_check(hipMemcpyAsync(out, to.get(), es * nout, hipMemcpyDeviceToDevice, lib_stream()), "copy");
+}
+
+// intAMGflex::Precondition / dfAMG::Precondition on level-ordered vectors.
+void DevAmg::vcycle_lo(const void* in, void* out) const {
   smooth(0, in, out, false);
   spmv(lv[0].a, SpmvMode::kSub, out, in, r0.get());  // r = in - A out
   if (depth == 0) return;
@@ -969,7 +1084,7 @@ DevSolver::Result DevSolver::solve(const double* b_dev, double nrm_b, double* x_
         hip_check(dev::launch_checksum(n, static_cast<const int32_t*>(zj), k, s), "cs");
       }
       spmv_timer.begin();
-      amg->spmv(amg->lv[0].a, SpmvMode::kPlain, zj, nullptr, w);
+      amg->spmv(amg->a0, SpmvMode::kPlain, zj, nullptr, w);
       spmv_timer.end();
       hip_check(dev::launch_dot_first(elem, n, Vb, w, k, j, seq, s), "dot");
       for (int kk = 0; kk <= j; ++kk)

@@ -69,6 +69,7 @@ struct DevMat {
 // Level-set schedule of one triangular factor.
 struct TriPlan {
   dev::TriKind kind = dev::TriKind::kLD;
+  bool identity = false;  // level order == row order (the matrix was renumbered)
   DevBuf perm, lev_ptr, chain_lev;
   std::vector<dev::TriSegment> segs;
   int nlevels = 0;    // longest dependency chain (levels of the deepest chain)
@@ -82,7 +83,7 @@ struct TriPlan {
   DevBuf slice, boff, ddinv, flags, err;
   mutable int epoch = 0;  // value the blocks raise their flags to in the last launch
   // level-ordered layout used by the level-scheduled kernels
-  DevBuf lrow, lbeg, lcol, lval, ldiag, ecol, eval, lpos, epos, lent, lrd, lev_eptr;
+  DevBuf lrow, lbeg, lcol, lval, ldiag, ecol, eval, lpos, epos, lent, lev_eptr;
   dev::TriLayout layout;
 };
 
@@ -103,9 +104,12 @@ struct EventTimer {
   int used = 0;
 };
 
+// One level on the device.  Its unknowns are numbered in the smoother's
+// level order (position in the LD schedule); perm[new] = original row.
 struct DevLevel {
   DevMat a, ld, u, r, p;
   TriPlan ld_plan;
+  DevBuf perm;
   int n = 0;
 };
 
@@ -126,7 +130,9 @@ class DevAmg {
   int n_coarse = 0;
   // work vectors: index = level (1..depth); level 0 uses r0 / temp0
   std::vector<DevBuf> ee, xx, rr, tmp;
-  DevBuf r0, tmpc, rows_scratch;  // rows_scratch: level-ordered row data (int4 per row)
+  DevBuf r0, tmpc;
+  DevMat a0;             // fine-level operator in the original numbering (Arnoldi)
+  DevBuf vin0, vout0;    // V-cycle input / output in level-0 order
   LaunchCfg cfg;
   mutable EventTimer trsv0_timer;  // level-0 LD solves (benchmark hook)
 
@@ -143,7 +149,10 @@ class DevAmg {
   void trsv(const TriPlan& plan, const DevMat& m, const void* in, void* out) const;
   void smooth(int l, const void* in, void* out, bool inited) const;
   void coarse(const void* in, void* out) const;
+  // One V-cycle on vectors in the original numbering.
   void vcycle(const void* in, void* out) const;
+  // Level operation (sc23ad_level_op) on vectors in the original numbering.
+  void level_op(int level, int op, const void* in, void* out) const;
   // Algorithmic HBM bytes of one V-cycle (sum over its kernels).
   double vcycle_bytes() const;
   // Throws if a kernel with a bounded wait gave up (after a stream sync).
@@ -152,6 +161,10 @@ class DevAmg {
  private:
   void alloc_work();
   void tune();  // SpMV launch shapes per matrix
+  void vcycle_lo(const void* in, void* out) const;  // vectors in level order
+  const int* perm_of(int level) const {  // nullptr: original order (coarsest grid)
+    return level < depth ? lv[level].perm.as<int>() : nullptr;
+  }
 };
 
 struct SolverOpts {

@@ -704,6 +704,34 @@ hipError_t launch_checksum(int n, const int32_t* v, const KState& k, hipStream_t
   return hipGetLastError();
 }
 
+template <typename T, bool kGather>
+__global__ __launch_bounds__(B) void k_permute(int n, const int* __restrict__ perm,
+                                               const T* __restrict__ src, T* __restrict__ dst) {
+  const int k = blockIdx.x * B + threadIdx.x;
+  if (k >= n) return;
+  if (kGather) dst[k] = src[perm[k]];
+  else dst[perm[k]] = src[k];
+}
+
+hipError_t launch_permute(Elem e, int n, const int* perm, const void* src, void* dst,
+                          bool gather, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int grid = (n + B - 1) / B;
+  auto go = [&](auto tag) {
+    using T = decltype(tag);
+    if (gather)
+      hipLaunchKernelGGL((k_permute<T, true>), dim3(grid), dim3(B), 0, s, n, perm,
+                         static_cast<const T*>(src), static_cast<T*>(dst));
+    else
+      hipLaunchKernelGGL((k_permute<T, false>), dim3(grid), dim3(B), 0, s, n, perm,
+                         static_cast<const T*>(src), static_cast<T*>(dst));
+  };
+  if (e == Elem::kF64) go(double{});
+  else if (e == Elem::kF32) go(float{});
+  else go(int32_t{});
+  return hipGetLastError();
+}
+
 hipError_t launch_copy(Elem e, int n, const void* src, void* dst, hipStream_t s) {
   const size_t bytes = static_cast<size_t>(n) * (e == Elem::kF64 ? 8 : 4);
   const size_t n4 = bytes / 4;

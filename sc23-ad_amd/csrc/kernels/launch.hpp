@@ -73,32 +73,26 @@ struct TriLayout {
   const int* lpos = nullptr;
   const int* epos = nullptr;
   int ring = 0;  // > 0: LDS ring of this many values (power of two)
-  // entry-parallel mode (integer factors): packed entries {dep, val, slot, 0}
-  // and row data {row, inverse diagonal} (see k_tri_chain_ep), entry offset
-  // of each level, waves per chain
+  // entry-parallel mode (integer factors, factor numbered in level order):
+  // packed entries {dep, val, slot, 0} (see k_tri_chain_ep), entry offset of
+  // each level, waves per chain
   const void* lent = nullptr;
-  const void* lrd = nullptr;
   const int* lev_eptr = nullptr;
   int ep_waves = 0;  // > 0: use the entry-parallel kernel
   int ep_far = 0;    // entries with dep < 0 (read from global memory) exist
   int ep_narrow = 0; // 4 entries and 1 row per lane in registers (else 8 and 2)
-  int ep_staged = 0; // right-hand sides staged in LDS per chain (else gathered)
   int ep_maxlev = 0; // most levels in one chain
-  int ep_maxrows = 0; // most rows in one chain
   int n = 0;
 };
 // Entry-parallel integer solve (one workgroup per chain).
 constexpr int kEpEntriesPerLane = 8;  // entry-parallel kernel: entries held in registers
 constexpr int kEpFarLevels = 40;  // least level distance of a dependency read from global memory
-// rows_scratch: n int4 (row data gathered into level order) unless staged.
 hipError_t launch_trsv_ep(TriKind kind, const TriLayout& L, const TriSegment& seg, int bit,
-                          int dbit, const void* in, void* out, void* rows_scratch,
-                          hipStream_t s);
+                          int dbit, const void* in, void* out, hipStream_t s);
 // LDS bytes of the entry-parallel kernel.
-inline size_t ep_lds_bytes(int ring, int maxlev, int staged_rows) {
+inline size_t ep_lds_bytes(int ring, int maxlev) {
   return sizeof(uint64_t) * 1024 + sizeof(int32_t) * static_cast<size_t>(ring) +
-         2 * sizeof(int) * (static_cast<size_t>(maxlev) + 1) +
-         sizeof(int32_t) * static_cast<size_t>(staged_rows);
+         2 * sizeof(int) * (static_cast<size_t>(maxlev) + 1);
 }
 hipError_t launch_trsv_layout(Elem e, TriKind kind, const TriLayout& L,
                               const TriSegment& seg, int bit, int dbit, const void* in,
@@ -184,6 +178,9 @@ hipError_t launch_relres(const DCsr& a64, const double* b, const double* x,
                          double* out, hipStream_t s);
 // V-cycle vector helpers
 hipError_t launch_copy(Elem e, int n, const void* src, void* dst, hipStream_t s);
+// gather: dst[k] = src[perm[k]]; otherwise scatter: dst[perm[k]] = src[k].
+hipError_t launch_permute(Elem e, int n, const int* perm, const void* src, void* dst,
+                          bool gather, hipStream_t s);
 hipError_t launch_fill_zero(size_t bytes, void* dst, hipStream_t s);
 
 }  // namespace sc23ad::dev

@@ -832,26 +832,23 @@ __global__ __launch_bounds__(kChainBlock) void k_tri_chain_pre(
 // a whole level (contiguous in the level-ordered layout) are spread over the
 // lanes: each lane multiplies its entries by solution values and adds the
 // products into the row's LDS accumulator with 64-bit LDS atomics; then one
-// lane per row finishes the row.
+// lane per row finishes the row.  The factor is numbered in level order (row
+// = position), so a level's right-hand sides, inverse diagonals and results
+// are contiguous.
 //
 // Entries are packed as int4 {dep, val, slot, 0}: dep >= 0 is the position of
 // a dependency whose value sits in the LDS ring; dep < 0 (kFar kernels only)
-// names row -dep-1, a dependency at least kEpFarLevels levels back that the
-// ring cannot hold, read from `out` in global memory.  A full workgroup fence
-// every kEpFence levels makes such values visible before any load of them is
-// issued.  Right-hand sides reach the kernel in one of two ways:
-//   staged (kStaged): row data are int2 {row, inverse diagonal} per position
-//     and the chain's right-hand side values are gathered into LDS when the
-//     kernel starts;
-//   gathered: row data are int4 {rhs, inverse diagonal, row, 0} per
-//     position, gathered into level order by k_gather_rows before the launch.
-// The chain's level offsets are staged in LDS too, so the loads of a level
-// depend only on LDS data: the entries and row data of level L + kEpDepth are
-// loaded once level L is done (into the registers it has just freed), and
-// far values of level L + kEpFarAhead (whose addresses need the entries) at
-// the start of level L.  Every load is issued unconditionally (lanes past a
-// level's end read element 0 and ignore it): loads under divergent branches
-// make the compiler wait for all outstanding loads at the join.
+// names position -dep-1, a dependency at least kEpFarLevels levels back that
+// the ring cannot hold, read from `out` in global memory.  A full workgroup
+// fence every kEpFence levels makes such values visible before any load of
+// them is issued.  The chain's level offsets are staged in LDS, so the loads
+// of a level depend only on LDS data: entries, right-hand sides and inverse
+// diagonals of level L + kEpDepth are loaded once level L is done (into the
+// registers it has just freed), far values of level L + kEpFarAhead (whose
+// addresses need the entries) at the start of level L.  Every load is issued
+// unconditionally (lanes past a level's end read element 0 and ignore it):
+// loads under divergent branches make the compiler wait for all outstanding
+// loads at the join.
 
 constexpr int kEpMaxLevel = 1024;
 constexpr int kEpDepth = 4;     // prefetch distance in levels
@@ -867,14 +864,15 @@ struct EpPre {
   int32_t val[kE];
   int slot[kE];
   int32_t xfar[kE];
-  int row[kR];
   int32_t dinv[kR];
-  int32_t in[kR];  // gathered mode
+  int32_t in[kR];
 };
 
-template <int kThreads, bool kStaged, int kE, int kR>
+template <int kThreads, int kE, int kR>
 __device__ __forceinline__ void ep_load(EpPre<kE, kR>& r, int2 lo, int2 hi,
-                                        const int4* __restrict__ lent, const void* rowdata) {
+                                        const int4* __restrict__ lent,
+                                        const int32_t* __restrict__ dinv,
+                                        const int32_t* __restrict__ in) {
   const int t = threadIdx.x;
 #pragma unroll
   for (int q = 0; q < kE; ++q) {
@@ -888,16 +886,8 @@ __device__ __forceinline__ void ep_load(EpPre<kE, kR>& r, int2 lo, int2 hi,
   for (int q = 0; q < kR; ++q) {
     const int idx = lo.x + t + q * kThreads;
     const int id = idx < hi.x ? idx : 0;
-    if (kStaged) {
-      const int2 w = static_cast<const int2*>(rowdata)[id];
-      r.row[q] = w.x;
-      r.dinv[q] = w.y;
-    } else {
-      const int4 w = static_cast<const int4*>(rowdata)[id];
-      r.in[q] = w.x;
-      r.dinv[q] = w.y;
-      r.row[q] = w.z;
-    }
+    r.dinv[q] = dinv[id];
+    r.in[q] = in[id];
   }
 }
 
@@ -923,9 +913,7 @@ __device__ __forceinline__ void ep_sync() {
 }
 
 struct EpShared {
-  const int2* lvo;     // level offsets {row position, entry position}, LDS
-  const int32_t* inl;  // staged right-hand sides by position - chain start, LDS
-  int pos0;            // first position of the chain
+  const int2* lvo;  // level offsets {position, entry}, LDS
   uint64_t* acc;
   int32_t* ring;
   int rmask;
@@ -934,10 +922,11 @@ struct EpShared {
 // One level of the chain; `cur` holds the level's prefetched data, `ahead`
 // the buffer of level L + kEpFarAhead.  Level indices past the chain's end
 // are clamped to an empty level.
-template <TriKind K, int kWaves, bool kFar, bool kStaged, int kE, int kR>
+template <TriKind K, int kWaves, bool kFar, int kE, int kR>
 __device__ __forceinline__ void ep_level(EpPre<kE, kR>& cur, EpPre<kE, kR>& ahead, int L,
                                          int nl, const EpShared& sh,
-                                         const int4* __restrict__ lent, const void* rowdata,
+                                         const int4* __restrict__ lent,
+                                         const int32_t* __restrict__ dinv,
                                          const int32_t* __restrict__ in, int32_t* out, int bit,
                                          int dbit) {
   constexpr int kThreads = 64 * kWaves;
@@ -978,63 +967,43 @@ __device__ __forceinline__ void ep_level(EpPre<kE, kR>& cur, EpPre<kE, kR>& ahea
   }
   ep_sync<kWaves>();
   uint64_t av[kR];
-  int32_t rin[kR];
 #pragma unroll
-  for (int q = 0; q < kR; ++q) {
-    av[q] = acc[t + q * kThreads];
-    if (kStaged) {
-      const int k = s + t + q * kThreads - sh.pos0;
-      rin[q] = sh.inl[k < e - sh.pos0 ? k : 0];
-    } else {
-      rin[q] = cur.in[q];
-    }
-  }
+  for (int q = 0; q < kR; ++q) av[q] = acc[t + q * kThreads];
 #pragma unroll
   for (int q = 0; q < kR; ++q) {
     const int k = t + q * kThreads;
     if (s + k < e) {
-      const uint64_t tt = (static_cast<uint64_t>(static_cast<int64_t>(rin[q])) << bit) - av[q];
+      const uint64_t tt =
+          (static_cast<uint64_t>(static_cast<int64_t>(cur.in[q])) << bit) - av[q];
       acc[k] = 0;
       const int32_t v = acc_final<int32_t, K>(tt, cur.dinv[q], bit, dbit);
       ring[(s + k) & rmask] = v;
-      out[cur.row[q]] = v;
+      out[s + k] = v;
     }
   }
   for (int k = t + kR * kThreads; s + k < e; k += kThreads) {
-    int row, dv;
-    int32_t rv;
-    if (kStaged) {
-      const int2 w = static_cast<const int2*>(rowdata)[s + k];
-      row = w.x;
-      dv = w.y;
-      rv = sh.inl[s + k - sh.pos0];
-    } else {
-      const int4 w = static_cast<const int4*>(rowdata)[s + k];
-      rv = w.x;
-      dv = w.y;
-      row = w.z;
-    }
-    const uint64_t tt = (static_cast<uint64_t>(static_cast<int64_t>(rv)) << bit) - acc[k];
+    const uint64_t tt =
+        (static_cast<uint64_t>(static_cast<int64_t>(in[s + k])) << bit) - acc[k];
     acc[k] = 0;
-    const int32_t v = acc_final<int32_t, K>(tt, dv, bit, dbit);
+    const int32_t v = acc_final<int32_t, K>(tt, dinv[s + k], bit, dbit);
     ring[(s + k) & rmask] = v;
-    out[row] = v;
+    out[s + k] = v;
   }
   if (kFar && (L % kEpFence) == kEpFence - 1)
     __syncthreads();  // full fence: global results visible to later loads
   else
     ep_sync<kWaves>();
   const int Lp = min(L + kEpDepth, nl);
-  ep_load<kThreads, kStaged>(cur, sh.lvo[Lp], sh.lvo[min(Lp + 1, nl)], lent, rowdata);
+  ep_load<kThreads>(cur, sh.lvo[Lp], sh.lvo[min(Lp + 1, nl)], lent, dinv, in);
 }
 
-// LDS: kEpMaxLevel uint64 accumulators, the ring, the chain's level offsets
-// (max_levels + 1 int2), then (staged) the chain's right-hand sides.
-template <TriKind K, int kWaves, bool kFar, bool kStaged, int kE, int kR>
+// LDS: kEpMaxLevel uint64 accumulators, the ring, then the chain's level
+// offsets (max_levels + 1 int2).
+template <TriKind K, int kWaves, bool kFar, int kE, int kR>
 __global__ __launch_bounds__(64 * kWaves) void k_tri_chain_ep(
-    const int4* __restrict__ lent, const void* __restrict__ rowdata,
+    const int4* __restrict__ lent, const int32_t* __restrict__ dinv,
     const int* __restrict__ lev_ptr, const int* __restrict__ lev_eptr,
-    const int* __restrict__ chain_lev, int rmask, int maxlev, int bit, int dbit,
+    const int* __restrict__ chain_lev, int rmask, int bit, int dbit,
     const int32_t* __restrict__ in, int32_t* out) {
   constexpr int kThreads = 64 * kWaves;
   extern __shared__ __align__(16) unsigned char smem_e[];
@@ -1043,41 +1012,18 @@ __global__ __launch_bounds__(64 * kWaves) void k_tri_chain_ep(
   sh.ring = reinterpret_cast<int32_t*>(sh.acc + kEpMaxLevel);
   sh.rmask = rmask;
   int2* lvo = reinterpret_cast<int2*>(sh.ring + rmask + 1);
-  int32_t* inl = reinterpret_cast<int32_t*>(lvo + maxlev + 1);
   sh.lvo = lvo;
-  sh.inl = inl;
   const int t = threadIdx.x;
   const int lb = chain_lev[2 * blockIdx.x], le = chain_lev[2 * blockIdx.x + 1];
   const int nl = le - lb;
   for (int k = t; k < kEpMaxLevel; k += kThreads) sh.acc[k] = 0;
   for (int k = t; k <= nl; k += kThreads) lvo[k] = make_int2(lev_ptr[lb + k], lev_eptr[lb + k]);
-  sh.pos0 = lev_ptr[lb];
-  if (kStaged) {
-    // Gather in batches of 8 per thread, so that the latency of the two
-    // dependent loads is paid once per batch.
-    const int n_c = lev_ptr[le] - sh.pos0;
-    const int2* rd = static_cast<const int2*>(rowdata) + sh.pos0;
-    for (int k0 = t; k0 < n_c; k0 += 8 * kThreads) {
-      int rw[8];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int k = k0 + j * kThreads;
-        rw[j] = rd[k < n_c ? k : 0].x;
-      }
-      int32_t v[8];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) v[j] = in[rw[j]];
-#pragma unroll
-      for (int j = 0; j < 8; ++j)
-        if (k0 + j * kThreads < n_c) inl[k0 + j * kThreads] = v[j];
-    }
-  }
   __syncthreads();
   EpPre<kE, kR> buf[kEpDepth];
 #pragma unroll
   for (int d = 0; d < kEpDepth; ++d) {
     const int Lc = min(d, nl);
-    ep_load<kThreads, kStaged>(buf[d], lvo[Lc], lvo[min(Lc + 1, nl)], lent, rowdata);
+    ep_load<kThreads>(buf[d], lvo[Lc], lvo[min(Lc + 1, nl)], lent, dinv, in);
   }
   // Dependencies stay inside a chain, so its first kEpFarLevels levels have
   // no far entries: the first far loads are issued from the loop.
@@ -1085,63 +1031,42 @@ __global__ __launch_bounds__(64 * kWaves) void k_tri_chain_ep(
   for (; L0 + kEpDepth <= nl; L0 += kEpDepth) {
 #pragma unroll
     for (int d = 0; d < kEpDepth; ++d)
-      ep_level<K, kWaves, kFar, kStaged>(buf[d], buf[(d + kEpFarAhead) % kEpDepth], L0 + d, nl,
-                                         sh, lent, rowdata, in, out, bit, dbit);
+      ep_level<K, kWaves, kFar>(buf[d], buf[(d + kEpFarAhead) % kEpDepth], L0 + d, nl, sh, lent,
+                                dinv, in, out, bit, dbit);
   }
 #pragma unroll
   for (int d = 0; d < kEpDepth; ++d)
     if (L0 + d < nl)
-      ep_level<K, kWaves, kFar, kStaged>(buf[d], buf[(d + kEpFarAhead) % kEpDepth], L0 + d, nl,
-                                         sh, lent, rowdata, in, out, bit, dbit);
+      ep_level<K, kWaves, kFar>(buf[d], buf[(d + kEpFarAhead) % kEpDepth], L0 + d, nl, sh, lent,
+                                dinv, in, out, bit, dbit);
 }
 
-// rows[idx] = {in[lrow[idx]], ldiag[idx], lrow[idx], 0}
-__global__ __launch_bounds__(256) void k_gather_rows(int n, const int* __restrict__ lrow,
-                                                     const int32_t* __restrict__ ldiag,
-                                                     const int32_t* __restrict__ in,
-                                                     int4* __restrict__ rows) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < n) {
-    const int r = lrow[i];
-    rows[i] = make_int4(in[r], ldiag[i], r, 0);
-  }
-}
-
-template <TriKind K, int kWaves, bool kFar, bool kStaged, int kE, int kR>
+template <TriKind K, int kWaves, bool kFar, int kE, int kR>
 hipError_t tri_ep_go(const TriLayout& L, const TriSegment& seg, int bit, int dbit,
-                     const void* rowdata, const int32_t* in, int32_t* out, hipStream_t s) {
-  const size_t lds = ep_lds_bytes(L.ring, L.ep_maxlev, kStaged ? L.ep_maxrows : 0);
+                     const int32_t* in, int32_t* out, hipStream_t s) {
+  const size_t lds = ep_lds_bytes(L.ring, L.ep_maxlev);
   if (lds > static_cast<size_t>(kTriBlockedMaxLds)) return hipErrorInvalidValue;
   static bool attr = false;
   if (!attr) {
     hipError_t e = hipFuncSetAttribute(
-        reinterpret_cast<const void*>(&k_tri_chain_ep<K, kWaves, kFar, kStaged, kE, kR>),
+        reinterpret_cast<const void*>(&k_tri_chain_ep<K, kWaves, kFar, kE, kR>),
         hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kTriBlockedMaxLds));
     if (e != hipSuccess) return e;
     attr = true;
   }
-  hipLaunchKernelGGL((k_tri_chain_ep<K, kWaves, kFar, kStaged, kE, kR>), dim3(seg.nchain),
-                     dim3(64 * kWaves), lds, s, reinterpret_cast<const int4*>(L.lent), rowdata,
-                     L.lev_ptr, L.lev_eptr, seg.chain_lev, L.ring - 1, L.ep_maxlev, bit, dbit,
-                     in, out);
+  hipLaunchKernelGGL((k_tri_chain_ep<K, kWaves, kFar, kE, kR>), dim3(seg.nchain),
+                     dim3(64 * kWaves), lds, s, reinterpret_cast<const int4*>(L.lent),
+                     static_cast<const int32_t*>(L.ldiag), L.lev_ptr, L.lev_eptr, seg.chain_lev,
+                     L.ring - 1, bit, dbit, in, out);
   return hipGetLastError();
 }
 
 template <TriKind K, int kWaves>
 hipError_t tri_ep_launch(const TriLayout& L, const TriSegment& seg, int bit, int dbit,
-                         const int32_t* in, int32_t* out, void* rows_scratch, hipStream_t s) {
-  if (L.ep_staged) {
-    if (L.ep_narrow) return tri_ep_go<K, kWaves, false, true, 4, 1>(L, seg, bit, dbit, L.lrd, in, out, s);
-    return tri_ep_go<K, kWaves, false, true, 8, 2>(L, seg, bit, dbit, L.lrd, in, out, s);
-  }
-  int4* rows = static_cast<int4*>(rows_scratch);
-  if (L.n > 0)
-    hipLaunchKernelGGL(k_gather_rows, dim3((L.n + 255) / 256), dim3(256), 0, s, L.n, L.lrow,
-                       static_cast<const int32_t*>(L.ldiag), in, rows);
-  const hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  if (L.ep_far) return tri_ep_go<K, kWaves, true, false, 8, 2>(L, seg, bit, dbit, rows, in, out, s);
-  return tri_ep_go<K, kWaves, false, false, 8, 2>(L, seg, bit, dbit, rows, in, out, s);
+                         const int32_t* in, int32_t* out, hipStream_t s) {
+  if (L.ep_far) return tri_ep_go<K, kWaves, true, 8, 2>(L, seg, bit, dbit, in, out, s);
+  if (L.ep_narrow) return tri_ep_go<K, kWaves, false, 4, 1>(L, seg, bit, dbit, in, out, s);
+  return tri_ep_go<K, kWaves, false, 8, 2>(L, seg, bit, dbit, in, out, s);
 }
 
 // One level, many workgroups, level-ordered layout.
@@ -1317,28 +1242,26 @@ hipError_t launch_trsv_layout(Elem e, TriKind kind, const TriLayout& L,
 }
 
 hipError_t launch_trsv_ep(TriKind kind, const TriLayout& L, const TriSegment& seg, int bit,
-                          int dbit, const void* in_v, void* out_v, void* rows_scratch,
-                          hipStream_t s) {
+                          int dbit, const void* in_v, void* out_v, hipStream_t s) {
   const int32_t* in = static_cast<const int32_t*>(in_v);
   int32_t* o = static_cast<int32_t*>(out_v);
-  void* rs = rows_scratch;
   const int w = L.ep_waves;
   if (kind == TriKind::kLD) {
-    if (w <= 1) return tri_ep_launch<TriKind::kLD, 1>(L, seg, bit, dbit, in, o, rs, s);
-    if (w <= 2) return tri_ep_launch<TriKind::kLD, 2>(L, seg, bit, dbit, in, o, rs, s);
-    if (w <= 4) return tri_ep_launch<TriKind::kLD, 4>(L, seg, bit, dbit, in, o, rs, s);
-    return tri_ep_launch<TriKind::kLD, 8>(L, seg, bit, dbit, in, o, rs, s);
+    if (w <= 1) return tri_ep_launch<TriKind::kLD, 1>(L, seg, bit, dbit, in, o, s);
+    if (w <= 2) return tri_ep_launch<TriKind::kLD, 2>(L, seg, bit, dbit, in, o, s);
+    if (w <= 4) return tri_ep_launch<TriKind::kLD, 4>(L, seg, bit, dbit, in, o, s);
+    return tri_ep_launch<TriKind::kLD, 8>(L, seg, bit, dbit, in, o, s);
   }
   if (kind == TriKind::kL) {
-    if (w <= 1) return tri_ep_launch<TriKind::kL, 1>(L, seg, bit, dbit, in, o, rs, s);
-    if (w <= 2) return tri_ep_launch<TriKind::kL, 2>(L, seg, bit, dbit, in, o, rs, s);
-    if (w <= 4) return tri_ep_launch<TriKind::kL, 4>(L, seg, bit, dbit, in, o, rs, s);
-    return tri_ep_launch<TriKind::kL, 8>(L, seg, bit, dbit, in, o, rs, s);
+    if (w <= 1) return tri_ep_launch<TriKind::kL, 1>(L, seg, bit, dbit, in, o, s);
+    if (w <= 2) return tri_ep_launch<TriKind::kL, 2>(L, seg, bit, dbit, in, o, s);
+    if (w <= 4) return tri_ep_launch<TriKind::kL, 4>(L, seg, bit, dbit, in, o, s);
+    return tri_ep_launch<TriKind::kL, 8>(L, seg, bit, dbit, in, o, s);
   }
-  if (w <= 1) return tri_ep_launch<TriKind::kDU, 1>(L, seg, bit, dbit, in, o, rs, s);
-  if (w <= 2) return tri_ep_launch<TriKind::kDU, 2>(L, seg, bit, dbit, in, o, rs, s);
-  if (w <= 4) return tri_ep_launch<TriKind::kDU, 4>(L, seg, bit, dbit, in, o, rs, s);
-  return tri_ep_launch<TriKind::kDU, 8>(L, seg, bit, dbit, in, o, rs, s);
+  if (w <= 1) return tri_ep_launch<TriKind::kDU, 1>(L, seg, bit, dbit, in, o, s);
+  if (w <= 2) return tri_ep_launch<TriKind::kDU, 2>(L, seg, bit, dbit, in, o, s);
+  if (w <= 4) return tri_ep_launch<TriKind::kDU, 4>(L, seg, bit, dbit, in, o, s);
+  return tri_ep_launch<TriKind::kDU, 8>(L, seg, bit, dbit, in, o, s);
 }
 
 hipError_t launch_trsv_segment(Elem e, TriKind kind, const DCsr& a,
