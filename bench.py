@@ -164,6 +164,75 @@ def cpu_baseline(args, unscaled_arrays, b_unscaled, smoother):
     return None
 
 
+def run_batch(args):
+    """Batched uneven systems: `--systems` 2D convection-diffusion systems
+    (5-point first-order upwind; side uniform in [32, 256], Peclet in [1, 100],
+    flow angle in [0, 2 pi) drawn from seed 3), 4 right-hand sides each
+    (b = A x*, x* ~ U[-1,1]), int-AMG-FGMRES(20) to rtol.  Reports systems per
+    second for all solves issued at once (each on its own stream) against the
+    same solves one at a time, and checks that both give identical x."""
+    import sc23ad as S
+    K, R = args.systems, 4
+    ns, pes, angs = S.convdiff_draws(K, 3)
+    t0 = time.time()
+    prm = S.amg_param(args.smoother)
+    solvers, rhs, nrms, amgs = [], [], [], []
+    rows = 0
+    for i in range(K):
+        a = S.CsrMat.generate(S.GEN_CONVDIFF2D, int(ns[i]), float(pes[i]), float(angs[i]))
+        rptr, cind, val = a.arrays()
+        rmax = np.maximum.reduceat(np.abs(val), rptr[:-1])
+        bs = [a.matvec(S.xstar(a.n, 1000 * (i + 1) + k)) for k in range(R)]
+        a.lmax1_scaling(bs[0].copy())
+        bs = [b / rmax for b in bs]
+        hi, hd = S.Hierarchy.build_multi(a, [S.INT, S.FP64], prm)
+        am, ad = S.Amg(hi), S.Amg(hd)
+        for b in bs:
+            nrm = float(np.sqrt(np.add.accumulate(b * b)[-1]))
+            bit = ad.default_frac_bits(b, nrm)
+            solvers.append(S.Solver(S.INT, a, am, m=args.m, eps=args.rtol, bit=bit))
+            rhs.append(b)
+            nrms.append(nrm)
+        amgs.append((am, ad, a))
+        rows += a.n
+        if (i + 1) % 32 == 0:
+            log(f"batch setup: {i + 1}/{K} systems ({time.time() - t0:.1f} s)")
+    log(f"batch: {K} systems, {rows} rows, {len(solvers)} solves; setup {time.time() - t0:.1f} s")
+    # warm-up (first launches, code objects)
+    S.solve_batch(solvers[:16], rhs[:16], nrms[:16], threads=args.threads)
+    t = time.perf_counter()
+    xs_seq, cvs = [], []
+    for s_, b, nb in zip(solvers, rhs, nrms):
+        x, cv = s_.solve(b, nb)
+        xs_seq.append(x)
+        cvs.append(cv)
+    t_seq = time.perf_counter() - t
+    log(f"one at a time: {t_seq:.2f} s")
+    t = time.perf_counter()
+    xs_bat, cvb = S.solve_batch(solvers, rhs, nrms, threads=args.threads)
+    t_bat = time.perf_counter() - t
+    log(f"all at once: {t_bat:.2f} s")
+    same = all(np.array_equal(p.view(np.uint64), q.view(np.uint64)) for p, q in zip(xs_seq, xs_bat))
+    conv = sum(c.is_converged for c in cvb)
+    iters = [c.iter for c in cvb]
+    rec = {
+        "metric": "batched uneven systems: systems per second (all at once vs one at a time)",
+        "value": K / t_bat, "unit": "systems/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": t_bat * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32/int64 (FP64 outer refinement)",
+        "data": "synthetic (2D convection-diffusion family, seed 3)",
+        "config": {"workload": f"{K} 2D convection-diffusion systems (side 32..256), {R} RHS each, "
+                               f"int-AMG-FGMRES({args.m}), rtol {args.rtol}",
+                   "systems": K, "rhs_per_system": R, "rows_total": rows,
+                   "threads": args.threads or 16},
+        "one_at_a_time_systems_per_s": K / t_seq,
+        "speedup_vs_one_at_a_time": t_seq / t_bat,
+        "identical_results": bool(same), "converged": int(conv), "solves": len(cvb),
+        "iterations_mean": float(np.mean(iters)), "iterations_max": int(np.max(iters)),
+    }
+    print(json.dumps(rec), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__,
                                  formatter_class=argparse.RawDescriptionHelpFormatter)
@@ -180,7 +249,14 @@ def main():
     ap.add_argument("--cpu-timeout", type=float, default=400.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run the copy kernel (MIB MiB per launch) for counter calibration")
+    ap.add_argument("--workload", default="laplace3d", choices=("laplace3d", "batch"),
+                    help="laplace3d: the 3D 7-point solve (default); batch: uneven 2D "
+                         "convection-diffusion systems solved at once")
+    ap.add_argument("--systems", type=int, default=256, help="batch: number of systems")
+    ap.add_argument("--threads", type=int, default=16, help="batch: host threads")
     args = ap.parse_args()
+    if args.workload == "batch" and args.m == 30 and "--m" not in sys.argv:
+        args.m = 20
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path "
@@ -194,6 +270,9 @@ def main():
         sys.exit(3)
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy, 9)
+        return
+    if args.workload == "batch":
+        run_batch(args)
         return
 
     n = args.n

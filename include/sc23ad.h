@@ -184,6 +184,15 @@ void sc23ad_solver_destroy(sc23ad_solver* s);
    selects device pointers for b and x (FP64, fine size). */
 int sc23ad_solver_solve(sc23ad_solver* s, const double* b, double nrm_b,
                         double* x, int on_device, sc23ad_converge* out);
+/* Several independent solves at once (no counterpart in the reference, whose
+   drivers solve one system at a time: Work/main.cpp:167-169).  Solver k runs
+   on its own stream with its own work buffers, taken by one of `threads` host
+   threads (0: min(count, 16)); solvers may share a hierarchy (several
+   right-hand sides).  b[k], x[k]: host or device pointers (on_device);
+   out[k] (optional) receives solve k's record.  Each solver may appear once. */
+int sc23ad_solve_batch(int count, sc23ad_solver* const* solvers, const double* const* b,
+                       const double* nrm_b, double* const* x, int on_device, int threads,
+                       sc23ad_converge* out);
 /* FP reduction order of the solver's norms and inner products:
    0 = the fixed blocked order (default, 1024 x 256 partial sums, halving
    trees); 1 = strictly left to right in one thread, the order of the

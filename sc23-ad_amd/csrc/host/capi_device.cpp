@@ -1,7 +1,12 @@
 // Device half of the C ABI: device buffers, the raw integer SpMV, the
 // device-resident preconditioner and the solver.
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
 #include <memory>
 #include <new>
 
@@ -154,15 +159,15 @@ int sc23ad_amg_precondition(sc23ad_amg* m, const void* in, void* out, int on_dev
     DevAmg& a = *m->d;
     const size_t bytes = a.esize() * static_cast<size_t>(a.n0());
     if (on_device) {
-      a.vcycle(in, out);
+      a.vcycle(in, out, a.own_work());
     } else {
       DevBuf di(bytes), dout(bytes);
       di.upload(in, bytes);
-      a.vcycle(di.get(), dout.get());
+      a.vcycle(di.get(), dout.get(), a.own_work());
       dout.download(out, bytes);
     }
     hip_check(hipStreamSynchronize(lib_stream()), "sync");
-    a.check_errors();
+    a.check_errors(a.own_work());
     return SC23AD_OK;
   });
 }
@@ -178,9 +183,9 @@ int sc23ad_amg_level_op(sc23ad_amg* m, int level, int op, const void* in, void* 
       if (op < SC23AD_OP_SPMV_A || op > SC23AD_OP_SMOOTH1)
         return fail(SC23AD_BAD_PARAM, "level_op: bad op");
     }
-    a.level_op(level, op, in, out);
+    a.level_op(level, op, in, out, a.own_work());
     hip_check(hipStreamSynchronize(lib_stream()), "sync");
-    a.check_errors();
+    a.check_errors(a.own_work());
     return SC23AD_OK;
   });
 }
@@ -208,11 +213,16 @@ int sc23ad_solver_create(int precision, const sc23ad_csrmat* a, sc23ad_amg* m,
 
 void sc23ad_solver_destroy(sc23ad_solver* s) { delete s; }
 
-int sc23ad_solver_solve(sc23ad_solver* s, const double* b, double nrm_b, double* x,
-                        int on_device, sc23ad_converge* out) {
+namespace {
+
+// One solve; host pointers are staged through the solver's buffers on its
+// own stream, so that solves of different solvers can run concurrently.
+int solve_one(sc23ad_solver* s, const double* b, double nrm_b, double* x, int on_device,
+              sc23ad_converge* out) {
   if (!s || !b || !x) return fail(SC23AD_BAD_PARAM, "solve: null");
   return guarded([&]() -> int {
     DevSolver& d = *s->s;
+    StreamScope scope(d.stream);
     const int n = d.amg->n0();
     const size_t bytes = sizeof(double) * n;
     DevSolver::Result r;
@@ -239,6 +249,43 @@ int sc23ad_solver_solve(sc23ad_solver* s, const double* b, double nrm_b, double*
     }
     return SC23AD_OK;
   });
+}
+
+}  // namespace
+
+int sc23ad_solver_solve(sc23ad_solver* s, const double* b, double nrm_b, double* x,
+                        int on_device, sc23ad_converge* out) {
+  return solve_one(s, b, nrm_b, x, on_device, out);
+}
+
+int sc23ad_solve_batch(int count, sc23ad_solver* const* solvers, const double* const* b,
+                       const double* nrm_b, double* const* x, int on_device, int threads,
+                       sc23ad_converge* out) {
+  if (count < 0 || (count > 0 && (!solvers || !b || !nrm_b || !x)))
+    return fail(SC23AD_BAD_PARAM, "solve_batch: bad arguments");
+  if (count == 0) return SC23AD_OK;
+  for (int k = 0; k < count; ++k)
+    for (int q = 0; q < k; ++q)
+      if (solvers[k] == solvers[q])
+        return fail(SC23AD_BAD_PARAM, "solve_batch: a solver appears twice");
+  int nt = threads > 0 ? threads : 16;
+  nt = std::max(1, std::min(nt, count));
+  std::atomic<int> next{0};
+  std::vector<int> rc(count, SC23AD_OK);
+  std::vector<std::string> msg(count);
+  auto worker = [&]() {
+    for (int k = next.fetch_add(1); k < count; k = next.fetch_add(1)) {
+      rc[k] = solve_one(solvers[k], b[k], nrm_b[k], x[k], on_device, out ? out + k : nullptr);
+      if (rc[k] != SC23AD_OK) msg[k] = g_last_error;
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; ++t) pool.emplace_back(worker);
+  worker();
+  for (auto& th : pool) th.join();
+  for (int k = 0; k < count; ++k)
+    if (rc[k] != SC23AD_OK) return fail(rc[k], "solve_batch: system " + std::to_string(k) + ": " + msg[k]);
+  return SC23AD_OK;
 }
 
 int sc23ad_solver_trace(sc23ad_solver* s, uint64_t* buf, int cap, int* len) {
@@ -329,7 +376,8 @@ int sc23ad_default_frac_bits(sc23ad_amg* m, const double* b, double nrm_b, int* 
     const int n = a.n0();
     DevBuf db(sizeof(double) * n), dc(sizeof(double) * n);
     db.upload(b, sizeof(double) * n);
-    a.vcycle(db.get(), dc.get());
+    a.vcycle(db.get(), dc.get(), a.own_work());
+    a.check_errors(a.own_work());
     std::vector<double> c(n);
     dc.download(c.data(), sizeof(double) * n);
     double c_max = 0;

@@ -17,7 +17,15 @@ void hip_check(hipError_t e, const char* what) {
     throw DevError{5, std::string(what) + ": " + hipGetErrorString(e)};
 }
 
+namespace {
+thread_local hipStream_t tl_stream = nullptr;
+}
+
+StreamScope::StreamScope(hipStream_t s) : prev_(tl_stream) { tl_stream = s; }
+StreamScope::~StreamScope() { tl_stream = prev_; }
+
 hipStream_t lib_stream() {
+  if (tl_stream) return tl_stream;
   static hipStream_t s = nullptr;
   if (!s) {
     int n = 0;
@@ -512,11 +520,6 @@ void build_tiles(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rpt
   plan.boff.upload(toff.data(), sizeof(long long) * toff.size());
   plan.ddinv.alloc(sizeof(int32_t) * dv.size());
   plan.ddinv.upload(dv.data(), sizeof(int32_t) * dv.size());
-  std::vector<int> zero(static_cast<size_t>(nb) + 1, 0);
-  plan.flags.alloc(sizeof(int) * zero.size());
-  plan.flags.upload(zero.data(), sizeof(int) * zero.size());
-  plan.err.alloc(sizeof(int));
-  plan.err.upload(zero.data(), sizeof(int));
   plan.nb = nb;
   plan.dense = true;
 }
@@ -681,24 +684,34 @@ void DevAmg::tune() {
     for (DevMat* m : {&L.a, &L.u, &L.r, &L.p}) tune_spmv(*m, elem);
 }
 
-void DevAmg::alloc_work() {
+std::unique_ptr<DevAmg::Work> DevAmg::make_work() const {
+  auto w = std::make_unique<Work>();
   const size_t es = esize();
-  ee.resize(depth + 1);
-  xx.resize(depth + 1);
-  rr.resize(depth + 1);
-  tmp.resize(depth + 1);
+  w->ee.resize(depth + 1);
+  w->xx.resize(depth + 1);
+  w->rr.resize(depth + 1);
+  w->tmp.resize(depth + 1);
   for (int l = 1; l <= depth; ++l) {
     const size_t n = static_cast<size_t>(std::max(level_n(l), 1));
-    ee[l].alloc(es * n);
-    xx[l].alloc(es * n);
-    rr[l].alloc(es * n);
-    tmp[l].alloc(es * n);
+    w->ee[l].alloc(es * n);
+    w->xx[l].alloc(es * n);
+    w->rr[l].alloc(es * n);
+    w->tmp[l].alloc(es * n);
   }
-  r0.alloc(es * std::max(n0(), 1));
-  vin0.alloc(es * std::max(n0(), 1));
-  vout0.alloc(es * std::max(n0(), 1));
-  tmp[0].alloc(es * std::max(n0(), 1));
-  tmpc.alloc(es * std::max(n_coarse, 1));
+  const size_t n0s = static_cast<size_t>(std::max(n0(), 1));
+  w->r0.alloc(es * n0s);
+  w->vin0.alloc(es * n0s);
+  w->vout0.alloc(es * n0s);
+  w->tmp[0].alloc(es * n0s);
+  w->tmpc.alloc(es * std::max(n_coarse, 1));
+  std::vector<int> zero(static_cast<size_t>(std::max({lc_plan.nb, uc_plan.nb, 1})) + 1, 0);
+  for (DevBuf* f : {&w->lflags, &w->uflags}) {
+    f->alloc(sizeof(int) * zero.size());
+    f->upload(zero.data(), sizeof(int) * zero.size());
+  }
+  w->err.alloc(sizeof(int));
+  w->err.upload(zero.data(), sizeof(int));
+  return w;
 }
 
 std::unique_ptr<DevAmg> DevAmg::upload(const IntHierarchy& h, const LaunchCfg& c) {
@@ -726,7 +739,7 @@ std::unique_ptr<DevAmg> DevAmg::upload(const IntHierarchy& h, const LaunchCfg& c
     build_tiles(a->lc_plan, TriKind::kL, h.lc.n, h.lc.rptr, h.lc.cind, h.lc.val);
     build_tiles(a->uc_plan, TriKind::kDU, h.uc.n, h.uc.rptr, h.uc.cind, h.uc.val);
   }
-  a->alloc_work();
+  a->work_ = a->make_work();
   a->tune();
   return a;
 }
@@ -755,7 +768,7 @@ std::unique_ptr<DevAmg> DevAmg::upload_fp(const FpHierarchy<T>& h, const LaunchC
              c.single_wg_max_rows);
   build_blocked(a->lc_plan, TriKind::kL, h.lc.n, h.lc.rptr, h.lc.cind, sizeof(T), sizeof(T));
   build_blocked(a->uc_plan, TriKind::kDU, h.uc.n, h.uc.rptr, h.uc.cind, sizeof(T), sizeof(T));
-  a->alloc_work();
+  a->work_ = a->make_work();
   a->tune();
   return a;
 }
@@ -770,14 +783,14 @@ void DevAmg::spmv(const DevMat& m, SpmvMode mode, const void* x, const void* z,
   hip_check(dev::launch_spmv(elem, m.d, mode, x, z, y, lib_stream()), "spmv");
 }
 
-void DevAmg::trsv(const TriPlan& plan, const DevMat& m, const void* in,
-                  void* out) const {
-  if (plan.dense && elem == Elem::kI32) {
-    plan.epoch = plan.epoch == INT32_MAX ? 1 : plan.epoch + 1;
+void DevAmg::trsv(const TriPlan& plan, const DevMat& m, const void* in, void* out,
+                  int* flags, int* epoch, int* err) const {
+  if (plan.dense && elem == Elem::kI32 && flags) {
+    *epoch = *epoch == INT32_MAX ? 1 : *epoch + 1;
     hip_check(dev::launch_trsv_tiles(plan.kind, m.d.n, plan.nb, plan.slice.as<int32_t>(),
                                      plan.boff.as<long long>(), plan.ddinv.as<int32_t>(),
-                                     m.d.bit, m.d.dbit, in, out, plan.flags.as<int>(),
-                                     plan.epoch, plan.err.as<int>(), lib_stream()),
+                                     m.d.bit, m.d.dbit, in, out, flags, *epoch, err,
+                                     lib_stream()),
               "trsv tiles");
     return;
   }
@@ -802,54 +815,52 @@ void DevAmg::trsv(const TriPlan& plan, const DevMat& m, const void* in,
 // SOR / BlockSOR Smooth (Library/smoother/stationary.hpp:415-430, 997-1012):
 // inited: iter x { t = in - U out; out = LD^-1 t }; otherwise out = LD^-1 in
 // first.
-void DevAmg::smooth(int l, const void* in, void* out, bool inited) const {
+void DevAmg::smooth(int l, const void* in, void* out, bool inited, Work& w) const {
   const DevLevel& L = lv[l];
   int k0 = 0;
   EventTimer* tm = l == 0 ? &trsv0_timer : nullptr;
   if (!inited) {
     if (tm) tm->begin();
-    trsv(L.ld_plan, L.ld, in, out);
+    trsv(L.ld_plan, L.ld, in, out, nullptr, nullptr, nullptr);
     if (tm) tm->end();
     k0 = 1;
   }
-  void* t = tmp[l].get();
+  void* t = w.tmp[l].get();
   for (int k = k0; k < iter; ++k) {
     spmv(L.u, SpmvMode::kSub, out, in, t);
     if (tm) tm->begin();
-    trsv(L.ld_plan, L.ld, t, out);
+    trsv(L.ld_plan, L.ld, t, out, nullptr, nullptr, nullptr);
     if (tm) tm->end();
   }
 }
 
-void DevAmg::check_errors() const {
-  for (const TriPlan* p : {&lc_plan, &uc_plan}) {
-    if (!p->dense) continue;
-    int e = 0;
-    hip_check(hipMemcpyAsync(&e, p->err.get(), sizeof(int), hipMemcpyDeviceToHost, lib_stream()),
-              "error flag");
-    hip_check(hipStreamSynchronize(lib_stream()), "sync");
-    if (e != 0) throw DevError{5, "coarse solve: a block waited too long for its dependencies"};
-  }
+void DevAmg::check_errors(const Work& w) const {
+  if (!lc_plan.dense && !uc_plan.dense) return;
+  int e = 0;
+  hip_check(hipMemcpyAsync(&e, w.err.get(), sizeof(int), hipMemcpyDeviceToHost, lib_stream()),
+            "error flag");
+  hip_check(hipStreamSynchronize(lib_stream()), "sync");
+  if (e != 0) throw DevError{5, "coarse solve: a block waited too long for its dependencies"};
 }
 
 // direct::LUflex / LU<T>::Solve (Library/solver/lu.hpp:91-94, 33-36).
-void DevAmg::coarse(const void* in, void* out) const {
-  trsv(lc_plan, lc, in, tmpc.get());
-  trsv(uc_plan, uc, tmpc.get(), out);
+void DevAmg::coarse(const void* in, void* out, Work& w) const {
+  trsv(lc_plan, lc, in, w.tmpc.get(), w.lflags.as<int>(), &w.lepoch, w.err.as<int>());
+  trsv(uc_plan, uc, w.tmpc.get(), out, w.uflags.as<int>(), &w.uepoch, w.err.as<int>());
 }
 
 // intAMGflex::Precondition / dfAMG::Precondition (INTAMG.hpp:269-293,
 // 108-133).  Coarse work vectors are indexed by level (1..depth).
-void DevAmg::vcycle(const void* in, void* out) const {
+void DevAmg::vcycle(const void* in, void* out, Work& w) const {
   const int n = n0();
-  hip_check(dev::launch_permute(elem, n, perm_of(0), in, vin0.get(), true, lib_stream()),
+  hip_check(dev::launch_permute(elem, n, perm_of(0), in, w.vin0.get(), true, lib_stream()),
             "permute in");
-  vcycle_lo(vin0.get(), vout0.get());
-  hip_check(dev::launch_permute(elem, n, perm_of(0), vout0.get(), out, false, lib_stream()),
+  vcycle_lo(w.vin0.get(), w.vout0.get(), w);
+  hip_check(dev::launch_permute(elem, n, perm_of(0), w.vout0.get(), out, false, lib_stream()),
             "permute out");
 }
 
-void DevAmg::level_op(int level, int op, const void* in, void* out) const {
+void DevAmg::level_op(int level, int op, const void* in, void* out, Work& w) const {
   // Numbering of the operand and of the result (nullptr: original order).
   const int* pin = op == 7 ? nullptr : op == 3 ? perm_of(level + 1) : perm_of(level);
   const int* pout = op == 7 ? nullptr : op == 2 ? perm_of(level + 1) : perm_of(level);
@@ -864,7 +875,7 @@ void DevAmg::level_op(int level, int op, const void* in, void* out) const {
   to_level(pin, nin, in, ti.get());
   if (op == 6) to_level(pout, nout, out, to.get());  // Smooth(true) reads out too
   if (op == 7) {
-    coarse(ti.get(), to.get());
+    coarse(ti.get(), to.get(), w);
   } else {
     const DevLevel& L = lv[level];
     switch (op) {
@@ -872,9 +883,9 @@ void DevAmg::level_op(int level, int op, const void* in, void* out) const {
       case 1: spmv(L.u, SpmvMode::kPlain, ti.get(), nullptr, to.get()); break;
       case 2: spmv(L.r, SpmvMode::kPlain, ti.get(), nullptr, to.get()); break;
       case 3: spmv(L.p, SpmvMode::kPlain, ti.get(), nullptr, to.get()); break;
-      case 4: trsv(L.ld_plan, L.ld, ti.get(), to.get()); break;
-      case 5: smooth(level, ti.get(), to.get(), false); break;
-      case 6: smooth(level, ti.get(), to.get(), true); break;
+      case 4: trsv(L.ld_plan, L.ld, ti.get(), to.get(), nullptr, nullptr, nullptr); break;
+      case 5: smooth(level, ti.get(), to.get(), false, w); break;
+      case 6: smooth(level, ti.get(), to.get(), true, w); break;
       default: throw DevError{1, "level_op: bad op"};
     }
   }
@@ -883,23 +894,23 @@ void DevAmg::level_op(int level, int op, const void* in, void* out) const {
 }
 
 // intAMGflex::Precondition / dfAMG::Precondition on level-ordered vectors.
-void DevAmg::vcycle_lo(const void* in, void* out) const {
-  smooth(0, in, out, false);
-  spmv(lv[0].a, SpmvMode::kSub, out, in, r0.get());  // r = in - A out
+void DevAmg::vcycle_lo(const void* in, void* out, Work& w) const {
+  smooth(0, in, out, false, w);
+  spmv(lv[0].a, SpmvMode::kSub, out, in, w.r0.get());  // r = in - A out
   if (depth == 0) return;
-  spmv(lv[0].r, SpmvMode::kPlain, r0.get(), nullptr, ee[1].get());
+  spmv(lv[0].r, SpmvMode::kPlain, w.r0.get(), nullptr, w.ee[1].get());
   for (int l = 1; l < depth; ++l) {
-    smooth(l, ee[l].get(), xx[l].get(), false);
-    spmv(lv[l].a, SpmvMode::kSub, xx[l].get(), ee[l].get(), rr[l].get());
-    spmv(lv[l].r, SpmvMode::kPlain, rr[l].get(), nullptr, ee[l + 1].get());
+    smooth(l, w.ee[l].get(), w.xx[l].get(), false, w);
+    spmv(lv[l].a, SpmvMode::kSub, w.xx[l].get(), w.ee[l].get(), w.rr[l].get());
+    spmv(lv[l].r, SpmvMode::kPlain, w.rr[l].get(), nullptr, w.ee[l + 1].get());
   }
-  coarse(ee[depth].get(), xx[depth].get());
+  coarse(w.ee[depth].get(), w.xx[depth].get(), w);
   for (int l = depth - 1; l >= 1; --l) {
-    spmv(lv[l].p, SpmvMode::kAdd, xx[l + 1].get(), xx[l].get(), xx[l].get());
-    smooth(l, ee[l].get(), xx[l].get(), true);
+    spmv(lv[l].p, SpmvMode::kAdd, w.xx[l + 1].get(), w.xx[l].get(), w.xx[l].get());
+    smooth(l, w.ee[l].get(), w.xx[l].get(), true, w);
   }
-  spmv(lv[0].p, SpmvMode::kAdd, xx[1].get(), out, out);
-  smooth(0, in, out, true);
+  spmv(lv[0].p, SpmvMode::kAdd, w.xx[1].get(), out, out);
+  smooth(0, in, out, true, w);
 }
 
 double DevAmg::vcycle_bytes() const {
@@ -927,6 +938,10 @@ double DevAmg::vcycle_bytes() const {
 DevSolver::DevSolver(Elem e, const HostCsr& a, DevAmg* amg_, int m_, double eps_,
                      int bit_, int max_outer_)
     : elem(e), m(m_), eps(eps_), bit(bit_), max_outer(max_outer_), amg(amg_) {
+  lib_stream();  // device set up
+  hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+  StreamScope scope(stream);
+  work = amg->make_work();
   const int n = a.n;
   upload_mat(a64, a.n, a.m, 0, -1, a.rptr, a.cind, a.val, 1);
   const size_t es = amg->esize();
@@ -972,7 +987,10 @@ DevSolver::DevSolver(Elem e, const HostCsr& a, DevAmg* amg_, int m_, double eps_
 }
 
 DevSolver::~DevSolver() {
-  if (host_flags) hipHostFree(host_flags);
+  if (stream) (void)hipStreamSynchronize(stream);
+  if (host_flags) (void)hipHostFree(host_flags);
+  work.reset();
+  if (stream) (void)hipStreamDestroy(stream);
 }
 
 void EventTimer::begin() {
@@ -1023,6 +1041,7 @@ void DevSolver::read_flags() {
 }
 
 void DevSolver::set_trace(int cap) {
+  StreamScope scope(stream);
   if (cap <= 0) {
     tracing = false;
     return;
@@ -1036,14 +1055,18 @@ void DevSolver::set_trace(int cap) {
 }
 
 void DevSolver::read_trace(uint64_t* out, int cap, int* len) {
+  StreamScope scope(stream);
   int l = 0;
-  hip_check(hipMemcpy(&l, k.trace_len, sizeof(int), hipMemcpyDeviceToHost), "trace len");
+  hip_check(hipMemcpyAsync(&l, k.trace_len, sizeof(int), hipMemcpyDeviceToHost, lib_stream()),
+            "trace len");
+  hip_check(hipStreamSynchronize(lib_stream()), "sync");
   const int cnt = std::min(std::min(l, cap), trace_cap);
   if (cnt > 0) pinned_dummy.download(out, sizeof(uint64_t) * cnt);
   *len = cnt;
 }
 
 DevSolver::Result DevSolver::solve(const double* b_dev, double nrm_b, double* x_dev) {
+  StreamScope scope(stream);
   Result res;
   hipStream_t s = lib_stream();
   const int n = amg->n0();
@@ -1078,7 +1101,7 @@ DevSolver::Result DevSolver::solve(const double* b_dev, double nrm_b, double* x_
       void* vj = Vb + es * static_cast<size_t>(j) * n;
       void* zj = Zb + es * static_cast<size_t>(j) * n;
       void* w = Vb + es * static_cast<size_t>(j + 1) * n;
-      amg->vcycle(vj, zj);
+      amg->vcycle(vj, zj, *work);
       if (tracing && elem == Elem::kI32) {
         hip_check(dev::launch_checksum(n, static_cast<const int32_t*>(vj), k, s), "cs");
         hip_check(dev::launch_checksum(n, static_cast<const int32_t*>(zj), k, s), "cs");
@@ -1126,13 +1149,14 @@ DevSolver::Result DevSolver::solve(const double* b_dev, double nrm_b, double* x_
   res.implicit_relres = sc[5];
   spmv_timer.collect();
   amg->trsv0_timer.collect();
-  amg->check_errors();
+  amg->check_errors(*work);
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   return res;
 }
 
 double DevSolver::relres(const double* b_dev, double nrm_b, const double* x_dev) {
+  StreamScope scope(stream);
   hip_check(dev::launch_relres(a64.d, b_dev, x_dev, r.as<double>(), k.dpart,
                                k.counter + 4, k.scal + 6, lib_stream()),
             "relres");

@@ -22,7 +22,20 @@ struct DevError {
 };
 
 void hip_check(hipError_t e, const char* what);
-hipStream_t lib_stream();  // library stream on device 0 (created on first use)
+// The stream library work goes to: the calling thread's current stream (see
+// StreamScope), else the library's default stream on device 0.
+hipStream_t lib_stream();
+// Makes `s` the calling thread's current stream for the scope's lifetime.
+class StreamScope {
+ public:
+  explicit StreamScope(hipStream_t s);
+  ~StreamScope();
+  StreamScope(const StreamScope&) = delete;
+  StreamScope& operator=(const StreamScope&) = delete;
+
+ private:
+  hipStream_t prev_;
+};
 
 class DevBuf {
  public:
@@ -78,10 +91,10 @@ struct TriPlan {
   bool blocked = false;
   DevBuf bptr;
   int nb = 0;
-  // 64 x 64 tiles, one workgroup per block (integer coarse factors)
+  // 64 x 64 tiles, one workgroup per block (integer coarse factors); the
+  // hand-off flags live with the caller's work context
   bool dense = false;
-  DevBuf slice, boff, ddinv, flags, err;
-  mutable int epoch = 0;  // value the blocks raise their flags to in the last launch
+  DevBuf slice, boff, ddinv;
   // level-ordered layout used by the level-scheduled kernels
   DevBuf lrow, lbeg, lcol, lval, ldiag, ecol, eval, lpos, epos, lent, lev_eptr;
   dev::TriLayout layout;
@@ -128,11 +141,7 @@ class DevAmg {
   DevMat lc, uc;
   TriPlan lc_plan, uc_plan;
   int n_coarse = 0;
-  // work vectors: index = level (1..depth); level 0 uses r0 / temp0
-  std::vector<DevBuf> ee, xx, rr, tmp;
-  DevBuf r0, tmpc;
   DevMat a0;             // fine-level operator in the original numbering (Arnoldi)
-  DevBuf vin0, vout0;    // V-cycle input / output in level-0 order
   LaunchCfg cfg;
   mutable EventTimer trsv0_timer;  // level-0 LD solves (benchmark hook)
 
@@ -144,24 +153,37 @@ class DevAmg {
   int n0() const { return lv.empty() ? 0 : lv[0].n; }
   int level_n(int l) const { return l < depth ? lv[l].n : n_coarse; }
 
+  // Work vectors and coarse-solve hand-off flags of one V-cycle in flight.
+  // Every concurrent user of the hierarchy (a solver, a batch member) owns
+  // one; the hierarchy's own serves the preconditioner and level-op calls.
+  struct Work {
+    std::vector<DevBuf> ee, xx, rr, tmp;  // index = level (1..depth); tmp[0] fine
+    DevBuf r0, tmpc, vin0, vout0;         // fine residual, coarse temp, level-0 order in/out
+    DevBuf lflags, uflags, err;           // tiled coarse solves: block flags, error count
+    int lepoch = 0, uepoch = 0;           // last flag values raised
+  };
+  std::unique_ptr<Work> make_work() const;
+  Work& own_work() const { return *work_; }
+
   void spmv(const DevMat& m, dev::SpmvMode mode, const void* x, const void* z,
             void* y) const;
-  void trsv(const TriPlan& plan, const DevMat& m, const void* in, void* out) const;
-  void smooth(int l, const void* in, void* out, bool inited) const;
-  void coarse(const void* in, void* out) const;
+  void trsv(const TriPlan& plan, const DevMat& m, const void* in, void* out, int* flags,
+            int* epoch, int* err) const;
+  void smooth(int l, const void* in, void* out, bool inited, Work& w) const;
+  void coarse(const void* in, void* out, Work& w) const;
   // One V-cycle on vectors in the original numbering.
-  void vcycle(const void* in, void* out) const;
+  void vcycle(const void* in, void* out, Work& w) const;
   // Level operation (sc23ad_level_op) on vectors in the original numbering.
-  void level_op(int level, int op, const void* in, void* out) const;
+  void level_op(int level, int op, const void* in, void* out, Work& w) const;
   // Algorithmic HBM bytes of one V-cycle (sum over its kernels).
   double vcycle_bytes() const;
   // Throws if a kernel with a bounded wait gave up (after a stream sync).
-  void check_errors() const;
+  void check_errors(const Work& w) const;
 
  private:
-  void alloc_work();
+  std::unique_ptr<Work> work_;
   void tune();  // SpMV launch shapes per matrix
-  void vcycle_lo(const void* in, void* out) const;  // vectors in level order
+  void vcycle_lo(const void* in, void* out, Work& w) const;  // vectors in level order
   const int* perm_of(int level) const {  // nullptr: original order (coarsest grid)
     return level < depth ? lv[level].perm.as<int>() : nullptr;
   }
@@ -182,6 +204,8 @@ class DevSolver {
   DevMat a64;             // FP64 fine operator (outer residual)
   SolverOpts opts;
   DevBuf V, Z, r, b, x, state, pinned_dummy;
+  hipStream_t stream = nullptr;             // this solver's stream (default priority)
+  std::unique_ptr<DevAmg::Work> work;       // its own V-cycle work context
   dev::KState k{};
   int* host_flags = nullptr;  // pinned: flags[0..3], then the implicit residual
   // tracing / timing

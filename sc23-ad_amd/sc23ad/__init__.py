@@ -98,6 +98,9 @@ def _load() -> C.CDLL:
                                           C.POINTER(Converge)]),
         "sc23ad_solver_trace": (C.c_int, [vp, vp, C.c_int, ip]),
         "sc23ad_solver_set_reduction": (C.c_int, [vp, C.c_int]),
+        "sc23ad_solve_batch": (C.c_int, [C.c_int, C.POINTER(vp), C.POINTER(vp),
+                                         C.POINTER(C.c_double), C.POINTER(vp), C.c_int,
+                                         C.c_int, C.POINTER(Converge)]),
         "sc23ad_solver_history": (C.c_int, [vp, vp, C.c_int, ip]),
         "sc23ad_solver_time_spmv": (C.c_int, [vp, C.c_int]),
         "sc23ad_solver_spmv_stats": (C.c_int, [vp, ip, dp]),
@@ -447,6 +450,23 @@ class Solver:
         if getattr(self, "h", None) and self.h.value:
             lib.sc23ad_solver_destroy(self.h)
             self.h = C.c_void_p()
+
+
+def solve_batch(solvers, bs, nrm_bs, xs=None, threads: int = 0):
+    """Solves solver k for right-hand side bs[k] (host arrays), all at once:
+    each solver on its own stream, `threads` host threads (0: up to 16).
+    Returns (xs, converge records)."""
+    k = len(solvers)
+    bs = [np.ascontiguousarray(b, dtype=np.float64) for b in bs]
+    if xs is None:
+        xs = [np.zeros(s.n) for s in solvers]
+    hs = (C.c_void_p * k)(*[s.h.value for s in solvers])
+    bp = (C.c_void_p * k)(*[b.ctypes.data for b in bs])
+    xp = (C.c_void_p * k)(*[x.ctypes.data for x in xs])
+    nb = (C.c_double * k)(*nrm_bs)
+    out = (Converge * k)()
+    check(lib.sc23ad_solve_batch(k, hs, bp, nb, xp, 0, threads, out))
+    return xs, list(out)
 
 
 def spmv_int(rptr, cind, val, bit, x) -> np.ndarray:

@@ -234,3 +234,30 @@ def test_spmv_int_brute_force(shape):
     for bit in (0, 17, 30):
         assert np.array_equal(S.spmv_int(rptr, cind, val, bit, x),
                               _spmv_ref(rptr, cind, val, bit, x))
+
+
+def test_batch_equals_individual_solves():
+    """All solves issued at once (each solver on its own stream, several
+    right-hand sides sharing one hierarchy) give the same x as one at a time."""
+    draws = [(S.GEN_CONVDIFF2D, (40, 20.0, 0.3)), (S.GEN_POISSON2D, (50,)),
+             (S.GEN_LAPLACE3D, (12,))]
+    solvers, rhs, nrms, keep = [], [], [], []
+    for kind, params in draws:
+        a = S.CsrMat.generate(kind, *params)
+        rptr, cind, val = a.arrays()
+        rmax = np.maximum.reduceat(np.abs(val), rptr[:-1])
+        bs = [a.matvec(S.xstar(a.n, 7 + k)) for k in range(3)]
+        a.lmax1_scaling(bs[0].copy())
+        hi = S.Hierarchy(a, S.INT, S.amg_param(S.BLOCK_SOR))
+        amg = S.Amg(hi)
+        keep.append((a, amg))
+        for b in bs:
+            b = b / rmax
+            solvers.append(S.Solver(S.INT, a, amg, m=20, eps=1e-10, bit=28))
+            rhs.append(b)
+            nrms.append(float(np.sqrt(np.add.accumulate(b * b)[-1])))
+    seq = [s.solve(b, nb) for s, b, nb in zip(solvers, rhs, nrms)]
+    xs, cvs = S.solve_batch(solvers, rhs, nrms, threads=4)
+    for (x1, c1), x2, c2 in zip(seq, xs, cvs):
+        assert c1.iter == c2.iter and c1.is_converged == c2.is_converged
+        assert _same(x1, x2)
