@@ -233,6 +233,71 @@ def run_batch(args):
     print(json.dumps(rec), flush=True)
 
 
+def run_learned(args):
+    """Learned per-level omega / bit caps against the reference's defaults on
+    the 3D 27-point variable-coefficient diffusion system (side --n, default
+    96: coefficient exp(2 g) on 8^3-cell blocks, anisotropy 1:1:1e-3 in z,
+    seed 2), int-AMG-FGMRES(30) to rtol: seconds per solve for both, with b and
+    x resident in HBM, and the iterations; the learned setup falls back to
+    the defaults when its solve does not converge."""
+    import sc23ad as S
+    from sc23ad import learned as LR
+    n = args.n
+    t0 = time.time()
+    sy = S.make_system(S.GEN_DIFFUSION27, (n, 8, 1.0, 1.0, 1e-3, 2), 2)
+    prm = S.amg_param(args.smoother)
+    bi, bd = S.Hierarchy.build_multi(sy.a, [S.INT, S.FP64], prm)
+    log(f"diffusion27 {n}^3: N={sy.a.n}, depth {bi.info()[0]} ({time.time() - t0:.1f} s)")
+    mlp = LR.load_mlp()
+    ls = LR.learned_setup(bi, bd, mlp)
+    log(f"learned omega {np.round(ls.omega, 3).tolist()} bits {ls.bits.tolist()} "
+        f"saturated {ls.saturated}")
+    b_dev = S.DevArray(sy.b)
+    x_dev = S.DevArray(np.zeros(sy.a.n))
+    zeros = np.zeros(sy.a.n)
+
+    def timed(hint, hfp):
+        bit = S.Amg(hfp).default_frac_bits(sy.b, sy.nrm_b)
+        sol = S.Solver(S.INT, sy.a, S.Amg(hint), m=args.m, eps=args.rtol, bit=bit)
+        for _ in range(args.warmup):
+            x_dev.set(zeros)
+            sol.solve_device(b_dev, sy.nrm_b, x_dev)
+        ts = []
+        for _ in range(args.steps):
+            x_dev.set(zeros)
+            S.check(S.lib.sc23ad_dev_sync())
+            t = time.perf_counter()
+            cv = sol.solve_device(b_dev, sy.nrm_b, x_dev)
+            S.check(S.lib.sc23ad_dev_sync())
+            ts.append(time.perf_counter() - t)
+        rel = sol.relres(sy.b, sy.nrm_b, x_dev.get())
+        return float(np.mean(ts)), cv, rel, bit
+
+    t_def, cv_def, rel_def, bit_def = timed(bi, bd)
+    log(f"default: {t_def * 1e3:.1f} ms, {cv_def.iter} iterations, relres {rel_def:.3e}")
+    t_l, cv_l, rel_l, bit_l = timed(ls.hier_int, ls.hier_fp64)
+    fallback = bool(ls.saturated > 0)
+    if not (cv_l.is_converged and cv_l.status == S.OK):
+        fallback = True
+        t_l, cv_l, rel_l, bit_l = t_def, cv_def, rel_def, bit_def
+    log(f"learned: {t_l * 1e3:.1f} ms, {cv_l.iter} iterations, relres {rel_l:.3e}, "
+        f"fallback {fallback}")
+    rec = {
+        "metric": "learned per-level omega/bits vs reference defaults: solve time to 1e-8",
+        "value": t_l, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t_l * 1e3, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32/int64 (FP64 outer refinement)",
+        "data": "synthetic (27-point variable-coefficient diffusion, seed 2)",
+        "config": {"workload": f"int-AMG-FGMRES({args.m}), 3D 27-point diffusion {n}^3 "
+                               f"(N={sy.a.n}), anisotropy 1:1:1e-3, rtol {args.rtol}"},
+        "default_s": t_def, "default_iterations": cv_def.iter, "default_relres": rel_def,
+        "learned_iterations": cv_l.iter, "learned_relres": rel_l,
+        "learned_omega": [float(v) for v in ls.omega], "learned_bits": [int(v) for v in ls.bits],
+        "fallback_used": fallback, "gain_vs_default": t_def / t_l,
+    }
+    print(json.dumps(rec), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__,
                                  formatter_class=argparse.RawDescriptionHelpFormatter)
@@ -249,9 +314,10 @@ def main():
     ap.add_argument("--cpu-timeout", type=float, default=400.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run the copy kernel (MIB MiB per launch) for counter calibration")
-    ap.add_argument("--workload", default="laplace3d", choices=("laplace3d", "batch"),
+    ap.add_argument("--workload", default="laplace3d", choices=("laplace3d", "batch", "learned"),
                     help="laplace3d: the 3D 7-point solve (default); batch: uneven 2D "
-                         "convection-diffusion systems solved at once")
+                         "convection-diffusion systems solved at once; learned: learned "
+                         "per-level parameters on the 27-point diffusion system (--n 96)")
     ap.add_argument("--systems", type=int, default=256, help="batch: number of systems")
     ap.add_argument("--threads", type=int, default=16, help="batch: host threads")
     args = ap.parse_args()
@@ -273,6 +339,11 @@ def main():
         return
     if args.workload == "batch":
         run_batch(args)
+        return
+    if args.workload == "learned":
+        if "--n" not in sys.argv:
+            args.n = 96
+        run_learned(args)
         return
 
     n = args.n

@@ -221,6 +221,41 @@ int sc23ad_solver_relres(sc23ad_solver* s, const double* b, double nrm_b,
 int sc23ad_default_frac_bits(sc23ad_amg* m_fp64, const double* b, double nrm_b,
                              int* bits);
 
+/* ---- learned per-level smoother parameters --------------------------------
+   No counterpart in the reference, which uses one omega for every level
+   (SORParam / BlockSORParam, Work/main.cpp:82-83) and the CSRflex bit rule
+   (matrix/spmat.hpp:108).  A small network (8 -> 32 -> 32 -> 2, tanh,
+   sigmoid outputs) maps per-level matrix features to omega_l in [0.6, 1.9]
+   and a cap bits_l in [8, 30] on the fractional bits of A_l, U_l and LD_l's
+   off-diagonal part; the integer kernels are unchanged. */
+typedef struct sc23ad_mlp_s sc23ad_mlp;
+/* Hierarchy with other smoother parameters from base's coarsening (no
+   re-coarsening): level_omega / level_bits (nlevels entries, either may be
+   NULL; bits <= 0 = no cap). */
+int sc23ad_hier_retune(const sc23ad_hier* base, const double* level_omega,
+                       const int* level_bits, int nlevels, int precision,
+                       sc23ad_hier** out);
+/* Levels whose requested bits exceeded the flex rule's maximum (the rule's
+   value was kept: the default is the fallback for a saturated prediction). */
+int sc23ad_hier_saturated(const sc23ad_hier* h, int* levels);
+/* 8 features per level (feat: cap x 8 floats, NULL to query nlevels). */
+int sc23ad_hier_features(const sc23ad_hier* h, float* feat, int cap, int* nlevels);
+int sc23ad_mlp_create(uint64_t seed, sc23ad_mlp** out);
+void sc23ad_mlp_destroy(sc23ad_mlp* m);
+int sc23ad_mlp_nparams(int* n);
+/* Copy the parameters out (get) and/or in (set; resets the optimiser). */
+int sc23ad_mlp_params(sc23ad_mlp* m, float* get, const float* set);
+/* y (batch x 2, in [0,1]) = network(x (batch x 8)); host arrays. */
+int sc23ad_mlp_forward(sc23ad_mlp* m, const float* x, int batch, float* y);
+/* Mean over the batch of sum_k (y_k - t_k)^2 and its gradient (nparams). */
+int sc23ad_mlp_grad(sc23ad_mlp* m, const float* x, const float* t, int batch, float* loss,
+                    float* grad);
+/* `steps` full-batch Adam steps on the GPU; *loss = loss afterwards. */
+int sc23ad_mlp_train(sc23ad_mlp* m, const float* x, const float* t, int batch, int steps,
+                     float lr, float* loss);
+/* The network's omega_l and bits_l for every level of h (cap entries). */
+int sc23ad_learned_levels(sc23ad_mlp* m, const sc23ad_hier* h, double* level_omega,
+                          int* level_bits, int cap, int* nlevels);
 /* Order-free checksum used by the traces: sum_i mix64(i * 0x9E3779B97F4A7C15 ^
    (uint32)v_i) mod 2^64, mix64 = splitmix64's finaliser.  Host arrays. */
 uint64_t sc23ad_checksum_i32(const int32_t* v, int n);

@@ -13,12 +13,18 @@
 #include "../../../include/sc23ad.h"
 #include "capi_internal.hpp"
 #include "device.hpp"
+#include "learn.hpp"
 
 using namespace sc23ad;
 
 struct sc23ad_amg_s {
   int precision = 2;
   std::unique_ptr<DevAmg> d;
+};
+
+struct sc23ad_mlp_s {
+  DevBuf P, G, M, V, loss;
+  int step = 0;
 };
 
 struct sc23ad_solver_s {
@@ -389,6 +395,127 @@ int sc23ad_default_frac_bits(sc23ad_amg* m, const double* b, double nrm_b, int* 
                 : 31 - (static_cast<int>(std::ceil(std::log2(static_cast<int>(c_max / nrm_b) + 1))) + 1);
     return SC23AD_OK;
   });
+}
+
+
+// ---- learned per-level smoother parameters ----------------------------------
+
+int sc23ad_mlp_create(uint64_t seed, sc23ad_mlp** out) {
+  if (!out) return fail(SC23AD_BAD_PARAM, "mlp_create: null");
+  return guarded([&]() -> int {
+    if (dev::mlp_param_count() != kMlpParams)
+      return fail(SC23AD_BAD_PARAM, "mlp_create: parameter layout mismatch");
+    auto m = std::make_unique<sc23ad_mlp>();
+    std::vector<float> p(kMlpParams), z(kMlpParams, 0.0f);
+    mlp_init(seed, p.data());
+    for (DevBuf* b : {&m->P, &m->G, &m->M, &m->V}) b->alloc(sizeof(float) * kMlpParams);
+    m->P.upload(p.data(), sizeof(float) * kMlpParams);
+    m->M.upload(z.data(), sizeof(float) * kMlpParams);
+    m->V.upload(z.data(), sizeof(float) * kMlpParams);
+    m->loss.alloc(sizeof(float));
+    *out = m.release();
+    return SC23AD_OK;
+  });
+}
+
+void sc23ad_mlp_destroy(sc23ad_mlp* m) { delete m; }
+
+int sc23ad_mlp_nparams(int* n) {
+  if (!n) return fail(SC23AD_BAD_PARAM, "mlp_nparams: null");
+  *n = kMlpParams;
+  return SC23AD_OK;
+}
+
+int sc23ad_mlp_params(sc23ad_mlp* m, float* get, const float* set) {
+  if (!m) return fail(SC23AD_BAD_PARAM, "mlp_params: null");
+  return guarded([&]() -> int {
+    if (set) {
+      m->P.upload(set, sizeof(float) * kMlpParams);
+      m->step = 0;
+      std::vector<float> z(kMlpParams, 0.0f);
+      m->M.upload(z.data(), sizeof(float) * kMlpParams);
+      m->V.upload(z.data(), sizeof(float) * kMlpParams);
+    }
+    if (get) m->P.download(get, sizeof(float) * kMlpParams);
+    return SC23AD_OK;
+  });
+}
+
+int sc23ad_mlp_forward(sc23ad_mlp* m, const float* x, int batch, float* y) {
+  if (!m || batch < 0 || (batch > 0 && (!x || !y))) return fail(SC23AD_BAD_PARAM, "mlp_forward: bad arguments");
+  return guarded([&]() -> int {
+    if (batch == 0) return SC23AD_OK;
+    DevBuf dx(sizeof(float) * batch * kMlpIn), dy(sizeof(float) * batch * kMlpOut);
+    dx.upload(x, sizeof(float) * batch * kMlpIn);
+    hip_check(dev::launch_mlp_forward(batch, dx.as<float>(), m->P.as<float>(), dy.as<float>(),
+                                      lib_stream()),
+              "mlp forward");
+    dy.download(y, sizeof(float) * batch * kMlpOut);
+    return SC23AD_OK;
+  });
+}
+
+int sc23ad_mlp_grad(sc23ad_mlp* m, const float* x, const float* t, int batch, float* loss,
+                    float* grad) {
+  if (!m || batch < 1 || !x || !t) return fail(SC23AD_BAD_PARAM, "mlp_grad: bad arguments");
+  return guarded([&]() -> int {
+    DevBuf dx(sizeof(float) * batch * kMlpIn), dt(sizeof(float) * batch * kMlpOut);
+    dx.upload(x, sizeof(float) * batch * kMlpIn);
+    dt.upload(t, sizeof(float) * batch * kMlpOut);
+    hip_check(dev::launch_mlp_grad(batch, dx.as<float>(), dt.as<float>(), m->P.as<float>(),
+                                   m->G.as<float>(), m->loss.as<float>(), lib_stream()),
+              "mlp grad");
+    if (loss) m->loss.download(loss, sizeof(float));
+    if (grad) m->G.download(grad, sizeof(float) * kMlpParams);
+    return SC23AD_OK;
+  });
+}
+
+int sc23ad_mlp_train(sc23ad_mlp* m, const float* x, const float* t, int batch, int steps,
+                     float lr, float* loss) {
+  if (!m || batch < 1 || !x || !t || steps < 0 || !(lr > 0))
+    return fail(SC23AD_BAD_PARAM, "mlp_train: bad arguments");
+  return guarded([&]() -> int {
+    DevBuf dx(sizeof(float) * batch * kMlpIn), dt(sizeof(float) * batch * kMlpOut);
+    dx.upload(x, sizeof(float) * batch * kMlpIn);
+    dt.upload(t, sizeof(float) * batch * kMlpOut);
+    for (int k = 0; k < steps; ++k) {
+      hip_check(dev::launch_mlp_grad(batch, dx.as<float>(), dt.as<float>(), m->P.as<float>(),
+                                     m->G.as<float>(), m->loss.as<float>(), lib_stream()),
+                "mlp grad");
+      hip_check(dev::launch_adam(kMlpParams, m->P.as<float>(), m->G.as<float>(),
+                                 m->M.as<float>(), m->V.as<float>(), lr, ++m->step,
+                                 lib_stream()),
+                "adam");
+    }
+    if (loss) {  // loss at the final parameters
+      hip_check(dev::launch_mlp_grad(batch, dx.as<float>(), dt.as<float>(), m->P.as<float>(),
+                                     m->G.as<float>(), m->loss.as<float>(), lib_stream()),
+                "mlp grad");
+      m->loss.download(loss, sizeof(float));
+    }
+    return SC23AD_OK;
+  });
+}
+
+int sc23ad_learned_levels(sc23ad_mlp* m, const sc23ad_hier* h, double* level_omega,
+                          int* level_bits, int cap, int* nlevels) {
+  if (!m || !h || !nlevels) return fail(SC23AD_BAD_PARAM, "learned_levels: null");
+  int nl = 0;
+  int rc = sc23ad_hier_features(h, nullptr, 0, &nl);
+  if (rc != SC23AD_OK) return rc;
+  *nlevels = nl;
+  if (nl == 0 || (!level_omega && !level_bits)) return SC23AD_OK;
+  std::vector<float> f(static_cast<size_t>(nl) * kLevelFeatures), y(static_cast<size_t>(nl) * kMlpOut);
+  rc = sc23ad_hier_features(h, f.data(), nl, &nl);
+  if (rc != SC23AD_OK) return rc;
+  rc = sc23ad_mlp_forward(m, f.data(), nl, y.data());
+  if (rc != SC23AD_OK) return rc;
+  for (int l = 0; l < nl && l < cap; ++l) {
+    if (level_omega) level_omega[l] = mlp_omega(y[2 * l]);
+    if (level_bits) level_bits[l] = mlp_bits(y[2 * l + 1]);
+  }
+  return SC23AD_OK;
 }
 
 }  // extern "C"

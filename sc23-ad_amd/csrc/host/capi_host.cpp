@@ -10,6 +10,7 @@
 
 #include "../../../include/sc23ad.h"
 #include "capi_internal.hpp"
+#include "learn.hpp"
 #include "gen.hpp"
 #include "hier_io.hpp"
 #include "hierarchy.hpp"
@@ -185,6 +186,24 @@ static AmgParams to_params(const sc23ad_amg_param* p) {
   return prm;
 }
 
+// A hierarchy of `precision` from a kept setup state with parameters prm
+// (the state's own or per-level overrides of them).
+static std::unique_ptr<sc23ad_hier> derive_hierarchy(
+    const std::shared_ptr<const Sc23adSetupState>& st, const AmgParams& prm, int precision) {
+  auto h = std::make_unique<sc23ad_hier>();
+  h->precision = precision;
+  h->warnings = st->chain.warnings;
+  h->fine_nnz = st->fine.nnz();
+  h->setup = st;
+  if (precision == SC23AD_INT)
+    h->ih = build_int_hierarchy(st->fine, st->chain, prm);
+  else if (precision == SC23AD_FP64)
+    h->dh = build_fp_hierarchy<double>(st->fine, st->chain, prm);
+  else
+    h->fh = build_fp_hierarchy<float>(st->fine, st->chain, prm);
+  return h;
+}
+
 int sc23ad_hier_build_multi(const sc23ad_csrmat* a, const sc23ad_amg_param* p,
                             const int* precisions, int count, sc23ad_hier** outs) {
   if (!a || !p || !precisions || !outs || count < 1)
@@ -199,19 +218,12 @@ int sc23ad_hier_build_multi(const sc23ad_csrmat* a, const sc23ad_amg_param* p,
   if (a->a.n != a->a.m) return fail(SC23AD_BAD_PARAM, "hier_build: matrix must be square");
   const AmgParams prm = to_params(p);
   try {
-    AmgChain ch = build_chain(a->a, prm);
+    auto st = std::make_shared<Sc23adSetupState>();
+    st->fine = a->a;
+    st->prm = prm;
+    st->chain = build_chain(a->a, prm);
     for (int k = 0; k < count; ++k) {
-      auto h = std::make_unique<sc23ad_hier>();
-      h->precision = precisions[k];
-      h->warnings = ch.warnings;
-      h->fine_nnz = a->a.nnz();
-      if (precisions[k] == SC23AD_INT)
-        h->ih = build_int_hierarchy(a->a, ch, prm);
-      else if (precisions[k] == SC23AD_FP64)
-        h->dh = build_fp_hierarchy<double>(a->a, ch, prm);
-      else
-        h->fh = build_fp_hierarchy<float>(a->a, ch, prm);
-      outs[k] = h.release();
+      outs[k] = derive_hierarchy(st, prm, precisions[k]).release();
     }
   } catch (const SetupError& e) {
     for (int k = 0; k < count; ++k) {
@@ -233,6 +245,49 @@ int sc23ad_hier_build(const sc23ad_csrmat* a, const sc23ad_amg_param* p,
                       int precision, sc23ad_hier** out) {
   if (!out) return fail(SC23AD_BAD_PARAM, "hier_build: bad arguments");
   return sc23ad_hier_build_multi(a, p, &precision, 1, out);
+}
+
+int sc23ad_hier_retune(const sc23ad_hier* base, const double* level_omega,
+                       const int* level_bits, int nlevels, int precision,
+                       sc23ad_hier** out) {
+  if (!base || !out || nlevels < 0 || precision < 0 || precision > 2)
+    return fail(SC23AD_BAD_PARAM, "hier_retune: bad arguments");
+  *out = nullptr;
+  if (!base->setup) return fail(SC23AD_BAD_PARAM, "hier_retune: hierarchy has no setup state");
+  AmgParams prm = base->setup->prm;
+  if (level_omega) {
+    prm.level_omega.assign(level_omega, level_omega + nlevels);
+    for (double w : prm.level_omega)
+      if (!(w > 0 && w < 2)) return fail(SC23AD_BAD_PARAM, "hier_retune: omega outside (0, 2)");
+  }
+  if (level_bits) prm.level_bits.assign(level_bits, level_bits + nlevels);
+  try {
+    *out = derive_hierarchy(base->setup, prm, precision).release();
+  } catch (const SetupError& e) {
+    return fail(e.code, "hier_retune: " + e.what);
+  } catch (const std::bad_alloc&) {
+    return fail(SC23AD_SETUP_FAIL, "hier_retune: out of host memory");
+  }
+  return SC23AD_OK;
+}
+
+int sc23ad_hier_saturated(const sc23ad_hier* h, int* levels) {
+  if (!h || !levels) return fail(SC23AD_BAD_PARAM, "hier_saturated: null");
+  *levels = h->precision == SC23AD_INT ? h->ih.saturated : 0;
+  return SC23AD_OK;
+}
+
+int sc23ad_hier_features(const sc23ad_hier* h, float* feat, int cap, int* nlevels) {
+  if (!h || !nlevels) return fail(SC23AD_BAD_PARAM, "hier_features: null");
+  if (!h->setup) return fail(SC23AD_BAD_PARAM, "hier_features: hierarchy has no setup state");
+  const AmgChain& ch = h->setup->chain;
+  *nlevels = ch.depth;
+  if (!feat) return SC23AD_OK;
+  for (int l = 0; l < ch.depth && l < cap; ++l) {
+    const HostCsr& a = l == 0 ? h->setup->fine : ch.coarse[l - 1];
+    level_features(a, l, ch.depth, feat + static_cast<size_t>(l) * kLevelFeatures);
+  }
+  return SC23AD_OK;
 }
 
 int sc23ad_hier_info(const sc23ad_hier* h, int* depth, int* level_n,

@@ -2,6 +2,7 @@
 // halves of the library.
 #pragma once
 
+#include <memory>
 #include <string>
 
 #include "hierarchy.hpp"
@@ -10,10 +11,19 @@ struct sc23ad_csrmat_s {
   sc23ad::HostCsr a;
 };
 
+// The precision-independent part of a setup, kept so that hierarchies with
+// other smoother parameters can be derived without coarsening again.
+struct Sc23adSetupState {
+  sc23ad::HostCsr fine;
+  sc23ad::AmgChain chain;
+  sc23ad::AmgParams prm;
+};
+
 struct sc23ad_hier_s {
   int precision = 2;
   int warnings = 0;
   int fine_nnz = 0;
+  std::shared_ptr<const Sc23adSetupState> setup;
   sc23ad::IntHierarchy ih;
   sc23ad::FpHierarchy<double> dh;
   sc23ad::FpHierarchy<float> fh;

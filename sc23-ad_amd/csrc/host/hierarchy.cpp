@@ -55,9 +55,11 @@ AmgChain build_chain(const HostCsr& fine, const AmgParams& prm) {
 // ceil(n/b_num) rows; LD keeps the diagonal / omega and the lower entries of
 // the row's own block, U everything else (stationary.hpp:934-978).
 void smoother_split(const HostCsr& a, const AmgParams& prm, HostCsr& ld,
-                    HostCsr& u) {
+                    HostCsr& u, int level) {
   const int n = a.n;
-  const double omega = prm.omega;
+  const double omega = static_cast<size_t>(level) < prm.level_omega.size()
+                           ? prm.level_omega[level]
+                           : prm.omega;
   ld.reset(n, a.m);
   u.reset(n, a.m);
   if (prm.smoother == SmootherKind::kSOR) {
@@ -239,13 +241,17 @@ void check_bits(int bit, const char* what) {
 
 }  // namespace
 
-IntCsr to_flex(const HostCsr& a) {
+IntCsr to_flex(const HostCsr& a, int cap, int* saturated) {
   IntCsr o = int_shell(a);
   double mx = 0;
   for (double v : a.val)
     if (std::fabs(v) > mx) mx = std::fabs(v);
   if (mx >= 2147483647.0) throw SetupError{kOverflow, "CSRflex: |a| too large"};
   o.bit = flex_bits(mx);
+  if (cap > 0) {
+    if (cap > o.bit && saturated) *saturated = 1;
+    o.bit = std::min(o.bit, cap);
+  }
   check_bits(o.bit, "CSRflex");
   for (size_t p = 0; p < a.val.size(); ++p) o.val[p] = to_fixed(a.val[p], o.bit);
   return o;
@@ -260,7 +266,7 @@ IntCsr to_fixed_csr(const HostCsr& a, int bit) {
 
 // ldtri::CSRflex / BJCSRflex: separate bit counts for the stored inverse
 // diagonal (dbit) and the off-diagonal entries (bit), both by the flex rule.
-IntCsr to_ld_flex(const HostCsr& l) {
+IntCsr to_ld_flex(const HostCsr& l, int cap, int* saturated) {
   IntCsr o = int_shell(l);
   double mx = 0, dmx = 0;
   for (int i = 0; i < l.n; ++i)
@@ -276,6 +282,10 @@ IntCsr to_ld_flex(const HostCsr& l) {
     throw SetupError{kOverflow, "ldtri::CSRflex: |a| too large"};
   o.dbit = flex_bits(dmx);
   o.bit = flex_bits(mx);
+  if (cap > 0) {
+    if (cap > o.bit && saturated) *saturated = 1;
+    o.bit = std::min(o.bit, cap);
+  }
   check_bits(o.bit, "ldtri::CSRflex");
   check_bits(o.dbit, "ldtri::CSRflex");
   for (int i = 0; i < l.n; ++i)
@@ -348,14 +358,17 @@ IntHierarchy build_int_hierarchy(const HostCsr& fine, const AmgChain& chain,
   for (int l = 0; l < d; ++l) {
     IntLevel& L = h.lv[l];
     const HostCsr& src = (l == 0) ? fine : ca[l - 1];
-    L.a = (l == 0) ? to_fixed_csr(fine, 30) : to_flex(src);
+    const int cap = static_cast<size_t>(l) < prm.level_bits.size() ? prm.level_bits[l] : 0;
+    int sat = 0;
+    L.a = (l == 0) ? to_fixed_csr(fine, 30) : to_flex(src, cap, &sat);
     lap("  A");
     HostCsr ld, u;
-    smoother_split(src, prm, ld, u);
+    smoother_split(src, prm, ld, u, l);
     lap("  split");
-    L.ld = to_ld_flex(extract_ld_inv(ld));
+    L.ld = to_ld_flex(extract_ld_inv(ld), cap, &sat);
     lap("  LD");
-    L.u = to_flex(u);
+    L.u = to_flex(u, cap, &sat);
+    h.saturated += sat;
     L.r = to_flex(rt[l]);
     L.p = to_flex(chain.interp[l]);
     lap("  U R P");
@@ -396,7 +409,7 @@ FpHierarchy<T> build_fp_hierarchy(const HostCsr& fine, const AmgChain& chain,
     const HostCsr& src = (l == 0) ? fine : chain.coarse[l - 1];
     L.a = to_fp<T>(src);
     HostCsr ld, u;
-    smoother_split(src, prm, ld, u);
+    smoother_split(src, prm, ld, u, l);
     L.ld = to_fp<T>(extract_ld_inv(ld));
     L.u = to_fp<T>(u);
     L.r = to_fp<T>(chain.restrict_[l]);

@@ -36,6 +36,10 @@ struct AmgParams {
   int iter = 1;          // sweeps per Smooth call
   int b_num = 40;        // BlockSORParam::b_num
   int coarse_n = 2000;   // coarsening stops once a level has <= coarse_n rows
+  // Per-level overrides (learned parameters); empty = the reference's rule.
+  std::vector<double> level_omega;  // SOR weight of level l's smoother
+  std::vector<int> level_bits;      // cap on the fractional bits of A_l, U_l and
+                                    // LD_l's off-diagonal part (<= 0: none)
 };
 
 // The unscaled Galerkin chain every precision shares: coarse[l], interp[l] and
@@ -54,6 +58,7 @@ struct IntLevel {
 struct IntHierarchy {
   AmgParams prm;
   int depth = 0;
+  int saturated = 0;  // levels whose requested bits exceeded the flex rule's maximum
   std::vector<IntLevel> lv;  // depth entries
   IntCsr lc, uc;             // coarsest-grid factors
   int n_coarse = 0;
@@ -80,14 +85,17 @@ FpHierarchy<T> build_fp_hierarchy(const HostCsr& fine, const AmgChain& chain,
 
 // Pieces exposed for the unit tests.
 void smoother_split(const HostCsr& a, const AmgParams& prm, HostCsr& ld,
-                    HostCsr& u);
+                    HostCsr& u, int level = 0);
 HostCsr extract_ld_inv(const HostCsr& ld);  // helper_matrix.hpp:634-664
 HostCsr extract_l(const HostCsr& a);        // helper_matrix.hpp:606-632
 HostCsr extract_dinv_u(const HostCsr& a);   // helper_matrix.hpp:698-728
 HostCsr complete_lu(const HostCsr& a);      // helper_matrix.hpp:978-1060
-IntCsr to_flex(const HostCsr& a);           // CSRflex (spmat.hpp:96-121)
+// CSRflex (spmat.hpp:96-121); cap > 0 limits the bits (saturated = 1 when
+// the cap exceeds the rule's maximum and is not applied).
+IntCsr to_flex(const HostCsr& a, int cap = 0, int* saturated = nullptr);
 IntCsr to_fixed_csr(const HostCsr& a, int bit);  // CSR<Fixed<bit>>
-IntCsr to_ld_flex(const HostCsr& ld_inv);   // ldtri::CSRflex (ldtri.hpp:127-165)
+IntCsr to_ld_flex(const HostCsr& ld_inv, int cap = 0,
+                  int* saturated = nullptr);  // ldtri::CSRflex (ldtri.hpp:127-165)
 IntCsr to_l_flex(const HostCsr& l);         // ltri::CSRflex (ltri.hpp:95-118)
 IntCsr to_dinvu_flex(const HostCsr& du);    // dutri::CSRflex (dutri.hpp:105-139)
 

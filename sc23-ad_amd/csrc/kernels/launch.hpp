@@ -183,4 +183,13 @@ hipError_t launch_permute(Elem e, int n, const int* perm, const void* src, void*
                           bool gather, hipStream_t s);
 hipError_t launch_fill_zero(size_t bytes, void* dst, hipStream_t s);
 
+// Learned-parameter network (mlp.hip): parameters P (mlp_param_count()
+// floats), inputs X (B x 8), targets T and outputs Y (B x 2).
+int mlp_param_count();
+hipError_t launch_mlp_forward(int B, const float* X, const float* P, float* Y, hipStream_t s);
+hipError_t launch_mlp_grad(int B, const float* X, const float* T, const float* P, float* G,
+                           float* loss, hipStream_t s);
+hipError_t launch_adam(int n, float* P, const float* G, float* M, float* V, float lr, int step,
+                       hipStream_t s);
+
 }  // namespace sc23ad::dev

@@ -1,0 +1,61 @@
+"""Trains the learned-parameter network on the GPU (the library's own HIP
+forward/backward kernels, full-batch Adam) from the solver's sweep data:
+
+    python sc23-ad_amd/learn/train.py [--data ...npz] [--steps 3000] [--out ...npy]
+
+Instances are split 80/20 into training and validation sets (by instance);
+the script reports both losses and how often the predicted omega, rounded to
+the sweep grid, is the instance's best, and writes the parameters (float32).
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+import sc23ad as S  # noqa: E402
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--data", default=os.path.join(HERE, "data", "diffusion27_sweeps.npz"))
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--lr", type=float, default=3e-3)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--out", default=os.path.join(HERE, "weights_diffusion27.npy"))
+    args = ap.parse_args()
+    d = np.load(args.data, allow_pickle=False)
+    X, T, inst = d["X"], d["T"], d["instance"]
+    ids = np.unique(inst)
+    rng = np.random.default_rng(args.seed)
+    val_ids = set(rng.choice(ids, max(1, len(ids) // 5), replace=False).tolist())
+    va = np.array([i in val_ids for i in inst])
+    m = S.Mlp(args.seed)
+    l0 = m.grad(X[~va], T[~va])[0]
+    done = 0
+    while done < args.steps:
+        k = min(500, args.steps - done)
+        lt = m.train(X[~va], T[~va], steps=k, lr=args.lr)
+        done += k
+        lv = m.grad(X[va], T[va])[0] if va.any() else float("nan")
+        print(f"step {done}: train loss {lt:.5f} validation loss {lv:.5f} (initial {l0:.5f})",
+              flush=True)
+    y = m.forward(X)
+    om_pred = 0.6 + 1.3 * y[:, 0]
+    om_true = 0.6 + 1.3 * T[:, 0]
+    grid = np.array(sorted(set(np.round(om_true, 6))))
+    snap = grid[np.abs(om_pred[:, None] - grid[None, :]).argmin(1)]
+    for name, sel in (("train", ~va), ("validation", va)):
+        if sel.any():
+            acc = np.mean(np.isclose(snap[sel], om_true[sel]))
+            print(f"{name}: omega on the best grid point for {100 * acc:.1f}% of levels", flush=True)
+    np.save(args.out, m.params())
+    print("wrote", args.out, flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -76,6 +76,9 @@ def _load() -> C.CDLL:
         "sc23ad_hier_export": (C.c_int, [vp, C.c_char_p]),
         "sc23ad_hier_destroy": (None, [vp]),
         "sc23ad_checksum_i32": (C.c_uint64, [vp, C.c_int]),
+        "sc23ad_hier_retune": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, C.POINTER(vp)]),
+        "sc23ad_hier_saturated": (C.c_int, [vp, ip]),
+        "sc23ad_hier_features": (C.c_int, [vp, vp, C.c_int, ip]),
     }
     optional = {
         "sc23ad_device_count": (C.c_int, [ip]),
@@ -107,6 +110,15 @@ def _load() -> C.CDLL:
         "sc23ad_solver_trsv_stats": (C.c_int, [vp, ip, dp]),
         "sc23ad_solver_relres": (C.c_int, [vp, vp, C.c_double, vp, C.c_int, dp]),
         "sc23ad_default_frac_bits": (C.c_int, [vp, vp, C.c_double, ip]),
+        "sc23ad_mlp_create": (C.c_int, [C.c_uint64, C.POINTER(vp)]),
+        "sc23ad_mlp_destroy": (None, [vp]),
+        "sc23ad_mlp_nparams": (C.c_int, [ip]),
+        "sc23ad_mlp_params": (C.c_int, [vp, vp, vp]),
+        "sc23ad_mlp_forward": (C.c_int, [vp, vp, C.c_int, vp]),
+        "sc23ad_mlp_grad": (C.c_int, [vp, vp, vp, C.c_int, C.POINTER(C.c_float), vp]),
+        "sc23ad_mlp_train": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, C.c_float,
+                                       C.POINTER(C.c_float)]),
+        "sc23ad_learned_levels": (C.c_int, [vp, vp, vp, vp, C.c_int, ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -251,6 +263,31 @@ class Hierarchy:
         nnz = np.empty(depth.value + 1, np.int32)
         check(lib.sc23ad_hier_info(self.h, C.byref(depth), ptr(n), ptr(nnz)))
         return depth.value, n, nnz
+
+    def retune(self, level_omega=None, level_bits=None, precision: int | None = None):
+        """Hierarchy with per-level smoother weights / bit caps, from this
+        one's coarsening."""
+        nl = self.info()[0]
+        om = None if level_omega is None else np.ascontiguousarray(level_omega, np.float64)
+        bi = None if level_bits is None else np.ascontiguousarray(level_bits, np.int32)
+        n = len(om) if om is not None else (len(bi) if bi is not None else nl)
+        h = C.c_void_p()
+        prec = self.precision if precision is None else precision
+        check(lib.sc23ad_hier_retune(self.h, ptr(om), ptr(bi), n, prec, C.byref(h)))
+        return Hierarchy(None, prec, _handle=h)
+
+    def saturated(self) -> int:
+        v = C.c_int()
+        check(lib.sc23ad_hier_saturated(self.h, C.byref(v)))
+        return v.value
+
+    def features(self) -> np.ndarray:
+        """8 features per level (levels x 8, float32)."""
+        nl = C.c_int()
+        check(lib.sc23ad_hier_features(self.h, None, 0, C.byref(nl)))
+        f = np.zeros((max(nl.value, 1), 8), np.float32)
+        check(lib.sc23ad_hier_features(self.h, ptr(f), nl.value, C.byref(nl)))
+        return f[:nl.value]
 
     def export(self, path: str) -> None:
         check(lib.sc23ad_hier_export(self.h, path.encode()))
@@ -449,6 +486,64 @@ class Solver:
     def __del__(self):
         if getattr(self, "h", None) and self.h.value:
             lib.sc23ad_solver_destroy(self.h)
+            self.h = C.c_void_p()
+
+
+class Mlp:
+    """The learned-parameter network (parameters on the GPU)."""
+
+    def __init__(self, seed: int = 0):
+        h = C.c_void_p()
+        check(lib.sc23ad_mlp_create(seed, C.byref(h)))
+        self.h = h
+        n = C.c_int()
+        check(lib.sc23ad_mlp_nparams(C.byref(n)))
+        self.nparams = n.value
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.nparams, np.float32)
+        check(lib.sc23ad_mlp_params(self.h, ptr(p), None))
+        return p
+
+    def set_params(self, p) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.nparams
+        check(lib.sc23ad_mlp_params(self.h, None, ptr(p)))
+
+    def forward(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 8)
+        y = np.zeros((x.shape[0], 2), np.float32)
+        check(lib.sc23ad_mlp_forward(self.h, ptr(x), x.shape[0], ptr(y)))
+        return y
+
+    def grad(self, x, t):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 8)
+        t = np.ascontiguousarray(t, np.float32).reshape(-1, 2)
+        g = np.zeros(self.nparams, np.float32)
+        loss = C.c_float()
+        check(lib.sc23ad_mlp_grad(self.h, ptr(x), ptr(t), x.shape[0], C.byref(loss), ptr(g)))
+        return loss.value, g
+
+    def train(self, x, t, steps: int, lr: float = 1e-2) -> float:
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 8)
+        t = np.ascontiguousarray(t, np.float32).reshape(-1, 2)
+        loss = C.c_float()
+        check(lib.sc23ad_mlp_train(self.h, ptr(x), ptr(t), x.shape[0], steps, lr,
+                                   C.byref(loss)))
+        return loss.value
+
+    def levels(self, hier: "Hierarchy"):
+        """(omega per level, bit cap per level) predicted for hier's levels."""
+        nl = hier.info()[0]
+        om = np.zeros(max(nl, 1))
+        bi = np.zeros(max(nl, 1), np.int32)
+        n = C.c_int()
+        check(lib.sc23ad_learned_levels(self.h, hier.h, ptr(om), ptr(bi), nl, C.byref(n)))
+        return om[:n.value], bi[:n.value]
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib.sc23ad_mlp_destroy(self.h)
             self.h = C.c_void_p()
 
 
