@@ -51,15 +51,21 @@ def sptrsv_int_bytes(n: int, nnz_ld: int) -> int:
     return 8 * nnz_ld + 4 * (n + 1) + 8 * n
 
 
+SMOOTHER_NAMES = {0: "SOR", 1: "BlockSOR(40)", 2: "JacobiSOR(1200)"}
+
+
 def ld_nnz(rptr, cind, smoother: int, b_num: int = 40) -> int:
     """Entries of the smoother's LD factor (diagonal included): col <= i for
     SOR; left <= col <= i inside each of the b_num row blocks of ceil(N / b_num)
-    rows for BlockSOR (smoother/stationary.hpp:934-978)."""
+    rows for BlockSOR (smoother/stationary.hpp:934-978); col == i or col below
+    the row's 1200-row block for JacobiSOR (stationary.hpp:586-631)."""
     n = len(rptr) - 1
     row = np.repeat(np.arange(n, dtype=np.int64), np.diff(rptr))
     col = np.asarray(cind, np.int64)
     if smoother == 0:
         return int(np.count_nonzero(col <= row))
+    if smoother == 2:
+        return int(np.count_nonzero((col == row) | (col < (row // 1200) * 1200)))
     size = -(-n // b_num)
     left = (row // size) * size
     return int(np.count_nonzero((col <= row) & (col >= left)))
@@ -117,14 +123,15 @@ def cpu_baseline(args, unscaled_arrays, b_unscaled, smoother):
     as Work/main.cpp times it.  Falls back to the oracle's CPU restatement
     (single thread) when the reference build is absent."""
     import oracle.oracle as O  # noqa: F401  (checker; cpu_baseline leg only)
-    exe = O.MULTI_REF if smoother == 1 else O.SEQ_REF
+    omp = smoother in (1, 2)   # BlockSOR and JacobiSOR are the OpenMP paths
+    exe = O.MULTI_REF if omp else O.SEQ_REF
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
     threads = min(threads, 16)
     if os.path.exists(exe):
         sysf = os.path.join("/tmp", f"sc23ad_sys_{os.getpid()}.bin")
         write_system_file(sysf, unscaled_arrays, b_unscaled)
         env = dict(os.environ)
-        env["OMP_NUM_THREADS"] = str(threads if smoother == 1 else 1)
+        env["OMP_NUM_THREADS"] = str(threads if omp else 1)
         cmd = [exe, sysf, str(smoother), "2", str(args.m), "0", repr(args.rtol)]
         log("cpu baseline:", " ".join(cmd))
         t0 = time.time()
@@ -154,9 +161,9 @@ def cpu_baseline(args, unscaled_arrays, b_unscaled, smoother):
             log("cpu baseline failed", rc)
             return None
         return {"value": float(tm.group(1)), "unit": "s",
-                "cores": threads if smoother == 1 else 1, "kind": "reference",
+                "cores": threads if omp else 1, "kind": "reference",
                 "sample": (f"full Restarted::Solve of the same system "
-                           f"({'multi.exe, BlockSOR(40)' if smoother == 1 else 'seq.exe, SOR'},"
+                           f"({'multi.exe' if omp else 'seq.exe'}, {SMOOTHER_NAMES[smoother]},"
                            f" solver 2, m={args.m}, eps={args.rtol}); "
                            f"iterations {it.group(1) if it else '?'}, explicit relres "
                            f"{rs.group(1) if rs else '?'}; process wall incl. setup {wall:.1f} s"),
@@ -305,8 +312,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=128, help="grid side of the 3D 7-point system")
-    ap.add_argument("--smoother", type=int, default=1, choices=(0, 1),
-                    help="0: SOR (seq.exe ordering), 1: BlockSOR(40) (multi.exe)")
+    ap.add_argument("--smoother", type=int, default=1, choices=(0, 1, 2),
+                    help="0: SOR (seq.exe ordering), 1: BlockSOR(40) (multi.exe), "
+                         "2: JacobiSOR (1200-row blocks)")
     ap.add_argument("--m", type=int, default=30)
     ap.add_argument("--rtol", type=float, default=1e-8)
     ap.add_argument("--no-fp", action="store_true", help="skip the FP64/FP32 solver runs")
@@ -438,21 +446,26 @@ def main():
         "data": "synthetic (generated 3D 7-point Laplacian, x* ~ U[-1,1] seed 1)",
         "config": {"workload": f"int-AMG-FGMRES({args.m}) solve, 3D 7-point Laplacian "
                                f"{n}^3 (N={N}, nnz={nnz}), "
-                               f"{'BlockSOR(40)' if args.smoother == 1 else 'SOR'} smoother, "
+                               f"{SMOOTHER_NAMES[args.smoother]} smoother, "
                                f"rtol {args.rtol}",
                    "solver": 2, "restart_m": args.m, "frac_bits": bit,
-                   "smoother": "BlockSOR(40)" if args.smoother == 1 else "SOR",
+                   "smoother": SMOOTHER_NAMES[args.smoother],
                    "amg_levels": [int(v) for v in level_n]},
         "iterations": cv_int.iter,
         "converged": bool(cv_int.is_converged),
         "explicit_relres": rel_int,
-        "roofline": {"bound": "hbm", "kernel": "fixed-point CSR SpMV, level 0 (A)",
+        "roofline": {"bound": "hbm",
+                     "kernel": "fixed-point SpMV of A0 (Arnoldi step; SELL-64 or CSR, "
+                               "per-matrix autotuned)",
+                     "timing": "hipEvents bound to each launch's dispatch (hipExtLaunchKernel)",
                      "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBPS) if achieved else None,
                      "traffic": traffic, "algorithmic_bytes": spmv_bytes,
                      "launches_timed": launches, "mean_ms": spmv_ms},
         "roofline_sptrsv": {"bound": "hbm",
                             "kernel": "integer LD SpTRSV, level 0 (smoother)",
+                            "timing": "hipEvents bound to each launch's dispatch "
+                                      "(hipExtLaunchKernel)",
                             "achieved": trsv_achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                             "frac": (trsv_achieved / HBM_PEAK_GBPS) if trsv_achieved else None,
                             "traffic": trsv_traffic, "algorithmic_bytes": trsv_bytes,

@@ -56,7 +56,14 @@ typedef enum {
 } sc23ad_status;
 
 typedef enum { SC23AD_FP64 = 0, SC23AD_FP32 = 1, SC23AD_INT = 2 } sc23ad_precision;
-typedef enum { SC23AD_SOR = 0, SC23AD_BLOCK_SOR = 1 } sc23ad_smoother;
+/* SOR (seq path), BlockSOR(b_num) (multi path) and the reference's opt-in
+   JacobiSOR: 1200-row blocks, Gauss-Seidel across blocks, Jacobi inside one
+   (JacobiSORflex / JacobiSOR<T>, Library/smoother/stationary.hpp:447-676). */
+typedef enum {
+  SC23AD_SOR = 0,
+  SC23AD_BLOCK_SOR = 1,
+  SC23AD_JACOBI_SOR = 2
+} sc23ad_smoother;
 
 /* Matrix kinds for sc23ad_csrmat_gen (parameters in brackets). */
 typedef enum {
@@ -201,10 +208,12 @@ int sc23ad_solver_set_reduction(sc23ad_solver* s, int sequential);
 /* Record an order-free checksum of every V-cycle input and output (two
    uint64 per Arnoldi step) during the next solves; cap = capacity. */
 int sc23ad_solver_trace(sc23ad_solver* s, uint64_t* buf, int cap, int* len);
-/* Bracket every fine-level SpMV of the Arnoldi steps (A0 Z_j, gmres.hpp:240)
+/* Time every fine-level SpMV of the Arnoldi steps (A0 Z_j, gmres.hpp:240)
    and every fine-level LD solve of the smoother (ldtri.hpp:172-183,
    511-525) with hipEvents during the next solves; report their count and
-   mean duration. */
+   mean duration.  Single-launch operations bind the events to the kernel's
+   dispatch (hipExtLaunchKernel: execution time only); multi-launch ones are
+   timed on the stream around their launches. */
 int sc23ad_solver_time_spmv(sc23ad_solver* s, int enable);
 int sc23ad_solver_spmv_stats(sc23ad_solver* s, int* launches, double* mean_ms);
 int sc23ad_solver_trsv_stats(sc23ad_solver* s, int* launches, double* mean_ms);

@@ -5,7 +5,8 @@
 //
 // What it provides:
 //  * ref_build_hier(): builds the reference's intAMGflex / dfAMG<double|float>
-//    with SOR or BlockSOR smoothers on a given (already row-scaled) matrix and
+//    with SOR, BlockSOR or JacobiSOR smoothers on a given (already row-scaled)
+//    matrix and
 //    writes the resulting hierarchy in the "SC23AMG1" format, so the product's
 //    host setup can be compared with it byte for byte.
 //  * ref_amg_*(): the reference V-cycle on given inputs.
@@ -22,6 +23,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "timer.hpp"
@@ -76,6 +78,13 @@ using LDd = ldtri::CSR<double>;
 using LDf = ldtri::CSR<float>;
 using BJd = ldtri::BJCSR<double>;
 using BJf = ldtri::BJCSR<float>;
+SC23_CSR_FIELDS(mcflex, ldtri::MCCSRflex, int)
+SC23_ACCESS(mcflex_bit, ldtri::MCCSRflex, int, bit)
+SC23_ACCESS(mcflex_dbit, ldtri::MCCSRflex, int, dbit)
+using MCd = ldtri::MCCSR<double>;
+using MCf = ldtri::MCCSR<float>;
+SC23_CSR_FIELDS(mcd, MCd, double)
+SC23_CSR_FIELDS(mcf, MCf, float)
 SC23_CSR_FIELDS(ldd, LDd, double)
 SC23_CSR_FIELDS(ldf, LDf, float)
 SC23_CSR_FIELDS(bjd, BJd, double)
@@ -102,6 +111,15 @@ using SORd = SOR<ldtri::CSR<double>, CSR<double>>;
 using SORf = SOR<ldtri::CSR<float>, CSR<float>>;
 using BSORd = BlockSOR<ldtri::BJCSR<double>, CSR<double>>;
 using BSORf = BlockSOR<ldtri::BJCSR<float>, CSR<float>>;
+using JSORi = JacobiSORflex<ldtri::MCCSRflex, CSRflex>;
+using JSORd = JacobiSOR<ldtri::MCCSR<double>, CSR<double>>;
+using JSORf = JacobiSOR<ldtri::MCCSR<float>, CSR<float>>;
+SC23_ACCESS(jsori_ld, JSORi, ldtri::SpMat<int>*, LD)
+SC23_ACCESS(jsori_u, JSORi, SpMat<int>*, U)
+SC23_ACCESS(jsord_ld, JSORd, ldtri::SpMat<double>*, LD)
+SC23_ACCESS(jsord_u, JSORd, SpMat<double>*, U)
+SC23_ACCESS(jsorf_ld, JSORf, ldtri::SpMat<float>*, LD)
+SC23_ACCESS(jsorf_u, JSORf, SpMat<float>*, U)
 SC23_ACCESS(sori_ld, SORi, ldtri::SpMat<int>*, LD)
 SC23_ACCESS(sori_u, SORi, SpMat<int>*, U)
 SC23_ACCESS(bsori_ld, BSORi, ldtri::SpMat<int>*, LD)
@@ -144,6 +162,15 @@ SC23_ACCESS(iab_sm, IAmgBJ, BSORi**, smoother)
 SC23_ACCESS(iab_lu, IAmgBJ, direct::LUflex*, lu_solver)
 SC23_ACCESS(iab_depth, IAmgBJ, int, depth)
 SC23_ACCESS(iab_lastn, IAmgBJ, int, last_N)
+using IAmgJS = intAMGflex<CSRflex, CSRflex, JSORi>;
+SC23_ACCESS(iaj_a, IAmgJS, SpMat<int>*, A)
+SC23_ACCESS(iaj_ca, IAmgJS, CSRflex**, C_A)
+SC23_ACCESS(iaj_p, IAmgJS, CSRflex**, Intrpl)
+SC23_ACCESS(iaj_r, IAmgJS, CSRflex**, Intrpl_t)
+SC23_ACCESS(iaj_sm, IAmgJS, JSORi**, smoother)
+SC23_ACCESS(iaj_lu, IAmgJS, direct::LUflex*, lu_solver)
+SC23_ACCESS(iaj_depth, IAmgJS, int, depth)
+SC23_ACCESS(iaj_lastn, IAmgJS, int, last_N)
 
 template <typename T, typename SM>
 using DAmg = dfAMG<CSR<T>, SM, T>;
@@ -151,6 +178,8 @@ using DAmgSORd = DAmg<double, SORd>;
 using DAmgBJd = DAmg<double, BSORd>;
 using DAmgSORf = DAmg<float, SORf>;
 using DAmgBJf = DAmg<float, BSORf>;
+using DAmgJSd = DAmg<double, JSORd>;
+using DAmgJSf = DAmg<float, JSORf>;
 #define SC23_DAMG_FIELDS(PFX, CLASS, T, SM)                  \
   SC23_ACCESS(PFX##_a, CLASS, SpMat<T>*, A)                  \
   SC23_ACCESS(PFX##_ca, CLASS, CSR<T>**, C_A)                \
@@ -164,6 +193,8 @@ SC23_DAMG_FIELDS(das, DAmgSORd, double, SORd)
 SC23_DAMG_FIELDS(dab, DAmgBJd, double, BSORd)
 SC23_DAMG_FIELDS(fas, DAmgSORf, float, SORf)
 SC23_DAMG_FIELDS(fab, DAmgBJf, float, BSORf)
+SC23_DAMG_FIELDS(daj, DAmgJSd, double, JSORd)
+SC23_DAMG_FIELDS(faj, DAmgJSf, float, JSORf)
 
 // ---------------------------------------------------------------------------
 struct Out {
@@ -227,7 +258,7 @@ void dump_int(Out& o, IA* amg, int smoother, int b_num, double omega,
               CSRflex** (IA::*r_ptr), direct::LUflex* (IA::*lu_ptr),
               int(IA::*depth_ptr), int(IA::*lastn_ptr),
               ldtri::SpMat<int>*(SM::*sm_ld), SpMat<int>*(SM::*sm_u),
-              bool bj) {
+              int ldk) {
   const int depth = amg->*depth_ptr;
   put_header(o, 2, depth, smoother, b_num, amg->*lastn_ptr, omega, theta);
   for (int l = 0; l < depth; ++l) {
@@ -239,9 +270,12 @@ void dump_int(Out& o, IA* amg, int smoother, int b_num, double omega,
     }
     SM* s = (amg->*sm_ptr)[l];
     ldtri::SpMat<int>* ld = s->*sm_ld;
-    if (bj) {
+    if (ldk == 1) {
       auto& L = *static_cast<ldtri::BJCSRflex*>(ld);
       PUT_CSR(o, L, bjflex, L.*get(bjflex_bit()), L.*get(bjflex_dbit()));
+    } else if (ldk == 2) {
+      auto& L = *static_cast<ldtri::MCCSRflex*>(ld);
+      PUT_CSR(o, L, mcflex, L.*get(mcflex_bit()), L.*get(mcflex_dbit()));
     } else {
       auto& L = *static_cast<ldtri::CSRflex*>(ld);
       PUT_CSR(o, L, ldflex, L.*get(ldflex_bit()), L.*get(ldflex_dbit()));
@@ -302,6 +336,10 @@ SC23_DUMP_FP(dump_fas, DAmgSORf, fas, float, SORf, sorf_ld, sorf_u, LDf, ldf,
              lf_l, uf_u, luf_l, luf_u, csrf, 1)
 SC23_DUMP_FP(dump_fab, DAmgBJf, fab, float, BSORf, bsorf_ld, bsorf_u, BJf, bjf,
              lf_l, uf_u, luf_l, luf_u, csrf, 1)
+SC23_DUMP_FP(dump_daj, DAmgJSd, daj, double, JSORd, jsord_ld, jsord_u, MCd, mcd,
+             ld_l, ud_u, lud_l, lud_u, csrd, 0)
+SC23_DUMP_FP(dump_faj, DAmgJSf, faj, float, JSORf, jsorf_ld, jsorf_u, MCf, mcf,
+             lf_l, uf_u, luf_l, luf_u, csrf, 1)
 
 namespace {
 
@@ -331,21 +369,43 @@ struct TracePre : public Preconditioner<int> {
   }
 };
 
+// SOR and JacobiSOR take SORParam; BlockSOR takes BlockSORParam.
+SmootherParam* make_sp(int smoother, double omega, int b_num) {
+  if (smoother == 1) return new BlockSORParam(omega, 1, b_num);
+  return new SORParam(omega, 1);
+}
+
+template <typename T>
+Preconditioner<T>* make_damg(int smoother, CSRMat* m, CSR<T>* a, AMGParam* prm) {
+  if constexpr (std::is_same_v<T, double>) {
+    if (smoother == 0) return new DAmgSORd(m, a, prm);
+    if (smoother == 1) return new DAmgBJd(m, a, prm);
+    return new DAmgJSd(m, a, prm);
+  } else {
+    if (smoother == 0) return new DAmgSORf(m, a, prm);
+    if (smoother == 1) return new DAmgBJf(m, a, prm);
+    return new DAmgJSf(m, a, prm);
+  }
+}
+
+Preconditioner<int>* make_iamg(int smoother, CSRMat* m, CSRFix30* a, AMGParam* prm) {
+  if (smoother == 0) return new IAmgSOR(m, a, prm);
+  if (smoother == 1) return new IAmgBJ(m, a, prm);
+  return new IAmgJS(m, a, prm);
+}
+
 }  // namespace
 
 extern "C" {
 
 // Builds the reference hierarchy for solver `precision` (0 FP64, 1 FP32,
-// 2 INT) and smoother (0 SOR, 1 BlockSOR(omega, 1, b_num)) on the given
+// 2 INT) and smoother (0 SOR, 1 BlockSOR(omega, 1, b_num), 2 JacobiSOR) on the given
 // row-scaled matrix; writes it to `path`.  Returns depth.
 int ref_build_hier(int n, const int* rptr, const int* cind, const double* val,
                    int precision, int smoother, double omega, int b_num,
                    double theta, const char* path) {
   CSRMat* m = make_mat(n, rptr, cind, val);
-  auto* sor = new SORParam(omega, 1);
-  auto* bsor = new BlockSORParam(omega, 1, b_num);
-  SmootherParam* sp = smoother == 0 ? (SmootherParam*)sor : (SmootherParam*)bsor;
-  AMGParam* prm = new AMGParam(20, theta, 1, sp);
+  AMGParam* prm = new AMGParam(20, theta, 1, make_sp(smoother, omega, b_num));
   FILE* f = fopen(path, "wb");
   if (!f) return -1;
   Out o{f};
@@ -357,15 +417,23 @@ int ref_build_hier(int n, const int* rptr, const int* cind, const double* val,
       dump_int<IAmgSOR, SORi, ldtri::CSRflex>(
           o, amg, 0, b_num, omega, theta, ia, get(ias_sm()), get(ias_ca()),
           get(ias_p()), get(ias_r()), get(ias_lu()), get(ias_depth()),
-          get(ias_lastn()), get(sori_ld()), get(sori_u()), false);
+          get(ias_lastn()), get(sori_ld()), get(sori_u()), 0);
       depth = amg->*get(ias_depth());
+      delete amg;
+    } else if (smoother == 2) {
+      auto* amg = new IAmgJS(m, ia, prm);
+      dump_int<IAmgJS, JSORi, ldtri::MCCSRflex>(
+          o, amg, 2, b_num, omega, theta, ia, get(iaj_sm()), get(iaj_ca()),
+          get(iaj_p()), get(iaj_r()), get(iaj_lu()), get(iaj_depth()),
+          get(iaj_lastn()), get(jsori_ld()), get(jsori_u()), 2);
+      depth = amg->*get(iaj_depth());
       delete amg;
     } else {
       auto* amg = new IAmgBJ(m, ia, prm);
       dump_int<IAmgBJ, BSORi, ldtri::BJCSRflex>(
           o, amg, 1, b_num, omega, theta, ia, get(iab_sm()), get(iab_ca()),
           get(iab_p()), get(iab_r()), get(iab_lu()), get(iab_depth()),
-          get(iab_lastn()), get(bsori_ld()), get(bsori_u()), true);
+          get(iab_lastn()), get(bsori_ld()), get(bsori_u()), 1);
       depth = amg->*get(iab_depth());
       delete amg;
     }
@@ -376,6 +444,10 @@ int ref_build_hier(int n, const int* rptr, const int* cind, const double* val,
       auto* amg = new DAmgSORd(m, a, prm);
       dump_das(o, amg, 0, b_num, omega, theta, a);
       depth = amg->*get(das_depth());
+    } else if (smoother == 2) {
+      auto* amg = new DAmgJSd(m, a, prm);
+      dump_daj(o, amg, 2, b_num, omega, theta, a);
+      depth = amg->*get(daj_depth());
     } else {
       auto* amg = new DAmgBJd(m, a, prm);
       dump_dab(o, amg, 1, b_num, omega, theta, a);
@@ -387,6 +459,10 @@ int ref_build_hier(int n, const int* rptr, const int* cind, const double* val,
       auto* amg = new DAmgSORf(m, a, prm);
       dump_fas(o, amg, 0, b_num, omega, theta, a);
       depth = amg->*get(fas_depth());
+    } else if (smoother == 2) {
+      auto* amg = new DAmgJSf(m, a, prm);
+      dump_faj(o, amg, 2, b_num, omega, theta, a);
+      depth = amg->*get(faj_depth());
     } else {
       auto* amg = new DAmgBJf(m, a, prm);
       dump_fab(o, amg, 1, b_num, omega, theta, a);
@@ -411,13 +487,8 @@ void* ref_int_amg_create(int n, const int* rptr, const int* cind,
   h->m = make_mat(n, rptr, cind, val);
   h->ia = new CSRFix30(h->m);
   h->n = n;
-  SmootherParam* sp = smoother == 0 ? (SmootherParam*)new SORParam(omega, 1)
-                                    : (SmootherParam*)new BlockSORParam(omega, 1, b_num);
-  AMGParam* prm = new AMGParam(20, theta, 1, sp);
-  if (smoother == 0)
-    h->pre = new IAmgSOR(h->m, h->ia, prm);
-  else
-    h->pre = new IAmgBJ(h->m, h->ia, prm);
+  AMGParam* prm = new AMGParam(20, theta, 1, make_sp(smoother, omega, b_num));
+  h->pre = make_iamg(smoother, h->m, h->ia, prm);
   return h;
 }
 
@@ -450,36 +521,26 @@ int ref_solve(int n, const int* rptr, const int* cind, const double* val,
   double nrm_b = blas1::Nrm2<double>(b.data(), n);
   auto* dA = new CSR<double>(mat);
   const int outer = 1000 / m;
-  SmootherParam* sp = smoother == 0 ? (SmootherParam*)new SORParam(omega, 1)
-                                    : (SmootherParam*)new BlockSORParam(omega, 1, b_num);
-  AMGParam* prm = new AMGParam(20, theta, 1, sp);
+  AMGParam* prm = new AMGParam(20, theta, 1, make_sp(smoother, omega, b_num));
   utils::Set<double>(0, x, n);
   Converge cv(false, 0);
   *trace_len = 0;
   if (solver == 0) {
-    Preconditioner<double>* M =
-        smoother == 0 ? (Preconditioner<double>*)new DAmgSORd(mat, dA, prm)
-                      : (Preconditioner<double>*)new DAmgBJd(mat, dA, prm);
+    Preconditioner<double>* M = make_damg<double>(smoother, mat, dA, prm);
     auto* inner = new FGMRES<double>(dA, m, eps, M);
     auto* s = new Restarted(dA, inner, outer, eps);
     cv = s->Solve(b.data(), nrm_b, x);
   } else if (solver == 1) {
     auto* fA = new CSR<float>(mat);
-    Preconditioner<float>* M =
-        smoother == 0 ? (Preconditioner<float>*)new DAmgSORf(mat, fA, prm)
-                      : (Preconditioner<float>*)new DAmgBJf(mat, fA, prm);
+    Preconditioner<float>* M = make_damg<float>(smoother, mat, fA, prm);
     auto* inner = new FGMRES<float>(fA, m, eps, M);
     auto* s = new Restarted(dA, inner, outer, eps);
     cv = s->Solve(b.data(), nrm_b, x);
   } else {
     auto* iA = new CSRFix30(mat);
-    Preconditioner<int>* iM =
-        smoother == 0 ? (Preconditioner<int>*)new IAmgSOR(mat, iA, prm)
-                      : (Preconditioner<int>*)new IAmgBJ(mat, iA, prm);
+    Preconditioner<int>* iM = make_iamg(smoother, mat, iA, prm);
     if (bit < 0) {
-      Preconditioner<double>* M =
-          smoother == 0 ? (Preconditioner<double>*)new DAmgSORd(mat, dA, prm)
-                        : (Preconditioner<double>*)new DAmgBJd(mat, dA, prm);
+      Preconditioner<double>* M = make_damg<double>(smoother, mat, dA, prm);
       std::vector<double> c(n);
       M->Precondition(b.data(), c.data());
       double c_max = 0;
@@ -520,7 +581,7 @@ static void stage(const char* what) {
   fflush(stderr);
 }
 
-// CPU baseline: <system file> <smoother 0|1> <solver 0|1|2> <m> [bit]
+// CPU baseline: <system file> <smoother 0|1|2> <solver 0|1|2> <m> [bit]
 // The system file ("SC23SYS1", int32 n, nnz, rptr, cind, double val, b) holds
 // the unscaled matrix and right-hand side; scaling and nrm_b follow
 // Work/main.cpp:61-63.
@@ -553,16 +614,11 @@ int main(int argc, char** argv) {
   const int outer = 1000 / m;
   const double epsilon = argc > 6 ? atof(argv[6]) : 1.0e-10;
   double* x = utils::SafeMalloc<double>(n);
-  auto sor_param = new SORParam(1.0, 1);
-  auto hybrid_param = new BlockSORParam(1.0, 1, 40);
-  SmootherParam* sp = smoother_type == 0 ? (SmootherParam*)sor_param
-                                         : (SmootherParam*)hybrid_param;
+  SmootherParam* sp = make_sp(smoother_type, 1.0, 40);
   Converge cv(false, 0);
   if (solver_type == 0) {
     Preconditioner<double>* M =
-        smoother_type == 0
-            ? (Preconditioner<double>*)new DAmgSORd(Abase, dA, new AMGParam(20, 0.8, 1, sp))
-            : (Preconditioner<double>*)new DAmgBJd(Abase, dA, new AMGParam(20, 0.8, 1, sp));
+        make_damg<double>(smoother_type, Abase, dA, new AMGParam(20, 0.8, 1, sp));
     auto inner = new FGMRES<double>(dA, m, epsilon, M);
     auto s = new Restarted(dA, inner, outer, epsilon);
     utils::Set<double>(0, x, n);
@@ -572,9 +628,7 @@ int main(int argc, char** argv) {
   } else if (solver_type == 1) {
     auto* fA = new CSR<float>(Abase);
     Preconditioner<float>* M =
-        smoother_type == 0
-            ? (Preconditioner<float>*)new DAmgSORf(Abase, fA, new AMGParam(20, 0.8, 1, sp))
-            : (Preconditioner<float>*)new DAmgBJf(Abase, fA, new AMGParam(20, 0.8, 1, sp));
+        make_damg<float>(smoother_type, Abase, fA, new AMGParam(20, 0.8, 1, sp));
     auto inner = new FGMRES<float>(fA, m, epsilon, M);
     auto s = new Restarted(dA, inner, outer, epsilon);
     utils::Set<double>(0, x, n);
@@ -585,15 +639,9 @@ int main(int argc, char** argv) {
     auto iA = new CSRFix30(Abase);
     Preconditioner<int>* iM;
     Preconditioner<double>* M;
-    if (smoother_type == 0) {
-      M = new DAmgSORd(Abase, dA, new AMGParam(20, 0.8, 1, sp));
-      stage("FP64 AMG setup done");
-      iM = new IAmgSOR(Abase, iA, new AMGParam(20, 0.8, 1, sp));
-    } else {
-      M = new DAmgBJd(Abase, dA, new AMGParam(20, 0.8, 1, sp));
-      stage("FP64 AMG setup done");
-      iM = new IAmgBJ(Abase, iA, new AMGParam(20, 0.8, 1, sp));
-    }
+    M = make_damg<double>(smoother_type, Abase, dA, new AMGParam(20, 0.8, 1, sp));
+    stage("FP64 AMG setup done");
+    iM = make_iamg(smoother_type, Abase, iA, new AMGParam(20, 0.8, 1, sp));
     stage("integer AMG setup done");
     int bit;
     if (argc > 5 && atoi(argv[5]) > 0) {

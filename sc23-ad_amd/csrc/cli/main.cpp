@@ -45,6 +45,7 @@ int usage() {
   std::fprintf(stderr,
                "usage: sc23ad_run <matrix> <smoother> <solver> <m> [options]\n"
                "       sc23ad_run --sub <matrix> <smoother> <m> <bit> [options]\n"
+               "smoother: 0 SOR, 1 BlockSOR(40), 2 JacobiSOR (1200-row blocks)\n"
                "options: --eps E  --bit F  --seed S  --quiet\n");
   return 2;
 }
@@ -99,7 +100,7 @@ int main(int argc, char** argv) {
   const int solver = sub ? SC23AD_INT : std::atoi(pos[2].c_str());
   const int m = std::atoi((sub ? pos[2] : pos[3]).c_str());
   if (sub) bit_opt = std::atoi(pos[3].c_str());
-  if (smoother < 0 || smoother > 1 || solver < 0 || solver > 2 || m < 1) return usage();
+  if (smoother < 0 || smoother > 2 || solver < 0 || solver > 2 || m < 1) return usage();
 
   sc23ad_csrmat* A = nullptr;
   const bool gen = generated(mat, &A);

@@ -213,8 +213,8 @@ int sc23ad_hier_build_multi(const sc23ad_csrmat* a, const sc23ad_amg_param* p,
     if (precisions[k] < 0 || precisions[k] > 2)
       return fail(SC23AD_BAD_PARAM, "hier_build: precision must be 0, 1 or 2");
   }
-  if (p->smoother != 0 && p->smoother != 1)
-    return fail(SC23AD_BAD_PARAM, "hier_build: smoother must be 0 or 1");
+  if (p->smoother < 0 || p->smoother > 2)
+    return fail(SC23AD_BAD_PARAM, "hier_build: smoother must be 0, 1 or 2");
   if (a->a.n != a->a.m) return fail(SC23AD_BAD_PARAM, "hier_build: matrix must be square");
   const AmgParams prm = to_params(p);
   try {

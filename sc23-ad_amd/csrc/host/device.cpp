@@ -46,7 +46,7 @@ void DevBuf::alloc(size_t bytes) {
 }
 
 void DevBuf::release() {
-  if (p_) hipFree(p_);
+  if (p_) (void)hipFree(p_);
   p_ = nullptr;
   bytes_ = 0;
 }
@@ -1000,12 +1000,19 @@ void EventTimer::begin() {
     hip_check(hipEventCreate(&e), "event");
     ev.push_back(e);
   }
+  // The start event is recorded on the stream; a timing-aware launcher that
+  // takes the armed pair re-binds both events to its own dispatch.
   hip_check(hipEventRecord(ev[2 * used], lib_stream()), "record");
+  dev::arm_dispatch_events(ev[2 * used], ev[2 * used + 1]);
 }
 
 void EventTimer::end() {
   if (!on) return;
-  hip_check(hipEventRecord(ev[2 * used + 1], lib_stream()), "record");
+  if (dev::take_dispatch_events().start) {
+    // no launcher took the events: time the stream interval instead
+    hip_check(hipEventRecord(ev[2 * used + 1], lib_stream()), "record");
+    ++stream_timed;
+  }
   ++used;
 }
 
@@ -1023,11 +1030,12 @@ void EventTimer::collect() {
 void EventTimer::reset() {
   used = 0;
   launches = 0;
+  stream_timed = 0;
   ms_total = 0;
 }
 
 EventTimer::~EventTimer() {
-  for (hipEvent_t e : ev) hipEventDestroy(e);
+  for (hipEvent_t e : ev) (void)hipEventDestroy(e);
 }
 
 void DevSolver::read_flags() {
@@ -1150,8 +1158,8 @@ DevSolver::Result DevSolver::solve(const double* b_dev, double nrm_b, double* x_
   spmv_timer.collect();
   amg->trsv0_timer.collect();
   amg->check_errors(*work);
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   return res;
 }
 

@@ -100,11 +100,14 @@ struct TriPlan {
   dev::TriLayout layout;
 };
 
-// hipEvent pairs bracketing launches on the library stream; elapsed times
-// are read by collect() once the stream has been synchronised.
+// hipEvent pairs timing launches on the library stream: bound to the kernel's
+// own dispatch when the launcher supports it (the kernel's execution time),
+// else recorded around the launch on the stream (stream_timed counts those).
+// Elapsed times are read by collect() once the stream has been synchronised.
 struct EventTimer {
   bool on = false;
   int launches = 0;
+  int stream_timed = 0;
   double ms_total = 0;
   void begin();
   void end();

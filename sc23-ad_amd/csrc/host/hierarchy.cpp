@@ -88,6 +88,31 @@ void smoother_split(const HostCsr& a, const AmgParams& prm, HostCsr& ld,
     }
     return;
   }
+  if (prm.smoother == SmootherKind::kJacobiSOR) {
+    // stationary.hpp:586-631: left = start of the row's 1200-row block.
+    for (int i = 0; i < n; ++i) {
+      const int left = (i / kJacobiSorBlock) * kJacobiSorBlock;
+      for (int p = a.rptr[i]; p < a.rptr[i + 1]; ++p) {
+        const int c = a.cind[p];
+        const double v = a.val[p];
+        if (c == i) {
+          ld.cind.push_back(c);
+          ld.val.push_back((1.0 / omega) * v);
+          u.cind.push_back(c);
+          u.val.push_back((1 - 1.0 / omega) * v);
+        } else if (c < left) {
+          ld.cind.push_back(c);
+          ld.val.push_back(v);
+        } else {
+          u.cind.push_back(c);
+          u.val.push_back(v);
+        }
+      }
+      ld.rptr[i + 1] = static_cast<int>(ld.cind.size());
+      u.rptr[i + 1] = static_cast<int>(u.cind.size());
+    }
+    return;
+  }
   const int bn = prm.b_num;
   const int size = (n + bn - 1) / bn;
   std::vector<int> bj(bn + 1);

@@ -24,7 +24,13 @@
 
 namespace sc23ad {
 
-enum class SmootherKind : int { kSOR = 0, kBlockSOR = 1 };
+// kJacobiSOR: JacobiSORflex / JacobiSOR<T> (Library/smoother/stationary.hpp:
+// 447-562, 564-676): rows in consecutive blocks of kJacobiSorBlock; LD keeps
+// the entries left of the row's block (plus the diagonal), so the rows of a
+// block are independent of each other (the multicolour LD of ldtri::MCCSR*,
+// matrix/trimat/ldtri.hpp:528-702).
+enum class SmootherKind : int { kSOR = 0, kBlockSOR = 1, kJacobiSOR = 2 };
+constexpr int kJacobiSorBlock = 1200;  // stationary.hpp:474, 587 (fixed there)
 enum class Precision : int { kFP64 = 0, kFP32 = 1, kINT = 2 };
 
 struct AmgParams {

@@ -31,6 +31,17 @@ struct DCsr {
   const void* sval = nullptr;
 };
 
+// Dispatch-recorded timing.  When armed, the next launch by a timing-aware
+// launcher on this thread (the SpMV launchers and the entry-parallel LD solve)
+// goes through hipExtLaunchKernel with these events, so their interval is the
+// kernel's own execution time, without the queue gap before it.  The launch
+// disarms them; take_dispatch_events() returns and disarms what is left.
+struct DispatchEvents {
+  hipEvent_t start = nullptr, stop = nullptr;
+};
+void arm_dispatch_events(hipEvent_t start, hipEvent_t stop);
+DispatchEvents take_dispatch_events();
+
 enum class Elem : int { kF64 = 0, kF32 = 1, kI32 = 2 };
 enum class SpmvMode : int { kPlain = 0, kSub = 1, kAdd = 2 };
 // Triangular factor layouts:

@@ -5,6 +5,8 @@
 // split across lanes.  FP kernels (CSR<T>::SpMV spmat.hpp:43-52 and the
 // triangular solves) keep each row's sum strictly in column order, one thread
 // per row, so that the device rounds exactly like the CPU.
+#include <hip/hip_ext.h>
+
 #include <type_traits>
 
 #include "dev_common.hpp"
@@ -13,8 +15,30 @@
 namespace sc23ad::dev {
 
 namespace {
+thread_local DispatchEvents t_armed;
+}  // namespace
+
+void arm_dispatch_events(hipEvent_t start, hipEvent_t stop) { t_armed = {start, stop}; }
+
+DispatchEvents take_dispatch_events() {
+  const DispatchEvents e = t_armed;
+  t_armed = {};
+  return e;
+}
+
+namespace {
 
 constexpr int kBlock = 256;
+
+// A launch that honours armed dispatch events (arm_dispatch_events).
+template <typename F, typename... A>
+void launch_timed(F kern, dim3 grid, dim3 block, uint32_t lds, hipStream_t s, A... args) {
+  const DispatchEvents ev = take_dispatch_events();
+  if (ev.start)
+    hipExtLaunchKernelGGL(kern, grid, block, lds, s, ev.start, ev.stop, 0u, args...);
+  else
+    hipLaunchKernelGGL(kern, grid, block, lds, s, args...);
+}
 
 template <int G, int MODE>
 __global__ __launch_bounds__(kBlock) void k_spmv_int(
@@ -143,10 +167,9 @@ hipError_t spmv_int_g(const DCsr& a, const void* x, const void* z, void* y,
   const long long threads = static_cast<long long>(a.n) * G;
   const int grid = static_cast<int>((threads + kBlock - 1) / kBlock);
   if (grid == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_spmv_int<G, MODE>), dim3(grid), dim3(kBlock), 0, s, a.n,
-                     a.rptr, a.cind, static_cast<const int32_t*>(a.val), a.bit,
-                     static_cast<const int32_t*>(x), static_cast<const int32_t*>(z),
-                     static_cast<int32_t*>(y));
+  launch_timed(&k_spmv_int<G, MODE>, dim3(grid), dim3(kBlock), 0, s, a.n, a.rptr, a.cind,
+               static_cast<const int32_t*>(a.val), a.bit, static_cast<const int32_t*>(x),
+               static_cast<const int32_t*>(z), static_cast<int32_t*>(y));
   return hipGetLastError();
 }
 
@@ -169,10 +192,9 @@ hipError_t spmv_fp(const DCsr& a, const void* x, const void* z, void* y,
                    hipStream_t s) {
   const int grid = (a.n + kBlock - 1) / kBlock;
   if (grid == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_spmv_fp<T, MODE>), dim3(grid), dim3(kBlock), 0, s, a.n,
-                     a.rptr, a.cind, static_cast<const T*>(a.val),
-                     static_cast<const T*>(x), static_cast<const T*>(z),
-                     static_cast<T*>(y));
+  launch_timed(&k_spmv_fp<T, MODE>, dim3(grid), dim3(kBlock), 0, s, a.n, a.rptr, a.cind,
+               static_cast<const T*>(a.val), static_cast<const T*>(x),
+               static_cast<const T*>(z), static_cast<T*>(y));
   return hipGetLastError();
 }
 
@@ -1054,10 +1076,10 @@ hipError_t tri_ep_go(const TriLayout& L, const TriSegment& seg, int bit, int dbi
     if (e != hipSuccess) return e;
     attr = true;
   }
-  hipLaunchKernelGGL((k_tri_chain_ep<K, kWaves, kFar, kE, kR>), dim3(seg.nchain),
-                     dim3(64 * kWaves), lds, s, reinterpret_cast<const int4*>(L.lent),
-                     static_cast<const int32_t*>(L.ldiag), L.lev_ptr, L.lev_eptr, seg.chain_lev,
-                     L.ring - 1, bit, dbit, in, out);
+  launch_timed(&k_tri_chain_ep<K, kWaves, kFar, kE, kR>, dim3(seg.nchain), dim3(64 * kWaves),
+               static_cast<uint32_t>(lds), s, reinterpret_cast<const int4*>(L.lent),
+               static_cast<const int32_t*>(L.ldiag), L.lev_ptr, L.lev_eptr, seg.chain_lev,
+               L.ring - 1, bit, dbit, in, out);
   return hipGetLastError();
 }
 
@@ -1141,14 +1163,13 @@ hipError_t spmv_sell(const DCsr& a, const void* x, const void* z, void* y, hipSt
   const int grid = (a.n + kBlock - 1) / kBlock;
   if (grid == 0) return hipSuccess;
   if constexpr (std::is_same_v<T, int32_t>) {
-    hipLaunchKernelGGL((k_spmv_sell_int<MODE>), dim3(grid), dim3(kBlock), 0, s, a.n, a.soff,
-                       a.scol, static_cast<const int32_t*>(a.sval), a.bit,
-                       static_cast<const int32_t*>(x), static_cast<const int32_t*>(z),
-                       static_cast<int32_t*>(y));
+    launch_timed(&k_spmv_sell_int<MODE>, dim3(grid), dim3(kBlock), 0, s, a.n, a.soff, a.scol,
+                 static_cast<const int32_t*>(a.sval), a.bit, static_cast<const int32_t*>(x),
+                 static_cast<const int32_t*>(z), static_cast<int32_t*>(y));
   } else {
-    hipLaunchKernelGGL((k_spmv_sell_fp<T, MODE>), dim3(grid), dim3(kBlock), 0, s, a.n, a.soff,
-                       a.scol, static_cast<const T*>(a.sval), static_cast<const T*>(x),
-                       static_cast<const T*>(z), static_cast<T*>(y));
+    launch_timed(&k_spmv_sell_fp<T, MODE>, dim3(grid), dim3(kBlock), 0, s, a.n, a.soff, a.scol,
+                 static_cast<const T*>(a.sval), static_cast<const T*>(x),
+                 static_cast<const T*>(z), static_cast<T*>(y));
   }
   return hipGetLastError();
 }

@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libsc23ad.so")
 OK, BAD_PARAM, SETUP_FAIL, OVERFLOW, STAGNATION, HIP_ERROR, NOT_CONVERGED, \
     IO_ERROR, NO_DEVICE = range(9)
 FP64, FP32, INT = 0, 1, 2
-SOR, BLOCK_SOR = 0, 1
+SOR, BLOCK_SOR, JACOBI_SOR = 0, 1, 2
 GEN_POISSON2D, GEN_LAPLACE3D, GEN_DIFFUSION27, GEN_CONVDIFF2D, \
     GEN_INCLUSIONS3D = range(5)
 OP_SPMV_A, OP_SPMV_U, OP_SPMV_R, OP_SPMV_P, OP_TRSV_LD, OP_SMOOTH0, \

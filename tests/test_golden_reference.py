@@ -41,7 +41,7 @@ def test_generated_systems_match_fixture(golden, systems):
 
 
 @pytest.mark.parametrize("prec", [0, 1, 2], ids=["fp64", "fp32", "int"])
-@pytest.mark.parametrize("smoother", [0, 1], ids=["sor", "blocksor"])
+@pytest.mark.parametrize("smoother", [0, 1, 2], ids=["sor", "blocksor", "jacobisor"])
 def test_setup_byte_identical_to_reference(golden, systems, prec, smoother):
     for c in _cases(golden):
         h = S.Hierarchy(systems[c["name"]].a, prec, S.amg_param(smoother))
@@ -57,7 +57,7 @@ def oracle_hiers(golden, systems):
     out = {}
     for c in _cases(golden):
         a = systems[c["name"]].a
-        for smoother in (0, 1):
+        for smoother in (0, 1, 2):
             hs = S.Hierarchy.build_multi(a, [S.INT, S.FP64, S.FP32], S.amg_param(smoother))
             for prec, h in zip((S.INT, S.FP64, S.FP32), hs):
                 path = export(h)
@@ -69,12 +69,12 @@ def oracle_hiers(golden, systems):
 def test_default_bits_match_reference(golden, systems, oracle_hiers):
     for c in _cases(golden):
         sy = systems[c["name"]]
-        for smoother in (0, 1):
+        for smoother in (0, 1, 2):
             bit = oracle_hiers[(c["name"], S.FP64, smoother)].default_bits(sy.b, sy.nrm_b)
             assert bit == c["solves"][f"2_{smoother}"]["bit"], c["name"]
 
 
-@pytest.mark.parametrize("smoother", [0, 1], ids=["sor", "blocksor"])
+@pytest.mark.parametrize("smoother", [0, 1, 2], ids=["sor", "blocksor", "jacobisor"])
 def test_oracle_int_solve_matches_reference(golden, systems, oracle_hiers, smoother):
     for c in _cases(golden):
         sy = systems[c["name"]]
@@ -88,7 +88,7 @@ def test_oracle_int_solve_matches_reference(golden, systems, oracle_hiers, smoot
 
 
 @pytest.mark.parametrize("solver", [0, 1], ids=["fp64", "fp32"])
-@pytest.mark.parametrize("smoother", [0, 1], ids=["sor", "blocksor"])
+@pytest.mark.parametrize("smoother", [0, 1, 2], ids=["sor", "blocksor", "jacobisor"])
 def test_oracle_fp_solve_matches_reference(golden, systems, oracle_hiers, solver, smoother):
     for c in _cases(golden):
         sy = systems[c["name"]]
@@ -116,7 +116,7 @@ def test_live_reference(case, tmp_path):
     kind, params, seed = case
     sy = S.make_system(kind, params, seed)
     arrs = sy.a.arrays()
-    for smoother in (0, 1):
+    for smoother in (0, 1, 2):
         for prec in (0, 2):
             ref_path = str(tmp_path / f"ref_{prec}_{smoother}.bin")
             O.ref_build_hier(arrs, prec, smoother, ref_path)

@@ -1,5 +1,6 @@
 """The 128^3 7-point Laplacian (2,097,152 rows), FGMRES(30), BlockSOR(40):
-the benchmark workload at full size (MI355X).
+the benchmark workload at full size (MI355X), plus one integer V-cycle
+with each single-chain smoother (SOR, JacobiSOR).
 
 Checked bit for bit against the oracle: the level-0 integer SpMV, one integer
 and one FP64 V-cycle, and the whole integer solve (every V-cycle input and
@@ -72,3 +73,21 @@ def test_int_solve_to_1e8(big):
     assert np.array_equal(trace, to)
     assert cv.iter == ro.iters
     assert np.array_equal(x.view(np.uint64), xo.view(np.uint64))
+
+
+@pytest.mark.parametrize("smoother", [S.SOR, S.JACOBI_SOR], ids=["sor", "jacobisor"])
+def test_int_vcycle_single_chain_smoothers(big, smoother):
+    """SOR (382 wavefront levels at 128^3) and JacobiSOR (1200-row blocks):
+    one integer V-cycle bit for bit against the oracle."""
+    sy = big["sy"]
+    print(f"[setup] int hierarchy, smoother {smoother}", flush=True)
+    h = S.Hierarchy(sy.a, S.INT, S.amg_param(smoother))
+    fd, path = tempfile.mkstemp(suffix=".bin")
+    os.close(fd)
+    h.export(path)
+    oh = O.OracleHier(path)
+    os.unlink(path)
+    amg = S.Amg(h)
+    v = np.random.default_rng(6).integers(-(2**29), 2**29, sy.a.n).astype(np.int32)
+    print(f"[vcycle] int smoother {smoother}", flush=True)
+    assert np.array_equal(amg.precondition(v), oh.vcycle(v))

@@ -33,7 +33,7 @@ def built(golden):
         print(f"[setup] {c['name']}", flush=True)
         sy = system(c)
         hs = {}
-        for smoother in (0, 1):
+        for smoother in (0, 1, 2):
             multi = S.Hierarchy.build_multi(sy.a, [S.INT, S.FP64, S.FP32],
                                             S.amg_param(smoother))
             for prec, h in zip((S.INT, S.FP64, S.FP32), multi):
@@ -59,7 +59,7 @@ def _same(a, b):
 def test_level_ops(built, name, prec):
     c, sy, hs = built[name]
     rng = np.random.default_rng(17)
-    for smoother in (0, 1):
+    for smoother in (0, 1, 2):
         amg, oh = hs[(prec, smoother)]
         depth = amg.depth
         for lv in range(depth):
@@ -86,7 +86,7 @@ def test_level_ops(built, name, prec):
 def test_vcycle(built, name, prec):
     c, sy, hs = built[name]
     rng = np.random.default_rng(23)
-    for smoother in (0, 1):
+    for smoother in (0, 1, 2):
         amg, oh = hs[(prec, smoother)]
         for _ in range(2):  # the second call reuses every work buffer
             v = _rand(prec, sy.a.n, rng)
@@ -96,13 +96,13 @@ def test_vcycle(built, name, prec):
 def test_default_frac_bits(built):
     for name in CASES:
         c, sy, hs = built[name]
-        for smoother in (0, 1):
+        for smoother in (0, 1, 2):
             amg, _ = hs[(S.FP64, smoother)]
             assert amg.default_frac_bits(sy.b, sy.nrm_b) == c["solves"][f"2_{smoother}"]["bit"]
 
 
 @pytest.mark.parametrize("name", CASES)
-@pytest.mark.parametrize("smoother", [0, 1], ids=["sor", "blocksor"])
+@pytest.mark.parametrize("smoother", [0, 1, 2], ids=["sor", "blocksor", "jacobisor"])
 def test_int_solve_matches_oracle(built, golden, name, smoother):
     c, sy, hs = built[name]
     amg, oh = hs[(S.INT, smoother)]
@@ -125,7 +125,7 @@ def test_int_solve_matches_oracle(built, golden, name, smoother):
 @pytest.mark.parametrize("prec", [S.FP64, S.FP32], ids=["fp64", "fp32"])
 def test_fp_solve_matches_oracle(built, golden, name, prec):
     c, sy, hs = built[name]
-    for smoother in (0, 1):
+    for smoother in (0, 1, 2):
         amg, oh = hs[(prec, smoother)]
         sol = S.Solver(prec, sy.a, amg, m=c["m"], eps=golden["eps"])
         x, cv = sol.solve(sy.b, sy.nrm_b)
@@ -140,7 +140,7 @@ def test_fp_solve_matches_oracle(built, golden, name, prec):
 def test_sequential_mode_reproduces_reference(built, golden, name, solver):
     """With sequential FP reductions the device solve is the reference's."""
     c, sy, hs = built[name]
-    for smoother in (0, 1):
+    for smoother in (0, 1, 2):
         g = c["solves"][f"{solver}_{smoother}"]
         amg, _ = hs[(solver, smoother)]
         kw = {"bit": g["bit"]} if solver == S.INT else {}

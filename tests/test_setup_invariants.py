@@ -1,8 +1,10 @@
 """Host-side AMG setup: invariants that fix its mathematics (CPU).
 
-  * SOR / BlockSOR split (smoother/stationary.hpp:385-396, 941-978):
-    A = strict lower part of LD + diag(1 / stored inverse) + U, where U holds
-    the explicit (1 - 1/omega) a_ii slot first in each row;
+  * SOR / BlockSOR / JacobiSOR split (smoother/stationary.hpp:385-396,
+    586-631, 941-978): A = strict lower part of LD + diag(1 / stored inverse)
+    + U, where U holds the explicit (1 - 1/omega) a_ii slot (first in each
+    SOR row, in column order otherwise); JacobiSOR's LD holds only columns
+    left of the row's 1200-row block, so one block's rows are independent;
   * BlockSOR with one block is SOR;
   * classical RS interpolation (helper/helper_multigrid.hpp:457-532): the
     rows of C points are unit rows, one per coarse column, in order;
@@ -48,7 +50,8 @@ def _hier(case, prec=S.FP64, smoother=S.SOR, **kw):
 
 @pytest.mark.parametrize("case", list(CASES))
 @pytest.mark.parametrize("smoother,omega", [(S.SOR, 1.0), (S.SOR, 1.3), (S.BLOCK_SOR, 1.0),
-                                            (S.BLOCK_SOR, 0.8)])
+                                            (S.BLOCK_SOR, 0.8), (S.JACOBI_SOR, 1.0),
+                                            (S.JACOBI_SOR, 1.2)])
 def test_split_sums_to_a(case, smoother, omega):
     _, hf = _hier(case, S.FP64, smoother, omega=omega)
     for lvl, L in enumerate(hf.levels):
@@ -60,6 +63,8 @@ def test_split_sums_to_a(case, smoother, omega):
             assert p1 > p0
             for p in range(p0, p1 - 1):
                 assert LD.cind[p] < i
+                if smoother == S.JACOBI_SOR:
+                    assert LD.cind[p] < (i // 1200) * 1200
                 rebuilt[i, LD.cind[p]] += LD.val[p]
             assert LD.cind[p1 - 1] == i          # inverse diagonal stored last
             diag_ld = 1.0 / LD.val[p1 - 1]       # a_ii / omega
@@ -165,3 +170,22 @@ def test_1d_poisson_alternates():
     for i in range(1, n - 1):
         if not is_c[i]:
             np.testing.assert_allclose(P.val[P.rptr[i]:P.rptr[i + 1]], [0.5, 0.5])
+
+
+def test_jacobisor_single_block_is_damped_jacobi():
+    """n <= 1200: LD = D / omega (diagonal only), U = A - D / omega."""
+    sy = S.make_system(S.GEN_POISSON2D, (32,), 0)     # 1024 rows, one smoothed level
+    path = export(S.Hierarchy(sy.a, S.FP64, S.amg_param(S.JACOBI_SOR, omega=0.7)))
+    try:
+        h = read_hier(path)
+    finally:
+        os.unlink(path)
+    L = h.levels[0]
+    assert L["A"].n <= 1200
+    A, LD, U = L["A"], L["LD"], L["U"]
+    assert np.array_equal(LD.rptr, np.arange(A.n + 1))
+    assert np.array_equal(LD.cind, np.arange(A.n))
+    dense = _csr(A).toarray()
+    np.testing.assert_allclose(1.0 / LD.val, np.diag(dense) / 0.7, rtol=1e-14)
+    np.testing.assert_allclose(_csr(U).toarray(), dense - np.diag(np.diag(dense)) / 0.7,
+                               rtol=1e-13, atol=1e-15)
