@@ -76,7 +76,7 @@ int sc23ad_dev_alloc(size_t bytes, void** ptr) {
 }
 
 int sc23ad_dev_free(void* ptr) {
-  if (ptr) hipFree(ptr);
+  if (ptr) (void)hipFree(ptr);
   return SC23AD_OK;
 }
 

@@ -346,6 +346,20 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
   plan.eval.upload(eval.data(), sizeof(V) * eval.size());
   plan.lpos.alloc(sizeof(int) * std::max<size_t>(lpos.size(), 1));
   plan.lpos.upload(lpos.data(), sizeof(int) * lpos.size());
+  // Integer level launches: lanes per row from the level's mean entry count
+  // (the smallest power of two >= mean, at most 64); FP rows stay one lane,
+  // which keeps the reference's summation order.
+  if (std::is_same_v<V, int32_t>) {
+    for (auto& sg : plan.segs) {
+      if (sg.chain_kernel) continue;
+      const int rows = sg.row_end - sg.row_begin;
+      const double mean =
+          rows > 0 ? static_cast<double>(lbeg[sg.row_end] - lbeg[sg.row_begin]) / rows : 0.0;
+      int g = 1;
+      while (g < 64 && g < mean) g *= 2;
+      sg.group = g;
+    }
+  }
   dev::TriLayout& L = plan.layout;
   L.lrow = plan.lrow.as<int>();
   L.lbeg = plan.lbeg.as<int>();
@@ -385,8 +399,27 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
     maxlev_chain = std::max(maxlev_chain, all_chain_lev[c + 1] - all_chain_lev[c]);
   int ep_ring = 2;
   while (ep_ring < 32768 && ep_ring < span) ep_ring *= 2;
-  if (std::is_same_v<V, int32_t> && plan.identity && one_launch_chains && maxlev_rows <= 1024 &&
-      dev::ep_lds_bytes(ep_ring, maxlev_chain) <= dev::kTriBlockedMaxLds) {
+  // Accumulator copies (LD factors only) where one instruction's lanes pile
+  // onto few rows: rows averaging at least 8 dependency entries and levels
+  // averaging at least 512 entries per chain (the dense coarse levels of the
+  // SOR ordering, 16-45 % faster there); on sparser or narrower levels the
+  // extra reads of the finish cost more than the collisions (BlockSOR's
+  // levels measured 5-15 % slower with them).  SC23AD_EP_COPIES=0/1 forces
+  // them off/on where they fit.
+  const size_t offdiag = lcol.size();
+  const size_t nlev_all = std::max<size_t>(lev_ptr.size() - 1, 1);
+  const char* env_cp = std::getenv("SC23AD_EP_COPIES");
+  bool copies = kind == TriKind::kLD && offdiag >= 8 * static_cast<size_t>(std::max(n, 1)) &&
+                offdiag >= 512 * nlev_all;
+  if (env_cp) copies = kind == TriKind::kLD && env_cp[0] == '1';
+  const int acc_rows = static_cast<int>((maxlev_rows + 63) / 64 * 64);
+  if (copies && dev::ep_lds_bytes(ep_ring, maxlev_chain, acc_rows * dev::kEpCopies) >
+                    dev::kTriBlockedMaxLds)
+    copies = false;
+  const int ep_acc = copies ? acc_rows * dev::kEpCopies : dev::kEpMaxRows;
+  if (std::is_same_v<V, int32_t> && plan.identity && one_launch_chains &&
+      maxlev_rows <= static_cast<size_t>(dev::kEpMaxRows) &&
+      dev::ep_lds_bytes(ep_ring, maxlev_chain, ep_acc) <= dev::kTriBlockedMaxLds) {
     std::vector<int> lev_of(static_cast<size_t>(std::max(n, 1)), 0);
     std::vector<int> lev_eptr(lev_ptr.size(), 0);
     for (size_t Lv = 0; Lv + 1 < lev_ptr.size(); ++Lv) {
@@ -433,6 +466,8 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
       L.ep_waves = w;
       L.ep_far = any_far ? 1 : 0;
       L.ep_maxlev = maxlev_chain;
+      L.ep_copies = copies ? dev::kEpCopies : 1;
+      L.ep_acc = ep_acc;
       // Narrow register layout (4 entries, 1 row per lane) when it holds
       // every row of a level and about as many entries.
       L.ep_narrow = (maxlev_rows <= static_cast<size_t>(w) * 64 && overflowing(w, 4) * 16 <= nlev)
@@ -445,10 +480,10 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
     std::fprintf(stderr,
                  "[sc23ad plan] kind %d n %d nnz %zu chains %d levels(max chain) %d "
                  "max level rows %zu launches %d ring %d span %lld max deps %d ep waves %d far %d "
-                 "narrow %d identity %d\n",
+                 "narrow %d copies %d identity %d\n",
                  static_cast<int>(kind), n, cind.size(), nchain, plan.nlevels, maxlev_rows,
                  plan.nlaunch, L.ring, span, maxdeps, L.ep_waves, L.ep_far, L.ep_narrow,
-                 plan.identity ? 1 : 0);
+                 L.ep_copies, plan.identity ? 1 : 0);
   }
 }
 

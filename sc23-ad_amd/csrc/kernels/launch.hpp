@@ -62,6 +62,7 @@ struct TriSegment {
   int nchain;
   const int* chain_lev;  // device, 2 * nchain ints: level range of each chain
   int row_begin, row_end;
+  int group = 1;  // chain_kernel 0, integer factors: lanes per row (power of two <= 64)
 };
 hipError_t launch_trsv_segment(Elem e, TriKind kind, const DCsr& a,
                                const int* perm, const int* lev_ptr,
@@ -93,16 +94,25 @@ struct TriLayout {
   int ep_far = 0;    // entries with dep < 0 (read from global memory) exist
   int ep_narrow = 0; // 4 entries and 1 row per lane in registers (else 8 and 2)
   int ep_maxlev = 0; // most levels in one chain
+  int ep_copies = 1; // accumulator copies per row (1 or kEpCopies)
+  int ep_acc = 0;    // accumulator slots in LDS
   int n = 0;
 };
 // Entry-parallel integer solve (one workgroup per chain).
 constexpr int kEpEntriesPerLane = 8;  // entry-parallel kernel: entries held in registers
 constexpr int kEpFarLevels = 40;  // least level distance of a dependency read from global memory
+constexpr int kEpMaxRows = 1024;  // most rows in one level of one chain
+// Rows with many entries: each row gets kEpCopies accumulators and lane t
+// adds into copy t % kEpCopies, so the lanes of one instruction, which hold
+// consecutive entries of few rows, collide kEpCopies times less in LDS.
+constexpr int kEpCopies = 8;
 hipError_t launch_trsv_ep(TriKind kind, const TriLayout& L, const TriSegment& seg, int bit,
                           int dbit, const void* in, void* out, hipStream_t s);
-// LDS bytes of the entry-parallel kernel.
-inline size_t ep_lds_bytes(int ring, int maxlev) {
-  return sizeof(uint64_t) * 1024 + sizeof(int32_t) * static_cast<size_t>(ring) +
+// LDS bytes of the entry-parallel kernel: acc_slots uint64 accumulators, the
+// ring, the chain's level offsets.
+inline size_t ep_lds_bytes(int ring, int maxlev, int acc_slots) {
+  return sizeof(uint64_t) * static_cast<size_t>(acc_slots) +
+         sizeof(int32_t) * static_cast<size_t>(ring) +
          2 * sizeof(int) * (static_cast<size_t>(maxlev) + 1);
 }
 hipError_t launch_trsv_layout(Elem e, TriKind kind, const TriLayout& L,

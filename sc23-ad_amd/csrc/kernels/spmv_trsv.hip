@@ -865,18 +865,19 @@ __global__ __launch_bounds__(kChainBlock) void k_tri_chain_pre(
 // fence every kEpFence levels makes such values visible before any load of
 // them is issued.  The chain's level offsets are staged in LDS, so the loads
 // of a level depend only on LDS data: entries, right-hand sides and inverse
-// diagonals of level L + kEpDepth are loaded once level L is done (into the
+// diagonals of level L + kD are loaded once level L is done (into the
 // registers it has just freed), far values of level L + kEpFarAhead (whose
 // addresses need the entries) at the start of level L.  Every load is issued
 // unconditionally (lanes past a level's end read element 0 and ignore it):
 // loads under divergent branches make the compiler wait for all outstanding
 // loads at the join.
 
-constexpr int kEpMaxLevel = 1024;
-constexpr int kEpDepth = 4;     // prefetch distance in levels
+// Prefetch distance kD in levels: 4 for the wide register layout (8 entries,
+// 2 rows per lane), 8 for the narrow one (4 entries, 1 row), whose buffers
+// are half the size.  A level costs at least the global-load latency / kD.
 constexpr int kEpFarAhead = 2;  // far-value distance in levels
 constexpr int kEpFence = 32;    // full fence period in levels (kFar kernels)
-static_assert(kEpFarAhead < kEpDepth, "far values need their entries loaded first");
+static_assert(kEpFarAhead < 4, "far values need their entries loaded first");
 static_assert(kEpFarLevels >= kEpFarAhead + kEpFence,
               "far dependencies must be fenced before their values are loaded");
 
@@ -943,8 +944,9 @@ struct EpShared {
 
 // One level of the chain; `cur` holds the level's prefetched data, `ahead`
 // the buffer of level L + kEpFarAhead.  Level indices past the chain's end
-// are clamped to an empty level.
-template <TriKind K, int kWaves, bool kFar, int kE, int kR>
+// are clamped to an empty level.  kC accumulator copies per row: lane t adds
+// into copy t % kC of its entry's row; the row's thread sums the copies.
+template <TriKind K, int kWaves, bool kFar, int kE, int kR, int kC, int kD>
 __device__ __forceinline__ void ep_level(EpPre<kE, kR>& cur, EpPre<kE, kR>& ahead, int L,
                                          int nl, const EpShared& sh,
                                          const int4* __restrict__ lent,
@@ -958,6 +960,7 @@ __device__ __forceinline__ void ep_level(EpPre<kE, kR>& cur, EpPre<kE, kR>& ahea
   uint64_t* acc = sh.acc;
   int32_t* ring = sh.ring;
   const int rmask = sh.rmask;
+  const int cp = t & (kC - 1);
   if (kFar) {
     const int Lf = min(L + kEpFarAhead, nl);
     ep_far<kThreads>(ahead, sh.lvo[Lf], sh.lvo[min(Lf + 1, nl)], out);
@@ -978,35 +981,54 @@ __device__ __forceinline__ void ep_level(EpPre<kE, kR>& cur, EpPre<kE, kR>& ahea
   for (int q = 0; q < kE; ++q) {
     const int p = e0 + t + q * kThreads;
     if (p < e1)
-      atomicAdd(reinterpret_cast<unsigned long long*>(&acc[cur.slot[q]]),
+      atomicAdd(reinterpret_cast<unsigned long long*>(&acc[cur.slot[q] * kC + cp]),
                 static_cast<unsigned long long>(prod64(cur.val[q], xv[q])));
   }
   for (int p = e0 + t + kE * kThreads; p < e1; p += kThreads) {
     const int4 w = lent[p];
     const int32_t x = (kFar && w.x < 0) ? out[-w.x - 1] : ring[w.x & rmask];
-    atomicAdd(reinterpret_cast<unsigned long long*>(&acc[w.z]),
+    atomicAdd(reinterpret_cast<unsigned long long*>(&acc[w.z * kC + cp]),
               static_cast<unsigned long long>(prod64(w.y, x)));
   }
   ep_sync<kWaves>();
   uint64_t av[kR];
+  if (kC == 1) {
 #pragma unroll
-  for (int q = 0; q < kR; ++q) av[q] = acc[t + q * kThreads];
+    for (int q = 0; q < kR; ++q) av[q] = acc[t + q * kThreads];
+  } else {
+#pragma unroll
+    for (int q = 0; q < kR; ++q) {
+      const int k = t + q * kThreads;
+      uint64_t sum = 0;
+      if (s + k < e) {
+#pragma unroll
+        for (int c = 0; c < kC; ++c) {
+          sum += acc[k * kC + c];
+          acc[k * kC + c] = 0;
+        }
+      }
+      av[q] = sum;
+    }
+  }
 #pragma unroll
   for (int q = 0; q < kR; ++q) {
     const int k = t + q * kThreads;
     if (s + k < e) {
       const uint64_t tt =
           (static_cast<uint64_t>(static_cast<int64_t>(cur.in[q])) << bit) - av[q];
-      acc[k] = 0;
+      if (kC == 1) acc[k] = 0;
       const int32_t v = acc_final<int32_t, K>(tt, cur.dinv[q], bit, dbit);
       ring[(s + k) & rmask] = v;
       out[s + k] = v;
     }
   }
   for (int k = t + kR * kThreads; s + k < e; k += kThreads) {
-    const uint64_t tt =
-        (static_cast<uint64_t>(static_cast<int64_t>(in[s + k])) << bit) - acc[k];
-    acc[k] = 0;
+    uint64_t sum = 0;
+    for (int c = 0; c < kC; ++c) {
+      sum += acc[k * kC + c];
+      acc[k * kC + c] = 0;
+    }
+    const uint64_t tt = (static_cast<uint64_t>(static_cast<int64_t>(in[s + k])) << bit) - sum;
     const int32_t v = acc_final<int32_t, K>(tt, dinv[s + k], bit, dbit);
     ring[(s + k) & rmask] = v;
     out[s + k] = v;
@@ -1015,80 +1037,93 @@ __device__ __forceinline__ void ep_level(EpPre<kE, kR>& cur, EpPre<kE, kR>& ahea
     __syncthreads();  // full fence: global results visible to later loads
   else
     ep_sync<kWaves>();
-  const int Lp = min(L + kEpDepth, nl);
+  const int Lp = min(L + kD, nl);
   ep_load<kThreads>(cur, sh.lvo[Lp], sh.lvo[min(Lp + 1, nl)], lent, dinv, in);
 }
 
-// LDS: kEpMaxLevel uint64 accumulators, the ring, then the chain's level
-// offsets (max_levels + 1 int2).
-template <TriKind K, int kWaves, bool kFar, int kE, int kR>
+// LDS: acc_slots uint64 accumulators (kC per row of the widest level), the
+// ring, then the chain's level offsets (max_levels + 1 int2).
+template <TriKind K, int kWaves, bool kFar, int kE, int kR, int kC, int kD>
 __global__ __launch_bounds__(64 * kWaves) void k_tri_chain_ep(
     const int4* __restrict__ lent, const int32_t* __restrict__ dinv,
     const int* __restrict__ lev_ptr, const int* __restrict__ lev_eptr,
-    const int* __restrict__ chain_lev, int rmask, int bit, int dbit,
+    const int* __restrict__ chain_lev, int rmask, int acc_slots, int bit, int dbit,
     const int32_t* __restrict__ in, int32_t* out) {
   constexpr int kThreads = 64 * kWaves;
   extern __shared__ __align__(16) unsigned char smem_e[];
   EpShared sh;
-  sh.acc = reinterpret_cast<uint64_t*>(smem_e);  // kEpMaxLevel
-  sh.ring = reinterpret_cast<int32_t*>(sh.acc + kEpMaxLevel);
+  sh.acc = reinterpret_cast<uint64_t*>(smem_e);
+  sh.ring = reinterpret_cast<int32_t*>(sh.acc + acc_slots);
   sh.rmask = rmask;
   int2* lvo = reinterpret_cast<int2*>(sh.ring + rmask + 1);
   sh.lvo = lvo;
   const int t = threadIdx.x;
   const int lb = chain_lev[2 * blockIdx.x], le = chain_lev[2 * blockIdx.x + 1];
   const int nl = le - lb;
-  for (int k = t; k < kEpMaxLevel; k += kThreads) sh.acc[k] = 0;
+  for (int k = t; k < acc_slots; k += kThreads) sh.acc[k] = 0;
   for (int k = t; k <= nl; k += kThreads) lvo[k] = make_int2(lev_ptr[lb + k], lev_eptr[lb + k]);
   __syncthreads();
-  EpPre<kE, kR> buf[kEpDepth];
+  EpPre<kE, kR> buf[kD];
 #pragma unroll
-  for (int d = 0; d < kEpDepth; ++d) {
+  for (int d = 0; d < kD; ++d) {
     const int Lc = min(d, nl);
     ep_load<kThreads>(buf[d], lvo[Lc], lvo[min(Lc + 1, nl)], lent, dinv, in);
   }
   // Dependencies stay inside a chain, so its first kEpFarLevels levels have
   // no far entries: the first far loads are issued from the loop.
   int L0 = 0;
-  for (; L0 + kEpDepth <= nl; L0 += kEpDepth) {
+  for (; L0 + kD <= nl; L0 += kD) {
 #pragma unroll
-    for (int d = 0; d < kEpDepth; ++d)
-      ep_level<K, kWaves, kFar>(buf[d], buf[(d + kEpFarAhead) % kEpDepth], L0 + d, nl, sh, lent,
-                                dinv, in, out, bit, dbit);
+    for (int d = 0; d < kD; ++d)
+      ep_level<K, kWaves, kFar, kE, kR, kC, kD>(buf[d], buf[(d + kEpFarAhead) % kD], L0 + d, nl,
+                                                sh, lent, dinv, in, out, bit, dbit);
   }
 #pragma unroll
-  for (int d = 0; d < kEpDepth; ++d)
+  for (int d = 0; d < kD; ++d)
     if (L0 + d < nl)
-      ep_level<K, kWaves, kFar>(buf[d], buf[(d + kEpFarAhead) % kEpDepth], L0 + d, nl, sh, lent,
-                                dinv, in, out, bit, dbit);
+      ep_level<K, kWaves, kFar, kE, kR, kC, kD>(buf[d], buf[(d + kEpFarAhead) % kD], L0 + d, nl,
+                                                sh, lent, dinv, in, out, bit, dbit);
 }
 
-template <TriKind K, int kWaves, bool kFar, int kE, int kR>
+template <TriKind K, int kWaves, bool kFar, int kE, int kR, int kC, int kD>
 hipError_t tri_ep_go(const TriLayout& L, const TriSegment& seg, int bit, int dbit,
                      const int32_t* in, int32_t* out, hipStream_t s) {
-  const size_t lds = ep_lds_bytes(L.ring, L.ep_maxlev);
-  if (lds > static_cast<size_t>(kTriBlockedMaxLds)) return hipErrorInvalidValue;
+  const size_t lds = ep_lds_bytes(L.ring, L.ep_maxlev, L.ep_acc);
+  if (lds > static_cast<size_t>(kTriBlockedMaxLds) || L.ep_acc < kC) return hipErrorInvalidValue;
   static bool attr = false;
   if (!attr) {
     hipError_t e = hipFuncSetAttribute(
-        reinterpret_cast<const void*>(&k_tri_chain_ep<K, kWaves, kFar, kE, kR>),
+        reinterpret_cast<const void*>(&k_tri_chain_ep<K, kWaves, kFar, kE, kR, kC, kD>),
         hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kTriBlockedMaxLds));
     if (e != hipSuccess) return e;
     attr = true;
   }
-  launch_timed(&k_tri_chain_ep<K, kWaves, kFar, kE, kR>, dim3(seg.nchain), dim3(64 * kWaves),
-               static_cast<uint32_t>(lds), s, reinterpret_cast<const int4*>(L.lent),
-               static_cast<const int32_t*>(L.ldiag), L.lev_ptr, L.lev_eptr, seg.chain_lev,
-               L.ring - 1, bit, dbit, in, out);
+  launch_timed(&k_tri_chain_ep<K, kWaves, kFar, kE, kR, kC, kD>, dim3(seg.nchain),
+               dim3(64 * kWaves), static_cast<uint32_t>(lds), s,
+               reinterpret_cast<const int4*>(L.lent), static_cast<const int32_t*>(L.ldiag),
+               L.lev_ptr, L.lev_eptr, seg.chain_lev, L.ring - 1, L.ep_acc, bit, dbit, in, out);
   return hipGetLastError();
 }
 
+template <TriKind K, int kWaves, int kC>
+hipError_t tri_ep_launch_c(const TriLayout& L, const TriSegment& seg, int bit, int dbit,
+                           const int32_t* in, int32_t* out, hipStream_t s) {
+  if (L.ep_far) return tri_ep_go<K, kWaves, true, 8, 2, kC, 4>(L, seg, bit, dbit, in, out, s);
+  if (L.ep_narrow)
+    return tri_ep_go<K, kWaves, false, 4, 1, kC, 8>(L, seg, bit, dbit, in, out, s);
+  return tri_ep_go<K, kWaves, false, 8, 2, kC, 4>(L, seg, bit, dbit, in, out, s);
+}
+
+// Accumulator copies only for the smoother's LD factors (rows with many
+// entries on coarse levels).
 template <TriKind K, int kWaves>
 hipError_t tri_ep_launch(const TriLayout& L, const TriSegment& seg, int bit, int dbit,
                          const int32_t* in, int32_t* out, hipStream_t s) {
-  if (L.ep_far) return tri_ep_go<K, kWaves, true, 8, 2>(L, seg, bit, dbit, in, out, s);
-  if (L.ep_narrow) return tri_ep_go<K, kWaves, false, 4, 1>(L, seg, bit, dbit, in, out, s);
-  return tri_ep_go<K, kWaves, false, 8, 2>(L, seg, bit, dbit, in, out, s);
+  if constexpr (K == TriKind::kLD) {
+    if (L.ep_copies == kEpCopies)
+      return tri_ep_launch_c<K, kWaves, kEpCopies>(L, seg, bit, dbit, in, out, s);
+  }
+  return tri_ep_launch_c<K, kWaves, 1>(L, seg, bit, dbit, in, out, s);
 }
 
 // One level, many workgroups, level-ordered layout.
@@ -1107,9 +1142,63 @@ __global__ __launch_bounds__(kBlock) void k_tri_level_pre(
   out[r.row] = pre_eval<T, K, false>(r, bit, dbit, lcol, lval, out, nullptr, 0);
 }
 
+// One level of an integer factor, `G` lanes per row: each lane sums a
+// strided share of the row's entries (integer sums are order-free), the
+// group reduces with shuffles and its first lane finishes the row.
+template <TriKind K, int G>
+__global__ __launch_bounds__(kBlock) void k_tri_level_g(
+    const int* __restrict__ lrow, const int* __restrict__ lbeg, const int* __restrict__ lcol,
+    const int32_t* __restrict__ lval, const int32_t* __restrict__ ldiag, int bit, int dbit,
+    int row_begin, int row_end, const int32_t* __restrict__ in, int32_t* out) {
+  const int gid = blockIdx.x * kBlock + threadIdx.x;
+  const int idx = row_begin + gid / G;
+  const int lane = gid & (G - 1);
+  uint64_t acc = 0;
+  if (idx < row_end) {
+    const int pb = lbeg[idx], pe = lbeg[idx + 1];
+    for (int p = pb + lane; p < pe; p += G) acc += prod64(lval[p], out[lcol[p]]);
+  }
+#pragma unroll
+  for (int m = G / 2; m > 0; m >>= 1) acc += shfl_xor_u64(acc, m);
+  if (idx < row_end && lane == 0) {
+    const int row = lrow[idx];
+    const uint64_t tt = (static_cast<uint64_t>(static_cast<int64_t>(in[row])) << bit) - acc;
+    out[row] = acc_final<int32_t, K>(tt, ldiag[idx], bit, dbit);
+  }
+}
+
+template <TriKind K>
+hipError_t tri_level_group(const TriLayout& L, const TriSegment& seg, int bit, int dbit,
+                           const int32_t* in, int32_t* out, hipStream_t s) {
+  const long long threads = static_cast<long long>(seg.row_end - seg.row_begin) * seg.group;
+  const int grid = static_cast<int>((threads + kBlock - 1) / kBlock);
+  if (grid <= 0) return hipSuccess;
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, s, L.lrow, L.lbeg, L.lcol,
+                       static_cast<const int32_t*>(L.lval),
+                       static_cast<const int32_t*>(L.ldiag), bit, dbit, seg.row_begin,
+                       seg.row_end, in, out);
+    return hipGetLastError();
+  };
+  switch (seg.group) {
+    case 2: return go(&k_tri_level_g<K, 2>);
+    case 4: return go(&k_tri_level_g<K, 4>);
+    case 8: return go(&k_tri_level_g<K, 8>);
+    case 16: return go(&k_tri_level_g<K, 16>);
+    case 32: return go(&k_tri_level_g<K, 32>);
+    case 64: return go(&k_tri_level_g<K, 64>);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 template <typename T, TriKind K>
 hipError_t tri_pre_seg(const TriLayout& L, const TriSegment& seg, int bit, int dbit,
                        const void* in, void* out, hipStream_t s) {
+  if constexpr (std::is_same_v<T, int32_t>) {
+    if (!seg.chain_kernel && seg.group > 1)
+      return tri_level_group<K>(L, seg, bit, dbit, static_cast<const int32_t*>(in),
+                                static_cast<int32_t*>(out), s);
+  }
   const T* vin = static_cast<const T*>(in);
   T* vout = static_cast<T*>(out);
   const T* lval = static_cast<const T*>(L.lval);
