@@ -429,8 +429,10 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run the copy kernel (MIB MiB per launch) for counter calibration")
     ap.add_argument("--workload", default="laplace3d",
-                    choices=("laplace3d", "batch", "learned", "scale"),
-                    help="laplace3d: the 3D 7-point solve (default); batch: uneven 2D "
+                    choices=("laplace3d", "poisson2d", "batch", "learned", "scale"),
+                    help="laplace3d: the 3D 7-point solve (default); poisson2d: the 2D "
+                         "5-point 256^2 case with SOR (the reference's CPU-sized case); "
+                         "batch: uneven 2D "
                          "convection-diffusion systems solved at once; learned: learned "
                          "per-level parameters on the 27-point diffusion system (--n 96); "
                          "scale: learned + batched right-hand sides on the 160^3 "
@@ -468,12 +470,22 @@ def main():
         return
 
     n = args.n
+    if args.workload == "poisson2d":
+        # the reference's own CPU-sized case: 2D 5-point 256^2, x* seed 0, SOR
+        n = args.n if "--n" in sys.argv else 256
+        if "--smoother" not in sys.argv:
+            args.smoother = 0
+        gen, seed, tag = S.GEN_POISSON2D, 0, "poisson2d"
+        desc = f"2D 5-point Poisson {n}^2"
+    else:
+        gen, seed, tag = S.GEN_LAPLACE3D, 1, "laplace3d"
+        desc = f"3D 7-point Laplacian {n}^3"
     t0 = time.time()
-    sy = S.make_system(S.GEN_LAPLACE3D, (n,), 1)
+    sy = S.make_system(gen, (n,), seed)
     N, _, nnz = sy.a.info()
-    log(f"system: 3D 7-point {n}^3, N={N}, nnz={nnz} ({time.time() - t0:.1f} s)")
+    log(f"system: {desc}, N={N}, nnz={nnz} ({time.time() - t0:.1f} s)")
     # unscaled copy for the CPU baseline (the reference scales it itself)
-    unscaled = S.CsrMat.generate(S.GEN_LAPLACE3D, n).arrays()
+    unscaled = S.CsrMat.generate(gen, n).arrays()
 
     t0 = time.time()
     prm = S.amg_param(args.smoother)
@@ -534,9 +546,9 @@ def main():
                  "solver1_fp32_s": t1s, "solver1_iter": cv1.iter, "solver1_relres": rel1}
 
     traffic = read_traffic(os.path.join(HERE, "profiles",
-                                        f"spmv_int_level0_laplace3d_{n}_pmc.txt"))
+                                        f"spmv_int_level0_{tag}_{n}_pmc.txt"))
     trsv_traffic = read_traffic(os.path.join(HERE, "profiles",
-                                             f"sptrsv_int_level0_laplace3d_{n}_pmc.txt"))
+                                             f"sptrsv_int_level0_{tag}_{n}_pmc.txt"))
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -556,9 +568,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int32/int64 (FP64 outer refinement)",
-        "data": "synthetic (generated 3D 7-point Laplacian, x* ~ U[-1,1] seed 1)",
-        "config": {"workload": f"int-AMG-FGMRES({args.m}) solve, 3D 7-point Laplacian "
-                               f"{n}^3 (N={N}, nnz={nnz}), "
+        "data": f"synthetic (generated {desc}, x* ~ U[-1,1] seed {seed})",
+        "config": {"workload": f"int-AMG-FGMRES({args.m}) solve, {desc} "
+                               f"(N={N}, nnz={nnz}), "
                                f"{SMOOTHER_NAMES[args.smoother]} smoother, "
                                f"rtol {args.rtol}",
                    "solver": 2, "restart_m": args.m, "frac_bits": bit,

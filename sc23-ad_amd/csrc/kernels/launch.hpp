@@ -108,10 +108,11 @@ constexpr int kEpMaxRows = 1024;  // most rows in one level of one chain
 constexpr int kEpCopies = 8;
 hipError_t launch_trsv_ep(TriKind kind, const TriLayout& L, const TriSegment& seg, int bit,
                           int dbit, const void* in, void* out, hipStream_t s);
-// LDS bytes of the entry-parallel kernel: acc_slots uint64 accumulators, the
-// ring, the chain's level offsets.
+// LDS bytes of the entry-parallel kernel: acc_slots uint64 accumulators and
+// kEpDummy spare ones, the ring, the chain's level offsets.
+constexpr int kEpDummy = 64;
 inline size_t ep_lds_bytes(int ring, int maxlev, int acc_slots) {
-  return sizeof(uint64_t) * static_cast<size_t>(acc_slots) +
+  return sizeof(uint64_t) * (static_cast<size_t>(acc_slots) + kEpDummy) +
          sizeof(int32_t) * static_cast<size_t>(ring) +
          2 * sizeof(int) * (static_cast<size_t>(maxlev) + 1);
 }

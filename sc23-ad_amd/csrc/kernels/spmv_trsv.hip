@@ -883,6 +883,7 @@ static_assert(kEpFarLevels >= kEpFarAhead + kEpFence,
 
 template <int kE, int kR>
 struct EpPre {
+  int2 lo, hi;  // the level's {position, entry} bounds
   int dep[kE];
   int32_t val[kE];
   int slot[kE];
@@ -897,6 +898,8 @@ __device__ __forceinline__ void ep_load(EpPre<kE, kR>& r, int2 lo, int2 hi,
                                         const int32_t* __restrict__ dinv,
                                         const int32_t* __restrict__ in) {
   const int t = threadIdx.x;
+  r.lo = lo;
+  r.hi = hi;
 #pragma unroll
   for (int q = 0; q < kE; ++q) {
     const int p = lo.y + t + q * kThreads;
@@ -915,13 +918,13 @@ __device__ __forceinline__ void ep_load(EpPre<kE, kR>& r, int2 lo, int2 hi,
 }
 
 template <int kThreads, int kE, int kR>
-__device__ __forceinline__ void ep_far(EpPre<kE, kR>& r, int2 lo, int2 hi, const int32_t* out) {
+__device__ __forceinline__ void ep_far(EpPre<kE, kR>& r, const int32_t* out) {
   const int t = threadIdx.x;
 #pragma unroll
   for (int q = 0; q < kE; ++q) {
-    const int p = lo.y + t + q * kThreads;
+    const int p = r.lo.y + t + q * kThreads;
     const int d = r.dep[q];
-    r.xfar[q] = out[(p < hi.y && d < 0) ? -d - 1 : 0];
+    r.xfar[q] = out[(p < r.hi.y && d < 0) ? -d - 1 : 0];
   }
 }
 
@@ -937,9 +940,10 @@ __device__ __forceinline__ void ep_sync() {
 
 struct EpShared {
   const int2* lvo;  // level offsets {position, entry}, LDS
-  uint64_t* acc;
+  uint64_t* acc;    // accumulators, then kEpDummy slots for lanes past a level's end
   int32_t* ring;
   int rmask;
+  int dummy;        // index of the first dummy slot
 };
 
 // One level of the chain; `cur` holds the level's prefetched data, `ahead`
@@ -955,16 +959,16 @@ __device__ __forceinline__ void ep_level(EpPre<kE, kR>& cur, EpPre<kE, kR>& ahea
                                          int dbit) {
   constexpr int kThreads = 64 * kWaves;
   const int t = threadIdx.x;
-  const int2 lo = sh.lvo[L], hi = sh.lvo[L + 1];
-  const int s = lo.x, e = hi.x, e0 = lo.y, e1 = hi.y;
+  const int s = cur.lo.x, e = cur.hi.x, e0 = cur.lo.y, e1 = cur.hi.y;
   uint64_t* acc = sh.acc;
   int32_t* ring = sh.ring;
   const int rmask = sh.rmask;
   const int cp = t & (kC - 1);
-  if (kFar) {
-    const int Lf = min(L + kEpFarAhead, nl);
-    ep_far<kThreads>(ahead, sh.lvo[Lf], sh.lvo[min(Lf + 1, nl)], out);
-  }
+  // Offsets of the level prefetched at the end of this one, read now so
+  // their LDS latency overlaps this level's work.
+  const int Lp = min(L + kD, nl);
+  const int2 plo = sh.lvo[Lp], phi = sh.lvo[min(Lp + 1, nl)];
+  if (kFar) ep_far<kThreads>(ahead, out);
   // All ring reads first (lanes past the level's end read a valid, unused
   // slot), then the atomics: one LDS round trip for the whole level.
   int32_t xv[kE];
@@ -977,12 +981,14 @@ __device__ __forceinline__ void ep_level(EpPre<kE, kR>& cur, EpPre<kE, kR>& ahea
     for (int q = 0; q < kE; ++q)
       if (__any(cur.dep[q] < 0)) xv[q] = cur.dep[q] < 0 ? cur.xfar[q] : xv[q];
   }
+  // Unconditional atomics: lanes past the level's end add into a dummy slot
+  // of their own, which keeps the LDS traffic free of branches and waits.
 #pragma unroll
   for (int q = 0; q < kE; ++q) {
     const int p = e0 + t + q * kThreads;
-    if (p < e1)
-      atomicAdd(reinterpret_cast<unsigned long long*>(&acc[cur.slot[q] * kC + cp]),
-                static_cast<unsigned long long>(prod64(cur.val[q], xv[q])));
+    const int a = p < e1 ? cur.slot[q] * kC + cp : sh.dummy + (t & (kEpDummy - 1));
+    atomicAdd(reinterpret_cast<unsigned long long*>(&acc[a]),
+              static_cast<unsigned long long>(prod64(cur.val[q], xv[q])));
   }
   for (int p = e0 + t + kE * kThreads; p < e1; p += kThreads) {
     const int4 w = lent[p];
@@ -1037,8 +1043,7 @@ __device__ __forceinline__ void ep_level(EpPre<kE, kR>& cur, EpPre<kE, kR>& ahea
     __syncthreads();  // full fence: global results visible to later loads
   else
     ep_sync<kWaves>();
-  const int Lp = min(L + kD, nl);
-  ep_load<kThreads>(cur, sh.lvo[Lp], sh.lvo[min(Lp + 1, nl)], lent, dinv, in);
+  ep_load<kThreads>(cur, plo, phi, lent, dinv, in);
 }
 
 // LDS: acc_slots uint64 accumulators (kC per row of the widest level), the
@@ -1053,7 +1058,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_tri_chain_ep(
   extern __shared__ __align__(16) unsigned char smem_e[];
   EpShared sh;
   sh.acc = reinterpret_cast<uint64_t*>(smem_e);
-  sh.ring = reinterpret_cast<int32_t*>(sh.acc + acc_slots);
+  sh.dummy = acc_slots;
+  sh.ring = reinterpret_cast<int32_t*>(sh.acc + acc_slots + kEpDummy);
   sh.rmask = rmask;
   int2* lvo = reinterpret_cast<int2*>(sh.ring + rmask + 1);
   sh.lvo = lvo;
