@@ -54,6 +54,19 @@ def sptrsv_int_bytes(n: int, nnz_ld: int) -> int:
 SMOOTHER_NAMES = {0: "SOR", 1: "BlockSOR(40)", 2: "JacobiSOR(1200)"}
 
 
+def _hier_sha(h) -> str:
+    import hashlib
+    import tempfile
+    fd, path = tempfile.mkstemp(suffix=".bin")
+    os.close(fd)
+    try:
+        h.export(path)
+        with open(path, "rb") as f:
+            return hashlib.sha256(f.read()).hexdigest()
+    finally:
+        os.unlink(path)
+
+
 def ld_nnz(rptr, cind, smoother: int, b_num: int = 40) -> int:
     """Entries of the smoother's LD factor (diagonal included): col <= i for
     SOR; left <= col <= i inside each of the b_num row blocks of ceil(N / b_num)
@@ -492,8 +505,16 @@ def main():
     precs = [S.INT, S.FP64] + ([] if args.no_fp else [S.FP32])
     hs = S.Hierarchy.build_multi(sy.a, precs, prm)
     hint, h64 = hs[0], hs[1]
+    setup_host = time.time() - t0
     depth, level_n, level_nnz = hint.info()
-    log(f"setup: depth {depth}, level rows {list(level_n)} ({time.time() - t0:.1f} s)")
+    log(f"setup: depth {depth}, level rows {list(level_n)} ({setup_host:.1f} s)")
+    # the same setup with the interpolation and Galerkin products on the GPU
+    t0 = time.time()
+    hs_dev = S.Hierarchy.build_multi(sy.a, precs, prm, device_setup=True)
+    setup_dev = time.time() - t0
+    same_setup = all(_hier_sha(h) == _hier_sha(d) for h, d in zip(hs, hs_dev))
+    del hs_dev
+    log(f"setup on the GPU: {setup_dev:.1f} s, identical hierarchies: {same_setup}")
 
     amg64 = S.Amg(h64)
     bit = amg64.default_frac_bits(sy.b, sy.nrm_b)
@@ -597,6 +618,8 @@ def main():
                             "nnz_ld": nnz_ld, "launches_timed": trsv_launches,
                             "mean_ms": trsv_ms},
         "cpu_baseline": cpu,
+        "setup_s": {"host": setup_host, "gpu_interp_galerkin": setup_dev,
+                    "identical": bool(same_setup), "precisions": len(precs)},
     }
     rec.update(extra)
     if cpu and cpu.get("value"):

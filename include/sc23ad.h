@@ -58,11 +58,14 @@ typedef enum {
 typedef enum { SC23AD_FP64 = 0, SC23AD_FP32 = 1, SC23AD_INT = 2 } sc23ad_precision;
 /* SOR (seq path), BlockSOR(b_num) (multi path) and the reference's opt-in
    JacobiSOR: 1200-row blocks, Gauss-Seidel across blocks, Jacobi inside one
-   (JacobiSORflex / JacobiSOR<T>, Library/smoother/stationary.hpp:447-676). */
+   (JacobiSORflex / JacobiSOR<T>, Library/smoother/stationary.hpp:447-676),
+   and damped Jacobi for the FP64 / FP32 solvers (Jacobi<CSR<T>>,
+   stationary.hpp:15-89; omega = JacobiParam::omega). */
 typedef enum {
   SC23AD_SOR = 0,
   SC23AD_BLOCK_SOR = 1,
-  SC23AD_JACOBI_SOR = 2
+  SC23AD_JACOBI_SOR = 2,
+  SC23AD_JACOBI = 3
 } sc23ad_smoother;
 
 /* Matrix kinds for sc23ad_csrmat_gen (parameters in brackets). */
@@ -135,6 +138,14 @@ int sc23ad_hier_build(const sc23ad_csrmat* a, const sc23ad_amg_param* p,
    for precisions[k]. */
 int sc23ad_hier_build_multi(const sc23ad_csrmat* a, const sc23ad_amg_param* p,
                             const int* precisions, int count, sc23ad_hier** outs);
+/* The same setup with the classical interpolation weights and both Galerkin
+   products of every coarsening step computed on the GPU
+   (get_interpolation_matrix and get_coarse_matrix,
+   Library/helper/helper_multigrid.hpp:457-586); the strength graph and the
+   C/F passes stay on the host.  Results are identical to
+   sc23ad_hier_build_multi.  NO_DEVICE without a GPU. */
+int sc23ad_hier_build_multi_dev(const sc23ad_csrmat* a, const sc23ad_amg_param* p,
+                                const int* precisions, int count, sc23ad_hier** outs);
 /* depth (number of coarsening steps); level_n / level_nnz receive depth+1
    entries (operator rows / nnz per level, the last one the coarsest grid). */
 int sc23ad_hier_info(const sc23ad_hier* h, int* depth, int* level_n,

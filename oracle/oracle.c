@@ -370,8 +370,11 @@ static void op_add(int prec, const void* x, void* y, int n) {
 
 static size_t esz(int prec) { return elem_size(prec); }
 
-/* Smooth(in, out, isInited) of SOR / BlockSOR (stationary.hpp:415-430,
-   997-1012). */
+/* Smooth(in, out, isInited) of SOR / BlockSOR / JacobiSOR
+   (stationary.hpp:415-430, 997-1012, 647-660).  Jacobi<CSR<T>>
+   (stationary.hpp:50-74; smoother 3, FP only): LD holds d_inv as its only
+   entry per row, U is A; first out = d_inv * in, then
+   out[i] = out[i] + d_inv[i] * (in[i] - (A out)[i]). */
 static void smooth(const or_hier* h, int l, const void* in, void* out,
                    int inited, void* temp) {
   const or_mat* ld = LV(h, l, MLD);
@@ -385,7 +388,19 @@ static void smooth(const or_hier* h, int l, const void* in, void* out,
   for (int k = k0; k < h->iter; ++k) {
     op_spmv(prec, u, out, temp);
     op_sub(prec, in, temp, n);
-    op_ld(prec, ld, temp, out);
+    if (h->smoother == 3 && prec == 0) {
+      const double* d = ld->val;
+      const double* t = temp;
+      double* o = out;
+      for (int i = 0; i < n; ++i) o[i] = o[i] + d[ld->rptr[i]] * t[i];
+    } else if (h->smoother == 3 && prec == 1) {
+      const float* d = ld->val;
+      const float* t = temp;
+      float* o = out;
+      for (int i = 0; i < n; ++i) o[i] = o[i] + d[ld->rptr[i]] * t[i];
+    } else {
+      op_ld(prec, ld, temp, out);
+    }
   }
 }
 

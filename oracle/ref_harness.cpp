@@ -120,6 +120,12 @@ SC23_ACCESS(jsord_ld, JSORd, ldtri::SpMat<double>*, LD)
 SC23_ACCESS(jsord_u, JSORd, SpMat<double>*, U)
 SC23_ACCESS(jsorf_ld, JSORf, ldtri::SpMat<float>*, LD)
 SC23_ACCESS(jsorf_u, JSORf, SpMat<float>*, U)
+using JACd = Jacobi<CSR<double>>;
+using JACf = Jacobi<CSR<float>>;
+SC23_ACCESS(jacd_lu, JACd, SpMat<double>*, LU)
+SC23_ACCESS(jacd_dinv, JACd, double*, d_inv)
+SC23_ACCESS(jacf_lu, JACf, SpMat<float>*, LU)
+SC23_ACCESS(jacf_dinv, JACf, float*, d_inv)
 SC23_ACCESS(sori_ld, SORi, ldtri::SpMat<int>*, LD)
 SC23_ACCESS(sori_u, SORi, SpMat<int>*, U)
 SC23_ACCESS(bsori_ld, BSORi, ldtri::SpMat<int>*, LD)
@@ -180,6 +186,8 @@ using DAmgSORf = DAmg<float, SORf>;
 using DAmgBJf = DAmg<float, BSORf>;
 using DAmgJSd = DAmg<double, JSORd>;
 using DAmgJSf = DAmg<float, JSORf>;
+using DAmgJacd = DAmg<double, JACd>;
+using DAmgJacf = DAmg<float, JACf>;
 #define SC23_DAMG_FIELDS(PFX, CLASS, T, SM)                  \
   SC23_ACCESS(PFX##_a, CLASS, SpMat<T>*, A)                  \
   SC23_ACCESS(PFX##_ca, CLASS, CSR<T>**, C_A)                \
@@ -195,6 +203,8 @@ SC23_DAMG_FIELDS(fas, DAmgSORf, float, SORf)
 SC23_DAMG_FIELDS(fab, DAmgBJf, float, BSORf)
 SC23_DAMG_FIELDS(daj, DAmgJSd, double, JSORd)
 SC23_DAMG_FIELDS(faj, DAmgJSf, float, JSORf)
+SC23_DAMG_FIELDS(dac, DAmgJacd, double, JACd)
+SC23_DAMG_FIELDS(fac, DAmgJacf, float, JACf)
 
 // ---------------------------------------------------------------------------
 struct Out {
@@ -341,6 +351,43 @@ SC23_DUMP_FP(dump_daj, DAmgJSd, daj, double, JSORd, jsord_ld, jsord_u, MCd, mcd,
 SC23_DUMP_FP(dump_faj, DAmgJSf, faj, float, JSORf, jsorf_ld, jsorf_u, MCf, mcf,
              lf_l, uf_u, luf_l, luf_u, csrf, 1)
 
+// Jacobi hierarchies: the "LD" factor is the diagonal d_inv (one entry per
+// row) and the "U" factor is the smoother's LU = A, as the product stores them.
+#define SC23_DUMP_JAC(FN, DA, PFX, T, SM, SMLU, SMDINV, LPFX, UPFX, LUL, LUU,    \
+                      CSRPFX, PREC)                                            \
+  static void FN(Out& o, DA* amg, int b_num, double omega, double theta,       \
+                 CSR<T>* a0) {                                                 \
+    const int depth = amg->*get(PFX##_depth());                                \
+    put_header(o, PREC, depth, 3, b_num, amg->*get(PFX##_lastn()), omega,      \
+               theta);                                                         \
+    for (int l = 0; l < depth; ++l) {                                          \
+      CSR<T>& a = (l == 0) ? *a0 : *(amg->*get(PFX##_ca()))[l - 1];            \
+      PUT_CSR(o, a, CSRPFX, 0, -1);                                            \
+      SM* s = (amg->*get(PFX##_sm()))[l];                                      \
+      CSR<T>& lu = *static_cast<CSR<T>*>(s->*get(SMLU()));                     \
+      const int n = lu.*get(CSRPFX##_n());                                     \
+      std::vector<int> rp(n + 1), ci(n);                                       \
+      for (int i = 0; i < n; ++i) rp[i] = ci[i] = i;                           \
+      rp[n] = n;                                                               \
+      put_mat(o, n, n, rp.data(), ci.data(), s->*get(SMDINV()), 0, -1);        \
+      PUT_CSR(o, lu, CSRPFX, 0, -1);                                           \
+      CSR<T>& r = *(amg->*get(PFX##_r()))[l];                                  \
+      PUT_CSR(o, r, CSRPFX, 0, -1);                                            \
+      CSR<T>& p = *(amg->*get(PFX##_p()))[l];                                  \
+      PUT_CSR(o, p, CSRPFX, 0, -1);                                            \
+    }                                                                          \
+    direct::LU<T>* lu = amg->*get(PFX##_lu());                                 \
+    auto& L = *(lu->*get(LUL()));                                              \
+    auto& U = *(lu->*get(LUU()));                                              \
+    PUT_CSR(o, L, LPFX, 0, -1);                                                \
+    PUT_CSR(o, U, UPFX, 0, -1);                                                \
+  }
+
+SC23_DUMP_JAC(dump_dac, DAmgJacd, dac, double, JACd, jacd_lu, jacd_dinv, ld_l, ud_u, lud_l,
+              lud_u, csrd, 0)
+SC23_DUMP_JAC(dump_fac, DAmgJacf, fac, float, JACf, jacf_lu, jacf_dinv, lf_l, uf_u, luf_l,
+              luf_u, csrf, 1)
+
 namespace {
 
 uint64_t mix64(uint64_t z) {
@@ -369,9 +416,10 @@ struct TracePre : public Preconditioner<int> {
   }
 };
 
-// SOR and JacobiSOR take SORParam; BlockSOR takes BlockSORParam.
+// SOR and JacobiSOR take SORParam, BlockSOR BlockSORParam, Jacobi JacobiParam.
 SmootherParam* make_sp(int smoother, double omega, int b_num) {
   if (smoother == 1) return new BlockSORParam(omega, 1, b_num);
+  if (smoother == 3) return new JacobiParam(omega, 1);
   return new SORParam(omega, 1);
 }
 
@@ -380,10 +428,12 @@ Preconditioner<T>* make_damg(int smoother, CSRMat* m, CSR<T>* a, AMGParam* prm) 
   if constexpr (std::is_same_v<T, double>) {
     if (smoother == 0) return new DAmgSORd(m, a, prm);
     if (smoother == 1) return new DAmgBJd(m, a, prm);
+    if (smoother == 3) return new DAmgJacd(m, a, prm);
     return new DAmgJSd(m, a, prm);
   } else {
     if (smoother == 0) return new DAmgSORf(m, a, prm);
     if (smoother == 1) return new DAmgBJf(m, a, prm);
+    if (smoother == 3) return new DAmgJacf(m, a, prm);
     return new DAmgJSf(m, a, prm);
   }
 }
@@ -410,7 +460,23 @@ int ref_build_hier(int n, const int* rptr, const int* cind, const double* val,
   if (!f) return -1;
   Out o{f};
   int depth = 0;
-  if (precision == 2) {
+  if (smoother == 3 && precision == 2) {
+    fclose(f);
+    return -1;  // the reference's integer Jacobi needs Fixed<bit> matrices
+  }
+  if (smoother == 3) {
+    if (precision == 0) {
+      auto* a = new CSR<double>(m);
+      auto* amg = new DAmgJacd(m, a, prm);
+      dump_dac(o, amg, b_num, omega, theta, a);
+      depth = amg->*get(dac_depth());
+    } else {
+      auto* a = new CSR<float>(m);
+      auto* amg = new DAmgJacf(m, a, prm);
+      dump_fac(o, amg, b_num, omega, theta, a);
+      depth = amg->*get(fac_depth());
+    }
+  } else if (precision == 2) {
     auto* ia = new CSRFix30(m);
     if (smoother == 0) {
       auto* amg = new IAmgSOR(m, ia, prm);
@@ -483,6 +549,7 @@ struct RefIntAmg {
 void* ref_int_amg_create(int n, const int* rptr, const int* cind,
                          const double* val, int smoother, double omega,
                          int b_num, double theta) {
+  if (smoother == 3) return nullptr;  // no integer Jacobi (FP only)
   auto* h = new RefIntAmg;
   h->m = make_mat(n, rptr, cind, val);
   h->ia = new CSRFix30(h->m);
@@ -516,6 +583,7 @@ int ref_solve(int n, const int* rptr, const int* cind, const double* val,
               int bit, double omega, int b_num, double theta, double* x,
               int* iters, int* converged, double* relres, int* bit_used,
               uint64_t* trace, int trace_cap, int* trace_len) {
+  if (smoother == 3 && solver == 2) return -1;  // no integer Jacobi (FP only)
   CSRMat* mat = make_mat(n, rptr, cind, val);
   std::vector<double> b(b_in, b_in + n);
   double nrm_b = blas1::Nrm2<double>(b.data(), n);
@@ -606,6 +674,7 @@ int main(int argc, char** argv) {
   CSRMat* Abase = new CSRMat(val, cind, rptr, n, n, MMShape::General, MMType::Real);
   int smoother_type = atoi(argv[2]);
   int solver_type = atoi(argv[3]);
+  if (smoother_type == 3 && solver_type == 2) return 4;  // no integer Jacobi
   int m = atoi(argv[4]);
   Abase->LMax1Scaling(b);
   double nrm_b = blas1::Nrm2<double>(b, n);

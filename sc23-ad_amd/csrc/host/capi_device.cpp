@@ -66,6 +66,14 @@ int sc23ad_device_count(int* count) {
   return SC23AD_OK;
 }
 
+int sc23ad_hier_build_multi_dev(const sc23ad_csrmat* a, const sc23ad_amg_param* p,
+                                const int* precisions, int count, sc23ad_hier** outs) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n < 1)
+    return fail(SC23AD_NO_DEVICE, "hier_build_multi_dev: no GPU");
+  return build_multi_impl(a, p, precisions, count, outs, true);
+}
+
 int sc23ad_dev_alloc(size_t bytes, void** ptr) {
   if (!ptr) return fail(SC23AD_BAD_PARAM, "dev_alloc: null");
   return guarded([&]() -> int {

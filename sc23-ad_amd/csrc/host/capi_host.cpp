@@ -10,6 +10,7 @@
 
 #include "../../../include/sc23ad.h"
 #include "capi_internal.hpp"
+#include "device.hpp"
 #include "learn.hpp"
 #include "gen.hpp"
 #include "hier_io.hpp"
@@ -206,6 +207,15 @@ static std::unique_ptr<sc23ad_hier> derive_hierarchy(
 
 int sc23ad_hier_build_multi(const sc23ad_csrmat* a, const sc23ad_amg_param* p,
                             const int* precisions, int count, sc23ad_hier** outs) {
+  return sc23ad::build_multi_impl(a, p, precisions, count, outs, false);
+}
+
+}  // extern "C"
+
+namespace sc23ad {
+
+int build_multi_impl(const sc23ad_csrmat* a, const sc23ad_amg_param* p, const int* precisions,
+                     int count, sc23ad_hier** outs, bool device_setup) {
   if (!a || !p || !precisions || !outs || count < 1)
     return fail(SC23AD_BAD_PARAM, "hier_build: bad arguments");
   for (int k = 0; k < count; ++k) {
@@ -213,10 +223,14 @@ int sc23ad_hier_build_multi(const sc23ad_csrmat* a, const sc23ad_amg_param* p,
     if (precisions[k] < 0 || precisions[k] > 2)
       return fail(SC23AD_BAD_PARAM, "hier_build: precision must be 0, 1 or 2");
   }
-  if (p->smoother < 0 || p->smoother > 2)
-    return fail(SC23AD_BAD_PARAM, "hier_build: smoother must be 0, 1 or 2");
+  if (p->smoother < 0 || p->smoother > 3)
+    return fail(SC23AD_BAD_PARAM, "hier_build: smoother must be 0, 1, 2 or 3");
+  for (int k = 0; k < count; ++k)
+    if (p->smoother == 3 && precisions[k] == SC23AD_INT)
+      return fail(SC23AD_BAD_PARAM, "hier_build: the Jacobi smoother is FP64 / FP32 only");
   if (a->a.n != a->a.m) return fail(SC23AD_BAD_PARAM, "hier_build: matrix must be square");
-  const AmgParams prm = to_params(p);
+  AmgParams prm = to_params(p);
+  prm.device_setup = device_setup;
   try {
     auto st = std::make_shared<Sc23adSetupState>();
     st->fine = a->a;
@@ -237,9 +251,19 @@ int sc23ad_hier_build_multi(const sc23ad_csrmat* a, const sc23ad_amg_param* p,
       outs[k] = nullptr;
     }
     return fail(SC23AD_SETUP_FAIL, "hier_build: out of host memory");
+  } catch (const DevError& e) {
+    for (int k = 0; k < count; ++k) {
+      delete outs[k];
+      outs[k] = nullptr;
+    }
+    return fail(e.code, "hier_build: " + e.what);
   }
   return SC23AD_OK;
 }
+
+}  // namespace sc23ad
+
+extern "C" {
 
 int sc23ad_hier_build(const sc23ad_csrmat* a, const sc23ad_amg_param* p,
                       int precision, sc23ad_hier** out) {

@@ -5,6 +5,7 @@
 #include <memory>
 #include <string>
 
+#include "../../../include/sc23ad.h"
 #include "hierarchy.hpp"
 
 struct sc23ad_csrmat_s {
@@ -32,4 +33,9 @@ struct sc23ad_hier_s {
 namespace sc23ad {
 extern thread_local std::string g_last_error;
 int fail(int code, const std::string& msg);
+// sc23ad_hier_build_multi[_dev]: one coarsening chain, then a hierarchy per
+// requested precision; device_setup runs the interpolation weights and the
+// Galerkin products on the GPU.
+int build_multi_impl(const sc23ad_csrmat* a, const sc23ad_amg_param* p, const int* precisions,
+                     int count, sc23ad_hier** outs, bool device_setup);
 }  // namespace sc23ad

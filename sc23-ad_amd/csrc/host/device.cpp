@@ -788,6 +788,7 @@ std::unique_ptr<DevAmg> DevAmg::upload_fp(const FpHierarchy<T>& h, const LaunchC
   a->iter = h.prm.iter;
   a->n_coarse = h.n_coarse;
   a->lv.resize(h.depth);
+  a->jacobi = h.prm.smoother == SmootherKind::kJacobi;
   const bool bj = h.prm.smoother == SmootherKind::kBlockSOR;
   LaunchCfg cf = c;
   cf.spmv_group = 1;
@@ -847,9 +848,10 @@ void DevAmg::trsv(const TriPlan& plan, const DevMat& m, const void* in, void* ou
               "trsv");
 }
 
-// SOR / BlockSOR Smooth (Library/smoother/stationary.hpp:415-430, 997-1012):
-// inited: iter x { t = in - U out; out = LD^-1 t }; otherwise out = LD^-1 in
-// first.
+// SOR / BlockSOR / JacobiSOR Smooth (Library/smoother/stationary.hpp:415-430,
+// 997-1012, 647-660): inited: iter x { t = in - U out; out = LD^-1 t };
+// otherwise out = LD^-1 in first.  Jacobi (stationary.hpp:50-74): LD^-1 is
+// the diagonal d_inv, U is A, and the update is out += d_inv t.
 void DevAmg::smooth(int l, const void* in, void* out, bool inited, Work& w) const {
   const DevLevel& L = lv[l];
   int k0 = 0;
@@ -863,6 +865,11 @@ void DevAmg::smooth(int l, const void* in, void* out, bool inited, Work& w) cons
   void* t = w.tmp[l].get();
   for (int k = k0; k < iter; ++k) {
     spmv(L.u, SpmvMode::kSub, out, in, t);
+    if (jacobi) {
+      hip_check(dev::launch_jacobi_update(elem, L.n, L.ld.d.val, t, out, lib_stream()),
+                "jacobi");
+      continue;
+    }
     if (tm) tm->begin();
     trsv(L.ld_plan, L.ld, t, out, nullptr, nullptr, nullptr);
     if (tm) tm->end();

@@ -144,6 +144,7 @@ class DevAmg {
   DevMat lc, uc;
   TriPlan lc_plan, uc_plan;
   int n_coarse = 0;
+  bool jacobi = false;   // Jacobi smoother: LD holds d_inv, U is A
   DevMat a0;             // fine-level operator in the original numbering (Arnoldi)
   LaunchCfg cfg;
   mutable EventTimer trsv0_timer;  // level-0 LD solves (benchmark hook)

@@ -31,7 +31,10 @@ AmgChain build_chain(const HostCsr& fine, const AmgParams& prm) {
   // INTAMG.hpp:162-195 / 38-51: coarsen until max_levels is reached or a
   // level has at most coarse_n rows.  Each step consumes the unscaled
   // previous coarse operator.
-  CoarsenStep st = rs_coarsen(fine, prm.theta);
+  auto coarsen = [&](const HostCsr& a) {
+    return prm.device_setup ? rs_coarsen_dev(a, prm.theta) : rs_coarsen(a, prm.theta);
+  };
+  CoarsenStep st = coarsen(fine);
   ch.warnings += st.warnings;
   ch.coarse.push_back(std::move(st.coarse));
   ch.interp.push_back(std::move(st.interp));
@@ -39,7 +42,7 @@ AmgChain build_chain(const HostCsr& fine, const AmgParams& prm) {
   ch.depth = 1;
   while (prm.max_levels > ch.depth && ch.coarse[ch.depth - 1].n > prm.coarse_n) {
     ++ch.depth;
-    CoarsenStep nx = rs_coarsen(ch.coarse[ch.depth - 2], prm.theta);
+    CoarsenStep nx = coarsen(ch.coarse[ch.depth - 2]);
     ch.warnings += nx.warnings;
     ch.coarse.push_back(std::move(nx.coarse));
     ch.interp.push_back(std::move(nx.interp));
@@ -352,6 +355,10 @@ IntCsr to_dinvu_flex(const HostCsr& du) {
 
 IntHierarchy build_int_hierarchy(const HostCsr& fine, const AmgChain& chain,
                                  const AmgParams& prm) {
+  if (prm.smoother == SmootherKind::kJacobi)
+    throw SetupError{kBadParam,
+                     "Jacobi smoother: FP precisions only (the reference's integer Jacobi "
+                     "needs Fixed<bit> matrices, which intAMGflex does not use)"};
   Lap lap;
   IntHierarchy h;
   h.prm = prm;
@@ -421,6 +428,33 @@ FpCsr<T> to_fp(const HostCsr& a) {
 }
 }  // namespace
 
+namespace {
+// Jacobi's d_inv (stationary.hpp:40-47): omega converted to T, divided in
+// double by the row's first diagonal entry, stored as T, as one diagonal
+// entry per row.
+template <typename T>
+FpCsr<T> jacobi_dinv(const HostCsr& a, double omega) {
+  FpCsr<T> o;
+  o.n = a.n;
+  o.m = a.m;
+  o.rptr.resize(static_cast<size_t>(a.n) + 1);
+  o.cind.resize(a.n);
+  o.val.resize(a.n);
+  const double w = static_cast<double>(static_cast<T>(omega));
+  for (int i = 0; i < a.n; ++i) {
+    int p = a.rptr[i];
+    while (p < a.rptr[i + 1] && a.cind[p] != i) ++p;
+    if (p == a.rptr[i + 1])
+      throw SetupError{kSetupFail, "Jacobi: row " + std::to_string(i) + " has no diagonal"};
+    o.rptr[i] = i;
+    o.cind[i] = i;
+    o.val[i] = static_cast<T>(w / a.val[p]);
+  }
+  o.rptr[a.n] = a.n;
+  return o;
+}
+}  // namespace
+
 template <typename T>
 FpHierarchy<T> build_fp_hierarchy(const HostCsr& fine, const AmgChain& chain,
                                   const AmgParams& prm) {
@@ -433,10 +467,17 @@ FpHierarchy<T> build_fp_hierarchy(const HostCsr& fine, const AmgChain& chain,
     FpLevel<T>& L = h.lv[l];
     const HostCsr& src = (l == 0) ? fine : chain.coarse[l - 1];
     L.a = to_fp<T>(src);
-    HostCsr ld, u;
-    smoother_split(src, prm, ld, u, l);
-    L.ld = to_fp<T>(extract_ld_inv(ld));
-    L.u = to_fp<T>(u);
+    if (prm.smoother == SmootherKind::kJacobi) {
+      const double omega =
+          static_cast<size_t>(l) < prm.level_omega.size() ? prm.level_omega[l] : prm.omega;
+      L.ld = jacobi_dinv<T>(src, omega);
+      L.u = L.a;
+    } else {
+      HostCsr ld, u;
+      smoother_split(src, prm, ld, u, l);
+      L.ld = to_fp<T>(extract_ld_inv(ld));
+      L.u = to_fp<T>(u);
+    }
     L.r = to_fp<T>(chain.restrict_[l]);
     L.p = to_fp<T>(chain.interp[l]);
   }

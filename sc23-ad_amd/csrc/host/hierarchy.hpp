@@ -29,7 +29,10 @@ namespace sc23ad {
 // the entries left of the row's block (plus the diagonal), so the rows of a
 // block are independent of each other (the multicolour LD of ldtri::MCCSR*,
 // matrix/trimat/ldtri.hpp:528-702).
-enum class SmootherKind : int { kSOR = 0, kBlockSOR = 1, kJacobiSOR = 2 };
+// kJacobi: Jacobi<CSR<T>> (stationary.hpp:15-89), FP hierarchies only:
+// out += d_inv (in - A out) with d_inv = omega / a_ii.  Its LD factor holds
+// d_inv as a diagonal "inverse" and its U factor is A itself.
+enum class SmootherKind : int { kSOR = 0, kBlockSOR = 1, kJacobiSOR = 2, kJacobi = 3 };
 constexpr int kJacobiSorBlock = 1200;  // stationary.hpp:474, 587 (fixed there)
 enum class Precision : int { kFP64 = 0, kFP32 = 1, kINT = 2 };
 
@@ -46,6 +49,7 @@ struct AmgParams {
   std::vector<double> level_omega;  // SOR weight of level l's smoother
   std::vector<int> level_bits;      // cap on the fractional bits of A_l, U_l and
                                     // LD_l's off-diagonal part (<= 0: none)
+  bool device_setup = false;        // interpolation and Galerkin products on the GPU
 };
 
 // The unscaled Galerkin chain every precision shares: coarse[l], interp[l] and

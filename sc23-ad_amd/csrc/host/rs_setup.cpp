@@ -231,8 +231,6 @@ int rs_second_pass(const HostCsr& s, std::vector<int>& cf) {
   return nc;
 }
 
-namespace {
-
 // sepalate_strong_to_coarse_fine (helper_multigrid.hpp:419-455).
 void split_strong(const HostCsr& s, const std::vector<int>& cf, HostCsr& cs,
                   HostCsr& ds) {
@@ -248,6 +246,8 @@ void split_strong(const HostCsr& s, const std::vector<int>& cf, HostCsr& cs,
     ds.rptr[i + 1] = static_cast<int>(ds.cind.size());
   }
 }
+
+namespace {
 
 // get_interpolation_matrix (helper_multigrid.hpp:457-532).  For an F point i
 // and each strong C neighbour j:

@@ -22,6 +22,9 @@ struct CoarsenStep {
 // Throws SetupError{kBadParam} for theta outside (0,1] and
 // SetupError{kSetupFail} where the reference calls exit().
 CoarsenStep rs_coarsen(const HostCsr& a, double theta);
+// The same step with the interpolation weights and both Galerkin products
+// computed on the GPU (setup_dev.cpp); identical results.
+CoarsenStep rs_coarsen_dev(const HostCsr& a, double theta);
 
 // Building blocks, exposed for the unit tests.
 void rs_strength(const HostCsr& a, double theta, HostCsr& strong,
@@ -30,6 +33,9 @@ std::vector<int> rs_first_pass(const HostCsr& strong, const HostCsr& strong_t,
                                int* warnings);
 int rs_second_pass(const HostCsr& strong, std::vector<int>& cf);
 HostCsr pattern_transpose(const HostCsr& s);
+// Strong connections to C points (cs) and to F points (ds), in row order.
+void split_strong(const HostCsr& s, const std::vector<int>& cf, HostCsr& cs,
+                  HostCsr& ds);
 HostCsr csr_transpose(const HostCsr& p);
 // out = L * R computed row by row as sum_k L(i,k) * R(k,:), merging in
 // ascending column order, y + alpha*x per shared column, dropping exact zeros

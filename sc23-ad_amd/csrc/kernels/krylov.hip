@@ -732,6 +732,33 @@ hipError_t launch_permute(Elem e, int n, const int* perm, const void* src, void*
   return hipGetLastError();
 }
 
+// Jacobi<CSR<T>>::Smooth update (stationary.hpp:56-61): out += d_inv * t,
+// with t = in - A out already formed; FP only, no contraction.
+template <typename T>
+__global__ __launch_bounds__(B) void k_jacobi_update(int n, const T* __restrict__ dinv,
+                                                     const T* __restrict__ t, T* out) {
+  const int k = blockIdx.x * B + threadIdx.x;
+  if (k >= n) return;
+  out[k] = out[k] + dinv[k] * t[k];
+}
+
+hipError_t launch_jacobi_update(Elem e, int n, const void* dinv, const void* t, void* out,
+                                hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int grid = (n + B - 1) / B;
+  if (e == Elem::kF64)
+    hipLaunchKernelGGL(k_jacobi_update<double>, dim3(grid), dim3(B), 0, s, n,
+                       static_cast<const double*>(dinv), static_cast<const double*>(t),
+                       static_cast<double*>(out));
+  else if (e == Elem::kF32)
+    hipLaunchKernelGGL(k_jacobi_update<float>, dim3(grid), dim3(B), 0, s, n,
+                       static_cast<const float*>(dinv), static_cast<const float*>(t),
+                       static_cast<float*>(out));
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
 hipError_t launch_copy(Elem e, int n, const void* src, void* dst, hipStream_t s) {
   const size_t bytes = static_cast<size_t>(n) * (e == Elem::kF64 ? 8 : 4);
   const size_t n4 = bytes / 4;

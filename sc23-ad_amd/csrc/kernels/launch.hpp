@@ -139,6 +139,30 @@ hipError_t launch_trsv_tiles(TriKind kind, int n, int nb, const int32_t* tiles,
 hipError_t launch_trsv_blocked(Elem e, TriKind kind, const DCsr& a, const int* bptr,
                                int nb, const void* in, void* out, hipStream_t s);
 
+// ---- AMG setup (setup.hip) -----------------------------------------------
+// Classical interpolation weights of the F rows `fine` into pval (P's CSR
+// values, row i starting at pptr[i]); status[0] |= 1 where an F-F connection
+// shares no C point, status[1] counts the amk = 0 substitutions.
+hipError_t launch_interp_weights(int nfine, const int* fine, const int* arp, const int* aci,
+                                 const double* av, const int* csp, const int* csc,
+                                 const double* csv, const int* dsp, const int* dsc,
+                                 const double* dsv, const int* wkp, const double* wkv,
+                                 const int* pptr, double* pval, int* status, hipStream_t s);
+// Row-merge product rows [row0, row1): row i hashes its terms in
+// tkey/tval[toff[i] - base_t ...) (toff[i+1] - toff[i] slots) and writes its
+// surviving entries, sorted, to ocol/oval[ooff[i] - base_o ...) and their
+// number to cnt[i - row0].
+hipError_t launch_rowmerge(int row0, int row1, const int* lp, const int* lc, const double* lv,
+                           const int* rp, const int* rc, const double* rv,
+                           const long long* toff, const long long* ooff, long long base_t,
+                           long long base_o, int* tkey, double* tval, int* ocol, double* oval,
+                           int* cnt, hipStream_t s);
+// Copies rows [row0, row1) of the compacted output to
+// cind/val[optr[i - row0] ...).
+hipError_t launch_rowmerge_gather(int row0, int row1, const long long* ooff, long long base_o,
+                                  const int* ocol, const double* oval, const int* optr,
+                                  int* cind, double* val, hipStream_t s);
+
 // ---- Krylov kernels ------------------------------------------------------
 // Scalars and small arrays of one FGMRES(m) solver, resident on the device.
 struct KState {
@@ -201,6 +225,9 @@ hipError_t launch_relres(const DCsr& a64, const double* b, const double* x,
 // V-cycle vector helpers
 hipError_t launch_copy(Elem e, int n, const void* src, void* dst, hipStream_t s);
 // gather: dst[k] = src[perm[k]]; otherwise scatter: dst[perm[k]] = src[k].
+// out[k] += dinv[k] * t[k] (FP; the Jacobi smoother's update).
+hipError_t launch_jacobi_update(Elem e, int n, const void* dinv, const void* t, void* out,
+                                hipStream_t s);
 hipError_t launch_permute(Elem e, int n, const int* perm, const void* src, void* dst,
                           bool gather, hipStream_t s);
 hipError_t launch_fill_zero(size_t bytes, void* dst, hipStream_t s);

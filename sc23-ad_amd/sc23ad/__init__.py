@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libsc23ad.so")
 OK, BAD_PARAM, SETUP_FAIL, OVERFLOW, STAGNATION, HIP_ERROR, NOT_CONVERGED, \
     IO_ERROR, NO_DEVICE = range(9)
 FP64, FP32, INT = 0, 1, 2
-SOR, BLOCK_SOR, JACOBI_SOR = 0, 1, 2
+SOR, BLOCK_SOR, JACOBI_SOR, JACOBI = 0, 1, 2, 3
 GEN_POISSON2D, GEN_LAPLACE3D, GEN_DIFFUSION27, GEN_CONVDIFF2D, \
     GEN_INCLUSIONS3D = range(5)
 OP_SPMV_A, OP_SPMV_U, OP_SPMV_R, OP_SPMV_P, OP_TRSV_LD, OP_SMOOTH0, \
@@ -73,6 +73,7 @@ def _load() -> C.CDLL:
         "sc23ad_hier_build": (C.c_int, [vp, C.POINTER(AmgParam), C.c_int, C.POINTER(vp)]),
         "sc23ad_hier_info": (C.c_int, [vp, ip, vp, vp]),
         "sc23ad_hier_build_multi": (C.c_int, [vp, C.POINTER(AmgParam), vp, C.c_int, vp]),
+        "sc23ad_hier_build_multi_dev": (C.c_int, [vp, C.POINTER(AmgParam), vp, C.c_int, vp]),
         "sc23ad_hier_export": (C.c_int, [vp, C.c_char_p]),
         "sc23ad_hier_destroy": (None, [vp]),
         "sc23ad_checksum_i32": (C.c_uint64, [vp, C.c_int]),
@@ -248,12 +249,16 @@ class Hierarchy:
         self.h = h
 
     @classmethod
-    def build_multi(cls, a: CsrMat, precisions, param: AmgParam | None = None):
-        """Hierarchies for several precisions from one coarsening chain."""
+    def build_multi(cls, a: CsrMat, precisions, param: AmgParam | None = None,
+                    device_setup: bool = False):
+        """Hierarchies for several precisions from one coarsening chain;
+        device_setup computes the interpolation weights and the Galerkin
+        products on the GPU (identical results)."""
         prm = param if param is not None else amg_param()
         pr = np.asarray(precisions, dtype=np.int32)
         outs = (C.c_void_p * len(pr))()
-        check(lib.sc23ad_hier_build_multi(a.h, C.byref(prm), ptr(pr), len(pr), outs))
+        fn = lib.sc23ad_hier_build_multi_dev if device_setup else lib.sc23ad_hier_build_multi
+        check(fn(a.h, C.byref(prm), ptr(pr), len(pr), outs))
         return [cls(None, int(p), _handle=C.c_void_p(outs[k])) for k, p in enumerate(pr)]
 
     def info(self):

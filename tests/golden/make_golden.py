@@ -67,13 +67,15 @@ def main() -> None:
                "n": sy.a.n, "nnz": sy.a.nnz, "a_sha": sha(*arrs), "b_sha": sha(sy.b),
                "nrm_b": sy.nrm_b, "hier": {}, "solves": {}}
         for prec in (0, 1, 2):
-            for sm in (0, 1, 2):
+            for sm in (0, 1, 2, 3):
+                if sm == 3 and prec == 2:
+                    continue  # Jacobi: FP precisions only
                 path = os.path.join(tmp, f"h_{prec}_{sm}.bin")
                 depth = O.ref_build_hier(arrs, prec, sm, path)
                 rec["hier"][f"{prec}_{sm}"] = file_sha(path)
                 rec["depth"] = depth
-        for sm in (0, 1, 2):
-            for solver in (2, 0, 1):
+        for sm in (0, 1, 2, 3):
+            for solver in ((0, 1) if sm == 3 else (2, 0, 1)):
                 r = O.ref_solve(arrs, sy.b, solver, sm, m=m, eps=EPS,
                                 trace_cap=8192 if solver == 2 else 0)
                 s = {"iters": r["iters"], "converged": r["converged"],

@@ -40,9 +40,23 @@ def test_generated_systems_match_fixture(golden, systems):
         assert sy.nrm_b == c["nrm_b"], c["name"]
 
 
+SMOOTHERS = [0, 1, 2, 3]
+SM_IDS = ["sor", "blocksor", "jacobisor", "jacobi"]
+
+
+def _precs(smoother):
+    """Jacobi exists for the FP hierarchies only."""
+    return (S.FP64, S.FP32) if smoother == 3 else (S.INT, S.FP64, S.FP32)
+
+
 @pytest.mark.parametrize("prec", [0, 1, 2], ids=["fp64", "fp32", "int"])
-@pytest.mark.parametrize("smoother", [0, 1, 2], ids=["sor", "blocksor", "jacobisor"])
+@pytest.mark.parametrize("smoother", SMOOTHERS, ids=SM_IDS)
 def test_setup_byte_identical_to_reference(golden, systems, prec, smoother):
+    if prec == S.INT and smoother == 3:
+        with pytest.raises(S.Sc23adError) as e:
+            S.Hierarchy(systems[golden["cases"][0]["name"]].a, prec, S.amg_param(smoother))
+        assert e.value.code == S.BAD_PARAM
+        return
     for c in _cases(golden):
         h = S.Hierarchy(systems[c["name"]].a, prec, S.amg_param(smoother))
         path = export(h)
@@ -57,9 +71,10 @@ def oracle_hiers(golden, systems):
     out = {}
     for c in _cases(golden):
         a = systems[c["name"]].a
-        for smoother in (0, 1, 2):
-            hs = S.Hierarchy.build_multi(a, [S.INT, S.FP64, S.FP32], S.amg_param(smoother))
-            for prec, h in zip((S.INT, S.FP64, S.FP32), hs):
+        for smoother in SMOOTHERS:
+            precs = _precs(smoother)
+            hs = S.Hierarchy.build_multi(a, list(precs), S.amg_param(smoother))
+            for prec, h in zip(precs, hs):
                 path = export(h)
                 out[(c["name"], prec, smoother)] = O.OracleHier(path)
                 os.unlink(path)
@@ -88,7 +103,7 @@ def test_oracle_int_solve_matches_reference(golden, systems, oracle_hiers, smoot
 
 
 @pytest.mark.parametrize("solver", [0, 1], ids=["fp64", "fp32"])
-@pytest.mark.parametrize("smoother", [0, 1, 2], ids=["sor", "blocksor", "jacobisor"])
+@pytest.mark.parametrize("smoother", SMOOTHERS, ids=SM_IDS)
 def test_oracle_fp_solve_matches_reference(golden, systems, oracle_hiers, solver, smoother):
     for c in _cases(golden):
         sy = systems[c["name"]]
@@ -116,14 +131,22 @@ def test_live_reference(case, tmp_path):
     kind, params, seed = case
     sy = S.make_system(kind, params, seed)
     arrs = sy.a.arrays()
-    for smoother in (0, 1, 2):
-        for prec in (0, 2):
+    for smoother in SMOOTHERS:
+        for prec in ((0, 1) if smoother == 3 else (0, 2)):
             ref_path = str(tmp_path / f"ref_{prec}_{smoother}.bin")
             O.ref_build_hier(arrs, prec, smoother, ref_path)
             h = S.Hierarchy(sy.a, prec, S.amg_param(smoother))
             mine = str(tmp_path / f"mine_{prec}_{smoother}.bin")
             h.export(mine)
             assert file_sha(mine) == file_sha(ref_path)
+        if smoother == 3:
+            # Jacobi: the FP64 solve through the oracle against the reference's
+            R = O.ref_solve(arrs, sy.b, 0, smoother, m=30, eps=1e-10)
+            oh = O.OracleHier(str(tmp_path / f"ref_0_{smoother}.bin"))
+            x, res, _ = oh.solve(arrs, sy.b, sy.nrm_b, m=30, eps=1e-10, mode=O.RED_SEQ)
+            assert res.iters == R["iters"]
+            assert np.array_equal(x, R["x"])
+            continue
         R = O.ref_solve(arrs, sy.b, 2, smoother, m=30, eps=1e-10, trace_cap=8192)
         oh = O.OracleHier(str(tmp_path / f"ref_2_{smoother}.bin"))
         x, res, trace = oh.solve(arrs, sy.b, sy.nrm_b, m=30, eps=1e-10, bit=R["bit"],

@@ -33,10 +33,10 @@ def built(golden):
         print(f"[setup] {c['name']}", flush=True)
         sy = system(c)
         hs = {}
-        for smoother in (0, 1, 2):
-            multi = S.Hierarchy.build_multi(sy.a, [S.INT, S.FP64, S.FP32],
-                                            S.amg_param(smoother))
-            for prec, h in zip((S.INT, S.FP64, S.FP32), multi):
+        for smoother in (0, 1, 2, 3):
+            precs = (S.FP64, S.FP32) if smoother == 3 else (S.INT, S.FP64, S.FP32)
+            multi = S.Hierarchy.build_multi(sy.a, list(precs), S.amg_param(smoother))
+            for prec, h in zip(precs, multi):
                 path = export(h)
                 hs[(prec, smoother)] = (S.Amg(h), O.OracleHier(path))
                 os.unlink(path)
@@ -59,7 +59,9 @@ def _same(a, b):
 def test_level_ops(built, name, prec):
     c, sy, hs = built[name]
     rng = np.random.default_rng(17)
-    for smoother in (0, 1, 2):
+    for smoother in (0, 1, 2, 3):
+        if (prec, smoother) not in hs:
+            continue  # Jacobi: FP only
         amg, oh = hs[(prec, smoother)]
         depth = amg.depth
         for lv in range(depth):
@@ -86,7 +88,9 @@ def test_level_ops(built, name, prec):
 def test_vcycle(built, name, prec):
     c, sy, hs = built[name]
     rng = np.random.default_rng(23)
-    for smoother in (0, 1, 2):
+    for smoother in (0, 1, 2, 3):
+        if (prec, smoother) not in hs:
+            continue  # Jacobi: FP only
         amg, oh = hs[(prec, smoother)]
         for _ in range(2):  # the second call reuses every work buffer
             v = _rand(prec, sy.a.n, rng)
@@ -125,7 +129,7 @@ def test_int_solve_matches_oracle(built, golden, name, smoother):
 @pytest.mark.parametrize("prec", [S.FP64, S.FP32], ids=["fp64", "fp32"])
 def test_fp_solve_matches_oracle(built, golden, name, prec):
     c, sy, hs = built[name]
-    for smoother in (0, 1, 2):
+    for smoother in (0, 1, 2, 3):
         amg, oh = hs[(prec, smoother)]
         sol = S.Solver(prec, sy.a, amg, m=c["m"], eps=golden["eps"])
         x, cv = sol.solve(sy.b, sy.nrm_b)
@@ -140,7 +144,9 @@ def test_fp_solve_matches_oracle(built, golden, name, prec):
 def test_sequential_mode_reproduces_reference(built, golden, name, solver):
     """With sequential FP reductions the device solve is the reference's."""
     c, sy, hs = built[name]
-    for smoother in (0, 1, 2):
+    for smoother in (0, 1, 2, 3):
+        if (solver, smoother) not in hs:
+            continue  # Jacobi: FP only
         g = c["solves"][f"{solver}_{smoother}"]
         amg, _ = hs[(solver, smoother)]
         kw = {"bit": g["bit"]} if solver == S.INT else {}

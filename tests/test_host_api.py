@@ -210,3 +210,12 @@ def test_errors_are_status_codes(tmp_path):
     with pytest.raises(S.Sc23adError) as e:
         S.Hierarchy(a, 7, S.amg_param(S.SOR))
     assert e.value.code == S.BAD_PARAM
+
+
+def test_device_setup_needs_a_gpu():
+    if S.device_count() > 0:
+        pytest.skip("a GPU is visible: the device setup is checked by the gpu tests")
+    a = S.CsrMat.generate(S.GEN_POISSON2D, 30)
+    with pytest.raises(S.Sc23adError) as e:
+        S.Hierarchy.build_multi(a, [S.INT], S.amg_param(S.SOR), device_setup=True)
+    assert e.value.code == S.NO_DEVICE

@@ -189,3 +189,28 @@ def test_jacobisor_single_block_is_damped_jacobi():
     np.testing.assert_allclose(1.0 / LD.val, np.diag(dense) / 0.7, rtol=1e-14)
     np.testing.assert_allclose(_csr(U).toarray(), dense - np.diag(np.diag(dense)) / 0.7,
                                rtol=1e-13, atol=1e-15)
+
+
+@pytest.mark.parametrize("omega", [1.0, 0.7])
+def test_jacobi_factors(omega):
+    """Jacobi (FP): LD holds omega / a_ii as its one entry per row (omega
+    converted to T, divided in double by the FP64 diagonal, stored as T, as
+    Jacobi<CSR<T>> computes it), U is A."""
+    sy = S.make_system(S.GEN_LAPLACE3D, (20,), 1)      # two smoothed levels
+    hs = S.Hierarchy.build_multi(sy.a, [S.FP64, S.FP32], S.amg_param(S.JACOBI, omega=omega))
+    paths = [export(h) for h in hs]
+    try:
+        h64, h32 = (read_hier(p) for p in paths)
+    finally:
+        for p in paths:
+            os.unlink(p)
+    assert h64.depth >= 2
+    for h, w in ((h64, omega), (h32, float(np.float32(omega)))):
+        for L, L64 in zip(h.levels, h64.levels):
+            A, LD, U = L["A"], L["LD"], L["U"]
+            assert np.array_equal(LD.rptr, np.arange(A.n + 1))
+            assert np.array_equal(LD.cind, np.arange(A.n))
+            diag = np.diag(_csr(L64["A"]).toarray())
+            np.testing.assert_array_equal(LD.val, (w / diag).astype(LD.val.dtype))
+            for f in ("rptr", "cind", "val"):
+                assert np.array_equal(getattr(U, f), getattr(A, f))
