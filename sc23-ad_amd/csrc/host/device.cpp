@@ -463,6 +463,8 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
       int w = 1;
       while (w < 8 && static_cast<size_t>(w) * 64 * 2 < maxlev_rows) w *= 2;
       while (w < 8 && overflowing(w, dev::kEpEntriesPerLane) * 16 > nlev) w *= 2;
+      if (const char* cap = std::getenv("SC23AD_EP_MAXWAVES"))  // tuning experiments
+        while (w > 1 && w > std::atoi(cap)) w /= 2;
       L.ep_waves = w;
       L.ep_far = any_far ? 1 : 0;
       L.ep_maxlev = maxlev_chain;
@@ -739,10 +741,11 @@ std::unique_ptr<DevAmg::Work> DevAmg::make_work() const {
   w->vout0.alloc(es * n0s);
   w->tmp[0].alloc(es * n0s);
   w->tmpc.alloc(es * std::max(n_coarse, 1));
-  std::vector<int> zero(static_cast<size_t>(std::max({lc_plan.nb, uc_plan.nb, 1})) + 1, 0);
+  // published-value words of the tiled coarse solves: epoch 0 = never written
+  std::vector<unsigned long long> zero(static_cast<size_t>(std::max(n_coarse, 1)), 0);
   for (DevBuf* f : {&w->lflags, &w->uflags}) {
-    f->alloc(sizeof(int) * zero.size());
-    f->upload(zero.data(), sizeof(int) * zero.size());
+    f->alloc(sizeof(unsigned long long) * zero.size());
+    f->upload(zero.data(), sizeof(unsigned long long) * zero.size());
   }
   w->err.alloc(sizeof(int));
   w->err.upload(zero.data(), sizeof(int));
@@ -820,7 +823,7 @@ void DevAmg::spmv(const DevMat& m, SpmvMode mode, const void* x, const void* z,
 }
 
 void DevAmg::trsv(const TriPlan& plan, const DevMat& m, const void* in, void* out,
-                  int* flags, int* epoch, int* err) const {
+                  unsigned long long* flags, int* epoch, int* err) const {
   if (plan.dense && elem == Elem::kI32 && flags) {
     *epoch = *epoch == INT32_MAX ? 1 : *epoch + 1;
     hip_check(dev::launch_trsv_tiles(plan.kind, m.d.n, plan.nb, plan.slice.as<int32_t>(),
@@ -887,8 +890,10 @@ void DevAmg::check_errors(const Work& w) const {
 
 // direct::LUflex / LU<T>::Solve (Library/solver/lu.hpp:91-94, 33-36).
 void DevAmg::coarse(const void* in, void* out, Work& w) const {
-  trsv(lc_plan, lc, in, w.tmpc.get(), w.lflags.as<int>(), &w.lepoch, w.err.as<int>());
-  trsv(uc_plan, uc, w.tmpc.get(), out, w.uflags.as<int>(), &w.uepoch, w.err.as<int>());
+  trsv(lc_plan, lc, in, w.tmpc.get(), w.lflags.as<unsigned long long>(), &w.lepoch,
+       w.err.as<int>());
+  trsv(uc_plan, uc, w.tmpc.get(), out, w.uflags.as<unsigned long long>(), &w.uepoch,
+       w.err.as<int>());
 }
 
 // intAMGflex::Precondition / dfAMG::Precondition (INTAMG.hpp:269-293,

@@ -163,7 +163,7 @@ class DevAmg {
   struct Work {
     std::vector<DevBuf> ee, xx, rr, tmp;  // index = level (1..depth); tmp[0] fine
     DevBuf r0, tmpc, vin0, vout0;         // fine residual, coarse temp, level-0 order in/out
-    DevBuf lflags, uflags, err;           // tiled coarse solves: block flags, error count
+    DevBuf lflags, uflags, err;           // tiled coarse solves: {epoch, value} words, errors
     int lepoch = 0, uepoch = 0;           // last flag values raised
   };
   std::unique_ptr<Work> make_work() const;
@@ -171,8 +171,8 @@ class DevAmg {
 
   void spmv(const DevMat& m, dev::SpmvMode mode, const void* x, const void* z,
             void* y) const;
-  void trsv(const TriPlan& plan, const DevMat& m, const void* in, void* out, int* flags,
-            int* epoch, int* err) const;
+  void trsv(const TriPlan& plan, const DevMat& m, const void* in, void* out,
+            unsigned long long* flags, int* epoch, int* err) const;
   void smooth(int l, const void* in, void* out, bool inited, Work& w) const;
   void coarse(const void* in, void* out, Work& w) const;
   // One V-cycle on vectors in the original numbering.

@@ -1,11 +1,17 @@
 // One Ruge-Stueben coarsening step with its two heavy numeric phases on the
-// GPU: the classical interpolation weights and both row-merge Galerkin
-// products (R A, then (R A) P).  The strength graph, the two C/F passes
-// (sequential greedy bookkeeping, helper_multigrid.hpp:138-266, 378-417) and
-// the transposes stay on the host.  Every FP64 value is computed in the same
-// operation order as the host code, so the step's result is identical to
-// rs_coarsen's bit for bit.
+// GPU: the classical interpolation weights and the row-merge Galerkin
+// products (R A, then (R A) P) of the large levels.  The strength graph, the
+// two C/F passes (sequential greedy bookkeeping, helper_multigrid.hpp:138-266,
+// 378-417) and the transposes stay on the host, and so do the products of
+// levels whose rows are few or long (one GPU thread per row then runs a long
+// dependent chain of table probes, and the host's threads are faster).
+// Every FP64 value is computed in the same operation order as the host code,
+// so the step's result is identical to rs_coarsen's bit for bit.
 #include <algorithm>
+#include <memory>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <numeric>
 #include <string>
 
@@ -88,7 +94,13 @@ HostCsr interpolation_dev(const DevCsr& da, const HostCsr& cs, const HostCsr& ds
 // a fixed scratch budget.
 constexpr long long kScratchSlots = 64ll << 20;  // 64 Mi slots of (int, double)
 
-HostCsr rowmerge_dev(const HostCsr& l, const DevCsr& dl, const HostCsr& r, const DevCsr& dr) {
+// The GPU product pays off on many short rows: at 128^3 the finest level's
+// products (1M rows, 7-25 terms each) ran 2-4x faster than the host's 16
+// threads, the next levels' (262k rows and fewer, or 100+ terms) slower.
+constexpr int kGpuProductMinRows = 1 << 20;
+constexpr double kGpuProductMaxTerms = 64.0;
+
+HostCsr rowmerge_auto(const HostCsr& l, const HostCsr& r, const DevCsr* dr_have) {
   const int n = l.n;
   std::vector<long long> toff(static_cast<size_t>(n) + 1, 0), ooff(toff);
   for (int i = 0; i < n; ++i) {
@@ -100,6 +112,15 @@ HostCsr rowmerge_dev(const HostCsr& l, const DevCsr& dl, const HostCsr& r, const
     toff[i + 1] = toff[i] + 2 * terms + 1;  // hash table: load factor <= 1/2
     ooff[i + 1] = ooff[i] + terms;
   }
+  const double mean_terms = n > 0 ? static_cast<double>(ooff[n]) / n : 0.0;
+  bool gpu = n >= kGpuProductMinRows && mean_terms <= kGpuProductMaxTerms;
+  if (const char* f = std::getenv("SC23AD_SETUP_GPU_PRODUCTS"))  // 1: always, 0: never
+    gpu = f[0] == '1';
+  if (!gpu) return rowmerge_product(l, r);
+  DevCsr dl(l);
+  std::unique_ptr<DevCsr> own_r;
+  if (!dr_have) own_r = std::make_unique<DevCsr>(r);
+  const DevCsr& dr = dr_have ? *dr_have : *own_r;
   DevBuf dtoff(sizeof(long long) * toff.size()), dooff(sizeof(long long) * ooff.size());
   dtoff.upload(toff.data(), sizeof(long long) * toff.size());
   dooff.upload(ooff.data(), sizeof(long long) * ooff.size());
@@ -158,23 +179,37 @@ HostCsr rowmerge_dev(const HostCsr& l, const DevCsr& dl, const HostCsr& r, const
 CoarsenStep rs_coarsen_dev(const HostCsr& a, double theta) {
   if (theta <= 0 || 1 < theta)
     throw SetupError{kBadParam, "theta must satisfy 0 < theta <= 1"};
+  const bool verbose = std::getenv("SC23AD_VERBOSE") != nullptr;
+  auto t = std::chrono::steady_clock::now();
+  auto lap = [&](const char* what) {
+    if (!verbose) return;
+    const auto now = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "[sc23ad setup/dev] n=%d %-14s %8.3f s\n", a.n, what,
+                 std::chrono::duration<double>(now - t).count());
+    t = now;
+  };
   CoarsenStep step;
   HostCsr strong, weak;
   rs_strength(a, theta, strong, weak);
   HostCsr strong_t = pattern_transpose(strong);
+  lap("strength");
   step.cf = rs_first_pass(strong, strong_t, &step.warnings);
   rs_second_pass(strong, step.cf);
+  lap("C/F passes");
   HostCsr cs, ds;
   split_strong(strong, step.cf, cs, ds);
   DevCsr da(a);
+  lap("split, upload");
   step.interp = interpolation_dev(da, cs, ds, weak, step.cf, &step.warnings);
+  lap("interpolation");
   step.restrict_ = csr_transpose(step.interp);
+  lap("transpose");
   // get_coarse_matrix (helper_multigrid.hpp:564-586): rows of P^T A first,
   // then (P^T A) P.
-  DevCsr dr(step.restrict_);
-  HostCsr rt_a = rowmerge_dev(step.restrict_, dr, a, da);
-  DevCsr drta(rt_a), dp(step.interp);
-  step.coarse = rowmerge_dev(rt_a, drta, step.interp, dp);
+  HostCsr rt_a = rowmerge_auto(step.restrict_, a, &da);
+  lap("R*A");
+  step.coarse = rowmerge_auto(rt_a, step.interp, nullptr);
+  lap("(RA)*P");
   return step;
 }
 

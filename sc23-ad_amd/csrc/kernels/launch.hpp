@@ -129,13 +129,14 @@ size_t tri_blocked_lds(int n, size_t elem, size_t acc);
 // Integer factor stored as dense 64-wide slices per column block (see
 // k_tri_dense_int); dinv[i] is row i's stored inverse diagonal (LD / DU).
 // Tiled integer solve of a coarse factor, one workgroup per 64-row block
-// (see k_tri_tiles_int); flags: nb ints, epoch: a value no flag holds yet,
-// err: incremented when a wait gives up.
+// (see k_tri_tiles_int); xp: n words {epoch, value} publishing the solved
+// rows, epoch: a value no word holds yet, err: incremented when a wait gives
+// up.
 constexpr int kTileMaxBlocks = 64;
 hipError_t launch_trsv_tiles(TriKind kind, int n, int nb, const int32_t* tiles,
                              const long long* toff, const int32_t* dinv, int bit, int dbit,
-                             const void* in, void* out, int* flags, int epoch, int* err,
-                             hipStream_t s);
+                             const void* in, void* out, unsigned long long* xp, int epoch,
+                             int* err, hipStream_t s);
 hipError_t launch_trsv_blocked(Elem e, TriKind kind, const DCsr& a, const int* bptr,
                                int nb, const void* in, void* out, hipStream_t s);
 

@@ -598,16 +598,17 @@ __global__ __launch_bounds__(kBlkThreads) void k_tri_blocked_int(
 // tiles (zeros where the factor has no entry), column-major (entry (row r,
 // column c) of a tile at c * 64 + r), in processing order: the off-diagonal
 // tiles of the blocks it depends on (forward: 0 .. b-1; backward: nb-1 ..
-// b+1), then its diagonal tile.  For each dependency block the workgroup
-// waits until that block's solution is published (flags[block] == epoch),
-// subtracts the tile's product (four waves, 16 columns each; integer sums
-// are order-free), then wave 0 sweeps the diagonal tile: at step c lane c
-// finishes its row and the value is broadcast to the other lanes.  The block
-// publishes its 64 values with an agent-scope release before raising its
-// flag.  Dependencies point to lower (forward) or higher (backward) blocks
-// only, and all blocks of a coarse grid (at most 2000 rows: 32 blocks) are
-// resident at once, so every wait ends; a wait longer than kTileMaxSpin
-// polls records an error and gives up instead of hanging.
+// b+1), then its diagonal tile.  A solved row is published as one 64-bit
+// word {epoch, value} (xp[row], an agent-scope atomic store), so a consumer
+// learns that a value is ready and reads it in the same load.  For each
+// dependency block, wave 0 polls the block's 64 words until all carry the
+// current epoch, the workgroup subtracts the tile's product (four waves, 16
+// columns each; integer sums are order-free), then wave 0 sweeps the
+// diagonal tile: at step c lane c finishes its row and the value is broadcast
+// to the other lanes.  Dependencies point to lower (forward) or higher
+// (backward) blocks only, and all blocks of a coarse grid (at most 2000
+// rows: 32 blocks) are resident at once, so every wait ends; a wait longer
+// than kTileMaxSpin polls records an error and gives up instead of hanging.
 constexpr int kTileThreads = 256;
 constexpr int kTileMaxSpin = 1 << 18;
 
@@ -615,7 +616,7 @@ template <TriKind K>
 __global__ __launch_bounds__(kTileThreads) void k_tri_tiles_int(
     int n, int nb, const int32_t* __restrict__ tiles, const long long* __restrict__ toff,
     const int32_t* __restrict__ dinv_g, int bit, int dbit, const int32_t* __restrict__ in,
-    int32_t* out, int* flags, int epoch, int* err) {
+    int32_t* out, unsigned long long* xp, int epoch, int* err) {
   constexpr bool fwd = (K != TriKind::kDU);
   __shared__ uint64_t part[kTileThreads / 64][64];
   __shared__ int32_t xsh[64];
@@ -633,20 +634,21 @@ __global__ __launch_bounds__(kTileThreads) void k_tri_tiles_int(
     int32_t tv[16];
 #pragma unroll
     for (int j = 0; j < 16; ++j) tv[j] = Tk[(16 * w + j) * kTB + lane];
-    if (tid == 0) {
+    if (w == 0) {
+      const int c = bp * kTB + lane;
+      unsigned long long v = 0;
       int spins = 0;
-      while (__hip_atomic_load(&flags[bp], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+      while (true) {
+        v = c < n ? __hip_atomic_load(&xp[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                  : static_cast<unsigned long long>(static_cast<unsigned>(epoch)) << 32;
+        if (__all(static_cast<int>(v >> 32) == epoch)) break;
         __builtin_amdgcn_s_sleep(1);
         if (++spins > kTileMaxSpin) {
-          atomicAdd(err, 1);
+          if (lane == 0) atomicAdd(err, 1);
           break;
         }
       }
-    }
-    __syncthreads();
-    if (tid < kTB) {
-      const int c = bp * kTB + tid;
-      xsh[tid] = c < n ? __hip_atomic_load(&out[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+      xsh[lane] = c < n ? static_cast<int32_t>(static_cast<uint32_t>(v)) : 0;
     }
     __syncthreads();
     uint64_t ps = 0;
@@ -676,9 +678,13 @@ __global__ __launch_bounds__(kTileThreads) void k_tri_tiles_int(
         t -= prod64(reg[c], xc);
       }
     }
-    if (lane < rows) out[r0 + lane] = xv;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    if (lane == 0) __hip_atomic_store(&flags[b], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (lane < rows) {
+      out[r0 + lane] = xv;
+      const unsigned long long word =
+          (static_cast<unsigned long long>(static_cast<unsigned>(epoch)) << 32) |
+          static_cast<uint32_t>(xv);
+      __hip_atomic_store(&xp[r0 + lane], word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
 }
 
@@ -1308,12 +1314,12 @@ size_t tri_blocked_lds(int n, size_t elem, size_t acc) {
 
 hipError_t launch_trsv_tiles(TriKind kind, int n, int nb, const int32_t* tiles,
                              const long long* toff, const int32_t* dinv, int bit, int dbit,
-                             const void* in, void* out, int* flags, int epoch, int* err,
-                             hipStream_t s) {
+                             const void* in, void* out, unsigned long long* xp, int epoch,
+                             int* err, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   auto go = [&](auto kern) -> hipError_t {
     hipLaunchKernelGGL(kern, dim3(nb), dim3(kTileThreads), 0, s, n, nb, tiles, toff, dinv, bit,
-                       dbit, static_cast<const int32_t*>(in), static_cast<int32_t*>(out), flags,
+                       dbit, static_cast<const int32_t*>(in), static_cast<int32_t*>(out), xp,
                        epoch, err);
     return hipGetLastError();
   };
