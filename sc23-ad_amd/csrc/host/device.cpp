@@ -465,6 +465,8 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
       while (w < 8 && overflowing(w, dev::kEpEntriesPerLane) * 16 > nlev) w *= 2;
       if (const char* cap = std::getenv("SC23AD_EP_MAXWAVES"))  // tuning experiments
         while (w > 1 && w > std::atoi(cap)) w /= 2;
+      if (const char* lo = std::getenv("SC23AD_EP_MINWAVES"))
+        while (w < 8 && w < std::atoi(lo) && static_cast<size_t>(w) * 64 < maxlev_rows) w *= 2;
       L.ep_waves = w;
       L.ep_far = any_far ? 1 : 0;
       L.ep_maxlev = maxlev_chain;

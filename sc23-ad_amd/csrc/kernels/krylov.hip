@@ -179,13 +179,38 @@ __device__ __forceinline__ uint64_t sum_parts(const int64_t* part, int G, uint64
 
 // ---- integer MGS ---------------------------------------------------------
 
+// Element loops of the integer MGS kernels: four consecutive elements per
+// step with 16-byte loads when every vector starts on a 16-byte boundary
+// (n % 4 == 0; the Krylov block is one allocation), else one at a time.
+// Integer sums are order-free, so the split does not change a bit.
+template <typename F>
+__device__ __forceinline__ void int_elems(int n, F&& f) {
+  const int stride = gridDim.x * B;
+  const int t = blockIdx.x * B + threadIdx.x;
+  if ((n & 3) == 0) {
+    for (int q = t; q < (n >> 2); q += stride) f.quad(q);
+  } else {
+    for (int i = t; i < n; i += stride) f.one(i);
+  }
+}
+
 __global__ __launch_bounds__(B) void k_dot_first_int(int n, const int32_t* __restrict__ v,
                                                      const int32_t* __restrict__ w,
                                                      int64_t* __restrict__ part) {
   __shared__ uint64_t sh[B / 64];
   uint64_t acc = 0;
-  const int stride = gridDim.x * B;
-  for (int i = blockIdx.x * B + threadIdx.x; i < n; i += stride) acc += prod64(v[i], w[i]);
+  struct {
+    const int32_t* v;
+    const int32_t* w;
+    uint64_t* acc;
+    __device__ void one(int i) { *acc += prod64(v[i], w[i]); }
+    __device__ void quad(int q) {
+      const int4 a = reinterpret_cast<const int4*>(v)[q];
+      const int4 b = reinterpret_cast<const int4*>(w)[q];
+      *acc += prod64(a.x, b.x) + prod64(a.y, b.y) + prod64(a.z, b.z) + prod64(a.w, b.w);
+    }
+  } body{v, w, &acc};
+  int_elems(n, body);
   const uint64_t p = block_sum_u64<B>(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = static_cast<int64_t>(p);
 }
@@ -255,12 +280,39 @@ __global__ __launch_bounds__(B) void k_axpy_dot_int(int n, const int32_t* __rest
   const int bit = k.bit;
   const bool norm = (kk == j);
   uint64_t acc = 0;
-  const int stride = gridDim.x * B;
-  for (int i = blockIdx.x * B + threadIdx.x; i < n; i += stride) {
-    const int32_t wi = add32(w[i], low32(mul_wrap(a, vk[i]) >> bit));
-    w[i] = wi;
-    acc += norm ? prod64(wi, wi) : prod64(vk1[i], wi);
-  }
+  struct {
+    const int32_t* vk;
+    const int32_t* vk1;
+    int32_t* w;
+    int64_t a;
+    int bit;
+    bool norm;
+    uint64_t* acc;
+    __device__ int32_t upd(int32_t wi, int32_t v) const {
+      return add32(wi, low32(mul_wrap(a, v) >> bit));
+    }
+    __device__ void one(int i) {
+      const int32_t wi = upd(w[i], vk[i]);
+      w[i] = wi;
+      *acc += norm ? prod64(wi, wi) : prod64(vk1[i], wi);
+    }
+    __device__ void quad(int q) {
+      const int4 v = reinterpret_cast<const int4*>(vk)[q];
+      int4 x = reinterpret_cast<const int4*>(w)[q];
+      x.x = upd(x.x, v.x);
+      x.y = upd(x.y, v.y);
+      x.z = upd(x.z, v.z);
+      x.w = upd(x.w, v.w);
+      reinterpret_cast<int4*>(w)[q] = x;
+      if (norm) {
+        *acc += prod64(x.x, x.x) + prod64(x.y, x.y) + prod64(x.z, x.z) + prod64(x.w, x.w);
+      } else {
+        const int4 u = reinterpret_cast<const int4*>(vk1)[q];
+        *acc += prod64(u.x, x.x) + prod64(u.y, x.y) + prod64(u.z, x.z) + prod64(u.w, x.w);
+      }
+    }
+  } body{vk, vk1, w, a, bit, norm, &acc};
+  int_elems(n, body);
   const uint64_t p = block_sum_u64<B>(acc, sh);
   if (threadIdx.x == 0) pout[blockIdx.x] = static_cast<int64_t>(p);
 }
@@ -283,10 +335,21 @@ __global__ __launch_bounds__(B) void k_scal_givens_int(int n, int32_t* __restric
     }
   }
   __syncthreads();
-  const int64_t hinv = hinv_sh;
-  const int stride = gridDim.x * B;
-  for (int i = blockIdx.x * B + threadIdx.x; i < n; i += stride)
-    w[i] = low32(mul_wrap(hinv, w[i]) >> k.bit);
+  struct {
+    int32_t* w;
+    int64_t hinv;
+    int bit;
+    __device__ void one(int i) { w[i] = low32(mul_wrap(hinv, w[i]) >> bit); }
+    __device__ void quad(int q) {
+      int4 x = reinterpret_cast<int4*>(w)[q];
+      x.x = low32(mul_wrap(hinv, x.x) >> bit);
+      x.y = low32(mul_wrap(hinv, x.y) >> bit);
+      x.z = low32(mul_wrap(hinv, x.z) >> bit);
+      x.w = low32(mul_wrap(hinv, x.w) >> bit);
+      reinterpret_cast<int4*>(w)[q] = x;
+    }
+  } body{w, hinv_sh, k.bit};
+  int_elems(n, body);
 }
 
 // ---- FP MGS (fixed blocked order) --------------------------------------------
