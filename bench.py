@@ -551,6 +551,13 @@ def main():
     spmv_bytes = spmv_int_bytes(N, N, nnz)
     achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9 if spmv_ms > 0 else None
     trsv_launches, trsv_ms = soli.trsv_stats()
+    # the whole V-cycle: 20 back-to-back cycles on device vectors
+    vin = np.random.default_rng(5).integers(-(2**29), 2**29, N).astype(np.int32)
+    vc_ms, _ = amgi.time_vcycles(vin, 20)
+    vc_bytes = amgi.vcycle_bytes()
+    vc_achieved = vc_bytes / (vc_ms * 1e-3) / 1e9 if vc_ms > 0 else None
+    log(f"V-cycle: {vc_ms:.3f} ms, {vc_bytes / 1e6:.1f} MB algorithmic, "
+        f"{vc_achieved:.0f} GB/s")
     rptr, cind, _ = sy.a.arrays()
     nnz_ld = ld_nnz(rptr, cind, args.smoother)
     trsv_bytes = sptrsv_int_bytes(N, nnz_ld)
@@ -617,6 +624,11 @@ def main():
                             "traffic": trsv_traffic, "algorithmic_bytes": trsv_bytes,
                             "nnz_ld": nnz_ld, "launches_timed": trsv_launches,
                             "mean_ms": trsv_ms},
+        "roofline_vcycle": {"bound": "hbm", "kernel": "one integer V-cycle (all its kernels)",
+                            "timing": "hipEvents around 20 back-to-back V-cycles",
+                            "achieved": vc_achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                            "frac": (vc_achieved / HBM_PEAK_GBPS) if vc_achieved else None,
+                            "algorithmic_bytes": vc_bytes, "mean_ms": vc_ms},
         "cpu_baseline": cpu,
         "setup_s": {"host": setup_host, "gpu_interp_galerkin": setup_dev,
                     "identical": bool(same_setup), "precisions": len(precs)},

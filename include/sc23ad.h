@@ -179,6 +179,17 @@ int sc23ad_amg_precision(const sc23ad_amg* m, int* precision);
 int sc23ad_amg_precondition(sc23ad_amg* m, const void* in, void* out,
                             int on_device);
 
+/* Algorithmic HBM bytes of one V-cycle: the sum over its kernels of the
+   SpTRSV and SpMV formulas (values, indices, row pointers, vectors read and
+   written once) and the vector passes.  No reference counterpart
+   (measurement of intAMGflex::Precondition, INTAMG.hpp:269-293). */
+int sc23ad_amg_vcycle_bytes(const sc23ad_amg* m, double* bytes);
+/* Mean duration of `reps` back-to-back V-cycles (hipEvents around them on
+   the library stream, after one warm-up cycle).  in, out: device vectors of
+   the fine level's element type; out receives the V-cycle of in. */
+int sc23ad_amg_time_vcycles(sc23ad_amg* m, int reps, const void* in, void* out,
+                            double* ms_per_cycle);
+
 /* Level operations (device vectors of the level's element type). */
 typedef enum {
   SC23AD_OP_SPMV_A = 0,    /* out = A_l in                                   */

@@ -186,6 +186,36 @@ int sc23ad_amg_precondition(sc23ad_amg* m, const void* in, void* out, int on_dev
   });
 }
 
+int sc23ad_amg_vcycle_bytes(const sc23ad_amg* m, double* bytes) {
+  if (!m || !bytes) return fail(SC23AD_BAD_PARAM, "vcycle_bytes: null");
+  *bytes = m->d->vcycle_bytes();
+  return SC23AD_OK;
+}
+
+int sc23ad_amg_time_vcycles(sc23ad_amg* m, int reps, const void* in, void* out,
+                            double* ms_per_cycle) {
+  if (!m || !in || !out || !ms_per_cycle || reps < 1)
+    return fail(SC23AD_BAD_PARAM, "time_vcycles: bad arguments");
+  return guarded([&]() -> int {
+    DevAmg& a = *m->d;
+    a.vcycle(in, out, a.own_work());  // warm-up
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "event");
+    hip_check(hipEventCreate(&e1), "event");
+    hip_check(hipEventRecord(e0, lib_stream()), "record");
+    for (int r = 0; r < reps; ++r) a.vcycle(in, out, a.own_work());
+    hip_check(hipEventRecord(e1, lib_stream()), "record");
+    hip_check(hipEventSynchronize(e1), "sync");
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_per_cycle = ms / reps;
+    a.check_errors(a.own_work());
+    return SC23AD_OK;
+  });
+}
+
 int sc23ad_amg_level_op(sc23ad_amg* m, int level, int op, const void* in, void* out) {
   if (!m || !in || !out) return fail(SC23AD_BAD_PARAM, "level_op: null");
   return guarded([&]() -> int {

@@ -74,6 +74,8 @@ def _load() -> C.CDLL:
         "sc23ad_hier_info": (C.c_int, [vp, ip, vp, vp]),
         "sc23ad_hier_build_multi": (C.c_int, [vp, C.POINTER(AmgParam), vp, C.c_int, vp]),
         "sc23ad_hier_build_multi_dev": (C.c_int, [vp, C.POINTER(AmgParam), vp, C.c_int, vp]),
+        "sc23ad_amg_vcycle_bytes": (C.c_int, [vp, dp]),
+        "sc23ad_amg_time_vcycles": (C.c_int, [vp, C.c_int, vp, vp, dp]),
         "sc23ad_hier_export": (C.c_int, [vp, C.c_char_p]),
         "sc23ad_hier_destroy": (None, [vp]),
         "sc23ad_checksum_i32": (C.c_uint64, [vp, C.c_int]),
@@ -402,6 +404,19 @@ class Amg:
         out = np.zeros(self.level_n[0], self.dtype)
         check(lib.sc23ad_amg_precondition(self.h, ptr(vin), ptr(out), 0))
         return out
+
+    def vcycle_bytes(self) -> float:
+        b = C.c_double()
+        check(lib.sc23ad_amg_vcycle_bytes(self.h, C.byref(b)))
+        return b.value
+
+    def time_vcycles(self, vin: np.ndarray, reps: int = 20):
+        """(mean milliseconds of one V-cycle on device vectors, its output)."""
+        din = DevArray(np.ascontiguousarray(vin, dtype=self.dtype))
+        dout = DevArray(np.zeros(self.level_n[0], self.dtype))
+        ms = C.c_double()
+        check(lib.sc23ad_amg_time_vcycles(self.h, reps, din.p, dout.p, C.byref(ms)))
+        return ms.value, dout.get()
 
     def level_op(self, level: int, op: int, vin: np.ndarray, out_n: int,
                  out_init: np.ndarray | None = None) -> np.ndarray:

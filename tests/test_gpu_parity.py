@@ -97,6 +97,18 @@ def test_vcycle(built, name, prec):
             assert _same(amg.precondition(v), oh.vcycle(v)), (name, smoother)
 
 
+@pytest.mark.parametrize("name", ["poisson2d_64", "inclusions3d_20"])
+def test_timed_vcycles_compute_the_vcycle(built, name):
+    """The V-cycle timing entry point runs the real V-cycle: its output is the
+    oracle's, and its per-cycle time and byte count are positive."""
+    c, sy, hs = built[name]
+    amg, oh = hs[(S.INT, 1)]
+    v = _rand(S.INT, sy.a.n, np.random.default_rng(29))
+    ms, out = amg.time_vcycles(v, 3)
+    assert _same(out, oh.vcycle(v))
+    assert ms > 0 and amg.vcycle_bytes() > 0
+
+
 def test_default_frac_bits(built):
     for name in CASES:
         c, sy, hs = built[name]
