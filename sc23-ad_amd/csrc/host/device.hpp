@@ -96,7 +96,7 @@ struct TriPlan {
   bool dense = false;
   DevBuf slice, boff, ddinv;
   // level-ordered layout used by the level-scheduled kernels
-  DevBuf lrow, lbeg, lcol, lval, ldiag, ecol, eval, lpos, epos, lent, lev_eptr;
+  DevBuf lrow, lbeg, lcol, lval, ldiag, ecol, eval, lpos, epos, lent, lev_eptr, rent;
   dev::TriLayout layout;
 };
 

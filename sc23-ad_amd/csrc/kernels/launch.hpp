@@ -96,6 +96,14 @@ struct TriLayout {
   int ep_maxlev = 0; // most levels in one chain
   int ep_copies = 1; // accumulator copies per row (1 or kEpCopies)
   int ep_acc = 0;    // accumulator slots in LDS
+  // row-parallel mode (LD factors whose rows have at most kRpMaxEntries
+  // entries): entries padded to rp_e per position as int2 {dep, val}, rp_rows
+  // rows per lane, rp_waves waves per chain; same chains, ring and far rule
+  // as the entry-parallel mode
+  const void* rent = nullptr;
+  int rp_waves = 0;  // > 0: use the row-parallel kernel
+  int rp_e = 0;
+  int rp_rows = 0;
   int n = 0;
 };
 // Entry-parallel integer solve (one workgroup per chain).
@@ -108,6 +116,14 @@ constexpr int kEpMaxRows = 1024;  // most rows in one level of one chain
 constexpr int kEpCopies = 8;
 hipError_t launch_trsv_ep(TriKind kind, const TriLayout& L, const TriSegment& seg, int bit,
                           int dbit, const void* in, void* out, hipStream_t s);
+// Row-parallel integer LD solve (one workgroup per chain, a lane per row).
+constexpr int kRpMaxEntries = 12;
+hipError_t launch_trsv_rp(const TriLayout& L, const TriSegment& seg, int bit, int dbit,
+                          const void* in, void* out, hipStream_t s);
+inline size_t rp_lds_bytes(int ring, int maxlev) {
+  return sizeof(int32_t) * static_cast<size_t>(ring) +
+         sizeof(int) * (static_cast<size_t>(maxlev) + 1);
+}
 // LDS bytes of the entry-parallel kernel: acc_slots uint64 accumulators and
 // kEpDummy spare ones, the ring, the chain's level offsets.
 constexpr int kEpDummy = 64;

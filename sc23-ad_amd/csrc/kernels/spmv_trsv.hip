@@ -1138,6 +1138,200 @@ hipError_t tri_ep_launch(const TriLayout& L, const TriSegment& seg, int bit, int
   return tri_ep_launch_c<K, kWaves, 1>(L, seg, bit, dbit, in, out, s);
 }
 
+// ---------------------------------------------------------------------------
+// Row-parallel integer LD solve: the chains, levels, LDS ring and far
+// dependencies of k_tri_chain_ep, for factors whose rows hold at most kE
+// dependency entries.  Lane t owns rows t, t + kThreads, ... (kR of them) of
+// each level and keeps their entries, padded to kE with zero values, in
+// registers; a level costs one round of ring reads, the row sums in
+// registers, the ring writes and one barrier (no accumulators, no atomics).
+// The integer sum is order-free (wrap-around), so the result is the same as
+// the sequential one bit for bit.  Entries are stored by position, kE int2
+// {dep, val} pairs each (dep as in k_tri_chain_ep; padding {0, 0}).
+template <int kE, int kR>
+struct RpPre {
+  int lo, hi;             // the level's position bounds
+  int4 ent[kR][kE / 2];   // {dep, val, dep, val}
+  int32_t xfar[kR][kE];
+  int32_t dinv[kR];
+  int32_t in[kR];
+};
+
+template <int kThreads, int kE, int kR>
+__device__ __forceinline__ void rp_load(RpPre<kE, kR>& r, int lo, int hi,
+                                        const int4* __restrict__ rent,
+                                        const int32_t* __restrict__ dinv,
+                                        const int32_t* __restrict__ in) {
+  const int t = threadIdx.x;
+  r.lo = lo;
+  r.hi = hi;
+#pragma unroll
+  for (int q = 0; q < kR; ++q) {
+    const int idx = lo + t + q * kThreads;
+    const int id = idx < hi ? idx : 0;  // lanes past the level's end load position 0
+#pragma unroll
+    for (int h = 0; h < kE / 2; ++h) r.ent[q][h] = rent[static_cast<size_t>(id) * (kE / 2) + h];
+    r.dinv[q] = dinv[id];
+    r.in[q] = in[id];
+  }
+}
+
+template <int kThreads, int kE, int kR>
+__device__ __forceinline__ void rp_far(RpPre<kE, kR>& r, const int32_t* out) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < kR; ++q) {
+    const bool ok = r.lo + t + q * kThreads < r.hi;
+#pragma unroll
+    for (int h = 0; h < kE / 2; ++h) {
+      const int d0 = r.ent[q][h].x, d1 = r.ent[q][h].z;
+      r.xfar[q][2 * h] = out[(ok && d0 < 0) ? -d0 - 1 : 0];
+      r.xfar[q][2 * h + 1] = out[(ok && d1 < 0) ? -d1 - 1 : 0];
+    }
+  }
+}
+
+template <TriKind K, int kWaves, bool kFar, int kE, int kR, int kD, int kA>
+__device__ __forceinline__ void rp_level(RpPre<kE, kR>& cur, RpPre<kE, kR>& ahead, int L, int nl,
+                                         const int* lvo, int32_t* ring, int rmask,
+                                         const int4* __restrict__ rent,
+                                         const int32_t* __restrict__ dinv,
+                                         const int32_t* __restrict__ in, int32_t* out, int bit,
+                                         int dbit) {
+  constexpr int kThreads = 64 * kWaves;
+  const int t = threadIdx.x;
+  const int s = cur.lo, e = cur.hi;
+  // bounds of the level prefetched at the end of this one
+  const int Lp = min(L + kD, nl);
+  const int plo = lvo[Lp], phi = lvo[min(Lp + 1, nl)];
+  if (kFar) rp_far<kThreads>(ahead, out);
+  int32_t xv[kR][kE];
+#pragma unroll
+  for (int q = 0; q < kR; ++q)
+#pragma unroll
+    for (int h = 0; h < kE / 2; ++h) {
+      xv[q][2 * h] = ring[cur.ent[q][h].x & rmask];
+      xv[q][2 * h + 1] = ring[cur.ent[q][h].z & rmask];
+    }
+  if (kFar) {
+#pragma unroll
+    for (int q = 0; q < kR; ++q)
+#pragma unroll
+      for (int h = 0; h < kE / 2; ++h) {
+        const int d0 = cur.ent[q][h].x, d1 = cur.ent[q][h].z;
+        if (__any(d0 < 0)) xv[q][2 * h] = d0 < 0 ? cur.xfar[q][2 * h] : xv[q][2 * h];
+        if (__any(d1 < 0)) xv[q][2 * h + 1] = d1 < 0 ? cur.xfar[q][2 * h + 1] : xv[q][2 * h + 1];
+      }
+  }
+#pragma unroll
+  for (int q = 0; q < kR; ++q) {
+    uint64_t acc = 0;
+#pragma unroll
+    for (int h = 0; h < kE / 2; ++h) {
+      acc += prod64(cur.ent[q][h].y, xv[q][2 * h]);
+      acc += prod64(cur.ent[q][h].w, xv[q][2 * h + 1]);
+    }
+    const int k = s + t + q * kThreads;
+    if (k < e) {
+      const uint64_t tt = (static_cast<uint64_t>(static_cast<int64_t>(cur.in[q])) << bit) - acc;
+      const int32_t v = acc_final<int32_t, K>(tt, cur.dinv[q], bit, dbit);
+      ring[k & rmask] = v;
+      out[k] = v;
+    }
+  }
+  if (kFar && (L % kEpFence) == kEpFence - 1)
+    __syncthreads();  // full fence: global results visible to later loads
+  else
+    ep_sync<kWaves>();
+  rp_load<kThreads>(cur, plo, phi, rent, dinv, in);
+}
+
+// LDS: the ring, then the chain's level offsets (max_levels + 1 ints).
+// Far values of level L + kA are loaded at the start of level L, from entries
+// loaded kD - kA levels before that.
+template <TriKind K, int kWaves, bool kFar, int kE, int kR, int kD, int kA>
+__global__ __launch_bounds__(64 * kWaves) void k_tri_chain_rp(
+    const int4* __restrict__ rent, const int32_t* __restrict__ dinv,
+    const int* __restrict__ lev_ptr, const int* __restrict__ chain_lev, int rmask, int bit,
+    int dbit, const int32_t* __restrict__ in, int32_t* out) {
+  constexpr int kThreads = 64 * kWaves;
+  extern __shared__ __align__(16) unsigned char smem_r[];
+  int32_t* ring = reinterpret_cast<int32_t*>(smem_r);
+  int* lvo = ring + rmask + 1;
+  const int t = threadIdx.x;
+  const int lb = chain_lev[2 * blockIdx.x], le = chain_lev[2 * blockIdx.x + 1];
+  const int nl = le - lb;
+  for (int k = t; k <= nl; k += kThreads) lvo[k] = lev_ptr[lb + k];
+  __syncthreads();
+  RpPre<kE, kR> buf[kD];
+#pragma unroll
+  for (int d = 0; d < kD; ++d) {
+    const int Lc = min(d, nl);
+    rp_load<kThreads>(buf[d], lvo[Lc], lvo[min(Lc + 1, nl)], rent, dinv, in);
+  }
+  static_assert(kA >= 1 && kA < kD, "far values need their entries loaded first");
+  static_assert(kEpFarLevels >= kA + kEpFence,
+                "far dependencies must be fenced before their values are loaded");
+  int L0 = 0;
+  for (; L0 + kD <= nl; L0 += kD) {
+#pragma unroll
+    for (int d = 0; d < kD; ++d)
+      rp_level<K, kWaves, kFar, kE, kR, kD, kA>(buf[d], buf[(d + kA) % kD], L0 + d, nl, lvo, ring,
+                                                rmask, rent, dinv, in, out, bit, dbit);
+  }
+#pragma unroll
+  for (int d = 0; d < kD; ++d)
+    if (L0 + d < nl)
+      rp_level<K, kWaves, kFar, kE, kR, kD, kA>(buf[d], buf[(d + kA) % kD], L0 + d, nl, lvo, ring,
+                                                rmask, rent, dinv, in, out, bit, dbit);
+}
+
+template <int kWaves, bool kFar, int kE, int kR, int kD, int kA>
+hipError_t tri_rp_go(const TriLayout& L, const TriSegment& seg, int bit, int dbit,
+                     const int32_t* in, int32_t* out, hipStream_t s) {
+  constexpr TriKind K = TriKind::kLD;
+  const size_t lds = rp_lds_bytes(L.ring, L.ep_maxlev);
+  if (lds > static_cast<size_t>(kTriBlockedMaxLds)) return hipErrorInvalidValue;
+  static bool attr = false;
+  if (!attr) {
+    hipError_t e = hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&k_tri_chain_rp<K, kWaves, kFar, kE, kR, kD, kA>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kTriBlockedMaxLds));
+    if (e != hipSuccess) return e;
+    attr = true;
+  }
+  launch_timed(&k_tri_chain_rp<K, kWaves, kFar, kE, kR, kD, kA>, dim3(seg.nchain), dim3(64 * kWaves),
+               static_cast<uint32_t>(lds), s, reinterpret_cast<const int4*>(L.rent),
+               static_cast<const int32_t*>(L.ldiag), L.lev_ptr, seg.chain_lev, L.ring - 1, bit,
+               dbit, in, out);
+  return hipGetLastError();
+}
+
+// Register layouts: 4 entries and 1 row per lane (prefetch distance 8
+// levels, far values 4 levels ahead: a level takes less than one global-load
+// latency, and a far value waits for two); 4 entries and 2 rows, 8 entries
+// and 1 row, 12 entries and 1 row (distance 4, far values 2 ahead).
+template <int kWaves, bool kFar>
+hipError_t tri_rp_layout(const TriLayout& L, const TriSegment& seg, int bit, int dbit,
+                         const int32_t* in, int32_t* out, hipStream_t s) {
+  if (L.rp_e == 4 && L.rp_rows == 1)
+    return tri_rp_go<kWaves, kFar, 4, 1, 8, 4>(L, seg, bit, dbit, in, out, s);
+  if (L.rp_e == 4 && L.rp_rows == 2)
+    return tri_rp_go<kWaves, kFar, 4, 2, 4, 2>(L, seg, bit, dbit, in, out, s);
+  if (L.rp_e == 8 && L.rp_rows == 1)
+    return tri_rp_go<kWaves, kFar, 8, 1, 4, 2>(L, seg, bit, dbit, in, out, s);
+  if (L.rp_e == 12 && L.rp_rows == 1)
+    return tri_rp_go<kWaves, kFar, 12, 1, 4, 2>(L, seg, bit, dbit, in, out, s);
+  return hipErrorInvalidValue;
+}
+
+template <int kWaves>
+hipError_t tri_rp_far(const TriLayout& L, const TriSegment& seg, int bit, int dbit,
+                      const int32_t* in, int32_t* out, hipStream_t s) {
+  if (L.ep_far) return tri_rp_layout<kWaves, true>(L, seg, bit, dbit, in, out, s);
+  return tri_rp_layout<kWaves, false>(L, seg, bit, dbit, in, out, s);
+}
+
 // One level, many workgroups, level-ordered layout.
 template <typename T, TriKind K>
 __global__ __launch_bounds__(kBlock) void k_tri_level_pre(
@@ -1384,6 +1578,17 @@ hipError_t launch_trsv_ep(TriKind kind, const TriLayout& L, const TriSegment& se
   if (w <= 2) return tri_ep_launch<TriKind::kDU, 2>(L, seg, bit, dbit, in, o, s);
   if (w <= 4) return tri_ep_launch<TriKind::kDU, 4>(L, seg, bit, dbit, in, o, s);
   return tri_ep_launch<TriKind::kDU, 8>(L, seg, bit, dbit, in, o, s);
+}
+
+hipError_t launch_trsv_rp(const TriLayout& L, const TriSegment& seg, int bit, int dbit,
+                          const void* in_v, void* out_v, hipStream_t s) {
+  const int32_t* in = static_cast<const int32_t*>(in_v);
+  int32_t* o = static_cast<int32_t*>(out_v);
+  const int w = L.rp_waves;
+  if (w <= 1) return tri_rp_far<1>(L, seg, bit, dbit, in, o, s);
+  if (w <= 2) return tri_rp_far<2>(L, seg, bit, dbit, in, o, s);
+  if (w <= 4) return tri_rp_far<4>(L, seg, bit, dbit, in, o, s);
+  return tri_rp_far<8>(L, seg, bit, dbit, in, o, s);
 }
 
 hipError_t launch_trsv_segment(Elem e, TriKind kind, const DCsr& a,
