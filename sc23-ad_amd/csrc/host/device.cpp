@@ -476,37 +476,59 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
       // every row of a level and about as many entries.
       L.ep_narrow = (maxlev_rows <= static_cast<size_t>(w) * 64 && overflowing(w, 4) * 16 <= nlev)
                         ? 1 : 0;
-      // Row-parallel mode for smoother factors with short rows (at most
-      // kRpMaxEntries dependencies): a lane per row, no LDS accumulators.
-      // SC23AD_TRI_RP=0 keeps the entry-parallel kernel; SC23AD_RP_ROWS=1/2
-      // sets the rows per lane of the 4-entry layout (default 1: at 128^3
-      // level 0 took 157 us with 8 waves of one row per lane, 174 us with 4
-      // waves of two).
+      // Row-parallel mode for smoother factors with short rows: one lane per
+      // row for at most kRpMaxEntries dependencies, else (no far entries)
+      // 2-16 lanes of 8 or 12 entries per row; no LDS accumulators.
+      // SC23AD_TRI_RP=0 keeps the entry-parallel kernel, SC23AD_TRI_RP=1
+      // uses one lane per row only; SC23AD_RP_ROWS=2 puts two rows on each
+      // lane of the 4-entry layout (default one: at 128^3 level 0 took 157 us
+      // with 8 waves of one row per lane, 174 us with 4 waves of two).
       int md = 0;
       for (int idx = 0; idx < n; ++idx) md = std::max(md, lbeg[idx + 1] - lbeg[idx]);
       const char* env_rp = std::getenv("SC23AD_TRI_RP");
-      if (kind == TriKind::kLD && md <= dev::kRpMaxEntries && !(env_rp && env_rp[0] == '0') &&
-          dev::rp_lds_bytes(ep_ring, maxlev_chain) <= dev::kTriBlockedMaxLds) {
-        const int e = md <= 4 ? 4 : md <= 8 ? 8 : 12;
-        int rows = 1;
-        if (const char* r = std::getenv("SC23AD_RP_ROWS"); r && e == 4) rows = r[0] == '1' ? 1 : 2;
-        int rw = 1;
-        while (rw < 8 && static_cast<size_t>(rw) * 64 * rows < maxlev_rows) rw *= 2;
-        if (static_cast<size_t>(rw) * 64 * rows >= maxlev_rows) {
-          std::vector<int> rent(static_cast<size_t>(std::max(n, 1)) * e * 2, 0);
-          for (int idx = 0; idx < n; ++idx)
-            for (int q = lbeg[idx]; q < lbeg[idx + 1]; ++q) {
-              const size_t at = (static_cast<size_t>(idx) * e + (q - lbeg[idx])) * 2;
-              rent[at] = lent[4 * static_cast<size_t>(q)];
-              rent[at + 1] = lent[4 * static_cast<size_t>(q) + 1];
+      int e = 0, lanes = 1, rows = 1;
+      const bool force_groups = env_rp && env_rp[0] == '2';  // tuning: lanes per row even for short rows
+      if (md <= dev::kRpMaxEntries && !(force_groups && !any_far && md > 4)) {
+        e = md <= 4 ? 4 : md <= 8 ? 8 : 12;
+        if (const char* r = std::getenv("SC23AD_RP_ROWS"); r && e == 4 && r[0] == '2') rows = 2;
+      } else if (!any_far && !(env_rp && env_rp[0] == '1')) {
+        // the fewest waves, then the fewest entries per row
+        size_t best = 0;
+        for (int g = 2; g <= dev::kRpMaxLanes; g *= 2)
+          for (int ge : {8, 12}) {
+            if (g * ge < md) continue;
+            const size_t w = (maxlev_rows * g + 63) / 64;
+            const size_t key = w * 4096 + static_cast<size_t>(g * ge);
+            if (w <= 8 && (best == 0 || key < best)) {
+              best = key;
+              lanes = g;
+              e = ge;
             }
-          plan.rent.alloc(sizeof(int) * rent.size());
-          plan.rent.upload(rent.data(), sizeof(int) * rent.size());
-          L.rent = plan.rent.get();
-          L.rp_e = e;
-          L.rp_rows = rows;
-          L.rp_waves = rw;
-        }
+          }
+      }
+      if (kind != TriKind::kLD || (env_rp && env_rp[0] == '0') ||
+          dev::rp_lds_bytes(ep_ring, maxlev_chain) > dev::kTriBlockedMaxLds)
+        e = 0;
+      const size_t lanes_needed = (maxlev_rows + rows - 1) / rows * lanes;
+      int rw = 1;
+      while (rw < 8 && static_cast<size_t>(rw) * 64 < lanes_needed) rw *= 2;
+      if (e > 0 && static_cast<size_t>(rw) * 64 >= lanes_needed) {
+        std::vector<int> rent(static_cast<size_t>(std::max(n, 1)) * lanes * e * 2, 0);
+        for (int idx = 0; idx < n; ++idx)
+          for (int q = lbeg[idx]; q < lbeg[idx + 1]; ++q) {
+            const int m = q - lbeg[idx];
+            const size_t at =
+                ((static_cast<size_t>(idx) * lanes + m % lanes) * e + m / lanes) * 2;
+            rent[at] = lent[4 * static_cast<size_t>(q)];
+            rent[at + 1] = lent[4 * static_cast<size_t>(q) + 1];
+          }
+        plan.rent.alloc(sizeof(int) * rent.size());
+        plan.rent.upload(rent.data(), sizeof(int) * rent.size());
+        L.rent = plan.rent.get();
+        L.rp_e = e;
+        L.rp_rows = rows;
+        L.rp_lanes = lanes;
+        L.rp_waves = rw;
       }
     }
   }
@@ -516,10 +538,10 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
     std::fprintf(stderr,
                  "[sc23ad plan] kind %d n %d nnz %zu chains %d levels(max chain) %d "
                  "max level rows %zu launches %d ring %d span %lld max deps %d ep waves %d far %d "
-                 "narrow %d copies %d rp waves %d entries %d rows %d identity %d\n",
+                 "narrow %d copies %d rp waves %d entries %d rows %d lanes %d identity %d\n",
                  static_cast<int>(kind), n, cind.size(), nchain, plan.nlevels, maxlev_rows,
                  plan.nlaunch, L.ring, span, maxdeps, L.ep_waves, L.ep_far, L.ep_narrow,
-                 L.ep_copies, L.rp_waves, L.rp_e, L.rp_rows, plan.identity ? 1 : 0);
+                 L.ep_copies, L.rp_waves, L.rp_e, L.rp_rows, L.rp_lanes, plan.identity ? 1 : 0);
   }
 }
 

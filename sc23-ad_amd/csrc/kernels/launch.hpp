@@ -96,14 +96,15 @@ struct TriLayout {
   int ep_maxlev = 0; // most levels in one chain
   int ep_copies = 1; // accumulator copies per row (1 or kEpCopies)
   int ep_acc = 0;    // accumulator slots in LDS
-  // row-parallel mode (LD factors whose rows have at most kRpMaxEntries
-  // entries): entries padded to rp_e per position as int2 {dep, val}, rp_rows
-  // rows per lane, rp_waves waves per chain; same chains, ring and far rule
-  // as the entry-parallel mode
+  // row-parallel mode (LD factors with short rows): rp_lanes lanes per row,
+  // each holding rp_e entries (int2 {dep, val}, padded) of its row, rp_rows
+  // rows per lane (one lane per row only), rp_waves waves per chain; same
+  // chains, ring and far rule as the entry-parallel mode
   const void* rent = nullptr;
   int rp_waves = 0;  // > 0: use the row-parallel kernel
   int rp_e = 0;
   int rp_rows = 0;
+  int rp_lanes = 1;
   int n = 0;
 };
 // Entry-parallel integer solve (one workgroup per chain).
@@ -116,8 +117,9 @@ constexpr int kEpMaxRows = 1024;  // most rows in one level of one chain
 constexpr int kEpCopies = 8;
 hipError_t launch_trsv_ep(TriKind kind, const TriLayout& L, const TriSegment& seg, int bit,
                           int dbit, const void* in, void* out, hipStream_t s);
-// Row-parallel integer LD solve (one workgroup per chain, a lane per row).
-constexpr int kRpMaxEntries = 12;
+// Row-parallel integer LD solve (one workgroup per chain, 1-16 lanes per row).
+constexpr int kRpMaxEntries = 12;  // one lane per row
+constexpr int kRpMaxLanes = 16;    // several lanes per row: 8 or 12 entries each
 hipError_t launch_trsv_rp(const TriLayout& L, const TriSegment& seg, int bit, int dbit,
                           const void* in, void* out, hipStream_t s);
 inline size_t rp_lds_bytes(int ring, int maxlev) {

@@ -1140,14 +1140,16 @@ hipError_t tri_ep_launch(const TriLayout& L, const TriSegment& seg, int bit, int
 
 // ---------------------------------------------------------------------------
 // Row-parallel integer LD solve: the chains, levels, LDS ring and far
-// dependencies of k_tri_chain_ep, for factors whose rows hold at most kE
-// dependency entries.  Lane t owns rows t, t + kThreads, ... (kR of them) of
-// each level and keeps their entries, padded to kE with zero values, in
-// registers; a level costs one round of ring reads, the row sums in
-// registers, the ring writes and one barrier (no accumulators, no atomics).
-// The integer sum is order-free (wrap-around), so the result is the same as
-// the sequential one bit for bit.  Entries are stored by position, kE int2
-// {dep, val} pairs each (dep as in k_tri_chain_ep; padding {0, 0}).
+// dependencies of k_tri_chain_ep, for factors whose rows hold at most kG * kE
+// dependency entries.  kG lanes serve one row (kG = 1: each lane serves kR
+// rows t, t + kThreads, ...); each lane keeps kE entries of its row, padded
+// with zero values, in registers.  A level costs one round of ring reads, the
+// products in registers, a sum over the row's kG lanes (DPP moves),
+// the ring writes and one barrier: no accumulators, no atomics.  The integer
+// sum is order-free (wrap-around), so the result equals the sequential one
+// bit for bit.  Entries are stored by position as kG * kE int2 {dep, val}
+// pairs, lane j's kE contiguous (a row's entry m goes to lane m % kG); dep is
+// encoded as in k_tri_chain_ep and padding is {0, 0}.
 template <int kE, int kR>
 struct RpPre {
   int lo, hi;             // the level's position bounds
@@ -1157,7 +1159,7 @@ struct RpPre {
   int32_t in[kR];
 };
 
-template <int kThreads, int kE, int kR>
+template <int kThreads, int kG, int kE, int kR>
 __device__ __forceinline__ void rp_load(RpPre<kE, kR>& r, int lo, int hi,
                                         const int4* __restrict__ rent,
                                         const int32_t* __restrict__ dinv,
@@ -1167,21 +1169,22 @@ __device__ __forceinline__ void rp_load(RpPre<kE, kR>& r, int lo, int hi,
   r.hi = hi;
 #pragma unroll
   for (int q = 0; q < kR; ++q) {
-    const int idx = lo + t + q * kThreads;
+    const int idx = lo + t / kG + q * (kThreads / kG);
     const int id = idx < hi ? idx : 0;  // lanes past the level's end load position 0
+    const size_t base = (static_cast<size_t>(id) * kG + t % kG) * (kE / 2);
 #pragma unroll
-    for (int h = 0; h < kE / 2; ++h) r.ent[q][h] = rent[static_cast<size_t>(id) * (kE / 2) + h];
+    for (int h = 0; h < kE / 2; ++h) r.ent[q][h] = rent[base + h];
     r.dinv[q] = dinv[id];
     r.in[q] = in[id];
   }
 }
 
-template <int kThreads, int kE, int kR>
+template <int kThreads, int kG, int kE, int kR>
 __device__ __forceinline__ void rp_far(RpPre<kE, kR>& r, const int32_t* out) {
   const int t = threadIdx.x;
 #pragma unroll
   for (int q = 0; q < kR; ++q) {
-    const bool ok = r.lo + t + q * kThreads < r.hi;
+    const bool ok = r.lo + t / kG + q * (kThreads / kG) < r.hi;
 #pragma unroll
     for (int h = 0; h < kE / 2; ++h) {
       const int d0 = r.ent[q][h].x, d1 = r.ent[q][h].z;
@@ -1191,7 +1194,31 @@ __device__ __forceinline__ void rp_far(RpPre<kE, kR>& r, const int32_t* out) {
   }
 }
 
-template <TriKind K, int kWaves, bool kFar, int kE, int kR, int kD, int kA>
+// Sum of a 64-bit value over aligned groups of kG <= 16 lanes, with DPP
+// moves inside rows of 16 lanes: swap neighbours (quad_perm 1,0,3,2), swap
+// pairs (quad_perm 2,3,0,1), then lane i with lane 7 - i of its 8 (row half
+// mirror) and lane 15 - i of its 16 (row mirror).  Every lane ends with its
+// group's sum.  All lanes must be active.
+template <int kCtrl>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(static_cast<uint32_t>(v)), kCtrl,
+                                             0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(
+      0, static_cast<int>(static_cast<uint32_t>(v >> 32)), kCtrl, 0xF, 0xF, false);
+  return (static_cast<uint64_t>(static_cast<uint32_t>(hi)) << 32) | static_cast<uint32_t>(lo);
+}
+
+template <int kG>
+__device__ __forceinline__ uint64_t group_sum(uint64_t v) {
+  static_assert(kG == 1 || kG == 2 || kG == 4 || kG == 8 || kG == 16, "groups of 1-16 lanes");
+  if (kG >= 2) v += dpp_u64<0xB1>(v);   // quad_perm(1, 0, 3, 2)
+  if (kG >= 4) v += dpp_u64<0x4E>(v);   // quad_perm(2, 3, 0, 1)
+  if (kG >= 8) v += dpp_u64<0x141>(v);  // row_half_mirror
+  if (kG >= 16) v += dpp_u64<0x140>(v); // row_mirror
+  return v;
+}
+
+template <TriKind K, int kWaves, bool kFar, int kG, int kE, int kR, int kD, int kA>
 __device__ __forceinline__ void rp_level(RpPre<kE, kR>& cur, RpPre<kE, kR>& ahead, int L, int nl,
                                          const int* lvo, int32_t* ring, int rmask,
                                          const int4* __restrict__ rent,
@@ -1204,7 +1231,7 @@ __device__ __forceinline__ void rp_level(RpPre<kE, kR>& cur, RpPre<kE, kR>& ahea
   // bounds of the level prefetched at the end of this one
   const int Lp = min(L + kD, nl);
   const int plo = lvo[Lp], phi = lvo[min(Lp + 1, nl)];
-  if (kFar) rp_far<kThreads>(ahead, out);
+  if (kFar) rp_far<kThreads, kG>(ahead, out);
   int32_t xv[kR][kE];
 #pragma unroll
   for (int q = 0; q < kR; ++q)
@@ -1225,14 +1252,15 @@ __device__ __forceinline__ void rp_level(RpPre<kE, kR>& cur, RpPre<kE, kR>& ahea
   }
 #pragma unroll
   for (int q = 0; q < kR; ++q) {
-    uint64_t acc = 0;
+    uint64_t acc = 0, acc2 = 0;  // two chains of multiply-adds
 #pragma unroll
     for (int h = 0; h < kE / 2; ++h) {
       acc += prod64(cur.ent[q][h].y, xv[q][2 * h]);
-      acc += prod64(cur.ent[q][h].w, xv[q][2 * h + 1]);
+      acc2 += prod64(cur.ent[q][h].w, xv[q][2 * h + 1]);
     }
-    const int k = s + t + q * kThreads;
-    if (k < e) {
+    acc = group_sum<kG>(acc + acc2);
+    const int k = s + t / kG + q * (kThreads / kG);
+    if (k < e && t % kG == 0) {
       const uint64_t tt = (static_cast<uint64_t>(static_cast<int64_t>(cur.in[q])) << bit) - acc;
       const int32_t v = acc_final<int32_t, K>(tt, cur.dinv[q], bit, dbit);
       ring[k & rmask] = v;
@@ -1243,18 +1271,22 @@ __device__ __forceinline__ void rp_level(RpPre<kE, kR>& cur, RpPre<kE, kR>& ahea
     __syncthreads();  // full fence: global results visible to later loads
   else
     ep_sync<kWaves>();
-  rp_load<kThreads>(cur, plo, phi, rent, dinv, in);
+  rp_load<kThreads, kG>(cur, plo, phi, rent, dinv, in);
 }
 
 // LDS: the ring, then the chain's level offsets (max_levels + 1 ints).
 // Far values of level L + kA are loaded at the start of level L, from entries
 // loaded kD - kA levels before that.
-template <TriKind K, int kWaves, bool kFar, int kE, int kR, int kD, int kA>
+template <TriKind K, int kWaves, bool kFar, int kG, int kE, int kR, int kD, int kA>
 __global__ __launch_bounds__(64 * kWaves) void k_tri_chain_rp(
     const int4* __restrict__ rent, const int32_t* __restrict__ dinv,
     const int* __restrict__ lev_ptr, const int* __restrict__ chain_lev, int rmask, int bit,
     int dbit, const int32_t* __restrict__ in, int32_t* out) {
   constexpr int kThreads = 64 * kWaves;
+  static_assert(kG == 1 || kR == 1, "several lanes per row serve one row each");
+  static_assert(kA >= 1 && kA < kD, "far values need their entries loaded first");
+  static_assert(kEpFarLevels >= kA + kEpFence,
+                "far dependencies must be fenced before their values are loaded");
   extern __shared__ __align__(16) unsigned char smem_r[];
   int32_t* ring = reinterpret_cast<int32_t*>(smem_r);
   int* lvo = ring + rmask + 1;
@@ -1267,26 +1299,27 @@ __global__ __launch_bounds__(64 * kWaves) void k_tri_chain_rp(
 #pragma unroll
   for (int d = 0; d < kD; ++d) {
     const int Lc = min(d, nl);
-    rp_load<kThreads>(buf[d], lvo[Lc], lvo[min(Lc + 1, nl)], rent, dinv, in);
+    rp_load<kThreads, kG>(buf[d], lvo[Lc], lvo[min(Lc + 1, nl)], rent, dinv, in);
   }
-  static_assert(kA >= 1 && kA < kD, "far values need their entries loaded first");
-  static_assert(kEpFarLevels >= kA + kEpFence,
-                "far dependencies must be fenced before their values are loaded");
+  // Whole rounds of kD levels, then the tail in a copy of the body.  (One
+  // body for all levels measured slower: with a test around each level the
+  // compiler waits for every outstanding load where the paths join, and
+  // without one the last round runs up to kD - 1 empty levels.)
   int L0 = 0;
   for (; L0 + kD <= nl; L0 += kD) {
 #pragma unroll
     for (int d = 0; d < kD; ++d)
-      rp_level<K, kWaves, kFar, kE, kR, kD, kA>(buf[d], buf[(d + kA) % kD], L0 + d, nl, lvo, ring,
-                                                rmask, rent, dinv, in, out, bit, dbit);
+      rp_level<K, kWaves, kFar, kG, kE, kR, kD, kA>(buf[d], buf[(d + kA) % kD], L0 + d, nl, lvo,
+                                                    ring, rmask, rent, dinv, in, out, bit, dbit);
   }
 #pragma unroll
   for (int d = 0; d < kD; ++d)
     if (L0 + d < nl)
-      rp_level<K, kWaves, kFar, kE, kR, kD, kA>(buf[d], buf[(d + kA) % kD], L0 + d, nl, lvo, ring,
-                                                rmask, rent, dinv, in, out, bit, dbit);
+      rp_level<K, kWaves, kFar, kG, kE, kR, kD, kA>(buf[d], buf[(d + kA) % kD], L0 + d, nl, lvo,
+                                                    ring, rmask, rent, dinv, in, out, bit, dbit);
 }
 
-template <int kWaves, bool kFar, int kE, int kR, int kD, int kA>
+template <int kWaves, bool kFar, int kG, int kE, int kR, int kD, int kA>
 hipError_t tri_rp_go(const TriLayout& L, const TriSegment& seg, int bit, int dbit,
                      const int32_t* in, int32_t* out, hipStream_t s) {
   constexpr TriKind K = TriKind::kLD;
@@ -1295,33 +1328,55 @@ hipError_t tri_rp_go(const TriLayout& L, const TriSegment& seg, int bit, int dbi
   static bool attr = false;
   if (!attr) {
     hipError_t e = hipFuncSetAttribute(
-        reinterpret_cast<const void*>(&k_tri_chain_rp<K, kWaves, kFar, kE, kR, kD, kA>),
+        reinterpret_cast<const void*>(&k_tri_chain_rp<K, kWaves, kFar, kG, kE, kR, kD, kA>),
         hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kTriBlockedMaxLds));
     if (e != hipSuccess) return e;
     attr = true;
   }
-  launch_timed(&k_tri_chain_rp<K, kWaves, kFar, kE, kR, kD, kA>, dim3(seg.nchain), dim3(64 * kWaves),
-               static_cast<uint32_t>(lds), s, reinterpret_cast<const int4*>(L.rent),
-               static_cast<const int32_t*>(L.ldiag), L.lev_ptr, seg.chain_lev, L.ring - 1, bit,
-               dbit, in, out);
+  launch_timed(&k_tri_chain_rp<K, kWaves, kFar, kG, kE, kR, kD, kA>, dim3(seg.nchain),
+               dim3(64 * kWaves), static_cast<uint32_t>(lds), s,
+               reinterpret_cast<const int4*>(L.rent), static_cast<const int32_t*>(L.ldiag),
+               L.lev_ptr, seg.chain_lev, L.ring - 1, bit, dbit, in, out);
   return hipGetLastError();
 }
 
-// Register layouts: 4 entries and 1 row per lane (prefetch distance 8
-// levels, far values 4 levels ahead: a level takes less than one global-load
-// latency, and a far value waits for two); 4 entries and 2 rows, 8 entries
-// and 1 row, 12 entries and 1 row (distance 4, far values 2 ahead).
+// Register layouts.  4 entries and 1 or 2 rows per lane: prefetch distance 8
+// levels and far values 4 levels ahead (a level takes less than one
+// global-load latency, and a far value waits for two: at 128^3 the finest
+// level's solve took 116 us, against 157 us at distance 4).  One lane with 8
+// or 12 entries, or 2, 4, 8 or 16 lanes of 8 or 12 entries per row (factors
+// without far dependencies): distance 4, far values 2 ahead (distance 8
+// measured no faster there).
 template <int kWaves, bool kFar>
 hipError_t tri_rp_layout(const TriLayout& L, const TriSegment& seg, int bit, int dbit,
                          const int32_t* in, int32_t* out, hipStream_t s) {
-  if (L.rp_e == 4 && L.rp_rows == 1)
-    return tri_rp_go<kWaves, kFar, 4, 1, 8, 4>(L, seg, bit, dbit, in, out, s);
-  if (L.rp_e == 4 && L.rp_rows == 2)
-    return tri_rp_go<kWaves, kFar, 4, 2, 4, 2>(L, seg, bit, dbit, in, out, s);
-  if (L.rp_e == 8 && L.rp_rows == 1)
-    return tri_rp_go<kWaves, kFar, 8, 1, 4, 2>(L, seg, bit, dbit, in, out, s);
-  if (L.rp_e == 12 && L.rp_rows == 1)
-    return tri_rp_go<kWaves, kFar, 12, 1, 4, 2>(L, seg, bit, dbit, in, out, s);
+  if (L.rp_lanes <= 1) {
+    if (L.rp_e == 4 && L.rp_rows == 1)
+      return tri_rp_go<kWaves, kFar, 1, 4, 1, 8, 4>(L, seg, bit, dbit, in, out, s);
+    if (L.rp_e == 4 && L.rp_rows == 2)
+      return tri_rp_go<kWaves, kFar, 1, 4, 2, 8, 4>(L, seg, bit, dbit, in, out, s);
+    if (L.rp_e == 8 && L.rp_rows == 1)
+      return tri_rp_go<kWaves, kFar, 1, 8, 1, 4, 2>(L, seg, bit, dbit, in, out, s);
+    if (L.rp_e == 12 && L.rp_rows == 1)
+      return tri_rp_go<kWaves, kFar, 1, 12, 1, 4, 2>(L, seg, bit, dbit, in, out, s);
+    return hipErrorInvalidValue;
+  }
+  if constexpr (!kFar) {
+    if (L.rp_e == 12 && L.rp_rows == 1) {
+      if (L.rp_lanes == 2) return tri_rp_go<kWaves, false, 2, 12, 1, 4, 2>(L, seg, bit, dbit, in, out, s);
+      if (L.rp_lanes == 4) return tri_rp_go<kWaves, false, 4, 12, 1, 4, 2>(L, seg, bit, dbit, in, out, s);
+      if (L.rp_lanes == 8) return tri_rp_go<kWaves, false, 8, 12, 1, 4, 2>(L, seg, bit, dbit, in, out, s);
+      if (L.rp_lanes == 16)
+        return tri_rp_go<kWaves, false, 16, 12, 1, 4, 2>(L, seg, bit, dbit, in, out, s);
+    }
+    if (L.rp_e == 8 && L.rp_rows == 1) {
+      if (L.rp_lanes == 2) return tri_rp_go<kWaves, false, 2, 8, 1, 4, 2>(L, seg, bit, dbit, in, out, s);
+      if (L.rp_lanes == 4) return tri_rp_go<kWaves, false, 4, 8, 1, 4, 2>(L, seg, bit, dbit, in, out, s);
+      if (L.rp_lanes == 8) return tri_rp_go<kWaves, false, 8, 8, 1, 4, 2>(L, seg, bit, dbit, in, out, s);
+      if (L.rp_lanes == 16)
+        return tri_rp_go<kWaves, false, 16, 8, 1, 4, 2>(L, seg, bit, dbit, in, out, s);
+    }
+  }
   return hipErrorInvalidValue;
 }
 
