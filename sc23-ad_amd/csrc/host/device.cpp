@@ -1,6 +1,7 @@
 #include "device.hpp"
 
 #include <algorithm>
+#include <numeric>
 #include <type_traits>
 #include <cstdio>
 #include <cstdlib>
@@ -105,20 +106,46 @@ void upload_mat(DevMat& dm, int n, int m, int bit, int dbit,
   dm.d.group = pick_group(n, dm.d.nnz, group);
   // SELL-64 copy when the rows of each 64-row slice have near-equal lengths
   // (padding at most 1/8 of the entries): one thread per row, coalesced.
-  // Whether it is used is decided by tune_spmv.
+  // Rows of irregular lengths (prolongation: one entry on coarse points,
+  // several on fine ones) are sorted by length, longest first, inside windows
+  // of kSellWindow rows, when that brings the padding under 1/2 of the
+  // entries (the one-lane-per-row CSR kernel reads such rows uncoalesced).
+  // Whether the copy is used is decided by tune_spmv.
   if (n >= 4096) {
     const int ns = (n + 63) / 64;
-    std::vector<long long> soff(static_cast<size_t>(ns) + 1, 0);
-    for (int sl = 0; sl < ns; ++sl) {
-      int w = 0;
-      for (int r = sl * 64; r < std::min(n, sl * 64 + 64); ++r) w = std::max(w, rptr[r + 1] - rptr[r]);
-      soff[sl + 1] = soff[sl] + 64LL * w;
+    auto slices = [&](const std::vector<int>& order, std::vector<long long>& soff, int num,
+                      int den) {
+      soff.assign(static_cast<size_t>(ns) + 1, 0);
+      for (int sl = 0; sl < ns; ++sl) {
+        int w = 0;
+        for (int k = sl * 64; k < std::min(n, sl * 64 + 64); ++k) {
+          const int r = order.empty() ? k : order[k];
+          w = std::max(w, rptr[r + 1] - rptr[r]);
+        }
+        soff[sl + 1] = soff[sl] + 64LL * w;
+      }
+      return soff[ns] * den <= static_cast<long long>(cind.size()) * num;
+    };
+    std::vector<int> order;
+    std::vector<long long> soff;
+    bool ok = slices(order, soff, 9, 8);
+    if (!ok) {
+      order.resize(n);
+      std::iota(order.begin(), order.end(), 0);
+      for (int w0 = 0; w0 < n; w0 += dev::kSellWindow) {
+        const int w1 = std::min(n, w0 + dev::kSellWindow);
+        std::stable_sort(order.begin() + w0, order.begin() + w1, [&](int a, int b) {
+          return rptr[a + 1] - rptr[a] > rptr[b + 1] - rptr[b];
+        });
+      }
+      ok = slices(order, soff, 3, 2);  // padding at most 1/2: the tuner decides
     }
-    if (soff[ns] * 8 <= static_cast<long long>(cind.size()) * 9) {
+    if (ok) {
       std::vector<int> scol(static_cast<size_t>(std::max<long long>(soff[ns], 1)), -1);
       std::vector<V> sval(scol.size(), V(0));
-      for (int i = 0; i < n; ++i) {
-        const long long b = soff[i / 64] + (i % 64);
+      for (int k = 0; k < n; ++k) {
+        const int i = order.empty() ? k : order[k];
+        const long long b = soff[k / 64] + (k % 64);
         for (int p = rptr[i]; p < rptr[i + 1]; ++p) {
           scol[static_cast<size_t>(b + 64LL * (p - rptr[i]))] = cind[p];
           sval[static_cast<size_t>(b + 64LL * (p - rptr[i]))] = val[p];
@@ -133,6 +160,11 @@ void upload_mat(DevMat& dm, int n, int m, int bit, int dbit,
       dm.d.soff = dm.soff.as<long long>();
       dm.d.scol = dm.scol.as<int>();
       dm.d.sval = dm.sval.get();
+      if (!order.empty()) {
+        dm.srow.alloc(sizeof(int) * order.size());
+        dm.srow.upload(order.data(), sizeof(int) * order.size());
+        dm.d.srow = dm.srow.as<int>();
+      }
     }
   }
 }
@@ -725,6 +757,7 @@ void tune_spmv(DevMat& m, Elem e) {
   std::vector<dev::DCsr> cands;
   dev::DCsr csr = m.d;
   csr.soff = nullptr;
+  csr.srow = nullptr;
   if (e == Elem::kI32) {
     const double avg = static_cast<double>(m.d.nnz) / m.d.n;
     for (int g = 1; g <= 64; g *= 2) {
@@ -755,6 +788,10 @@ void tune_spmv(DevMat& m, Elem e) {
     hip_check(hipEventSynchronize(e1), "sync");
     float ms = 0;
     hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+    if (std::getenv("SC23AD_VERBOSE"))
+      std::fprintf(stderr, "[sc23ad spmv tune] n %d nnz %d %s%s lanes %d: %.2f us\n", m.d.n,
+                   m.d.nnz, cands[i].soff ? "SELL" : "CSR", cands[i].srow ? " sorted" : "",
+                   cands[i].group, ms * 1000.0f / 5);
     if (ms < best) {
       best = ms;
       best_i = i;
@@ -767,6 +804,7 @@ void tune_spmv(DevMat& m, Elem e) {
     m.soff = DevBuf();
     m.scol = DevBuf();
     m.sval = DevBuf();
+    m.srow = DevBuf();
   }
 }
 

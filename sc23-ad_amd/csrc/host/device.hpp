@@ -75,7 +75,7 @@ class DevBuf {
 struct DevMat {
   dev::DCsr d;
   DevBuf rptr, cind, val;
-  DevBuf soff, scol, sval;  // SELL-64 copy (optional)
+  DevBuf soff, scol, sval, srow;  // SELL-64 copy (optional; srow: row order)
   size_t algo_bytes_spmv(size_t elem) const;
 };
 

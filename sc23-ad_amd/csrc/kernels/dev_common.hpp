@@ -36,6 +36,17 @@ __device__ __forceinline__ int64_t trunc_i64(double v) {
   return static_cast<int64_t>(v);
 }
 
+// The dispatcher hands workgroups to the 8 XCDs round-robin (workgroup b runs
+// on XCD b % 8), and each XCD has its own L2.  This renumbering gives every
+// XCD a contiguous range of workgroup indices (a bijection of [0, gridDim.x)),
+// so neighbouring rows, whose gathers share cache lines, run under one L2.
+__device__ __forceinline__ int xcd_block() {
+  constexpr int kXcd = 8;
+  const int nb = static_cast<int>(gridDim.x), b = static_cast<int>(blockIdx.x);
+  const int q = nb / kXcd, r = nb % kXcd, x = b % kXcd, k = b / kXcd;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
+}
+
 __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int mask) {
   const int lo = __shfl_xor(static_cast<int>(static_cast<uint32_t>(v)), mask);
   const int hi = __shfl_xor(static_cast<int>(static_cast<uint32_t>(v >> 32)), mask);

@@ -18,6 +18,7 @@ constexpr int kRedThreads = 256;
 constexpr int kRedTotal = kRedBlocks * kRedThreads;
 
 // Device view of a CSR matrix.  val points at int32 / float / double values.
+constexpr int kSellWindow = 1024;
 struct DCsr {
   int n = 0, m = 0, nnz = 0, bit = 0, dbit = -1;
   const int* rptr = nullptr;
@@ -26,9 +27,12 @@ struct DCsr {
   int group = 8;  // lanes per row for the integer SpMV (power of two <= 64)
   // SELL-64 copy (rows with near-uniform lengths): slices of 64 rows, entry k
   // of row r of slice s at soff[s] + 64 k + r, padded with column -1 / value 0.
+  // srow (optional): the slices hold rows srow[0], srow[1], ... (rows sorted
+  // by length inside windows of kSellWindow rows), else rows in order.
   const long long* soff = nullptr;
   const int* scol = nullptr;
   const void* sval = nullptr;
+  const int* srow = nullptr;
 };
 
 // Dispatch-recorded timing.  When armed, the next launch by a timing-aware

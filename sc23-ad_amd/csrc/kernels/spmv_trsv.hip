@@ -7,6 +7,7 @@
 // per row, so that the device rounds exactly like the CPU.
 #include <hip/hip_ext.h>
 
+#include <cstdlib>
 #include <type_traits>
 
 #include "dev_common.hpp"
@@ -30,6 +31,18 @@ namespace {
 
 constexpr int kBlock = 256;
 
+// SpMV workgroups renumbered per XCD (xcd_block) when SC23AD_XCD_REMAP=1.
+// Off by default: at 128^3 it cut the HBM fetches of the restriction and
+// prolongation by 30-45 % (their gathers stay under one L2) but not their
+// time, and the fine-level SpMV ran 4 % slower with it.
+int xcd_remap() {
+  static const int on = [] {
+    const char* e = std::getenv("SC23AD_XCD_REMAP");
+    return (e && e[0] == '1') ? 1 : 0;
+  }();
+  return on;
+}
+
 // A launch that honours armed dispatch events (arm_dispatch_events).
 template <typename F, typename... A>
 void launch_timed(F kern, dim3 grid, dim3 block, uint32_t lds, hipStream_t s, A... args) {
@@ -40,26 +53,47 @@ void launch_timed(F kern, dim3 grid, dim3 block, uint32_t lds, hipStream_t s, A.
     hipLaunchKernelGGL(kern, grid, block, lds, s, args...);
 }
 
+// CSR integer SpMV, G lanes per row.  Each lane takes the row's entries
+// lane, lane + G, ... eight at a time with every load unconditional (clamped
+// to the chunk's first entry), so a chunk costs two memory latencies (entries,
+// then the gathered x values) instead of two per entry.
 template <int G, int MODE>
 __global__ __launch_bounds__(kBlock) void k_spmv_int(
     int n, const int* __restrict__ rptr, const int* __restrict__ cind,
     const int32_t* __restrict__ val, int bit, const int32_t* __restrict__ x,
-    const int32_t* z, int32_t* y) {
-  const int gid = blockIdx.x * kBlock + threadIdx.x;
+    const int32_t* z, int32_t* y, int remap) {
+  const int blk = remap ? xcd_block() : static_cast<int>(blockIdx.x);
+  const int gid = blk * kBlock + threadIdx.x;
   const int row = gid / G;
   const int lane = gid & (G - 1);
   uint64_t acc = 0;
+  int32_t zv = 0;
   if (row < n) {
+    if (MODE != 0 && lane == 0) zv = z[row];
     const int rb = rptr[row], re = rptr[row + 1];
-    for (int p = rb + lane; p < re; p += G) acc += prod64(val[p], x[cind[p]]);
+    for (int p0 = rb + lane; p0 < re; p0 += 8 * G) {
+      int cc[8];
+      int32_t vv[8], xx[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int p = p0 + j * G < re ? p0 + j * G : p0;
+        cc[j] = cind[p];
+        vv[j] = val[p];
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) xx[j] = x[cc[j]];
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (p0 + j * G < re) acc += prod64(vv[j], xx[j]);
+    }
   }
 #pragma unroll
   for (int m = G / 2; m > 0; m >>= 1) acc += shfl_xor_u64(acc, m);
   if (row < n && lane == 0) {
     const int32_t s = low32(as_i64(acc) >> bit);
     if (MODE == 0) y[row] = s;
-    else if (MODE == 1) y[row] = sub32(z[row], s);
-    else y[row] = add32(z[row], s);
+    else if (MODE == 1) y[row] = sub32(zv, s);
+    else y[row] = add32(zv, s);
   }
 }
 
@@ -68,10 +102,11 @@ __global__ __launch_bounds__(kBlock) void k_spmv_int(
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_spmv_sell_int(
     int n, const long long* __restrict__ soff, const int* __restrict__ scol,
-    const int32_t* __restrict__ sval, int bit, const int32_t* __restrict__ x,
-    const int32_t* z, int32_t* y) {
-  const int row = blockIdx.x * kBlock + threadIdx.x;
+    const int32_t* __restrict__ sval, const int* __restrict__ srow, int bit,
+    const int32_t* __restrict__ x, const int32_t* z, int32_t* y, int remap) {
+  const int row = (remap ? xcd_block() : static_cast<int>(blockIdx.x)) * kBlock + threadIdx.x;
   if (row >= n) return;
+  const int orow = srow ? srow[row] : row;
   const long long b = soff[row >> 6];
   const int w = static_cast<int>((soff[(row >> 6) + 1] - b) >> 6);
   const int* c = scol + b + (row & 63);
@@ -93,9 +128,9 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell_int(
       if (k0 + j < w) acc += prod64(vv[j], xx[j]);
   }
   const int32_t s = low32(as_i64(acc) >> bit);
-  if (MODE == 0) y[row] = s;
-  else if (MODE == 1) y[row] = sub32(z[row], s);
-  else y[row] = add32(z[row], s);
+  if (MODE == 0) y[orow] = s;
+  else if (MODE == 1) y[orow] = sub32(z[orow], s);
+  else y[orow] = add32(z[orow], s);
 }
 
 // FP SELL-64 SpMV: the row's products are added in stored (column) order,
@@ -103,9 +138,11 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell_int(
 template <typename T, int MODE>
 __global__ __launch_bounds__(kBlock) void k_spmv_sell_fp(
     int n, const long long* __restrict__ soff, const int* __restrict__ scol,
-    const T* __restrict__ sval, const T* __restrict__ x, const T* z, T* y) {
-  const int row = blockIdx.x * kBlock + threadIdx.x;
+    const T* __restrict__ sval, const int* __restrict__ srow, const T* __restrict__ x,
+    const T* z, T* y, int remap) {
+  const int row = (remap ? xcd_block() : static_cast<int>(blockIdx.x)) * kBlock + threadIdx.x;
   if (row >= n) return;
+  const int orow = srow ? srow[row] : row;
   const long long b = soff[row >> 6];
   const int w = static_cast<int>((soff[(row >> 6) + 1] - b) >> 6);
   const int* c = scol + b + (row & 63);
@@ -126,16 +163,16 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell_fp(
     for (int j = 0; j < 8; ++j)
       if (k0 + j < w && cc[j] >= 0) t += vv[j] * xx[j];
   }
-  if (MODE == 0) y[row] = t;
-  else if (MODE == 1) y[row] = z[row] - t;
-  else y[row] = z[row] + t;
+  if (MODE == 0) y[orow] = t;
+  else if (MODE == 1) y[orow] = z[orow] - t;
+  else y[orow] = z[orow] + t;
 }
 
 template <typename T, int MODE>
 __global__ __launch_bounds__(kBlock) void k_spmv_fp(
     int n, const int* __restrict__ rptr, const int* __restrict__ cind,
-    const T* __restrict__ val, const T* __restrict__ x, const T* z, T* y) {
-  const int row = blockIdx.x * kBlock + threadIdx.x;
+    const T* __restrict__ val, const T* __restrict__ x, const T* z, T* y, int remap) {
+  const int row = (remap ? xcd_block() : static_cast<int>(blockIdx.x)) * kBlock + threadIdx.x;
   if (row >= n) return;
   T t = 0;
   const int rb = rptr[row], re = rptr[row + 1];
@@ -169,7 +206,7 @@ hipError_t spmv_int_g(const DCsr& a, const void* x, const void* z, void* y,
   if (grid == 0) return hipSuccess;
   launch_timed(&k_spmv_int<G, MODE>, dim3(grid), dim3(kBlock), 0, s, a.n, a.rptr, a.cind,
                static_cast<const int32_t*>(a.val), a.bit, static_cast<const int32_t*>(x),
-               static_cast<const int32_t*>(z), static_cast<int32_t*>(y));
+               static_cast<const int32_t*>(z), static_cast<int32_t*>(y), xcd_remap());
   return hipGetLastError();
 }
 
@@ -194,7 +231,7 @@ hipError_t spmv_fp(const DCsr& a, const void* x, const void* z, void* y,
   if (grid == 0) return hipSuccess;
   launch_timed(&k_spmv_fp<T, MODE>, dim3(grid), dim3(kBlock), 0, s, a.n, a.rptr, a.cind,
                static_cast<const T*>(a.val), static_cast<const T*>(x),
-               static_cast<const T*>(z), static_cast<T*>(y));
+               static_cast<const T*>(z), static_cast<T*>(y), xcd_remap());
   return hipGetLastError();
 }
 
@@ -1514,12 +1551,13 @@ hipError_t spmv_sell(const DCsr& a, const void* x, const void* z, void* y, hipSt
   if (grid == 0) return hipSuccess;
   if constexpr (std::is_same_v<T, int32_t>) {
     launch_timed(&k_spmv_sell_int<MODE>, dim3(grid), dim3(kBlock), 0, s, a.n, a.soff, a.scol,
-                 static_cast<const int32_t*>(a.sval), a.bit, static_cast<const int32_t*>(x),
-                 static_cast<const int32_t*>(z), static_cast<int32_t*>(y));
+                 static_cast<const int32_t*>(a.sval), a.srow, a.bit,
+                 static_cast<const int32_t*>(x),
+                 static_cast<const int32_t*>(z), static_cast<int32_t*>(y), xcd_remap());
   } else {
     launch_timed(&k_spmv_sell_fp<T, MODE>, dim3(grid), dim3(kBlock), 0, s, a.n, a.soff, a.scol,
-                 static_cast<const T*>(a.sval), static_cast<const T*>(x),
-                 static_cast<const T*>(z), static_cast<T*>(y));
+                 static_cast<const T*>(a.sval), a.srow, static_cast<const T*>(x),
+                 static_cast<const T*>(z), static_cast<T*>(y), xcd_remap());
   }
   return hipGetLastError();
 }
