@@ -608,15 +608,16 @@ def main():
         "converged": bool(cv_int.is_converged),
         "explicit_relres": rel_int,
         "roofline": {"bound": "hbm",
-                     "kernel": "fixed-point SpMV of A0 (Arnoldi step; SELL-64 or CSR, "
-                               "per-matrix autotuned)",
+                     "kernel": "fixed-point SpMV of A0 (Arnoldi step; SELL-64 with 16-bit "
+                               "column offsets, SELL-64 or CSR, per-matrix autotuned)",
                      "timing": "hipEvents bound to each launch's dispatch (hipExtLaunchKernel)",
                      "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBPS) if achieved else None,
                      "traffic": traffic, "algorithmic_bytes": spmv_bytes,
                      "launches_timed": launches, "mean_ms": spmv_ms},
         "roofline_sptrsv": {"bound": "hbm",
-                            "kernel": "integer LD SpTRSV, level 0 (smoother)",
+                            "kernel": "integer LD SpTRSV, level 0 (smoother; row-parallel, "
+                                      "one workgroup per BlockSOR chain)",
                             "timing": "hipEvents bound to each launch's dispatch "
                                       "(hipExtLaunchKernel)",
                             "achieved": trsv_achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",

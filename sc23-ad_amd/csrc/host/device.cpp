@@ -165,6 +165,27 @@ void upload_mat(DevMat& dm, int n, int m, int bit, int dbit,
         dm.srow.upload(order.data(), sizeof(int) * order.size());
         dm.d.srow = dm.srow.as<int>();
       }
+      // Integer square matrices whose entries lie within 32767 of the
+      // diagonal also get 16-bit column offsets (a tuner candidate).
+      if (std::is_same_v<V, int32_t> && order.empty() && n == m) {
+        std::vector<int16_t> s16(scol.size(), 0);
+        bool fits = true;
+        for (int k = 0; k < n && fits; ++k) {
+          const long long b = soff[k / 64] + (k % 64);
+          for (int p = rptr[k]; p < rptr[k + 1]; ++p) {
+            const int d = cind[p] - k;
+            if (d < -32767 || d > 32767) {
+              fits = false;
+              break;
+            }
+            s16[static_cast<size_t>(b + 64LL * (p - rptr[k]))] = static_cast<int16_t>(d);
+          }
+        }
+        if (fits) {
+          dm.scol16.alloc(sizeof(int16_t) * s16.size());
+          dm.scol16.upload(s16.data(), sizeof(int16_t) * s16.size());
+        }
+      }
     }
   }
 }
@@ -769,6 +790,22 @@ void tune_spmv(DevMat& m, Elem e) {
     cands.push_back(csr);
   }
   if (m.d.soff) cands.push_back(m.d);
+  if (m.d.soff && m.scol16.get()) {
+    dev::DCsr c16 = m.d;
+    c16.scol16 = m.scol16.as<int16_t>();
+    cands.push_back(c16);
+  }
+  // SC23AD_SPMV_FORMAT=csr|sell|sell16 keeps only that format's candidates
+  // where the matrix has it (tests run every kernel this way).
+  if (const char* f = std::getenv("SC23AD_SPMV_FORMAT")) {
+    const std::string want(f);
+    std::vector<dev::DCsr> keep;
+    for (const auto& c : cands) {
+      const char* k = !c.soff ? "csr" : c.scol16 ? "sell16" : "sell";
+      if (want == k) keep.push_back(c);
+    }
+    if (!keep.empty()) cands = keep;
+  }
   hipEvent_t e0, e1;
   hip_check(hipEventCreate(&e0), "event");
   hip_check(hipEventCreate(&e1), "event");
@@ -789,9 +826,9 @@ void tune_spmv(DevMat& m, Elem e) {
     float ms = 0;
     hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
     if (std::getenv("SC23AD_VERBOSE"))
-      std::fprintf(stderr, "[sc23ad spmv tune] n %d nnz %d %s%s lanes %d: %.2f us\n", m.d.n,
+      std::fprintf(stderr, "[sc23ad spmv tune] n %d nnz %d %s%s%s lanes %d: %.2f us\n", m.d.n,
                    m.d.nnz, cands[i].soff ? "SELL" : "CSR", cands[i].srow ? " sorted" : "",
-                   cands[i].group, ms * 1000.0f / 5);
+                   cands[i].scol16 ? " 16-bit" : "", cands[i].group, ms * 1000.0f / 5);
     if (ms < best) {
       best = ms;
       best_i = i;
@@ -806,6 +843,7 @@ void tune_spmv(DevMat& m, Elem e) {
     m.sval = DevBuf();
     m.srow = DevBuf();
   }
+  if (!m.d.scol16) m.scol16 = DevBuf();
 }
 
 }  // namespace
@@ -813,6 +851,7 @@ void tune_spmv(DevMat& m, Elem e) {
 void DevAmg::tune() {
   for (DevLevel& L : lv)
     for (DevMat* m : {&L.a, &L.u, &L.r, &L.p}) tune_spmv(*m, elem);
+  if (depth > 0) tune_spmv(a0, elem);  // the Arnoldi step's operator
 }
 
 std::unique_ptr<DevAmg::Work> DevAmg::make_work() const {

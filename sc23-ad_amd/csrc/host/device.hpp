@@ -76,6 +76,7 @@ struct DevMat {
   dev::DCsr d;
   DevBuf rptr, cind, val;
   DevBuf soff, scol, sval, srow;  // SELL-64 copy (optional; srow: row order)
+  DevBuf scol16;                   // 16-bit column offsets of the SELL-64 copy
   size_t algo_bytes_spmv(size_t elem) const;
 };
 

@@ -33,6 +33,9 @@ struct DCsr {
   const int* scol = nullptr;
   const void* sval = nullptr;
   const int* srow = nullptr;
+  // integer SELL-64 with 16-bit column offsets col - row (used instead of
+  // scol when set; square matrices, no srow)
+  const int16_t* scol16 = nullptr;
 };
 
 // Dispatch-recorded timing.  When armed, the next launch by a timing-aware

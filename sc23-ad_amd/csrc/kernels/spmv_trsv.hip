@@ -133,6 +133,43 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell_int(
   else y[orow] = add32(z[orow], s);
 }
 
+// SELL-64 integer SpMV with 16-bit column offsets: entry k of row r holds
+// col - r (square matrices whose entries all lie within 32767 of the
+// diagonal; padding is offset 0 with value 0, an exact zero product).  Six
+// bytes per entry instead of eight.
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_spmv_sell16_int(
+    int n, const long long* __restrict__ soff, const int16_t* __restrict__ scol,
+    const int32_t* __restrict__ sval, int bit, const int32_t* __restrict__ x, const int32_t* z,
+    int32_t* y, int remap) {
+  const int row = (remap ? xcd_block() : static_cast<int>(blockIdx.x)) * kBlock + threadIdx.x;
+  if (row >= n) return;
+  const long long b = soff[row >> 6];
+  const int w = static_cast<int>((soff[(row >> 6) + 1] - b) >> 6);
+  const int16_t* c = scol + b + (row & 63);
+  const int32_t* v = sval + b + (row & 63);
+  uint64_t acc = 0;
+  for (int k0 = 0; k0 < w; k0 += 8) {
+    int cc[8];
+    int32_t vv[8], xx[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int k = k0 + j < w ? k0 + j : k0;
+      cc[j] = row + c[64 * k];
+      vv[j] = v[64 * k];
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) xx[j] = x[cc[j]];
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (k0 + j < w) acc += prod64(vv[j], xx[j]);
+  }
+  const int32_t s = low32(as_i64(acc) >> bit);
+  if (MODE == 0) y[row] = s;
+  else if (MODE == 1) y[row] = sub32(z[row], s);
+  else y[row] = add32(z[row], s);
+}
+
 // FP SELL-64 SpMV: the row's products are added in stored (column) order,
 // as the CSR kernel adds them; padding is skipped, not added.
 template <typename T, int MODE>
@@ -1550,6 +1587,13 @@ hipError_t spmv_sell(const DCsr& a, const void* x, const void* z, void* y, hipSt
   const int grid = (a.n + kBlock - 1) / kBlock;
   if (grid == 0) return hipSuccess;
   if constexpr (std::is_same_v<T, int32_t>) {
+    if (a.scol16) {
+      launch_timed(&k_spmv_sell16_int<MODE>, dim3(grid), dim3(kBlock), 0, s, a.n, a.soff,
+                   a.scol16, static_cast<const int32_t*>(a.sval), a.bit,
+                   static_cast<const int32_t*>(x), static_cast<const int32_t*>(z),
+                   static_cast<int32_t*>(y), xcd_remap());
+      return hipGetLastError();
+    }
     launch_timed(&k_spmv_sell_int<MODE>, dim3(grid), dim3(kBlock), 0, s, a.n, a.soff, a.scol,
                  static_cast<const int32_t*>(a.sval), a.srow, a.bit,
                  static_cast<const int32_t*>(x),

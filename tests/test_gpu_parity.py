@@ -117,6 +117,29 @@ def test_default_frac_bits(built):
             assert amg.default_frac_bits(sy.b, sy.nrm_b) == c["solves"][f"2_{smoother}"]["bit"]
 
 
+@pytest.mark.parametrize("fmt", ["csr", "sell", "sell16"])
+def test_spmv_formats_match_oracle(fmt, monkeypatch):
+    """Every SpMV kernel the tuner can pick (CSR with 1-64 lanes, SELL-64,
+    length-sorted SELL-64, SELL-64 with 16-bit column offsets), forced in
+    turn, gives the oracle's A, U, R and P products bit for bit."""
+    monkeypatch.setenv("SC23AD_SPMV_FORMAT", fmt)
+    sy = S.make_system(S.GEN_INCLUSIONS3D, (24, 8, 2.0, 4.0, 1e4, 3), 5)  # 13,824 rows
+    for prec in (S.INT, S.FP64):
+        h = S.Hierarchy(sy.a, prec, S.amg_param(S.BLOCK_SOR))
+        path = export(h)
+        amg, oh = S.Amg(h), O.OracleHier(path)
+        os.unlink(path)
+        rng = np.random.default_rng(31)
+        for lv in range(amg.depth):
+            n, nxt = oh.level_n(lv), oh.level_n(lv + 1)
+            x, xc = _rand(prec, n, rng), _rand(prec, nxt, rng)
+            print(f"[spmv formats] {fmt} prec {prec} level {lv}", flush=True)
+            for op in (S.OP_SPMV_A, S.OP_SPMV_U):
+                assert _same(amg.level_op(lv, op, x, n), oh.level_op(lv, op, x, n)), (lv, op)
+            assert _same(amg.level_op(lv, S.OP_SPMV_R, x, nxt), oh.level_op(lv, S.OP_SPMV_R, x, nxt))
+            assert _same(amg.level_op(lv, S.OP_SPMV_P, xc, n), oh.level_op(lv, S.OP_SPMV_P, xc, n))
+
+
 @pytest.mark.parametrize("name", CASES)
 @pytest.mark.parametrize("smoother", [0, 1, 2], ids=["sor", "blocksor", "jacobisor"])
 def test_int_solve_matches_oracle(built, golden, name, smoother):
