@@ -110,7 +110,9 @@ struct EventTimer {
   int launches = 0;
   int stream_timed = 0;
   double ms_total = 0;
-  void begin();
+  // aware: the next launch binds the armed events to its dispatch (no
+  // marker on the stream before it); else the stream interval is timed.
+  void begin(bool aware);
   void end();
   void collect();
   void reset();
@@ -118,6 +120,7 @@ struct EventTimer {
 
  private:
   std::vector<hipEvent_t> ev;
+  bool armed_aware = false;
   int used = 0;
 };
 
@@ -172,6 +175,7 @@ class DevAmg {
 
   void spmv(const DevMat& m, dev::SpmvMode mode, const void* x, const void* z,
             void* y) const;
+  bool trsv_timing_aware(const TriPlan& plan) const;
   void trsv(const TriPlan& plan, const DevMat& m, const void* in, void* out,
             unsigned long long* flags, int* epoch, int* err) const;
   void smooth(int l, const void* in, void* out, bool inited, Work& w) const;

@@ -1414,10 +1414,11 @@ hipError_t tri_rp_go(const TriLayout& L, const TriSegment& seg, int bit, int dbi
   return hipGetLastError();
 }
 
-// Register layouts.  4 entries and 1 or 2 rows per lane: prefetch distance 8
+// Register layouts.  4 entries and 1 row per lane: prefetch distance 8
 // levels and far values 4 levels ahead (a level takes less than one
 // global-load latency, and a far value waits for two: at 128^3 the finest
-// level's solve took 116 us, against 157 us at distance 4).  One lane with 8
+// level's solve took 116 us, against 157 us at distance 4); 2 rows per lane:
+// distance 6, far values 3 ahead (the registers of 8 levels spill).  One lane with 8
 // or 12 entries, or 2, 4, 8 or 16 lanes of 8 or 12 entries per row (factors
 // without far dependencies): distance 4, far values 2 ahead (distance 8
 // measured no faster there).
@@ -1428,7 +1429,7 @@ hipError_t tri_rp_layout(const TriLayout& L, const TriSegment& seg, int bit, int
     if (L.rp_e == 4 && L.rp_rows == 1)
       return tri_rp_go<kWaves, kFar, 1, 4, 1, 8, 4>(L, seg, bit, dbit, in, out, s);
     if (L.rp_e == 4 && L.rp_rows == 2)
-      return tri_rp_go<kWaves, kFar, 1, 4, 2, 8, 4>(L, seg, bit, dbit, in, out, s);
+      return tri_rp_go<kWaves, kFar, 1, 4, 2, 6, 3>(L, seg, bit, dbit, in, out, s);
     if (L.rp_e == 8 && L.rp_rows == 1)
       return tri_rp_go<kWaves, kFar, 1, 8, 1, 4, 2>(L, seg, bit, dbit, in, out, s);
     if (L.rp_e == 12 && L.rp_rows == 1)
