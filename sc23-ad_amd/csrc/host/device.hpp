@@ -77,6 +77,11 @@ struct DevMat {
   DevBuf rptr, cind, val;
   DevBuf soff, scol, sval, srow;  // SELL-64 copy (optional; srow: row order)
   DevBuf scol16;                   // 16-bit column offsets of the SELL-64 copy
+  // second SELL-64 copy of a rectangular matrix, rows ordered by their first
+  // column (gathers of neighbouring rows share cache lines); d2 describes it
+  DevBuf soff2, scol2, sval2, srow2;
+  dev::DCsr d2;
+  bool has2 = false;
   size_t algo_bytes_spmv(size_t elem) const;
 };
 

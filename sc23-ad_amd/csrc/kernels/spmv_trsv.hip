@@ -107,6 +107,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell_int(
   const int row = (remap ? xcd_block() : static_cast<int>(blockIdx.x)) * kBlock + threadIdx.x;
   if (row >= n) return;
   const int orow = srow ? srow[row] : row;
+  const int32_t zv = MODE != 0 ? z[orow] : 0;  // issued before the entries
   const long long b = soff[row >> 6];
   const int w = static_cast<int>((soff[(row >> 6) + 1] - b) >> 6);
   const int* c = scol + b + (row & 63);
@@ -129,8 +130,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell_int(
   }
   const int32_t s = low32(as_i64(acc) >> bit);
   if (MODE == 0) y[orow] = s;
-  else if (MODE == 1) y[orow] = sub32(z[orow], s);
-  else y[orow] = add32(z[orow], s);
+  else if (MODE == 1) y[orow] = sub32(zv, s);
+  else y[orow] = add32(zv, s);
 }
 
 // SELL-64 integer SpMV with 16-bit column offsets: entry k of row r holds
@@ -144,6 +145,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell16_int(
     int32_t* y, int remap) {
   const int row = (remap ? xcd_block() : static_cast<int>(blockIdx.x)) * kBlock + threadIdx.x;
   if (row >= n) return;
+  const int32_t zv = MODE != 0 ? z[row] : 0;  // issued before the entries
   const long long b = soff[row >> 6];
   const int w = static_cast<int>((soff[(row >> 6) + 1] - b) >> 6);
   const int16_t* c = scol + b + (row & 63);
@@ -166,8 +168,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell16_int(
   }
   const int32_t s = low32(as_i64(acc) >> bit);
   if (MODE == 0) y[row] = s;
-  else if (MODE == 1) y[row] = sub32(z[row], s);
-  else y[row] = add32(z[row], s);
+  else if (MODE == 1) y[row] = sub32(zv, s);
+  else y[row] = add32(zv, s);
 }
 
 // FP SELL-64 SpMV: the row's products are added in stored (column) order,
@@ -180,6 +182,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell_fp(
   const int row = (remap ? xcd_block() : static_cast<int>(blockIdx.x)) * kBlock + threadIdx.x;
   if (row >= n) return;
   const int orow = srow ? srow[row] : row;
+  const T zv = MODE != 0 ? z[orow] : T(0);  // issued before the entries
   const long long b = soff[row >> 6];
   const int w = static_cast<int>((soff[(row >> 6) + 1] - b) >> 6);
   const int* c = scol + b + (row & 63);
@@ -201,8 +204,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv_sell_fp(
       if (k0 + j < w && cc[j] >= 0) t += vv[j] * xx[j];
   }
   if (MODE == 0) y[orow] = t;
-  else if (MODE == 1) y[orow] = z[orow] - t;
-  else y[orow] = z[orow] + t;
+  else if (MODE == 1) y[orow] = zv - t;
+  else y[orow] = zv + t;
 }
 
 template <typename T, int MODE>

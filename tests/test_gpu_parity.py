@@ -117,10 +117,11 @@ def test_default_frac_bits(built):
             assert amg.default_frac_bits(sy.b, sy.nrm_b) == c["solves"][f"2_{smoother}"]["bit"]
 
 
-@pytest.mark.parametrize("fmt", ["csr", "sell", "sell16"])
+@pytest.mark.parametrize("fmt", ["csr", "sell", "sell16", "sellcol"])
 def test_spmv_formats_match_oracle(fmt, monkeypatch):
     """Every SpMV kernel the tuner can pick (CSR with 1-64 lanes, SELL-64,
-    length-sorted SELL-64, SELL-64 with 16-bit column offsets), forced in
+    length-sorted SELL-64, SELL-64 with 16-bit column offsets, column-ordered
+    SELL-64 of the rectangular matrices), forced in
     turn, gives the oracle's A, U, R and P products bit for bit."""
     monkeypatch.setenv("SC23AD_SPMV_FORMAT", fmt)
     sy = S.make_system(S.GEN_INCLUSIONS3D, (24, 8, 2.0, 4.0, 1e4, 3), 5)  # 13,824 rows
