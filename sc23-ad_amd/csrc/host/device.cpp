@@ -635,6 +635,64 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
       }
     }
   }
+  // FP smoother factors (solvers 0 and 1): the row-parallel kernel with one
+  // lane per row keeps each row's stored order, so it applies to rows of at
+  // most 8 dependencies, with a ring of at most 128 KB and the same far rule.
+  // SC23AD_TRI_RP=0 keeps the level-pipelined kernel.
+  if constexpr (!std::is_same_v<V, int32_t>) {
+    int md = 0;
+    for (int idx = 0; idx < n; ++idx) md = std::max(md, lbeg[idx + 1] - lbeg[idx]);
+    const char* env_rp = std::getenv("SC23AD_TRI_RP");
+    int fring = 2;
+    while (fring * 2 * sizeof(V) <= 128 * 1024 && fring < span) fring *= 2;
+    size_t lds = sizeof(V) * static_cast<size_t>(fring) +
+                 sizeof(int) * (static_cast<size_t>(maxlev_chain) + 1);
+    if (kind == TriKind::kLD && plan.identity && one_launch_chains && md <= 8 &&
+        maxlev_rows <= 8 * 64 && lds <= dev::kTriBlockedMaxLds &&
+        !(env_rp && env_rp[0] == '0')) {
+      std::vector<int> lev_of(static_cast<size_t>(std::max(n, 1)), 0);
+      for (size_t Lv = 0; Lv + 1 < lev_ptr.size(); ++Lv)
+        for (int idx = lev_ptr[Lv]; idx < lev_ptr[Lv + 1]; ++idx)
+          lev_of[idx] = static_cast<int>(Lv);
+      const int e = md <= 4 ? 4 : 8;
+      std::vector<int> rdep(static_cast<size_t>(std::max(n, 1)) * e, 0),
+          rcnt(static_cast<size_t>(std::max(n, 1)), 0);
+      std::vector<V> rval(rdep.size(), V(0));
+      bool fits = true, any_far = false;
+      for (size_t Lv = 0; Lv + 1 < lev_ptr.size() && fits; ++Lv)
+        for (int idx = lev_ptr[Lv]; idx < lev_ptr[Lv + 1]; ++idx) {
+          rcnt[idx] = lbeg[idx + 1] - lbeg[idx];
+          for (int q = lbeg[idx]; q < lbeg[idx + 1]; ++q) {
+            const int dpos = lpos[q];
+            const bool in_ring = static_cast<long long>(lev_ptr[Lv + 1]) - dpos <= fring;
+            const bool far = !in_ring && lev_of[idx] - lev_of[dpos] >= dev::kEpFarLevels;
+            if (!in_ring && !far) fits = false;
+            any_far = any_far || far;
+            const size_t at = static_cast<size_t>(idx) * e + (q - lbeg[idx]);
+            rdep[at] = far ? -(lcol[q] + 1) : dpos;
+            rval[at] = lval[q];
+          }
+        }
+      if (fits) {
+        plan.rpf_dep.alloc(sizeof(int) * rdep.size());
+        plan.rpf_dep.upload(rdep.data(), sizeof(int) * rdep.size());
+        plan.rpf_val.alloc(sizeof(V) * rval.size());
+        plan.rpf_val.upload(rval.data(), sizeof(V) * rval.size());
+        plan.rpf_cnt.alloc(sizeof(int) * rcnt.size());
+        plan.rpf_cnt.upload(rcnt.data(), sizeof(int) * rcnt.size());
+        L.rpf_dep = plan.rpf_dep.as<int>();
+        L.rpf_val = plan.rpf_val.get();
+        L.rpf_cnt = plan.rpf_cnt.as<int>();
+        L.ring = fring;
+        L.ep_far = any_far ? 1 : 0;
+        L.ep_maxlev = maxlev_chain;
+        L.rp_e = e;
+        int rw = 1;
+        while (rw < 8 && static_cast<size_t>(rw) * 64 < maxlev_rows) rw *= 2;
+        L.rp_waves = rw;
+      }
+    }
+  }
   int maxdeps = 0;
   for (int idx = 0; idx < n; ++idx) maxdeps = std::max(maxdeps, lbeg[idx + 1] - lbeg[idx]);
   if (std::getenv("SC23AD_VERBOSE")) {
@@ -1056,6 +1114,11 @@ void DevAmg::trsv(const TriPlan& plan, const DevMat& m, const void* in, void* ou
     hip_check(dev::launch_trsv_rp(plan.layout, plan.segs[0], m.d.bit, m.d.dbit, in, out,
                                   lib_stream()),
               "trsv rp");
+    return;
+  }
+  if (plan.layout.rpf_dep && elem != Elem::kI32 && plan.segs.size() == 1) {
+    hip_check(dev::launch_trsv_rpf(elem, plan.layout, plan.segs[0], in, out, lib_stream()),
+              "trsv rp fp");
     return;
   }
   if (plan.layout.ep_waves > 0 && elem == Elem::kI32 && plan.segs.size() == 1) {

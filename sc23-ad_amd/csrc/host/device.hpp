@@ -103,6 +103,7 @@ struct TriPlan {
   DevBuf slice, boff, ddinv;
   // level-ordered layout used by the level-scheduled kernels
   DevBuf lrow, lbeg, lcol, lval, ldiag, ecol, eval, lpos, epos, lent, lev_eptr, rent;
+  DevBuf rpf_dep, rpf_val, rpf_cnt;  // FP row-parallel layout
   dev::TriLayout layout;
 };
 

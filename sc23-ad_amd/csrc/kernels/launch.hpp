@@ -112,6 +112,11 @@ struct TriLayout {
   int rp_e = 0;
   int rp_rows = 0;
   int rp_lanes = 1;
+  // FP row-parallel mode (one lane per row, rp_e entries in stored order):
+  // dependency codes and values by position, and each row's entry count
+  const int* rpf_dep = nullptr;
+  const void* rpf_val = nullptr;
+  const int* rpf_cnt = nullptr;
   int n = 0;
 };
 // Entry-parallel integer solve (one workgroup per chain).
@@ -129,6 +134,8 @@ constexpr int kRpMaxEntries = 12;  // one lane per row
 constexpr int kRpMaxLanes = 16;    // several lanes per row: 8 or 12 entries each
 hipError_t launch_trsv_rp(const TriLayout& L, const TriSegment& seg, int bit, int dbit,
                           const void* in, void* out, hipStream_t s);
+hipError_t launch_trsv_rpf(Elem e, const TriLayout& L, const TriSegment& seg, const void* in,
+                           void* out, hipStream_t s);
 inline size_t rp_lds_bytes(int ring, int maxlev) {
   return sizeof(int32_t) * static_cast<size_t>(ring) +
          sizeof(int) * (static_cast<size_t>(maxlev) + 1);

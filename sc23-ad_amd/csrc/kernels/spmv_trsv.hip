@@ -1465,6 +1465,172 @@ hipError_t tri_rp_far(const TriLayout& L, const TriSegment& seg, int bit, int db
   return tri_rp_layout<kWaves, false>(L, seg, bit, dbit, in, out, s);
 }
 
+// Row-parallel FP LD solve (solvers 0 and 1): the chains, levels, LDS ring and
+// far rule of k_tri_chain_rp with one lane per row.  A row's products are
+// subtracted in stored order, one at a time, as the sequential solve does
+// (entries past the row's count are skipped, not added as zeros), so the
+// result is the reference's bit for bit.  Entries are stored by position:
+// kE dependency codes (as in k_tri_chain_ep) and kE values, and the count.
+template <typename T, int kE>
+struct RpfPre {
+  int lo, hi;
+  int dep[kE];
+  T val[kE];
+  T xfar[kE];
+  T dinv, in;
+  int cnt;
+};
+
+template <int kThreads, typename T, int kE>
+__device__ __forceinline__ void rpf_load(RpfPre<T, kE>& r, int lo, int hi,
+                                         const int* __restrict__ rdep, const T* __restrict__ rval,
+                                         const int* __restrict__ rcnt,
+                                         const T* __restrict__ dinv,
+                                         const T* __restrict__ in) {
+  const int idx = lo + static_cast<int>(threadIdx.x);
+  const int id = idx < hi ? idx : 0;  // lanes past the level's end load position 0
+  r.lo = lo;
+  r.hi = hi;
+#pragma unroll
+  for (int q = 0; q < kE; ++q) {
+    r.dep[q] = rdep[static_cast<size_t>(id) * kE + q];
+    r.val[q] = rval[static_cast<size_t>(id) * kE + q];
+  }
+  r.cnt = rcnt[id];
+  r.dinv = dinv[id];
+  r.in = in[id];
+}
+
+template <int kThreads, typename T, int kE>
+__device__ __forceinline__ void rpf_far(RpfPre<T, kE>& r, const T* out) {
+  const bool ok = r.lo + static_cast<int>(threadIdx.x) < r.hi;
+#pragma unroll
+  for (int q = 0; q < kE; ++q) {
+    const int d = r.dep[q];
+    r.xfar[q] = out[(ok && d < 0) ? -d - 1 : 0];
+  }
+}
+
+template <typename T, TriKind K, int kWaves, bool kFar, int kE, int kD, int kA>
+__device__ __forceinline__ void rpf_level(RpfPre<T, kE>& cur, RpfPre<T, kE>& ahead, int L, int nl,
+                                          const int* lvo, T* ring, int rmask,
+                                          const int* __restrict__ rdep,
+                                          const T* __restrict__ rval,
+                                          const int* __restrict__ rcnt,
+                                          const T* __restrict__ dinv,
+                                          const T* __restrict__ in, T* out) {
+  constexpr int kThreads = 64 * kWaves;
+  const int Lp = min(L + kD, nl);
+  const int plo = lvo[Lp], phi = lvo[min(Lp + 1, nl)];
+  if (kFar) rpf_far<kThreads>(ahead, out);
+  T xv[kE];
+#pragma unroll
+  for (int q = 0; q < kE; ++q) xv[q] = ring[cur.dep[q] & rmask];
+  if (kFar) {
+#pragma unroll
+    for (int q = 0; q < kE; ++q)
+      if (__any(cur.dep[q] < 0)) xv[q] = cur.dep[q] < 0 ? cur.xfar[q] : xv[q];
+  }
+  T t = cur.in;
+#pragma unroll
+  for (int q = 0; q < kE; ++q) t = q < cur.cnt ? acc_sub<T>(t, cur.val[q], xv[q]) : t;
+  const int k = cur.lo + static_cast<int>(threadIdx.x);
+  if (k < cur.hi) {
+    const T v = acc_final<T, K>(t, cur.dinv, 0, 0);
+    ring[k & rmask] = v;
+    out[k] = v;
+  }
+  if (kFar && (L % kEpFence) == kEpFence - 1)
+    __syncthreads();
+  else
+    ep_sync<kWaves>();
+  rpf_load<kThreads>(cur, plo, phi, rdep, rval, rcnt, dinv, in);
+}
+
+template <typename T, TriKind K, int kWaves, bool kFar, int kE, int kD, int kA>
+__global__ __launch_bounds__(64 * kWaves) void k_tri_chain_rpf(
+    const int* __restrict__ rdep, const T* __restrict__ rval, const int* __restrict__ rcnt,
+    const T* __restrict__ dinv, const int* __restrict__ lev_ptr,
+    const int* __restrict__ chain_lev, int rmask, const T* __restrict__ in, T* out) {
+  constexpr int kThreads = 64 * kWaves;
+  static_assert(kA >= 1 && kA < kD, "far values need their entries loaded first");
+  static_assert(kEpFarLevels >= kA + kEpFence,
+                "far dependencies must be fenced before their values are loaded");
+  extern __shared__ __align__(16) unsigned char smem_f[];
+  T* ring = reinterpret_cast<T*>(smem_f);
+  int* lvo = reinterpret_cast<int*>(ring + rmask + 1);
+  const int t = threadIdx.x;
+  const int lb = chain_lev[2 * blockIdx.x], le = chain_lev[2 * blockIdx.x + 1];
+  const int nl = le - lb;
+  // The ring starts zeroed: a skipped entry's value is then never a
+  // leftover bit pattern.
+  for (int k = t; k <= rmask; k += kThreads) ring[k] = T(0);
+  for (int k = t; k <= nl; k += kThreads) lvo[k] = lev_ptr[lb + k];
+  __syncthreads();
+  RpfPre<T, kE> buf[kD];
+#pragma unroll
+  for (int d = 0; d < kD; ++d) {
+    const int Lc = min(d, nl);
+    rpf_load<kThreads>(buf[d], lvo[Lc], lvo[min(Lc + 1, nl)], rdep, rval, rcnt, dinv, in);
+  }
+  int L0 = 0;
+  for (; L0 + kD <= nl; L0 += kD) {
+#pragma unroll
+    for (int d = 0; d < kD; ++d)
+      rpf_level<T, K, kWaves, kFar, kE, kD, kA>(buf[d], buf[(d + kA) % kD], L0 + d, nl, lvo,
+                                                ring, rmask, rdep, rval, rcnt, dinv, in, out);
+  }
+#pragma unroll
+  for (int d = 0; d < kD; ++d)
+    if (L0 + d < nl)
+      rpf_level<T, K, kWaves, kFar, kE, kD, kA>(buf[d], buf[(d + kA) % kD], L0 + d, nl, lvo,
+                                                ring, rmask, rdep, rval, rcnt, dinv, in, out);
+}
+
+template <typename T, int kWaves, bool kFar, int kE>
+hipError_t tri_rpf_go(const TriLayout& L, const TriSegment& seg, const T* in, T* out,
+                      hipStream_t s) {
+  constexpr TriKind K = TriKind::kLD;
+  const size_t lds = sizeof(T) * static_cast<size_t>(L.ring) +
+                     sizeof(int) * (static_cast<size_t>(L.ep_maxlev) + 1);
+  if (lds > static_cast<size_t>(kTriBlockedMaxLds)) return hipErrorInvalidValue;
+  auto kern = &k_tri_chain_rpf<T, K, kWaves, kFar, kE, 4, 2>;
+  static bool attr = false;
+  if (!attr) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       static_cast<int>(kTriBlockedMaxLds));
+    if (e != hipSuccess) return e;
+    attr = true;
+  }
+  hipLaunchKernelGGL(kern, dim3(seg.nchain), dim3(64 * kWaves), static_cast<uint32_t>(lds), s,
+                     L.rpf_dep, static_cast<const T*>(L.rpf_val), L.rpf_cnt,
+                     static_cast<const T*>(L.ldiag), L.lev_ptr, seg.chain_lev, L.ring - 1, in,
+                     out);
+  return hipGetLastError();
+}
+
+template <typename T, int kWaves>
+hipError_t tri_rpf_layout(const TriLayout& L, const TriSegment& seg, const T* in, T* out,
+                          hipStream_t s) {
+  if (L.ep_far) {
+    if (L.rp_e == 4) return tri_rpf_go<T, kWaves, true, 4>(L, seg, in, out, s);
+    return tri_rpf_go<T, kWaves, true, 8>(L, seg, in, out, s);
+  }
+  if (L.rp_e == 4) return tri_rpf_go<T, kWaves, false, 4>(L, seg, in, out, s);
+  return tri_rpf_go<T, kWaves, false, 8>(L, seg, in, out, s);
+}
+
+template <typename T>
+hipError_t tri_rpf_waves(const TriLayout& L, const TriSegment& seg, const T* in, T* out,
+                         hipStream_t s) {
+  const int w = L.rp_waves;
+  if (w <= 1) return tri_rpf_layout<T, 1>(L, seg, in, out, s);
+  if (w <= 2) return tri_rpf_layout<T, 2>(L, seg, in, out, s);
+  if (w <= 4) return tri_rpf_layout<T, 4>(L, seg, in, out, s);
+  return tri_rpf_layout<T, 8>(L, seg, in, out, s);
+}
+
 // One level, many workgroups, level-ordered layout.
 template <typename T, TriKind K>
 __global__ __launch_bounds__(kBlock) void k_tri_level_pre(
@@ -1730,6 +1896,17 @@ hipError_t launch_trsv_rp(const TriLayout& L, const TriSegment& seg, int bit, in
   if (w <= 2) return tri_rp_far<2>(L, seg, bit, dbit, in, o, s);
   if (w <= 4) return tri_rp_far<4>(L, seg, bit, dbit, in, o, s);
   return tri_rp_far<8>(L, seg, bit, dbit, in, o, s);
+}
+
+hipError_t launch_trsv_rpf(Elem e, const TriLayout& L, const TriSegment& seg, const void* in,
+                           void* out, hipStream_t s) {
+  if (e == Elem::kF64)
+    return tri_rpf_waves<double>(L, seg, static_cast<const double*>(in),
+                                 static_cast<double*>(out), s);
+  if (e == Elem::kF32)
+    return tri_rpf_waves<float>(L, seg, static_cast<const float*>(in), static_cast<float*>(out),
+                                s);
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_trsv_segment(Elem e, TriKind kind, const DCsr& a,

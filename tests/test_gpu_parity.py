@@ -147,28 +147,32 @@ def test_spmv_formats_match_oracle(fmt, monkeypatch):
                          ids=["entry-parallel", "lane-per-row", "lanes-per-row", "two-rows",
                               "ep-copies"])
 def test_ld_solve_variants_match_oracle(knobs, monkeypatch):
-    """Every integer LD-solve kernel layout, forced in turn (entry-parallel
+    """Every LD-solve kernel layout, forced in turn (integer: entry-parallel
     with and without accumulator copies, row-parallel with one lane per row,
-    several lanes per row, two rows per lane), gives the oracle's LD solves,
+    several lanes per row, two rows per lane; FP64/FP32: row-parallel in
+    stored order, or level-pipelined), gives the oracle's LD solves,
     smoothing steps and V-cycles bit for bit."""
     for k, v in knobs.items():
         monkeypatch.setenv(k, v)
     for kind, params in ((S.GEN_LAPLACE3D, (24,)), (S.GEN_INCLUSIONS3D, (24, 8, 2.0, 4.0, 1e4, 3))):
         sy = S.make_system(kind, params, 7)
         for smoother in (S.SOR, S.BLOCK_SOR):
-            h = S.Hierarchy(sy.a, S.INT, S.amg_param(smoother))
-            path = export(h)
-            amg, oh = S.Amg(h), O.OracleHier(path)
-            os.unlink(path)
-            rng = np.random.default_rng(37)
-            print(f"[ld variants] {knobs} kind {kind} smoother {smoother}", flush=True)
-            for lv in range(amg.depth):
-                n = oh.level_n(lv)
-                x = _rand(S.INT, n, rng)
-                for op in (S.OP_TRSV_LD, S.OP_SMOOTH0):
-                    assert _same(amg.level_op(lv, op, x, n), oh.level_op(lv, op, x, n)), (lv, op)
-            v = _rand(S.INT, sy.a.n, rng)
-            assert _same(amg.precondition(v), oh.vcycle(v))
+            for prec in (S.INT, S.FP64, S.FP32):  # FP: row-parallel in stored order, or not
+                h = S.Hierarchy(sy.a, prec, S.amg_param(smoother))
+                path = export(h)
+                amg, oh = S.Amg(h), O.OracleHier(path)
+                os.unlink(path)
+                rng = np.random.default_rng(37)
+                print(f"[ld variants] {knobs} kind {kind} smoother {smoother} prec {prec}",
+                      flush=True)
+                for lv in range(amg.depth):
+                    n = oh.level_n(lv)
+                    x = _rand(prec, n, rng)
+                    for op in (S.OP_TRSV_LD, S.OP_SMOOTH0):
+                        assert _same(amg.level_op(lv, op, x, n), oh.level_op(lv, op, x, n)), \
+                            (lv, op)
+                v = _rand(prec, sy.a.n, rng)
+                assert _same(amg.precondition(v), oh.vcycle(v))
 
 
 @pytest.mark.parametrize("name", CASES)
