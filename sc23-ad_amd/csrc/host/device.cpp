@@ -592,7 +592,7 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
       int e = 0, lanes = 1, rows = 1;
       const bool force_groups = env_rp && env_rp[0] == '2';  // tuning: lanes per row even for short rows
       if (md <= dev::kRpMaxEntries && !(force_groups && !any_far && md > 4)) {
-        e = md <= 4 ? 4 : md <= 8 ? 8 : 12;
+        e = md <= 4 ? 4 : md <= 8 ? 8 : md <= 10 ? 10 : 12;
         if (const char* r = std::getenv("SC23AD_RP_ROWS"); r && e == 4 && r[0] == '2') rows = 2;
       } else if (!any_far && !(env_rp && env_rp[0] == '1')) {
         // the fewest waves, then the fewest entries per row

@@ -1421,8 +1421,8 @@ hipError_t tri_rp_go(const TriLayout& L, const TriSegment& seg, int bit, int dbi
 // levels and far values 4 levels ahead (a level takes less than one
 // global-load latency, and a far value waits for two: at 128^3 the finest
 // level's solve took 116 us, against 157 us at distance 4); 2 rows per lane:
-// distance 6, far values 3 ahead (the registers of 8 levels spill).  One lane with 8
-// or 12 entries, or 2, 4, 8 or 16 lanes of 8 or 12 entries per row (factors
+// distance 6, far values 3 ahead (the registers of 8 levels spill).  One lane with 8,
+// 10 or 12 entries, or 2, 4, 8 or 16 lanes of 8 or 12 entries per row (factors
 // without far dependencies): distance 4, far values 2 ahead (distance 8
 // measured no faster there).
 template <int kWaves, bool kFar>
@@ -1435,6 +1435,8 @@ hipError_t tri_rp_layout(const TriLayout& L, const TriSegment& seg, int bit, int
       return tri_rp_go<kWaves, kFar, 1, 4, 2, 6, 3>(L, seg, bit, dbit, in, out, s);
     if (L.rp_e == 8 && L.rp_rows == 1)
       return tri_rp_go<kWaves, kFar, 1, 8, 1, 4, 2>(L, seg, bit, dbit, in, out, s);
+    if (L.rp_e == 10 && L.rp_rows == 1)
+      return tri_rp_go<kWaves, kFar, 1, 10, 1, 4, 2>(L, seg, bit, dbit, in, out, s);
     if (L.rp_e == 12 && L.rp_rows == 1)
       return tri_rp_go<kWaves, kFar, 1, 12, 1, 4, 2>(L, seg, bit, dbit, in, out, s);
     return hipErrorInvalidValue;
