@@ -581,7 +581,7 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
                         ? 1 : 0;
       // Row-parallel mode for smoother factors with short rows: one lane per
       // row for at most kRpMaxEntries dependencies, else (no far entries)
-      // 2-16 lanes of 8 or 12 entries per row; no LDS accumulators.
+      // 2-16 lanes of 8, 10 or 12 entries per row; no LDS accumulators.
       // SC23AD_TRI_RP=0 keeps the entry-parallel kernel, SC23AD_TRI_RP=1
       // uses one lane per row only; SC23AD_RP_ROWS=2 puts two rows on each
       // lane of the 4-entry layout (default one: at 128^3 level 0 took 157 us
@@ -598,7 +598,7 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
         // the fewest waves, then the fewest entries per row
         size_t best = 0;
         for (int g = 2; g <= dev::kRpMaxLanes; g *= 2)
-          for (int ge : {8, 12}) {
+          for (int ge : {8, 10, 12}) {
             if (g * ge < md) continue;
             const size_t w = (maxlev_rows * g + 63) / 64;
             const size_t key = w * 4096 + static_cast<size_t>(g * ge);

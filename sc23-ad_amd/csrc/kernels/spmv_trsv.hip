@@ -1422,7 +1422,7 @@ hipError_t tri_rp_go(const TriLayout& L, const TriSegment& seg, int bit, int dbi
 // global-load latency, and a far value waits for two: at 128^3 the finest
 // level's solve took 116 us, against 157 us at distance 4); 2 rows per lane:
 // distance 6, far values 3 ahead (the registers of 8 levels spill).  One lane with 8,
-// 10 or 12 entries, or 2, 4, 8 or 16 lanes of 8 or 12 entries per row (factors
+// 10 or 12 entries, or 2, 4, 8 or 16 lanes of 8, 10 or 12 entries per row (factors
 // without far dependencies): distance 4, far values 2 ahead (distance 8
 // measured no faster there).
 template <int kWaves, bool kFar>
@@ -1442,6 +1442,13 @@ hipError_t tri_rp_layout(const TriLayout& L, const TriSegment& seg, int bit, int
     return hipErrorInvalidValue;
   }
   if constexpr (!kFar) {
+    if (L.rp_e == 10 && L.rp_rows == 1) {
+      if (L.rp_lanes == 2) return tri_rp_go<kWaves, false, 2, 10, 1, 4, 2>(L, seg, bit, dbit, in, out, s);
+      if (L.rp_lanes == 4) return tri_rp_go<kWaves, false, 4, 10, 1, 4, 2>(L, seg, bit, dbit, in, out, s);
+      if (L.rp_lanes == 8) return tri_rp_go<kWaves, false, 8, 10, 1, 4, 2>(L, seg, bit, dbit, in, out, s);
+      if (L.rp_lanes == 16)
+        return tri_rp_go<kWaves, false, 16, 10, 1, 4, 2>(L, seg, bit, dbit, in, out, s);
+    }
     if (L.rp_e == 12 && L.rp_rows == 1) {
       if (L.rp_lanes == 2) return tri_rp_go<kWaves, false, 2, 12, 1, 4, 2>(L, seg, bit, dbit, in, out, s);
       if (L.rp_lanes == 4) return tri_rp_go<kWaves, false, 4, 12, 1, 4, 2>(L, seg, bit, dbit, in, out, s);
