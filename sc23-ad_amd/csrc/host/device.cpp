@@ -613,8 +613,12 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
           dev::rp_lds_bytes(ep_ring, maxlev_chain) > dev::kTriBlockedMaxLds)
         e = 0;
       const size_t lanes_needed = (maxlev_rows + rows - 1) / rows * lanes;
+      // the fewest waves, 1 to 8 (SC23AD_RP_POW2=1: powers of two; at 128^3
+      // three waves instead of four took levels 1 and 3 from 94 and 109 us
+      // to 87 and 95 us per solve)
+      const bool pow2 = std::getenv("SC23AD_RP_POW2") != nullptr;
       int rw = 1;
-      while (rw < 8 && static_cast<size_t>(rw) * 64 < lanes_needed) rw *= 2;
+      while (rw < 8 && static_cast<size_t>(rw) * 64 < lanes_needed) rw = pow2 ? rw * 2 : rw + 1;
       if (e > 0 && static_cast<size_t>(rw) * 64 >= lanes_needed) {
         std::vector<int> rent(static_cast<size_t>(std::max(n, 1)) * lanes * e * 2, 0);
         for (int idx = 0; idx < n; ++idx)

@@ -1903,7 +1903,11 @@ hipError_t launch_trsv_rp(const TriLayout& L, const TriSegment& seg, int bit, in
   const int w = L.rp_waves;
   if (w <= 1) return tri_rp_far<1>(L, seg, bit, dbit, in, o, s);
   if (w <= 2) return tri_rp_far<2>(L, seg, bit, dbit, in, o, s);
+  if (w <= 3) return tri_rp_far<3>(L, seg, bit, dbit, in, o, s);
   if (w <= 4) return tri_rp_far<4>(L, seg, bit, dbit, in, o, s);
+  if (w <= 5) return tri_rp_far<5>(L, seg, bit, dbit, in, o, s);
+  if (w <= 6) return tri_rp_far<6>(L, seg, bit, dbit, in, o, s);
+  if (w <= 7) return tri_rp_far<7>(L, seg, bit, dbit, in, o, s);
   return tri_rp_far<8>(L, seg, bit, dbit, in, o, s);
 }
 
