@@ -621,14 +621,24 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
       while (rw < 8 && static_cast<size_t>(rw) * 64 < lanes_needed) rw = pow2 ? rw * 2 : rw + 1;
       if (e > 0 && static_cast<size_t>(rw) * 64 >= lanes_needed) {
         std::vector<int> rent(static_cast<size_t>(std::max(n, 1)) * lanes * e * 2, 0);
-        for (int idx = 0; idx < n; ++idx)
-          for (int q = lbeg[idx]; q < lbeg[idx + 1]; ++q) {
-            const int m = q - lbeg[idx];
+        int farq = 0;
+        for (int idx = 0; idx < n; ++idx) {
+          // far entries first (a row's integer sum is order-free), so the
+          // kernel loads far values for the first farq slots only
+          std::vector<int> ord;
+          for (int q = lbeg[idx]; q < lbeg[idx + 1]; ++q)
+            if (lent[4 * static_cast<size_t>(q)] < 0) ord.push_back(q);
+          farq = std::max(farq, static_cast<int>(ord.size()));
+          for (int q = lbeg[idx]; q < lbeg[idx + 1]; ++q)
+            if (lent[4 * static_cast<size_t>(q)] >= 0) ord.push_back(q);
+          for (int m = 0; m < static_cast<int>(ord.size()); ++m) {
+            const int q = ord[m];
             const size_t at =
                 ((static_cast<size_t>(idx) * lanes + m % lanes) * e + m / lanes) * 2;
             rent[at] = lent[4 * static_cast<size_t>(q)];
             rent[at + 1] = lent[4 * static_cast<size_t>(q) + 1];
           }
+        }
         plan.rent.alloc(sizeof(int) * rent.size());
         plan.rent.upload(rent.data(), sizeof(int) * rent.size());
         L.rent = plan.rent.get();
@@ -636,6 +646,7 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
         L.rp_rows = rows;
         L.rp_lanes = lanes;
         L.rp_waves = rw;
+        L.rp_farq = farq;
       }
     }
   }

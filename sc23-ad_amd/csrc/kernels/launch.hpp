@@ -112,6 +112,7 @@ struct TriLayout {
   int rp_e = 0;
   int rp_rows = 0;
   int rp_lanes = 1;
+  int rp_farq = 0;   // most far entries in one row (stored first; one-lane layouts)
   // FP row-parallel mode (one lane per row, rp_e entries in stored order):
   // dependency codes and values by position, and each row's entry count
   const int* rpf_dep = nullptr;

@@ -1256,7 +1256,9 @@ __device__ __forceinline__ void rp_load(RpPre<kE, kR>& r, int lo, int hi,
   }
 }
 
-template <int kThreads, int kG, int kE, int kR>
+// Far values of the first kFq entries of each row (the plan puts a row's far
+// entries first; kFq bounds their number).
+template <int kThreads, int kG, int kE, int kR, int kFq = kE>
 __device__ __forceinline__ void rp_far(RpPre<kE, kR>& r, const int32_t* out) {
   const int t = threadIdx.x;
 #pragma unroll
@@ -1265,8 +1267,8 @@ __device__ __forceinline__ void rp_far(RpPre<kE, kR>& r, const int32_t* out) {
 #pragma unroll
     for (int h = 0; h < kE / 2; ++h) {
       const int d0 = r.ent[q][h].x, d1 = r.ent[q][h].z;
-      r.xfar[q][2 * h] = out[(ok && d0 < 0) ? -d0 - 1 : 0];
-      r.xfar[q][2 * h + 1] = out[(ok && d1 < 0) ? -d1 - 1 : 0];
+      if (2 * h < kFq) r.xfar[q][2 * h] = out[(ok && d0 < 0) ? -d0 - 1 : 0];
+      if (2 * h + 1 < kFq) r.xfar[q][2 * h + 1] = out[(ok && d1 < 0) ? -d1 - 1 : 0];
     }
   }
 }
@@ -1295,7 +1297,7 @@ __device__ __forceinline__ uint64_t group_sum(uint64_t v) {
   return v;
 }
 
-template <TriKind K, int kWaves, bool kFar, int kG, int kE, int kR, int kD, int kA>
+template <TriKind K, int kWaves, bool kFar, int kG, int kE, int kR, int kD, int kA, int kFq>
 __device__ __forceinline__ void rp_level(RpPre<kE, kR>& cur, RpPre<kE, kR>& ahead, int L, int nl,
                                          const int* lvo, int32_t* ring, int rmask,
                                          const int4* __restrict__ rent,
@@ -1308,7 +1310,7 @@ __device__ __forceinline__ void rp_level(RpPre<kE, kR>& cur, RpPre<kE, kR>& ahea
   // bounds of the level prefetched at the end of this one
   const int Lp = min(L + kD, nl);
   const int plo = lvo[Lp], phi = lvo[min(Lp + 1, nl)];
-  if (kFar) rp_far<kThreads, kG>(ahead, out);
+  if (kFar) rp_far<kThreads, kG, kE, kR, kFq>(ahead, out);
   int32_t xv[kR][kE];
 #pragma unroll
   for (int q = 0; q < kR; ++q)
@@ -1323,8 +1325,10 @@ __device__ __forceinline__ void rp_level(RpPre<kE, kR>& cur, RpPre<kE, kR>& ahea
 #pragma unroll
       for (int h = 0; h < kE / 2; ++h) {
         const int d0 = cur.ent[q][h].x, d1 = cur.ent[q][h].z;
-        if (__any(d0 < 0)) xv[q][2 * h] = d0 < 0 ? cur.xfar[q][2 * h] : xv[q][2 * h];
-        if (__any(d1 < 0)) xv[q][2 * h + 1] = d1 < 0 ? cur.xfar[q][2 * h + 1] : xv[q][2 * h + 1];
+        if (2 * h < kFq && __any(d0 < 0))
+          xv[q][2 * h] = d0 < 0 ? cur.xfar[q][2 * h] : xv[q][2 * h];
+        if (2 * h + 1 < kFq && __any(d1 < 0))
+          xv[q][2 * h + 1] = d1 < 0 ? cur.xfar[q][2 * h + 1] : xv[q][2 * h + 1];
       }
   }
 #pragma unroll
@@ -1354,7 +1358,7 @@ __device__ __forceinline__ void rp_level(RpPre<kE, kR>& cur, RpPre<kE, kR>& ahea
 // LDS: the ring, then the chain's level offsets (max_levels + 1 ints).
 // Far values of level L + kA are loaded at the start of level L, from entries
 // loaded kD - kA levels before that.
-template <TriKind K, int kWaves, bool kFar, int kG, int kE, int kR, int kD, int kA>
+template <TriKind K, int kWaves, bool kFar, int kG, int kE, int kR, int kD, int kA, int kFq = kE>
 __global__ __launch_bounds__(64 * kWaves) void k_tri_chain_rp(
     const int4* __restrict__ rent, const int32_t* __restrict__ dinv,
     const int* __restrict__ lev_ptr, const int* __restrict__ chain_lev, int rmask, int bit,
@@ -1386,17 +1390,19 @@ __global__ __launch_bounds__(64 * kWaves) void k_tri_chain_rp(
   for (; L0 + kD <= nl; L0 += kD) {
 #pragma unroll
     for (int d = 0; d < kD; ++d)
-      rp_level<K, kWaves, kFar, kG, kE, kR, kD, kA>(buf[d], buf[(d + kA) % kD], L0 + d, nl, lvo,
-                                                    ring, rmask, rent, dinv, in, out, bit, dbit);
+      rp_level<K, kWaves, kFar, kG, kE, kR, kD, kA, kFq>(buf[d], buf[(d + kA) % kD], L0 + d, nl,
+                                                         lvo, ring, rmask, rent, dinv, in, out,
+                                                         bit, dbit);
   }
 #pragma unroll
   for (int d = 0; d < kD; ++d)
     if (L0 + d < nl)
-      rp_level<K, kWaves, kFar, kG, kE, kR, kD, kA>(buf[d], buf[(d + kA) % kD], L0 + d, nl, lvo,
-                                                    ring, rmask, rent, dinv, in, out, bit, dbit);
+      rp_level<K, kWaves, kFar, kG, kE, kR, kD, kA, kFq>(buf[d], buf[(d + kA) % kD], L0 + d, nl,
+                                                         lvo, ring, rmask, rent, dinv, in, out,
+                                                         bit, dbit);
 }
 
-template <int kWaves, bool kFar, int kG, int kE, int kR, int kD, int kA>
+template <int kWaves, bool kFar, int kG, int kE, int kR, int kD, int kA, int kFq = kE>
 hipError_t tri_rp_go(const TriLayout& L, const TriSegment& seg, int bit, int dbit,
                      const int32_t* in, int32_t* out, hipStream_t s) {
   constexpr TriKind K = TriKind::kLD;
@@ -1405,12 +1411,12 @@ hipError_t tri_rp_go(const TriLayout& L, const TriSegment& seg, int bit, int dbi
   static bool attr = false;
   if (!attr) {
     hipError_t e = hipFuncSetAttribute(
-        reinterpret_cast<const void*>(&k_tri_chain_rp<K, kWaves, kFar, kG, kE, kR, kD, kA>),
+        reinterpret_cast<const void*>(&k_tri_chain_rp<K, kWaves, kFar, kG, kE, kR, kD, kA, kFq>),
         hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kTriBlockedMaxLds));
     if (e != hipSuccess) return e;
     attr = true;
   }
-  launch_timed(&k_tri_chain_rp<K, kWaves, kFar, kG, kE, kR, kD, kA>, dim3(seg.nchain),
+  launch_timed(&k_tri_chain_rp<K, kWaves, kFar, kG, kE, kR, kD, kA, kFq>, dim3(seg.nchain),
                dim3(64 * kWaves), static_cast<uint32_t>(lds), s,
                reinterpret_cast<const int4*>(L.rent), static_cast<const int32_t*>(L.ldiag),
                L.lev_ptr, seg.chain_lev, L.ring - 1, bit, dbit, in, out);
@@ -1429,8 +1435,11 @@ template <int kWaves, bool kFar>
 hipError_t tri_rp_layout(const TriLayout& L, const TriSegment& seg, int bit, int dbit,
                          const int32_t* in, int32_t* out, hipStream_t s) {
   if (L.rp_lanes <= 1) {
-    if (L.rp_e == 4 && L.rp_rows == 1)
+    if (L.rp_e == 4 && L.rp_rows == 1) {
+      if (kFar && L.rp_farq == 1)  // one far entry at most, stored first
+        return tri_rp_go<kWaves, kFar, 1, 4, 1, 8, 4, 1>(L, seg, bit, dbit, in, out, s);
       return tri_rp_go<kWaves, kFar, 1, 4, 1, 8, 4>(L, seg, bit, dbit, in, out, s);
+    }
     if (L.rp_e == 4 && L.rp_rows == 2)
       return tri_rp_go<kWaves, kFar, 1, 4, 2, 6, 3>(L, seg, bit, dbit, in, out, s);
     if (L.rp_e == 8 && L.rp_rows == 1)
