@@ -365,7 +365,7 @@ int sc23ad_solver_history(sc23ad_solver* s, double* buf, int cap, int* len) {
 int sc23ad_solver_time_spmv(sc23ad_solver* s, int enable) {
   if (!s) return fail(SC23AD_BAD_PARAM, "time_spmv: null");
   return guarded([&]() -> int {
-    for (EventTimer* t : {&s->s->spmv_timer, &s->s->amg->trsv0_timer}) {
+    for (EventTimer* t : {&s->s->spmv_timer, &s->s->trsv0_timer}) {
       t->collect();
       t->reset();
       t->on = enable != 0;
@@ -390,7 +390,7 @@ int sc23ad_solver_spmv_stats(sc23ad_solver* s, int* launches, double* mean_ms) {
 
 int sc23ad_solver_trsv_stats(sc23ad_solver* s, int* launches, double* mean_ms) {
   if (!s) return fail(SC23AD_BAD_PARAM, "trsv_stats: null");
-  return timer_stats(s->s->amg->trsv0_timer, launches, mean_ms);
+  return timer_stats(s->s->trsv0_timer, launches, mean_ms);
 }
 
 int sc23ad_solver_relres(sc23ad_solver* s, const double* b, double nrm_b,

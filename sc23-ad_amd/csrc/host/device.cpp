@@ -1165,7 +1165,7 @@ void DevAmg::trsv(const TriPlan& plan, const DevMat& m, const void* in, void* ou
 void DevAmg::smooth(int l, const void* in, void* out, bool inited, Work& w) const {
   const DevLevel& L = lv[l];
   int k0 = 0;
-  EventTimer* tm = l == 0 ? &trsv0_timer : nullptr;
+  EventTimer* tm = l == 0 ? w.trsv_timer : nullptr;
   if (!inited) {
     if (tm) tm->begin(trsv_timing_aware(L.ld_plan));
     trsv(L.ld_plan, L.ld, in, out, nullptr, nullptr, nullptr);
@@ -1186,13 +1186,21 @@ void DevAmg::smooth(int l, const void* in, void* out, bool inited, Work& w) cons
   }
 }
 
-void DevAmg::check_errors(const Work& w) const {
+void DevAmg::check_errors(Work& w) const {
   if (!lc_plan.dense && !uc_plan.dense) return;
   int e = 0;
   hip_check(hipMemcpyAsync(&e, w.err.get(), sizeof(int), hipMemcpyDeviceToHost, lib_stream()),
             "error flag");
   hip_check(hipStreamSynchronize(lib_stream()), "sync");
-  if (e != 0) throw DevError{5, "coarse solve: a block waited too long for its dependencies"};
+  if (e != 0) {
+    // clear the word so that later calls on this work context start clean
+    const int zero = 0;
+    hip_check(hipMemcpyAsync(w.err.get(), &zero, sizeof(int), hipMemcpyHostToDevice,
+                             lib_stream()),
+              "error flag reset");
+    hip_check(hipStreamSynchronize(lib_stream()), "sync");
+    throw DevError{5, "coarse solve: a block waited too long for its dependencies"};
+  }
 }
 
 // direct::LUflex / LU<T>::Solve (Library/solver/lu.hpp:91-94, 33-36).
@@ -1296,6 +1304,7 @@ DevSolver::DevSolver(Elem e, const HostCsr& a, DevAmg* amg_, int m_, double eps_
   hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
   StreamScope scope(stream);
   work = amg->make_work();
+  work->trsv_timer = &trsv0_timer;
   const int n = a.n;
   upload_mat(a64, a.n, a.m, 0, -1, a.rptr, a.cind, a.val, 1);
   const size_t es = amg->esize();
@@ -1514,7 +1523,7 @@ DevSolver::Result DevSolver::solve(const double* b_dev, double nrm_b, double* x_
   hip_check(hipMemcpy(sc, k.scal, sizeof sc, hipMemcpyDeviceToHost), "scal");
   res.implicit_relres = sc[5];
   spmv_timer.collect();
-  amg->trsv0_timer.collect();
+  trsv0_timer.collect();
   amg->check_errors(*work);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

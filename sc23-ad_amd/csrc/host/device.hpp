@@ -157,7 +157,6 @@ class DevAmg {
   bool jacobi = false;   // Jacobi smoother: LD holds d_inv, U is A
   DevMat a0;             // fine-level operator in the original numbering (Arnoldi)
   LaunchCfg cfg;
-  mutable EventTimer trsv0_timer;  // level-0 LD solves (benchmark hook)
 
   static std::unique_ptr<DevAmg> upload(const IntHierarchy& h, const LaunchCfg& c);
   template <typename T>
@@ -175,6 +174,7 @@ class DevAmg {
     DevBuf r0, tmpc, vin0, vout0;         // fine residual, coarse temp, level-0 order in/out
     DevBuf lflags, uflags, err;           // tiled coarse solves: {epoch, value} words, errors
     int lepoch = 0, uepoch = 0;           // last flag values raised
+    EventTimer* trsv_timer = nullptr;     // times the level-0 LD solves (owner's hook)
   };
   std::unique_ptr<Work> make_work() const;
   Work& own_work() const { return *work_; }
@@ -192,8 +192,9 @@ class DevAmg {
   void level_op(int level, int op, const void* in, void* out, Work& w) const;
   // Algorithmic HBM bytes of one V-cycle (sum over its kernels).
   double vcycle_bytes() const;
-  // Throws if a kernel with a bounded wait gave up (after a stream sync).
-  void check_errors(const Work& w) const;
+  // Throws if a kernel with a bounded wait gave up (after a stream sync);
+  // the error word is cleared first, so only the call that timed out fails.
+  void check_errors(Work& w) const;
 
  private:
   std::unique_ptr<Work> work_;
@@ -227,6 +228,7 @@ class DevSolver {
   bool tracing = false;
   int trace_cap = 0;
   EventTimer spmv_timer;  // fine-level A SpMV of the Arnoldi steps (benchmark hook)
+  EventTimer trsv0_timer; // level-0 LD solves of this solver's V-cycles (benchmark hook)
   std::vector<double> history;          // |e_{j+1}| / nrm_b per Arnoldi step
 
   DevSolver(dev::Elem e, const HostCsr& a, DevAmg* amg, int m, double eps,

@@ -486,8 +486,8 @@ class Solver:
         return n.value, ms.value
 
     def solve(self, b: np.ndarray, nrm_b: float, x0: np.ndarray | None = None):
-        b = np.ascontiguousarray(b, dtype=np.float64)
-        x = np.zeros(self.n) if x0 is None else np.array(x0, dtype=np.float64)
+        b = _fine_vector(b, self.n, "b")
+        x = np.zeros(self.n) if x0 is None else _fine_vector(x0, self.n, "x0").copy()
         cv = Converge()
         check(lib.sc23ad_solver_solve(self.h, ptr(b), nrm_b, ptr(x), 0, C.byref(cv)))
         return x, cv
@@ -499,8 +499,8 @@ class Solver:
 
     def relres(self, b: np.ndarray, nrm_b: float, x: np.ndarray) -> float:
         out = C.c_double()
-        check(lib.sc23ad_solver_relres(self.h, ptr(np.ascontiguousarray(b)), nrm_b,
-                                       ptr(np.ascontiguousarray(x)), 0, C.byref(out)))
+        check(lib.sc23ad_solver_relres(self.h, ptr(_fine_vector(b, self.n, "b")), nrm_b,
+                                       ptr(_fine_vector(x, self.n, "x")), 0, C.byref(out)))
         return out.value
 
     def __del__(self):
@@ -572,16 +572,33 @@ def solve_batch(solvers, bs, nrm_bs, xs=None, threads: int = 0):
     each solver on its own stream, `threads` host threads (0: up to 16).
     Returns (xs, converge records)."""
     k = len(solvers)
-    bs = [np.ascontiguousarray(b, dtype=np.float64) for b in bs]
-    if xs is None:
-        xs = [np.zeros(s.n) for s in solvers]
+    if len(bs) != k or len(nrm_bs) != k or (xs is not None and len(xs) != k):
+        raise ValueError("solve_batch: one b, nrm_b (and x) per solver")
+    bs = [_fine_vector(b, s.n, "b") for s, b in zip(solvers, bs)]
+    # the library reads and writes solver.n doubles of every x: work on
+    # float64 contiguous copies and hand the results back in the caller's arrays
+    xw = [np.zeros(s.n) if xs is None else _fine_vector(x, s.n, "x").copy()
+          for s, x in zip(solvers, xs if xs is not None else [None] * k)]
     hs = (C.c_void_p * k)(*[s.h.value for s in solvers])
     bp = (C.c_void_p * k)(*[b.ctypes.data for b in bs])
-    xp = (C.c_void_p * k)(*[x.ctypes.data for x in xs])
+    xp = (C.c_void_p * k)(*[x.ctypes.data for x in xw])
     nb = (C.c_double * k)(*nrm_bs)
     out = (Converge * k)()
     check(lib.sc23ad_solve_batch(k, hs, bp, nb, xp, 0, threads, out))
+    if xs is None:
+        return xw, list(out)
+    for dst, src in zip(xs, xw):
+        dst[...] = src
     return xs, list(out)
+
+
+def _fine_vector(v, n: int, what: str) -> np.ndarray:
+    """v as a contiguous float64 array of length n (the library reads exactly
+    n doubles), or ValueError."""
+    a = np.ascontiguousarray(v, dtype=np.float64)
+    if a.ndim != 1 or a.shape[0] != n:
+        raise ValueError(f"{what}: expected {n} values, got shape {a.shape}")
+    return a
 
 
 def spmv_int(rptr, cind, val, bit, x) -> np.ndarray:

@@ -154,3 +154,93 @@ def test_live_reference(case, tmp_path):
         assert np.array_equal(trace, R["trace"])
         assert res.iters == R["iters"]
         assert np.array_equal(x, R["x"])
+
+
+# ---- relaxation weights other than 1 (tests/golden/reference_omega.json) ------
+#
+# With omega != 1 the SOR / BlockSOR split stores a_ii / omega in LD and a
+# non-zero (1 - 1/omega) a_ii in U (smoother/stationary.hpp:375-396,
+# 941-978).  Some weights make the reference's integer solver overflow: it
+# wraps in blas1::Add and Nrm2, then lsqrt(<= 0) = 0 and the division at
+# gmres.hpp:244 kills the process ("crashed" in the fixture).  The oracle
+# must stop with OVERFLOW exactly there.
+
+@pytest.fixture(scope="module")
+def golden_omega():
+    import json
+    with open(os.path.join(os.path.dirname(__file__), "golden", "reference_omega.json")) as f:
+        return json.load(f)
+
+
+def _omega_cases(golden_omega):
+    for c in golden_omega["cases"]:
+        yield c, system(c)
+
+
+def test_omega_setup_byte_identical_to_reference(golden_omega):
+    for c, sy in _omega_cases(golden_omega):
+        for om in golden_omega["omegas"]:
+            for sm in (0, 1):
+                hs = S.Hierarchy.build_multi(sy.a, [S.FP64, S.INT], S.amg_param(sm, omega=om))
+                for prec, h in zip((0, 2), hs):
+                    path = export(h)
+                    try:
+                        assert file_sha(path) == c["hier"][f"{om}_{prec}_{sm}"], (c["name"], om)
+                    finally:
+                        os.unlink(path)
+
+
+def test_omega_oracle_solves_match_reference(golden_omega):
+    for c, sy in _omega_cases(golden_omega):
+        for om in golden_omega["omegas"]:
+            for sm in (0, 1):
+                hs = S.Hierarchy.build_multi(sy.a, [S.FP64, S.INT], S.amg_param(sm, omega=om))
+                ohs = []
+                for h in hs:
+                    path = export(h)
+                    ohs.append(O.OracleHier(path))
+                    os.unlink(path)
+                # F by the driver's rule on the FP64 hierarchy (main.cpp:153-160)
+                rule_bit = ohs[0].default_bits(sy.b, sy.nrm_b)
+                for solver, oh in zip((0, 2), ohs):
+                    g = c["solves"][f"{om}_{solver}_{sm}"]
+                    if "bit" in g:
+                        assert g["bit"] == rule_bit, (c["name"], om, sm)
+                    bit = rule_bit
+                    x, res, trace = oh.solve(sy.a.arrays(), sy.b, sy.nrm_b, m=c["m"],
+                                             eps=golden_omega["eps"], bit=bit, mode=O.RED_SEQ,
+                                             trace_cap=8192)
+                    label = (c["name"], om, solver, sm)
+                    if "crashed" in g:
+                        assert solver == 2 and res.status == S.OVERFLOW, label
+                        continue
+                    assert (res.iters, res.converged) == (g["iters"], g["converged"]), label
+                    assert sha(x) == g["x_sha"], label
+                    if solver == 2:
+                        assert [format(int(v), "016x") for v in trace] == g["trace"], label
+
+
+def test_omega_fixture_covers_both_outcomes(golden_omega):
+    """The fixture holds converged, diverged and overflowing weights, so each
+    branch above is exercised."""
+    outs = [s for c in golden_omega["cases"] for s in c["solves"].values()]
+    assert any("crashed" in s for s in outs)
+    assert any(s.get("converged") == 1 for s in outs)
+    assert any(s.get("converged") == 0 for s in outs)
+
+
+def test_full_size_hierarchy_matches_reference():
+    """The host setup of the 128^3 Laplacian (BlockSOR, integer) writes the
+    reference's own hierarchy file (tests/golden/reference_laplace3d_128.json)."""
+    import json
+    with open(os.path.join(os.path.dirname(__file__), "golden",
+                           "reference_laplace3d_128.json")) as f:
+        g = json.load(f)
+    sy = S.make_system(g["kind"], tuple(g["params"]), g["seed"])
+    assert sha(*sy.a.arrays()) == g["a_sha"] and sha(sy.b) == g["b_sha"]
+    h = S.Hierarchy(sy.a, S.INT, S.amg_param(g["smoother"]))
+    path = export(h)
+    try:
+        assert file_sha(path) == g["hier_int"]
+    finally:
+        os.unlink(path)

@@ -5,10 +5,16 @@ with each single-chain smoother (SOR, JacobiSOR).
 Checked bit for bit against the oracle: the level-0 integer SpMV, one integer
 and one FP64 V-cycle, and the whole integer solve (every V-cycle input and
 output checksum, and x).  Size-independent properties: the explicit relative
-residual reaches 1e-8 and the iteration count equals the oracle's.
+residual reaches 1e-8 and the iteration count equals the oracle's.  Against
+the reference itself (tests/golden/reference_laplace3d_128.json, written by
+the reference compiled in the container): the integer hierarchy file's hash,
+and in the sequential FP reduction order every V-cycle checksum of the
+37-step solve, the iteration count and x.
 """
 from __future__ import annotations
 
+import hashlib
+import json
 import os
 import tempfile
 
@@ -34,8 +40,40 @@ def big():
         os.close(fd)
         h.export(path)
         out[key] = O.OracleHier(path)
+        if key == "or_i":
+            with open(path, "rb") as f:
+                out["hier_sha"] = hashlib.sha256(f.read()).hexdigest()
         os.unlink(path)
     return out
+
+
+def _sha(*arrays):
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def test_reference_trace_at_full_size(big):
+    """The GPU solve in the reference's sequential FP order reproduces the
+    reference's own 128^3 run bit for bit."""
+    with open(os.path.join(os.path.dirname(__file__), "golden",
+                           "reference_laplace3d_128.json")) as f:
+        g = json.load(f)
+    sy = big["sy"]
+    assert _sha(*sy.a.arrays()) == g["a_sha"] and _sha(sy.b) == g["b_sha"]
+    assert big["hier_sha"] == g["hier_int"]
+    bit = big["amg_d"].default_frac_bits(sy.b, sy.nrm_b)
+    assert bit == g["solve"]["bit"]
+    sol = S.Solver(S.INT, sy.a, big["amg_i"], m=g["m"], eps=g["eps"], bit=bit)
+    sol.set_sequential_reduction(True)
+    sol.trace(4096)
+    print(f"[solve] int bit {bit}, sequential reductions", flush=True)
+    x, cv = sol.solve(sy.b, sy.nrm_b)
+    trace = sol.read_trace(4096)
+    assert [format(int(v), "016x") for v in trace] == g["solve"]["trace"]
+    assert (cv.iter, cv.is_converged) == (g["solve"]["iters"], g["solve"]["converged"])
+    assert _sha(x) == g["solve"]["x_sha"]
 
 
 def test_spmv_and_vcycles(big):

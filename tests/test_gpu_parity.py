@@ -317,7 +317,7 @@ def test_batch_equals_individual_solves():
     right-hand sides sharing one hierarchy) give the same x as one at a time."""
     draws = [(S.GEN_CONVDIFF2D, (40, 20.0, 0.3)), (S.GEN_POISSON2D, (50,)),
              (S.GEN_LAPLACE3D, (12,))]
-    solvers, rhs, nrms, keep = [], [], [], []
+    solvers, rhs, nrms, keep, hiers = [], [], [], [], []
     for kind, params in draws:
         a = S.CsrMat.generate(kind, *params)
         rptr, cind, val = a.arrays()
@@ -327,6 +327,7 @@ def test_batch_equals_individual_solves():
         hi = S.Hierarchy(a, S.INT, S.amg_param(S.BLOCK_SOR))
         amg = S.Amg(hi)
         keep.append((a, amg))
+        hiers.append(hi)
         for b in bs:
             b = b / rmax
             solvers.append(S.Solver(S.INT, a, amg, m=20, eps=1e-10, bit=28))
@@ -337,3 +338,14 @@ def test_batch_equals_individual_solves():
     for (x1, c1), x2, c2 in zip(seq, xs, cvs):
         assert c1.iter == c2.iter and c1.is_converged == c2.is_converged
         assert _same(x1, x2)
+    # every batched answer is also the oracle's, in the device reduction order
+    for q, (a, amg) in enumerate(keep):
+        path = export(hiers[q])
+        oh = O.OracleHier(path)
+        os.unlink(path)
+        for k in range(3):
+            i = 3 * q + k
+            xo, ro, _ = oh.solve(a.arrays(), rhs[i], nrms[i], m=20, eps=1e-10, bit=28,
+                                 mode=O.RED_GPU)
+            assert cvs[i].iter == ro.iters and cvs[i].is_converged == ro.converged, (q, k)
+            assert _same(xs[i], xo), (q, k)

@@ -28,9 +28,12 @@ def _sha(h):
         os.unlink(path)
 
 
-def _compare(a, smoother, label):
+def _compare(a, smoother, label, golden_hashes=None):
     """Host setup against the device setup with the Galerkin products forced
-    onto the GPU at every level (by default only large levels use it)."""
+    onto the GPU at every level (by default only large levels use it); where
+    the reference's own hashes are known (golden_hashes: precision ->
+    SHA-256 of the reference's hierarchy file) the device setup must match
+    them too."""
     prm = S.amg_param(smoother)
     t = time.perf_counter()
     host = S.Hierarchy.build_multi(a, [S.INT, S.FP64, S.FP32], prm)
@@ -43,16 +46,22 @@ def _compare(a, smoother, label):
     finally:
         del os.environ["SC23AD_SETUP_GPU_PRODUCTS"]
     print(f"[setup] {label} smoother {smoother}: host {th:.2f} s, device {td:.2f} s", flush=True)
-    for h, d in zip(host, dev):
+    for prec, h, d in zip((S.INT, S.FP64, S.FP32), host, dev):
         assert h.info()[0] == d.info()[0], label
-        assert _sha(h) == _sha(d), label
+        dsha = _sha(d)
+        assert _sha(h) == dsha, label
+        if golden_hashes is not None:
+            assert dsha == golden_hashes[prec], (label, prec)
 
 
 def test_golden_systems(golden):
+    """Device setup against the reference's hierarchy files (written by the
+    reference compiled in the container: tests/golden/reference.json)."""
     for c in golden["cases"]:
         sy = system(c)
         for smoother in (0, 1):
-            _compare(sy.a, smoother, c["name"])
+            _compare(sy.a, smoother, c["name"],
+                     {prec: c["hier"][f"{prec}_{smoother}"] for prec in (0, 1, 2)})
 
 
 @pytest.mark.parametrize("kind,params,seed", [
