@@ -222,6 +222,50 @@ int sc23ad_solver_solve(sc23ad_solver* s, const double* b, double nrm_b,
 int sc23ad_solve_batch(int count, sc23ad_solver* const* solvers, const double* const* b,
                        const double* nrm_b, double* const* x, int on_device, int threads,
                        sc23ad_converge* out);
+/* ---- block solves: several right-hand sides of one hierarchy ------------
+   No counterpart in the reference, whose drivers solve one right-hand side
+   per Restarted::Solve (Work/main.cpp:167-169).  Up to SC23AD_BLOCK_MAX_RHS
+   integer solves on one hierarchy run in lockstep: every V-cycle
+   (INTAMG.hpp:269-293), fine-level SpMV and Gram-Schmidt pass of the step
+   is one launch for all of them, so the matrices and level schedules are
+   read once per step and the sequential levels of the LD solves
+   (ldtri.hpp:511-525) are walked once for four right-hand sides per
+   workgroup.  Each right-hand side's results (x, iterations, statuses) are
+   those of its own sc23ad_solver_solve, bit for bit. */
+#define SC23AD_BLOCK_MAX_RHS 64
+typedef struct sc23ad_block_solver_s sc23ad_block_solver;
+/* One V-cycle on nrhs vectors in the block layout: element (row i, vector r)
+   at ((r / 4) * n + i) * 4 + r % 4 (int32; ceil(nrhs / 4) * 4 * n words).
+   Integer hierarchies only. */
+int sc23ad_amg_precondition_block(sc23ad_amg* m, int nrhs, const int32_t* in, int32_t* out,
+                                  int on_device);
+/* 1 when every LD solve of the hierarchy has a block kernel (otherwise those
+   levels take the right-hand sides one at a time; results are the same). */
+int sc23ad_amg_block_native(const sc23ad_amg* m, int* native);
+/* frac_bits: F of each right-hand side (nrhs entries). */
+int sc23ad_block_solver_create(const sc23ad_csrmat* a, sc23ad_amg* m, int nrhs, int restart_m,
+                               double eps, const int* frac_bits, int max_restarts,
+                               sc23ad_block_solver** out);
+void sc23ad_block_solver_destroy(sc23ad_block_solver* s);
+/* b, x: nrhs FP64 columns of n (column r at r * n); x is the initial guess and
+   the result; nrm_b: nrhs norms; out (optional): nrhs records. */
+int sc23ad_block_solver_solve(sc23ad_block_solver* s, const double* b, const double* nrm_b,
+                              double* x, int on_device, sc23ad_converge* out);
+/* Several block solvers (e.g. one per system of a batch of uneven systems),
+   each on its own stream, taken by `threads` host threads (0: up to 16):
+   b[k], nrm_b[k], x[k], out[k] as for sc23ad_block_solver_solve. */
+int sc23ad_block_solve_batch(int count, sc23ad_block_solver* const* solvers,
+                             const double* const* b, const double* const* nrm_b,
+                             double* const* x, int on_device, int threads,
+                             sc23ad_converge* const* out);
+/* buf == NULL: record V-cycle checksums (as sc23ad_solver_trace) of every
+   right-hand side during the next solves, cap per right-hand side; else
+   read right-hand side rhs's trace. */
+int sc23ad_block_solver_trace(sc23ad_block_solver* s, int rhs, uint64_t* buf, int cap,
+                              int* len);
+int sc23ad_block_solver_history(sc23ad_block_solver* s, int rhs, double* buf, int cap,
+                                int* len);
+
 /* FP reduction order of the solver's norms and inner products:
    0 = the fixed blocked order (default, 1024 x 256 partial sums, halving
    trees); 1 = strictly left to right in one thread, the order of the

@@ -33,6 +33,11 @@ struct sc23ad_solver_s {
   DevBuf hb, hx;  // staging for host-pointer solves
 };
 
+struct sc23ad_block_solver_s {
+  std::unique_ptr<DevBlockSolver> s;
+  DevBuf hb, hx;  // staging for host-pointer solves
+};
+
 namespace {
 
 template <typename F>
@@ -329,6 +334,166 @@ int sc23ad_solve_batch(int count, sc23ad_solver* const* solvers, const double* c
   for (auto& th : pool) th.join();
   for (int k = 0; k < count; ++k)
     if (rc[k] != SC23AD_OK) return fail(rc[k], "solve_batch: system " + std::to_string(k) + ": " + msg[k]);
+  return SC23AD_OK;
+}
+
+// ---- block solves: several right-hand sides of one hierarchy --------------
+
+int sc23ad_amg_precondition_block(sc23ad_amg* m, int nrhs, const int32_t* in, int32_t* out,
+                                  int on_device) {
+  if (!m || !in || !out || nrhs < 1) return fail(SC23AD_BAD_PARAM, "precondition_block: bad arguments");
+  if (m->precision != SC23AD_INT)
+    return fail(SC23AD_BAD_PARAM, "precondition_block: integer hierarchies only");
+  return guarded([&]() -> int {
+    DevAmg& a = *m->d;
+    const int groups = (nrhs + 3) / 4;
+    const size_t bytes = 16 * static_cast<size_t>(groups) * a.n0();
+    auto w = a.make_block_work(groups);
+    if (on_device) {
+      a.vcycle_blk(in, out, *w);
+    } else {
+      DevBuf di(bytes), dout(bytes);
+      di.upload(in, bytes);
+      a.vcycle_blk(di.get(), dout.get(), *w);
+      dout.download(out, bytes);
+    }
+    hip_check(hipStreamSynchronize(lib_stream()), "sync");
+    a.check_errors(*w);
+    return SC23AD_OK;
+  });
+}
+
+int sc23ad_amg_block_native(const sc23ad_amg* m, int* native) {
+  if (!m || !native) return fail(SC23AD_BAD_PARAM, "amg_block_native: null");
+  *native = (m->precision == SC23AD_INT && m->d->block_native()) ? 1 : 0;
+  return SC23AD_OK;
+}
+
+int sc23ad_block_solver_create(const sc23ad_csrmat* a, sc23ad_amg* m, int nrhs, int restart_m,
+                               double eps, const int* frac_bits, int max_restarts,
+                               sc23ad_block_solver** out) {
+  if (!a || !m || !out || !frac_bits || nrhs < 1 || nrhs > SC23AD_BLOCK_MAX_RHS ||
+      restart_m < 1 || !(eps > 0))
+    return fail(SC23AD_BAD_PARAM, "block_solver_create: bad arguments");
+  if (m->precision != SC23AD_INT)
+    return fail(SC23AD_BAD_PARAM, "block_solver_create: integer hierarchies only");
+  for (int q = 0; q < nrhs; ++q)
+    if (frac_bits[q] < 1 || frac_bits[q] > 30)
+      return fail(SC23AD_BAD_PARAM, "block_solver_create: frac_bits must be in [1, 30]");
+  if (a->a.n != m->d->n0()) return fail(SC23AD_BAD_PARAM, "block_solver_create: size mismatch");
+  return guarded([&]() -> int {
+    auto s = std::make_unique<sc23ad_block_solver_s>();
+    const int outer = max_restarts > 0 ? max_restarts : 1000 / restart_m;
+    s->s = std::make_unique<DevBlockSolver>(a->a, m->d.get(), nrhs, restart_m, eps, frac_bits,
+                                            outer);
+    *out = s.release();
+    return SC23AD_OK;
+  });
+}
+
+void sc23ad_block_solver_destroy(sc23ad_block_solver* s) { delete s; }
+
+namespace {
+
+int block_solve_one(sc23ad_block_solver* s, const double* b, const double* nrm_b, double* x,
+                    int on_device, sc23ad_converge* out) {
+  if (!s || !b || !nrm_b || !x) return fail(SC23AD_BAD_PARAM, "block solve: null");
+  return guarded([&]() -> int {
+    DevBlockSolver& d = *s->s;
+    StreamScope scope(d.stream);
+    const size_t bytes = sizeof(double) * d.amg->n0() * static_cast<size_t>(d.count);
+    std::vector<DevSolver::Result> r(d.count);
+    if (on_device) {
+      d.solve(b, nrm_b, x, r.data());
+    } else {
+      if (s->hb.bytes() < bytes) s->hb.alloc(bytes);
+      if (s->hx.bytes() < bytes) s->hx.alloc(bytes);
+      s->hb.upload(b, bytes);
+      s->hx.upload(x, bytes);
+      d.solve(s->hb.as<double>(), nrm_b, s->hx.as<double>(), r.data());
+      s->hx.download(x, bytes);
+    }
+    if (out)
+      for (int q = 0; q < d.count; ++q) {
+        memset(out + q, 0, sizeof *out);
+        out[q].is_converged = r[q].converged;
+        out[q].iter = r[q].iter;
+        out[q].restarts = r[q].restarts;
+        out[q].status = r[q].status;
+        out[q].overflow_flag = r[q].overflow;
+        out[q].frac_bits = d.bits[q];
+        out[q].implicit_relres = r[q].implicit_relres;
+        out[q].solve_ms = r[q].solve_ms;
+      }
+    return SC23AD_OK;
+  });
+}
+
+}  // namespace
+
+int sc23ad_block_solver_solve(sc23ad_block_solver* s, const double* b, const double* nrm_b,
+                              double* x, int on_device, sc23ad_converge* out) {
+  return block_solve_one(s, b, nrm_b, x, on_device, out);
+}
+
+int sc23ad_block_solve_batch(int count, sc23ad_block_solver* const* solvers,
+                             const double* const* b, const double* const* nrm_b,
+                             double* const* x, int on_device, int threads,
+                             sc23ad_converge* const* out) {
+  if (count < 0 || (count > 0 && (!solvers || !b || !nrm_b || !x)))
+    return fail(SC23AD_BAD_PARAM, "block_solve_batch: bad arguments");
+  if (count == 0) return SC23AD_OK;
+  for (int k = 0; k < count; ++k)
+    for (int q = 0; q < k; ++q)
+      if (solvers[k] == solvers[q])
+        return fail(SC23AD_BAD_PARAM, "block_solve_batch: a solver appears twice");
+  int nt = threads > 0 ? threads : 16;
+  nt = std::max(1, std::min(nt, count));
+  std::atomic<int> next{0};
+  std::vector<int> rc(count, SC23AD_OK);
+  std::vector<std::string> msg(count);
+  auto worker = [&]() {
+    for (int k = next.fetch_add(1); k < count; k = next.fetch_add(1)) {
+      rc[k] = block_solve_one(solvers[k], b[k], nrm_b[k], x[k], on_device,
+                              out ? out[k] : nullptr);
+      if (rc[k] != SC23AD_OK) msg[k] = g_last_error;
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; ++t) pool.emplace_back(worker);
+  worker();
+  for (auto& th : pool) th.join();
+  for (int k = 0; k < count; ++k)
+    if (rc[k] != SC23AD_OK)
+      return fail(rc[k], "block_solve_batch: system " + std::to_string(k) + ": " + msg[k]);
+  return SC23AD_OK;
+}
+
+int sc23ad_block_solver_trace(sc23ad_block_solver* s, int rhs, uint64_t* buf, int cap,
+                              int* len) {
+  if (!s) return fail(SC23AD_BAD_PARAM, "block trace: null");
+  return guarded([&]() -> int {
+    if (buf == nullptr) {
+      s->s->set_trace(cap);
+      return SC23AD_OK;
+    }
+    if (rhs < 0 || rhs >= s->s->count) return fail(SC23AD_BAD_PARAM, "block trace: bad rhs");
+    int l = 0;
+    s->s->read_trace(rhs, buf, cap, &l);
+    if (len) *len = l;
+    return SC23AD_OK;
+  });
+}
+
+int sc23ad_block_solver_history(sc23ad_block_solver* s, int rhs, double* buf, int cap,
+                                int* len) {
+  if (!s || !len || rhs < 0 || rhs >= s->s->count)
+    return fail(SC23AD_BAD_PARAM, "block history: bad arguments");
+  const auto& h = s->s->history[rhs];
+  const int n = static_cast<int>(h.size());
+  *len = n;
+  if (buf)
+    for (int i = 0; i < n && i < cap; ++i) buf[i] = h[i];
   return SC23AD_OK;
 }
 

@@ -1297,6 +1297,10 @@ double DevAmg::vcycle_bytes() const {
 
 // ---------------------------------------------------------------------------
 
+void upload_fp64_operator(DevMat& dm, const HostCsr& a) {
+  upload_mat(dm, a.n, a.m, 0, -1, a.rptr, a.cind, a.val, 1);
+}
+
 DevSolver::DevSolver(Elem e, const HostCsr& a, DevAmg* amg_, int m_, double eps_,
                      int bit_, int max_outer_)
     : elem(e), m(m_), eps(eps_), bit(bit_), max_outer(max_outer_), amg(amg_) {

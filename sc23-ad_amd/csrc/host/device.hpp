@@ -196,14 +196,42 @@ class DevAmg {
   // the error word is cleared first, so only the call that timed out fails.
   void check_errors(Work& w) const;
 
+  // ---- block V-cycle: 4 * groups right-hand sides at once (integer only) ---
+  // Vectors hold element (row i, right-hand side r) at ((r / 4) * n + i) * 4
+  // + r % 4 (see block.hip).  One BlockWork per concurrent block user.
+  struct BlockWork {
+    int groups = 0;
+    std::vector<DevBuf> ee, xx, rr, tmp;   // per level (1..depth); tmp[0] fine
+    DevBuf r0, vin0, vout0, tmpc;
+    DevBuf lflags, uflags, tickets, err;   // tiled coarse solves
+    int lepoch = 0, uepoch = 0;
+    DevBuf col_in, col_out;                // one right-hand side (per-column fallback)
+  };
+  std::unique_ptr<BlockWork> make_block_work(int groups) const;
+  void vcycle_blk(const void* in, void* out, BlockWork& w) const;
+  void spmv_blk(const DevMat& m, dev::SpmvMode mode, const void* x, const void* z, void* y,
+                int groups) const;
+  // Whether every LD solve of the hierarchy has a block kernel (else those
+  // levels solve one right-hand side at a time).
+  bool block_native() const;
+  void check_errors(BlockWork& w) const;
+
  private:
   std::unique_ptr<Work> work_;
   void tune();  // SpMV launch shapes per matrix
   void vcycle_lo(const void* in, void* out, Work& w) const;  // vectors in level order
+  void vcycle_lo_blk(const void* in, void* out, BlockWork& w) const;
+  void smooth_blk(int l, const void* in, void* out, bool inited, BlockWork& w) const;
+  void trsv_blk(const TriPlan& plan, const DevMat& m, const void* in, void* out, BlockWork& w,
+                unsigned long long* flags, int* epoch, unsigned* tickets) const;
+  void coarse_blk(const void* in, void* out, BlockWork& w) const;
   const int* perm_of(int level) const {  // nullptr: original order (coarsest grid)
     return level < depth ? lv[level].perm.as<int>() : nullptr;
   }
 };
+
+// FP64 copy of the fine operator for the outer residual (the reference's dA).
+void upload_fp64_operator(DevMat& dm, const HostCsr& a);
 
 struct SolverOpts {
   int seq_reduction = 0;  // 1: FP reductions strictly left to right
@@ -245,6 +273,44 @@ class DevSolver {
 
  private:
   void read_flags();
+};
+
+// k integer solves on one hierarchy in lockstep: Restarted(FGMRESflex(m)) for
+// every right-hand side (irsolver.hpp:27-43, gmres.hpp:225-271), with the
+// V-cycles, the fine-level SpMV and the Gram-Schmidt passes of all of them
+// in shared launches.  Each right-hand side's arithmetic is its single
+// solve's, so its results are identical to DevSolver's.
+class DevBlockSolver {
+ public:
+  int count = 0;   // right-hand sides
+  int groups = 0;  // ceil(count / 4)
+  int m = 30;
+  double eps = 1e-10;
+  int max_outer = 33;
+  std::vector<int> bits;  // F per right-hand side
+  DevAmg* amg = nullptr;  // not owned
+  DevMat a64;
+  DevBuf V, Z, r, states, kst, live, upd, flagbuf, trace_buf;
+  hipStream_t stream = nullptr;
+  std::unique_ptr<DevAmg::BlockWork> work;
+  std::vector<dev::KState> ks;
+  int* host_flags = nullptr;  // pinned: 6 words per right-hand side
+  int* host_mask = nullptr;   // pinned: live then upd masks
+  bool tracing = false;
+  int trace_cap = 0;
+  std::vector<std::vector<double>> history;
+
+  DevBlockSolver(const HostCsr& a, DevAmg* amg, int count, int m, double eps,
+                 const int* bits, int max_outer);
+  ~DevBlockSolver();
+  // b, x: `count` FP64 columns of n on the device; res: count records
+  void solve(const double* b_dev, const double* nrm_b, double* x_dev, DevSolver::Result* res);
+  void set_trace(int cap);
+  void read_trace(int rhs, uint64_t* out, int cap, int* len);
+
+ private:
+  void read_flags();
+  void put_masks(const std::vector<int>& live_v, const std::vector<int>& upd_v);
 };
 
 }  // namespace sc23ad

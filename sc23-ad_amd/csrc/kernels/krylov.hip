@@ -476,7 +476,7 @@ __global__ __launch_bounds__(B) void k_scal_fp(int n, T* __restrict__ w, KState 
 
 // ---- solution update -----------------------------------------------------
 
-__global__ void k_trsv_int(KState k, int steps) {
+__device__ void trsv_int_body(const KState& k, int steps) {
   const int64_t* H = static_cast<const int64_t*>(k.H);
   const int64_t* e = static_cast<const int64_t*>(k.e);
   const int nn = k.m + 1;
@@ -489,6 +489,8 @@ __global__ void k_trsv_int(KState k, int steps) {
     k.y[i] = temp / (static_cast<double>(H[i * nn + i]) / fact);
   }
 }
+
+__global__ void k_trsv_int(KState k, int steps) { trsv_int_body(k, steps); }
 
 template <typename T>
 __global__ void k_trsv_fp(KState k, int steps, T* y) {
@@ -577,6 +579,306 @@ __global__ __launch_bounds__(B) void k_zero_bytes(size_t n4, uint32_t* __restric
   for (size_t i = blockIdx.x * static_cast<size_t>(B) + threadIdx.x; i < n4;
        i += static_cast<size_t>(gridDim.x) * B)
     b[i] = 0;
+}
+
+// ---- block Krylov kernels -----------------------------------------------------
+//
+// k integer solves on one hierarchy advance in lockstep (every solve that has
+// not finished is at the same Arnoldi step j, since all start together and
+// restart after exactly m steps); solve r keeps its own KState ks[r] (its F,
+// H, Givens pairs, e, scalars and partial sums), and its Krylov vectors are
+// right-hand side r of the block vectors V_j, Z_j (four solves per 16-byte
+// word, see block.hip).  live[r] marks the solves of the current cycle; a
+// solve drops out of the cycle at its converged / overflow flag.  Each
+// solve's arithmetic is its single solve's, element by element and in the
+// same order, so its results are identical.
+
+__device__ __forceinline__ bool blk_live(const KState* ks, const int* live, int r) {
+  return live[r] != 0 && ks[r].flags[0] == 0 && ks[r].flags[1] == 0;
+}
+
+// r_r = b_r - A x_r and the blocked-order partials of r_r^2 (blockIdx.y = r).
+__global__ __launch_bounds__(B) void k_outer_residual_blk(
+    int n, const int* __restrict__ rptr, const int* __restrict__ cind,
+    const double* __restrict__ val, const double* __restrict__ b_all,
+    const double* __restrict__ x_all, double* __restrict__ r_all, const KState* ks,
+    const int* live) {
+  __shared__ double sh[B];
+  const int r = blockIdx.y;
+  if (!live[r]) return;
+  const size_t off = static_cast<size_t>(r) * n;
+  const double* b = b_all + off;
+  const double* x = x_all + off;
+  double* rr = r_all + off;
+  dro_partials<double>(n, ks[r].dpart, sh, [&](int i) {
+    double t = 0;
+    const int re = rptr[i + 1];
+    for (int p = rptr[i]; p < re; ++p) t += val[p] * x[cind[p]];
+    const double ri = b[i] - t;
+    rr[i] = ri;
+    return ri * ri;
+  });
+}
+
+__global__ __launch_bounds__(B) void k_norm_start_blk(const KState* ks, const int* live) {
+  __shared__ double sh[B];
+  const int r = blockIdx.x;
+  if (!live[r]) return;
+  const KState k = ks[r];
+  const double tot = final_tree<double>(k.dpart, sh);
+  if (threadIdx.x == 0) start_scalars(k, Elem::kI32, sqrt(tot));
+}
+
+// V_0 of every live solve: trunc(r * e_inv * 2^F) (zero for the others).
+__global__ __launch_bounds__(B) void k_v0_blk(int n, const double* __restrict__ r_all,
+                                              int4* __restrict__ v0, const KState* ks,
+                                              const int* live) {
+  const int i = blockIdx.x * B + threadIdx.x;
+  const int g = blockIdx.y;
+  if (i >= n) return;
+  int32_t q[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int r = 4 * g + c;
+    q[c] = 0;
+    if (live[r]) {
+      const double e_inv = ks[r].scal[3];
+      const double fact = static_cast<double>(static_cast<int64_t>(1) << ks[r].bit);
+      q[c] = trunc_i32(r_all[static_cast<size_t>(r) * n + i] * e_inv * fact);
+    }
+  }
+  v0[static_cast<size_t>(g) * n + i] = make_int4(q[0], q[1], q[2], q[3]);
+}
+
+// Sum of four order-free values over the block; thread 0 gets them.
+__device__ __forceinline__ void block_sum4(uint64_t* acc, uint64_t (*sh)[B / 64]) {
+#pragma unroll
+  for (int c = 0; c < 4; ++c) acc[c] = wave_sum_u64(acc[c]);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) sh[c][wid] = acc[c];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      uint64_t t = 0;
+      for (int w = 0; w < B / 64; ++w) t += sh[c][w];
+      acc[c] = t;
+    }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(B) void k_dot_first_blk(int n, const int4* __restrict__ v_all,
+                                                     const int4* __restrict__ w_all,
+                                                     const KState* ks) {
+  __shared__ uint64_t sh[4][B / 64];
+  const int g = blockIdx.y;
+  const int4* v = v_all + static_cast<size_t>(g) * n;
+  const int4* w = w_all + static_cast<size_t>(g) * n;
+  uint64_t acc[4] = {0, 0, 0, 0};
+  for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) {
+    const int4 a = v[i], b = w[i];
+    acc[0] += prod64(a.x, b.x);
+    acc[1] += prod64(a.y, b.y);
+    acc[2] += prod64(a.z, b.z);
+    acc[3] += prod64(a.w, b.w);
+  }
+  block_sum4(acc, sh);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) ks[4 * g + c].ipart[blockIdx.x] = static_cast<int64_t>(acc[c]);
+}
+
+// w += (int32)((-H(kk,j) V_kk) >> F) for each solve of the group, then the
+// partials of <V_kk+1, w> (kk < j) or <w, w> (kk == j); H(kk,j) is finished
+// from the previous pass's partials (half kk & 1 of ipart, written to half
+// (kk + 1) & 1).
+__global__ __launch_bounds__(B) void k_axpy_dot_blk(int n, const int4* __restrict__ vk_all,
+                                                    const int4* __restrict__ vk1_all,
+                                                    int4* __restrict__ w_all, const KState* ks,
+                                                    const int* live, int kk, int j) {
+  __shared__ uint64_t sh[4][B / 64];
+  __shared__ int64_t hsh[4];
+  const int g = blockIdx.y;
+  const int G = gridDim.x;
+  const int pin = (kk & 1) * kRedBlocks, pout = ((kk + 1) & 1) * kRedBlocks;
+  {
+    uint64_t tot[4] = {0, 0, 0, 0};
+    for (int b = threadIdx.x; b < G; b += B)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) tot[c] += static_cast<uint64_t>(ks[4 * g + c].ipart[pin + b]);
+    block_sum4(tot, sh);
+    if (threadIdx.x == 0)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int r = 4 * g + c;
+        const int64_t h = as_i64(tot[c]) >> ks[r].bit;
+        hsh[c] = h;
+        if (blockIdx.x == 0 && blk_live(ks, live, r))
+          static_cast<int64_t*>(ks[r].H)[j * (ks[r].m + 1) + kk] = h;
+      }
+  }
+  __syncthreads();
+  int64_t a[4];
+  int bit[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    a[c] = as_i64(0 - static_cast<uint64_t>(hsh[c]));
+    bit[c] = ks[4 * g + c].bit;
+  }
+  const bool norm = (kk == j);
+  const size_t off = static_cast<size_t>(g) * n;
+  const int4* vk = vk_all + off;
+  const int4* vk1 = vk1_all + off;
+  int4* w = w_all + off;
+  uint64_t acc[4] = {0, 0, 0, 0};
+  for (int i = blockIdx.x * B + threadIdx.x; i < n; i += G * B) {
+    const int4 v = vk[i];
+    int4 x = w[i];
+    x.x = add32(x.x, low32(mul_wrap(a[0], v.x) >> bit[0]));
+    x.y = add32(x.y, low32(mul_wrap(a[1], v.y) >> bit[1]));
+    x.z = add32(x.z, low32(mul_wrap(a[2], v.z) >> bit[2]));
+    x.w = add32(x.w, low32(mul_wrap(a[3], v.w) >> bit[3]));
+    w[i] = x;
+    const int4 u = norm ? x : vk1[i];
+    acc[0] += prod64(u.x, x.x);
+    acc[1] += prod64(u.y, x.y);
+    acc[2] += prod64(u.z, x.z);
+    acc[3] += prod64(u.w, x.w);
+  }
+  block_sum4(acc, sh);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      ks[4 * g + c].ipart[pout + blockIdx.x] = static_cast<int64_t>(acc[c]);
+}
+
+// w = (int32)((H_inv w) >> F) per solve; block 0 finishes the Arnoldi step of
+// each live solve (lsqrt, H_inv, Givens, e, convergence flag).
+__global__ __launch_bounds__(B) void k_scal_givens_blk(int n, int4* __restrict__ w_all,
+                                                       const KState* ks, const int* live,
+                                                       int j) {
+  __shared__ uint64_t sh[4][B / 64];
+  __shared__ int64_t hinv_sh[4];
+  const int g = blockIdx.y;
+  const int G = gridDim.x;
+  const int pin = ((j + 1) & 1) * kRedBlocks;
+  uint64_t tot[4] = {0, 0, 0, 0};
+  for (int b = threadIdx.x; b < G; b += B)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) tot[c] += static_cast<uint64_t>(ks[4 * g + c].ipart[pin + b]);
+  block_sum4(tot, sh);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int r = 4 * g + c;
+      const KState k = ks[r];
+      const int64_t hn = lsqrt64(as_i64(tot[c]));
+      const int bit = k.bit;
+      hinv_sh[c] = hn == 0 ? 0 : ((static_cast<int64_t>(1) << 62) / hn) >> (62 - bit - bit);
+      if (blockIdx.x == 0 && blk_live(ks, live, r)) {
+        k.iscal[1] = hinv_sh[c];
+        givens_int(k, j, hn);
+      }
+    }
+  __syncthreads();
+  int64_t hinv[4];
+  int bit[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    hinv[c] = hinv_sh[c];
+    bit[c] = ks[4 * g + c].bit;
+  }
+  int4* w = w_all + static_cast<size_t>(g) * n;
+  for (int i = blockIdx.x * B + threadIdx.x; i < n; i += G * B) {
+    int4 x = w[i];
+    x.x = low32(mul_wrap(hinv[0], x.x) >> bit[0]);
+    x.y = low32(mul_wrap(hinv[1], x.y) >> bit[1]);
+    x.z = low32(mul_wrap(hinv[2], x.z) >> bit[2]);
+    x.w = low32(mul_wrap(hinv[3], x.w) >> bit[3]);
+    w[i] = x;
+  }
+}
+
+// y = H^-1 e for every solve whose x is updated (upd[r]), over the steps its
+// last cycle ran (flags[3]).
+__global__ void k_trsv_blk(const KState* ks, const int* upd) {
+  const int r = blockIdx.x;
+  if (threadIdx.x != 0 || !upd[r]) return;
+  const KState k = ks[r];
+  trsv_int_body(k, k.flags[3]);
+}
+
+// x_r += sum_q y_q Z_q(r) 2^-F, in order of q (blockIdx.y = r).
+__global__ __launch_bounds__(B) void k_update_x_blk(int n, int groups,
+                                                    const int32_t* __restrict__ Z,
+                                                    double* __restrict__ x_all,
+                                                    const KState* ks, const int* upd) {
+  const int r = blockIdx.y;
+  const int i = blockIdx.x * B + threadIdx.x;
+  if (i >= n || !upd[r]) return;
+  const KState& k = ks[r];
+  const int steps = k.flags[3];
+  const double fact_inv = 1.0 / static_cast<double>(static_cast<int64_t>(1) << k.bit);
+  const size_t vstride = static_cast<size_t>(groups) * n * 4;
+  const size_t at = (static_cast<size_t>(r >> 2) * n + i) * 4 + (r & 3);
+  double* x = x_all + static_cast<size_t>(r) * n;
+  double xi = x[i];
+  for (int q = 0; q < steps; ++q)
+    xi += k.y[q] * static_cast<double>(Z[q * vstride + at]) * fact_inv;
+  x[i] = xi;
+}
+
+// Flags and implicit residual of every solve into one array (5 words each:
+// converged, overflow, stagnated, steps, then the residual as a double in the
+// next two words).
+__global__ void k_flags_blk(const KState* ks, int count, int* out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= count) return;
+  const KState& k = ks[r];
+  int* o = out + 6 * r;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) o[q] = k.flags[q];
+  const double v = k.scal[5];
+  __builtin_memcpy(o + 4, &v, sizeof v);
+}
+
+// Order-free checksums of every live solve's column of a block vector,
+// appended to its trace (the single solve's checksum of the same vector).
+__global__ __launch_bounds__(B) void k_checksum_blk_part(int n, const int32_t* __restrict__ v,
+                                                         const KState* ks, const int* live) {
+  __shared__ uint64_t sh[4][B / 64];
+  const int g = blockIdx.y;
+  uint64_t acc[4] = {0, 0, 0, 0};
+  const int4* v4 = reinterpret_cast<const int4*>(v) + static_cast<size_t>(g) * n;
+  for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) {
+    const int4 x = v4[i];
+    const uint64_t base = static_cast<uint64_t>(i) * 0x9E3779B97F4A7C15ULL;
+    acc[0] += mix64(base ^ static_cast<uint64_t>(static_cast<uint32_t>(x.x)));
+    acc[1] += mix64(base ^ static_cast<uint64_t>(static_cast<uint32_t>(x.y)));
+    acc[2] += mix64(base ^ static_cast<uint64_t>(static_cast<uint32_t>(x.z)));
+    acc[3] += mix64(base ^ static_cast<uint64_t>(static_cast<uint32_t>(x.w)));
+  }
+  block_sum4(acc, sh);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      ks[4 * g + c].ipart[2 * kRedBlocks + blockIdx.x] = static_cast<int64_t>(acc[c]);
+}
+
+__global__ __launch_bounds__(B) void k_checksum_blk_finalize(const KState* ks, const int* live,
+                                                             int G) {
+  __shared__ uint64_t sh[B / 64];
+  const int r = blockIdx.x;
+  if (!blk_live(ks, live, r)) return;
+  const KState k = ks[r];
+  const uint64_t s = sum_parts(k.ipart + 2 * kRedBlocks, G, sh);
+  if (threadIdx.x == 0 && k.trace) {
+    const int len = *k.trace_len;
+    if (len < k.trace_cap) k.trace[len] = s;
+    *k.trace_len = len + 1;
+  }
 }
 
 inline int grid_for(int n) { return (n + B - 1) / B; }
@@ -838,6 +1140,74 @@ hipError_t launch_fill_zero(size_t bytes, void* dst, hipStream_t s) {
   const int grid = static_cast<int>(std::min<size_t>((n4 + B - 1) / B, 8192));
   hipLaunchKernelGGL(k_zero_bytes, dim3(grid), dim3(B), 0, s, n4,
                      static_cast<uint32_t*>(dst));
+  return hipGetLastError();
+}
+
+// ---- block launchers -------------------------------------------------------
+
+hipError_t launch_outer_residual_blk(const DCsr& a64, int count, const double* b,
+                                     const double* x, double* r, const KState* ks,
+                                     const int* live, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_outer_residual_blk, dim3(red_grid(a64.n), count), dim3(B), 0, s, a64.n,
+                     a64.rptr, a64.cind, static_cast<const double*>(a64.val), b, x, r, ks, live);
+  hipLaunchKernelGGL(k_norm_start_blk, dim3(count), dim3(B), 0, s, ks, live);
+  return hipGetLastError();
+}
+
+hipError_t launch_v0_blk(int n, int groups, const double* r, void* v0, const KState* ks,
+                         const int* live, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_v0_blk, dim3(grid_for(n), groups), dim3(B), 0, s, n, r,
+                     static_cast<int4*>(v0), ks, live);
+  return hipGetLastError();
+}
+
+hipError_t launch_dot_first_blk(int n, int groups, const void* v0, const void* w,
+                                const KState* ks, hipStream_t s) {
+  hipLaunchKernelGGL(k_dot_first_blk, dim3(red_grid(n), groups), dim3(B), 0, s, n,
+                     static_cast<const int4*>(v0), static_cast<const int4*>(w), ks);
+  return hipGetLastError();
+}
+
+hipError_t launch_axpy_dot_blk(int n, int groups, const void* V, void* w, const KState* ks,
+                               const int* live, int kk, int j, hipStream_t s) {
+  const size_t vs = static_cast<size_t>(groups) * n;
+  const int4* v = static_cast<const int4*>(V);
+  hipLaunchKernelGGL(k_axpy_dot_blk, dim3(red_grid(n), groups), dim3(B), 0, s, n, v + kk * vs,
+                     v + (kk + 1) * vs, static_cast<int4*>(w), ks, live, kk, j);
+  return hipGetLastError();
+}
+
+hipError_t launch_scal_blk(int n, int groups, void* w, const KState* ks, const int* live, int j,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(k_scal_givens_blk, dim3(red_grid(n), groups), dim3(B), 0, s, n,
+                     static_cast<int4*>(w), ks, live, j);
+  return hipGetLastError();
+}
+
+hipError_t launch_update_x_blk(int n, int groups, int count, const void* Z, double* x,
+                               const KState* ks, const int* upd, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_trsv_blk, dim3(count), dim3(64), 0, s, ks, upd);
+  if (n > 0)
+    hipLaunchKernelGGL(k_update_x_blk, dim3(grid_for(n), count), dim3(B), 0, s, n, groups,
+                       static_cast<const int32_t*>(Z), x, ks, upd);
+  return hipGetLastError();
+}
+
+hipError_t launch_flags_blk(const KState* ks, int count, int* out, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_flags_blk, dim3((count + 63) / 64), dim3(64), 0, s, ks, count, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_checksum_blk(int n, int groups, int count, const void* v, const KState* ks,
+                               const int* live, hipStream_t s) {
+  const int G = red_grid(n);
+  hipLaunchKernelGGL(k_checksum_blk_part, dim3(G, groups), dim3(B), 0, s, n,
+                     static_cast<const int32_t*>(v), ks, live);
+  hipLaunchKernelGGL(k_checksum_blk_finalize, dim3(count), dim3(B), 0, s, ks, live, G);
   return hipGetLastError();
 }
 

@@ -141,6 +141,32 @@ inline size_t rp_lds_bytes(int ring, int maxlev) {
   return sizeof(int32_t) * static_cast<size_t>(ring) +
          sizeof(int) * (static_cast<size_t>(maxlev) + 1);
 }
+// ---- block kernels (block.hip): several right-hand sides of one hierarchy --
+// A block vector of n rows and `groups` groups of four right-hand sides holds
+// element (row i, right-hand side r) at ((r / 4) * n + i) * 4 + r % 4.
+inline size_t rpb_lds_bytes(int ring, int maxlev) {
+  return 16 * static_cast<size_t>(ring) + sizeof(int) * (static_cast<size_t>(maxlev) + 1);
+}
+hipError_t launch_permute_blk(int n, int groups, const int* perm, const void* src, void* dst,
+                              bool gather, hipStream_t s);
+hipError_t launch_copy_blk(int n, int groups, const void* src, void* dst, hipStream_t s);
+// right-hand side r of a block <-> a plain int32 vector (get: block -> col)
+hipError_t launch_blk_column(int n, int r, void* blk, void* col, bool get, hipStream_t s);
+// y = spmv(x) | z - spmv(x) | z + spmv(x) for every right-hand side (y may alias z)
+hipError_t launch_spmv_blk(const DCsr& a, int groups, SpmvMode mode, const void* x,
+                           const void* z, void* y, hipStream_t s);
+// Row-parallel LD solve of a block (the layouts of launch_trsv_rp that the
+// block kernel has; see trsv_rp_blk_supported).
+bool trsv_rp_blk_supported(const TriLayout& L);
+hipError_t launch_trsv_rp_blk(const TriLayout& L, const TriSegment& seg, int groups, int bit,
+                              int dbit, const void* in, void* out, hipStream_t s);
+// Tiled coarse solve of a block: xp holds groups * n * 4 words, tickets two
+// zero-initialised counters that the kernel leaves at zero.
+hipError_t launch_trsv_tiles_blk(TriKind kind, int n, int nb, int groups, const int32_t* tiles,
+                                 const long long* toff, const int32_t* dinv, int bit, int dbit,
+                                 const void* in, void* out, unsigned long long* xp, int epoch,
+                                 int* err, unsigned* tickets, hipStream_t s);
+
 // LDS bytes of the entry-parallel kernel: acc_slots uint64 accumulators and
 // kEpDummy spare ones, the ring, the chain's level offsets.
 constexpr int kEpDummy = 64;
@@ -256,6 +282,29 @@ hipError_t launch_checksum(int n, const int32_t* v, const KState& k,
 hipError_t launch_relres(const DCsr& a64, const double* b, const double* x,
                          double* r, double* part, unsigned* counter,
                          double* out, hipStream_t s);
+// ---- block Krylov kernels (several solves on one hierarchy, in lockstep) ---
+// ks: one KState per right-hand side (4 * groups of them, in device memory);
+// live[r] != 0: solve r takes part in the current cycle; upd[r] != 0: its x
+// is updated at the end of the cycle over the steps it ran (flags[3]).
+// Block vectors as in block.hip; b, x, r: `count` FP64 columns of n.
+hipError_t launch_outer_residual_blk(const DCsr& a64, int count, const double* b,
+                                     const double* x, double* r, const KState* ks,
+                                     const int* live, hipStream_t s);
+hipError_t launch_v0_blk(int n, int groups, const double* r, void* v0, const KState* ks,
+                         const int* live, hipStream_t s);
+hipError_t launch_dot_first_blk(int n, int groups, const void* v0, const void* w,
+                                const KState* ks, hipStream_t s);
+hipError_t launch_axpy_dot_blk(int n, int groups, const void* V, void* w, const KState* ks,
+                               const int* live, int kk, int j, hipStream_t s);
+hipError_t launch_scal_blk(int n, int groups, void* w, const KState* ks, const int* live, int j,
+                           hipStream_t s);
+hipError_t launch_update_x_blk(int n, int groups, int count, const void* Z, double* x,
+                               const KState* ks, const int* upd, hipStream_t s);
+// flags[0..3] and the implicit residual (a double in words 4-5) of each solve
+hipError_t launch_flags_blk(const KState* ks, int count, int* out, hipStream_t s);
+hipError_t launch_checksum_blk(int n, int groups, int count, const void* v, const KState* ks,
+                               const int* live, hipStream_t s);
+
 // V-cycle vector helpers
 hipError_t launch_copy(Elem e, int n, const void* src, void* dst, hipStream_t s);
 // gather: dst[k] = src[perm[k]]; otherwise scatter: dst[perm[k]] = src[k].

@@ -12,6 +12,7 @@
 
 #include "dev_common.hpp"
 #include "launch.hpp"
+#include "tri_common.hpp"
 
 namespace sc23ad::dev {
 
@@ -417,58 +418,6 @@ hipError_t tri_seg(const DCsr& a, const int* perm, const int* lev_ptr,
 // bptr[i * (nb + 1) + b] is the first dependency entry of row i whose column
 // is >= 64 b.
 
-constexpr int kTB = 64;
-constexpr int kBlkThreads = 1024;
-
-template <typename T>
-struct AccOf {
-  using type = T;
-};
-template <>
-struct AccOf<int32_t> {
-  using type = uint64_t;
-};
-
-template <typename T>
-__device__ __forceinline__ typename AccOf<T>::type acc_init(T v, int bit) {
-  return v;
-}
-template <>
-__device__ __forceinline__ uint64_t acc_init<int32_t>(int32_t v, int bit) {
-  return static_cast<uint64_t>(static_cast<int64_t>(v)) << bit;
-}
-
-template <typename T>
-__device__ __forceinline__ typename AccOf<T>::type acc_sub(typename AccOf<T>::type t, T v,
-                                                           T x) {
-  return t - v * x;
-}
-template <>
-__device__ __forceinline__ uint64_t acc_sub<int32_t>(uint64_t t, int32_t v, int32_t x) {
-  return t - prod64(v, x);
-}
-
-template <typename T, TriKind K>
-__device__ __forceinline__ T acc_final(typename AccOf<T>::type t, T dinv, int bit, int dbit) {
-  if (K == TriKind::kL) return t;
-  return t * dinv;
-}
-template <>
-__device__ __forceinline__ int32_t acc_final<int32_t, TriKind::kL>(uint64_t t, int32_t, int bit,
-                                                                   int) {
-  return low32(as_i64(t) >> bit);
-}
-template <>
-__device__ __forceinline__ int32_t acc_final<int32_t, TriKind::kLD>(uint64_t t, int32_t dinv,
-                                                                    int bit, int dbit) {
-  return low32(mul_wrap(as_i64(t) >> bit, dinv) >> dbit);
-}
-template <>
-__device__ __forceinline__ int32_t acc_final<int32_t, TriKind::kDU>(uint64_t t, int32_t dinv,
-                                                                    int bit, int dbit) {
-  return low32(mul_wrap(as_i64(t) >> bit, dinv) >> dbit);
-}
-
 // Broadcast lane `src` (wave-uniform) to the whole wave with v_readlane.
 template <typename T>
 __device__ __forceinline__ T bcast_lane(T v, int src) {
@@ -686,8 +635,6 @@ __global__ __launch_bounds__(kBlkThreads) void k_tri_blocked_int(
 // (backward) blocks only, and all blocks of a coarse grid (at most 2000
 // rows: 32 blocks) are resident at once, so every wait ends; a wait longer
 // than kTileMaxSpin polls records an error and gives up instead of hanging.
-constexpr int kTileThreads = 256;
-constexpr int kTileMaxSpin = 1 << 18;
 
 template <TriKind K>
 __global__ __launch_bounds__(kTileThreads) void k_tri_tiles_int(
@@ -867,14 +814,6 @@ __device__ __forceinline__ T pre_eval(const PreRow<T>& r, int bit, int dbit,
   return acc_final<T, K>(t, r.dinv, bit, dbit);
 }
 
-// Workgroup barrier that orders LDS only: the solution is exchanged through
-// the LDS ring, so outstanding global loads (the prefetch of the next levels)
-// and stores (the result) may stay in flight across it.
-__device__ __forceinline__ void lds_barrier() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-  __builtin_amdgcn_s_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
 
 // kRing: dependency entries hold level-ordered positions; the values of the
 // last rmask+1 positions of the chain live in an LDS ring (the host checks
@@ -1011,15 +950,6 @@ __device__ __forceinline__ void ep_far(EpPre<kE, kR>& r, const int32_t* out) {
   }
 }
 
-template <int kWaves>
-__device__ __forceinline__ void ep_sync() {
-  if (kWaves == 1) {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront", "local");
-    __builtin_amdgcn_wave_barrier();
-  } else {
-    lds_barrier();
-  }
-}
 
 struct EpShared {
   const int2* lvo;  // level offsets {position, entry}, LDS
@@ -1273,29 +1203,6 @@ __device__ __forceinline__ void rp_far(RpPre<kE, kR>& r, const int32_t* out) {
   }
 }
 
-// Sum of a 64-bit value over aligned groups of kG <= 16 lanes, with DPP
-// moves inside rows of 16 lanes: swap neighbours (quad_perm 1,0,3,2), swap
-// pairs (quad_perm 2,3,0,1), then lane i with lane 7 - i of its 8 (row half
-// mirror) and lane 15 - i of its 16 (row mirror).  Every lane ends with its
-// group's sum.  All lanes must be active.
-template <int kCtrl>
-__device__ __forceinline__ uint64_t dpp_u64(uint64_t v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(static_cast<uint32_t>(v)), kCtrl,
-                                             0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(
-      0, static_cast<int>(static_cast<uint32_t>(v >> 32)), kCtrl, 0xF, 0xF, false);
-  return (static_cast<uint64_t>(static_cast<uint32_t>(hi)) << 32) | static_cast<uint32_t>(lo);
-}
-
-template <int kG>
-__device__ __forceinline__ uint64_t group_sum(uint64_t v) {
-  static_assert(kG == 1 || kG == 2 || kG == 4 || kG == 8 || kG == 16, "groups of 1-16 lanes");
-  if (kG >= 2) v += dpp_u64<0xB1>(v);   // quad_perm(1, 0, 3, 2)
-  if (kG >= 4) v += dpp_u64<0x4E>(v);   // quad_perm(2, 3, 0, 1)
-  if (kG >= 8) v += dpp_u64<0x141>(v);  // row_half_mirror
-  if (kG >= 16) v += dpp_u64<0x140>(v); // row_mirror
-  return v;
-}
 
 template <TriKind K, int kWaves, bool kFar, int kG, int kE, int kR, int kD, int kA, int kFq>
 __device__ __forceinline__ void rp_level(RpPre<kE, kR>& cur, RpPre<kE, kR>& ahead, int L, int nl,

@@ -122,6 +122,18 @@ def _load() -> C.CDLL:
         "sc23ad_mlp_train": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, C.c_float,
                                        C.POINTER(C.c_float)]),
         "sc23ad_learned_levels": (C.c_int, [vp, vp, vp, vp, C.c_int, ip]),
+        "sc23ad_amg_precondition_block": (C.c_int, [vp, C.c_int, vp, vp, C.c_int]),
+        "sc23ad_amg_block_native": (C.c_int, [vp, ip]),
+        "sc23ad_block_solver_create": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_double, vp,
+                                                 C.c_int, C.POINTER(vp)]),
+        "sc23ad_block_solver_destroy": (None, [vp]),
+        "sc23ad_block_solver_solve": (C.c_int, [vp, vp, vp, vp, C.c_int,
+                                                C.POINTER(Converge)]),
+        "sc23ad_block_solve_batch": (C.c_int, [C.c_int, C.POINTER(vp), C.POINTER(vp),
+                                               C.POINTER(vp), C.POINTER(vp), C.c_int, C.c_int,
+                                               C.POINTER(vp)]),
+        "sc23ad_block_solver_trace": (C.c_int, [vp, C.c_int, vp, C.c_int, ip]),
+        "sc23ad_block_solver_history": (C.c_int, [vp, C.c_int, vp, C.c_int, ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -599,6 +611,113 @@ def _fine_vector(v, n: int, what: str) -> np.ndarray:
     if a.ndim != 1 or a.shape[0] != n:
         raise ValueError(f"{what}: expected {n} values, got shape {a.shape}")
     return a
+
+
+def to_block(cols: np.ndarray) -> np.ndarray:
+    """(k, n) int32 vectors -> the block layout of the library (element (i, r)
+    at ((r // 4) * n + i) * 4 + r % 4; missing right-hand sides are zero)."""
+    cols = np.asarray(cols, dtype=np.int32)
+    k, n = cols.shape
+    g = (k + 3) // 4
+    pad = np.zeros((4 * g, n), np.int32)
+    pad[:k] = cols
+    return np.ascontiguousarray(pad.reshape(g, 4, n).transpose(0, 2, 1)).reshape(-1)
+
+
+def from_block(blk: np.ndarray, k: int, n: int) -> np.ndarray:
+    g = (k + 3) // 4
+    return np.asarray(blk).reshape(g, n, 4).transpose(0, 2, 1).reshape(4 * g, n)[:k].copy()
+
+
+def precondition_block(amg: "Amg", vins: np.ndarray) -> np.ndarray:
+    """One integer V-cycle on each row of vins ((k, n) int32), in one block
+    call; returns (k, n)."""
+    vins = np.asarray(vins, dtype=np.int32)
+    k, n = vins.shape
+    blk = to_block(vins)
+    out = np.zeros_like(blk)
+    check(lib.sc23ad_amg_precondition_block(amg.h, k, ptr(blk), ptr(out), 0))
+    return from_block(out, k, n)
+
+
+def block_native(amg: "Amg") -> bool:
+    v = C.c_int()
+    check(lib.sc23ad_amg_block_native(amg.h, C.byref(v)))
+    return bool(v.value)
+
+
+class BlockSolver:
+    """k integer Restarted(FGMRESflex(m)) solves on one hierarchy in lockstep
+    (one launch per V-cycle / SpMV / Gram-Schmidt pass for all of them)."""
+
+    def __init__(self, a: CsrMat, amg: Amg, bits, m: int = 30, eps: float = 1e-10,
+                 max_restarts: int = 0):
+        bits = np.ascontiguousarray(bits, dtype=np.int32)
+        h = C.c_void_p()
+        check(lib.sc23ad_block_solver_create(a.h, amg.h, len(bits), m, eps, ptr(bits),
+                                             max_restarts, C.byref(h)))
+        self.h = h
+        self.amg = amg
+        self.n = a.n
+        self.k = len(bits)
+
+    def solve(self, bs, nrm_bs, x0=None):
+        """bs: k right-hand sides (sequence or (k, n) array); returns ((k, n)
+        x, list of Converge)."""
+        b = np.ascontiguousarray(np.stack([_fine_vector(v, self.n, "b") for v in bs]))
+        if b.shape[0] != self.k or len(nrm_bs) != self.k:
+            raise ValueError(f"solve: expected {self.k} right-hand sides")
+        x = (np.zeros((self.k, self.n)) if x0 is None
+             else np.ascontiguousarray(np.asarray(x0, dtype=np.float64).reshape(self.k, self.n)).copy())
+        nb = np.ascontiguousarray(nrm_bs, dtype=np.float64)
+        out = (Converge * self.k)()
+        check(lib.sc23ad_block_solver_solve(self.h, ptr(b), ptr(nb), ptr(x), 0, out))
+        return x, list(out)
+
+    def trace(self, cap: int) -> None:
+        check(lib.sc23ad_block_solver_trace(self.h, 0, None, cap, None))
+
+    def read_trace(self, rhs: int, cap: int) -> np.ndarray:
+        buf = np.zeros(max(cap, 1), np.uint64)
+        n = C.c_int()
+        check(lib.sc23ad_block_solver_trace(self.h, rhs, ptr(buf), cap, C.byref(n)))
+        return buf[:n.value]
+
+    def history(self, rhs: int) -> np.ndarray:
+        n = C.c_int()
+        check(lib.sc23ad_block_solver_history(self.h, rhs, None, 0, C.byref(n)))
+        buf = np.zeros(max(n.value, 1))
+        check(lib.sc23ad_block_solver_history(self.h, rhs, ptr(buf), n.value, C.byref(n)))
+        return buf[:n.value]
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib.sc23ad_block_solver_destroy(self.h)
+            self.h = C.c_void_p()
+
+
+def block_solve_batch(solvers, bss, nrm_bss, threads: int = 0):
+    """Block solver k solves right-hand sides bss[k] (host arrays); all
+    solvers at once, each on its own stream.  Returns (xs, converge lists)."""
+    cnt = len(solvers)
+    if len(bss) != cnt or len(nrm_bss) != cnt:
+        raise ValueError("block_solve_batch: one set of right-hand sides per solver")
+    bl, xl, nl, ol = [], [], [], []
+    for s_, bs, nbs in zip(solvers, bss, nrm_bss):
+        b = np.ascontiguousarray(np.stack([_fine_vector(v, s_.n, "b") for v in bs]))
+        if b.shape[0] != s_.k or len(nbs) != s_.k:
+            raise ValueError("block_solve_batch: right-hand side count differs from the solver's")
+        bl.append(b)
+        xl.append(np.zeros((s_.k, s_.n)))
+        nl.append(np.ascontiguousarray(nbs, dtype=np.float64))
+        ol.append((Converge * s_.k)())
+    hs = (C.c_void_p * cnt)(*[s_.h.value for s_ in solvers])
+    bp = (C.c_void_p * cnt)(*[b.ctypes.data for b in bl])
+    np_ = (C.c_void_p * cnt)(*[v.ctypes.data for v in nl])
+    xp = (C.c_void_p * cnt)(*[x.ctypes.data for x in xl])
+    op = (C.c_void_p * cnt)(*[C.addressof(o) for o in ol])
+    check(lib.sc23ad_block_solve_batch(cnt, hs, bp, np_, xp, 0, threads, op))
+    return xl, [list(o) for o in ol]
 
 
 def spmv_int(rptr, cind, val, bit, x) -> np.ndarray:
