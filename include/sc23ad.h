@@ -310,6 +310,18 @@ typedef struct sc23ad_mlp_s sc23ad_mlp;
 int sc23ad_hier_retune(const sc23ad_hier* base, const double* level_omega,
                        const int* level_bits, int nlevels, int precision,
                        sc23ad_hier** out);
+/* Hierarchy with per-level smoother weights and per-level fractional bits of
+   the level vectors, from base's coarsening: level l's integer vectors carry
+   F + level_vbits[l] fractional bits (level_vbits: nlevels + 1 entries for
+   levels 0..depth, level_vbits[0] = 0).  The smoothers, residuals and the
+   coarse solve are homogeneous in the vector scale, so the integer kernels
+   are unchanged: only R_l's shift drops and P_l's grows by
+   d = level_vbits[l+1] - level_vbits[l] (INTAMG.hpp:278-290; a step outside
+   the shifts' range is cut to it and counted by sc23ad_hier_saturated).
+   level_omega as for sc23ad_hier_retune; either array may be NULL. */
+int sc23ad_hier_retune_levels(const sc23ad_hier* base, const double* level_omega,
+                              const int* level_vbits, int nlevels, int precision,
+                              sc23ad_hier** out);
 /* Levels whose requested bits exceeded the flex rule's maximum (the rule's
    value was kept: the default is the fallback for a saturated prediction). */
 int sc23ad_hier_saturated(const sc23ad_hier* h, int* levels);

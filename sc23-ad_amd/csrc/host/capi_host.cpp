@@ -295,6 +295,39 @@ int sc23ad_hier_retune(const sc23ad_hier* base, const double* level_omega,
   return SC23AD_OK;
 }
 
+int sc23ad_hier_retune_levels(const sc23ad_hier* base, const double* level_omega,
+                              const int* level_vbits, int nlevels, int precision,
+                              sc23ad_hier** out) {
+  if (!base || !out || nlevels < 0 || precision < 0 || precision > 2)
+    return fail(SC23AD_BAD_PARAM, "hier_retune_levels: bad arguments");
+  *out = nullptr;
+  if (!base->setup)
+    return fail(SC23AD_BAD_PARAM, "hier_retune_levels: hierarchy has no setup state");
+  AmgParams prm = base->setup->prm;
+  if (level_omega) {
+    prm.level_omega.assign(level_omega, level_omega + nlevels);
+    for (double w : prm.level_omega)
+      if (!(w > 0 && w < 2))
+        return fail(SC23AD_BAD_PARAM, "hier_retune_levels: omega outside (0, 2)");
+  }
+  if (level_vbits) {
+    if (level_vbits[0] != 0)
+      return fail(SC23AD_BAD_PARAM, "hier_retune_levels: level 0 keeps the solver's F");
+    prm.level_vbits.assign(level_vbits, level_vbits + nlevels + 1);
+    for (int v : prm.level_vbits)
+      if (v < -30 || v > 30)
+        return fail(SC23AD_BAD_PARAM, "hier_retune_levels: vector bits outside [-30, 30]");
+  }
+  try {
+    *out = derive_hierarchy(base->setup, prm, precision).release();
+  } catch (const SetupError& e) {
+    return fail(e.code, "hier_retune_levels: " + e.what);
+  } catch (const std::bad_alloc&) {
+    return fail(SC23AD_SETUP_FAIL, "hier_retune_levels: out of host memory");
+  }
+  return SC23AD_OK;
+}
+
 int sc23ad_hier_saturated(const sc23ad_hier* h, int* levels) {
   if (!h || !levels) return fail(SC23AD_BAD_PARAM, "hier_saturated: null");
   *levels = h->precision == SC23AD_INT ? h->ih.saturated : 0;

@@ -1,3 +1,4 @@
+#include <algorithm>
 #include "hierarchy.hpp"
 
 #include <chrono>
@@ -403,6 +404,22 @@ IntHierarchy build_int_hierarchy(const HostCsr& fine, const AmgChain& chain,
     h.saturated += sat;
     L.r = to_flex(rt[l]);
     L.p = to_flex(chain.interp[l]);
+    // per-level vector bits: R_l's result is shifted d bits less, P_l's d
+    // bits more (kept within the shifts' range [0, 62]; a step outside it
+    // counts as saturated and is cut to the range's edge)
+    if (!prm.level_vbits.empty()) {
+      auto vb = [&](int q) {
+        return static_cast<size_t>(q) < prm.level_vbits.size() ? prm.level_vbits[q] : 0;
+      };
+      int dv = vb(l + 1) - vb(l);
+      const int lo = std::max(L.p.bit - 62, -62), hi = L.r.bit;
+      if (dv < lo || dv > hi) {
+        dv = std::clamp(dv, lo, hi);
+        h.saturated += 1;
+      }
+      L.r.bit -= dv;
+      L.p.bit += dv;
+    }
     lap("  U R P");
   }
   lap("int levels");

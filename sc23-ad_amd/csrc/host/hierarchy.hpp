@@ -49,6 +49,12 @@ struct AmgParams {
   std::vector<double> level_omega;  // SOR weight of level l's smoother
   std::vector<int> level_bits;      // cap on the fractional bits of A_l, U_l and
                                     // LD_l's off-diagonal part (<= 0: none)
+  // Fractional bits of the level vectors relative to the fine level's F
+  // (entry l for level l = 0..depth, entry 0 is 0): level l's vectors carry
+  // F + level_vbits[l] bits.  The smoother, residual and coarse solves are
+  // homogeneous in the vector scale, so only the restriction and
+  // prolongation shifts change (by d = level_vbits[l+1] - level_vbits[l]).
+  std::vector<int> level_vbits;
   bool device_setup = false;        // interpolation and Galerkin products on the GPU
 };
 

@@ -81,6 +81,7 @@ def _load() -> C.CDLL:
         "sc23ad_checksum_i32": (C.c_uint64, [vp, C.c_int]),
         "sc23ad_hier_retune": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, C.POINTER(vp)]),
         "sc23ad_hier_saturated": (C.c_int, [vp, ip]),
+        "sc23ad_hier_retune_levels": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, C.POINTER(vp)]),
         "sc23ad_hier_features": (C.c_int, [vp, vp, C.c_int, ip]),
     }
     optional = {
@@ -293,6 +294,22 @@ class Hierarchy:
         h = C.c_void_p()
         prec = self.precision if precision is None else precision
         check(lib.sc23ad_hier_retune(self.h, ptr(om), ptr(bi), n, prec, C.byref(h)))
+        return Hierarchy(None, prec, _handle=h)
+
+    def retune_levels(self, level_omega=None, level_vbits=None,
+                      precision: int | None = None):
+        """Hierarchy with per-level smoother weights and per-level vector
+        fractional bits (level_vbits: depth + 1 entries, the first 0)."""
+        nl = self.info()[0]
+        om = None if level_omega is None else np.ascontiguousarray(level_omega, np.float64)
+        vb = None if level_vbits is None else np.ascontiguousarray(level_vbits, np.int32)
+        if om is not None and len(om) != nl:
+            raise ValueError(f"level_omega: expected {nl} values")
+        if vb is not None and len(vb) != nl + 1:
+            raise ValueError(f"level_vbits: expected {nl + 1} values")
+        h = C.c_void_p()
+        prec = self.precision if precision is None else precision
+        check(lib.sc23ad_hier_retune_levels(self.h, ptr(om), ptr(vb), nl, prec, C.byref(h)))
         return Hierarchy(None, prec, _handle=h)
 
     def saturated(self) -> int:

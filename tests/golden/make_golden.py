@@ -14,6 +14,8 @@ a parity failure.
 Run:  python tests/golden/make_golden.py            (reference.json)
       python tests/golden/make_golden.py --omega    (reference_omega.json)
       python tests/golden/make_golden.py --large    (reference_laplace3d_128.json)
+      python tests/golden/make_golden.py --inclusions K SOLVER
+                                       (reference_inclusions160_SOLVER_K.json)
 
 --omega records the same quantities for SOR / BlockSOR weights other than 1
 (the LD/U split of stationary.hpp:375-396 and :941-978 then carries a
@@ -160,9 +162,41 @@ def main_large() -> None:
     print("wrote", dst)
 
 
+def main_inclusions(k: int, solver: int) -> None:
+    """Right-hand side k of the 160^3 inclusions workload (3D 7-point
+    diffusion, k = 1 with 64 spheres of radius U[4, 12] cells and k = 1e6,
+    seed 4; b = A x*, x* from seed 400 + k, row-scaled), BlockSOR(40),
+    FGMRES(30), eps 1e-8, F by the driver's rule: the reference's outcome
+    (iterations, x hash, V-cycle checksums; or the signal that ended it) for
+    solver 2, or iterations and residual for solver 0.  Writes
+    reference_inclusions160_<solver>_<k>.json (about ten minutes each)."""
+    a = S.CsrMat.generate(S.GEN_INCLUSIONS3D, 160, 64, 4.0, 12.0, 1e6, 4)
+    rptr, cind, val = a.arrays()
+    rmax = np.maximum.reduceat(np.abs(val), rptr[:-1])
+    b = a.matvec(S.xstar(a.n, 400 + k)) / rmax
+    a.lmax1_scaling(np.ones(a.n))
+    arrs = a.arrays()
+    r = _ref_solve_isolated(arrs, b, solver, 1, m=30, eps=1e-8,
+                            trace_cap=8192 if solver == 2 else 0)
+    rec = {"name": "inclusions3d_160", "rhs": k, "solver": solver, "m": 30, "eps": 1e-8,
+           "smoother": 1, "n": a.n, "nnz": a.nnz, "a_sha": sha(*arrs), "b_sha": sha(b),
+           "solve": r if "crashed" in r else _solve_rec(r, solver)}
+    print("inclusions160", k, solver, {kk: v for kk, v in rec["solve"].items() if kk != "trace"},
+          flush=True)
+    dst = os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                       f"reference_inclusions160_{solver}_{k}.json")
+    with open(dst, "w") as f:
+        json.dump(rec, f, indent=1)
+    print("wrote", dst)
+
+
 def main() -> None:
     if not O.ref_available():
         sys.exit("oracle/_ref/libref.so is missing: run make -C oracle ref")
+    if "--inclusions" in sys.argv:
+        at = sys.argv.index("--inclusions")
+        main_inclusions(int(sys.argv[at + 1]), int(sys.argv[at + 2]))
+        return
     if "--omega" in sys.argv:
         main_omega()
         return

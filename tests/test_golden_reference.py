@@ -244,3 +244,47 @@ def test_full_size_hierarchy_matches_reference():
         assert file_sha(path) == g["hier_int"]
     finally:
         os.unlink(path)
+
+
+def test_batched_convdiff_outcomes_match_reference():
+    """A sample of the batched convection-diffusion workload
+    (tests/golden/reference_convdiff_batch.json): where the reference's
+    integer solver dies (the wrapped sum of squares and lsqrt(<= 0) = 0 as a
+    divisor, gmres.hpp:243-244) the oracle stops with OVERFLOW; where the
+    reference converges the oracle reproduces its iterations and x."""
+    import json
+    import importlib.util
+    here = os.path.dirname(__file__)
+    with open(os.path.join(here, "golden", "reference_convdiff_batch.json")) as f:
+        g = json.load(f)
+    spec = importlib.util.spec_from_file_location(
+        "make_batch_golden", os.path.join(here, "golden", "make_batch_golden.py"))
+    mk = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mk)
+    crashed = converged = 0
+    for c in g["cases"]:
+        a, rmax, _ = mk.system(c["system"])
+        b = mk.rhs(a, rmax, c["system"], c["rhs"])
+        a.lmax1_scaling(np.ones(a.n))
+        assert sha(*a.arrays()) == c["a_sha"] and sha(b) == c["b_sha"]
+        nrm = float(np.sqrt(np.add.accumulate(b * b)[-1]))
+        hi, hd = S.Hierarchy.build_multi(a, [S.INT, S.FP64], S.amg_param(g["smoother"]))
+        ohs = []
+        for h in (hi, hd):
+            path = export(h)
+            ohs.append(O.OracleHier(path))
+            os.unlink(path)
+        bit = ohs[1].default_bits(b, nrm)
+        x, res, _ = ohs[0].solve(a.arrays(), b, nrm, m=g["m"], eps=g["eps"], bit=bit,
+                                 mode=O.RED_SEQ)
+        label = (c["system"], c["rhs"])
+        if "crashed" in c["int"]:
+            assert res.status == S.OVERFLOW, label
+            crashed += 1
+        else:
+            assert bit == c["int"]["bit"], label
+            assert (res.iters, res.converged) == (c["int"]["iters"], c["int"]["converged"]), label
+            assert sha(x) == c["int"]["x_sha"], label
+            converged += 1
+        assert c["fp64"]["converged"] == 1, label
+    assert crashed > 0 and converged > 0

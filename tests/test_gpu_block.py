@@ -162,4 +162,4 @@ def test_block_solve_batch_over_systems():
         x1, c1 = one.solve(bss[q], nbs[q])
         assert np.array_equal(xs[q].view(np.uint64), x1.view(np.uint64)), q
         assert [c.iter for c in cvs[q]] == [c.iter for c in c1], q
-        assert all(c.is_converged for c in c1), q
+        assert [c.status for c in cvs[q]] == [c.status for c in c1], q
