@@ -31,6 +31,7 @@ sys.path.insert(0, os.path.join(HERE, "sc23-ad_amd"))
 METRIC = ("solve time to 1e-8 rel. residual; int-SpMV HBM GB/s; "
           "learned/batched vs plain")
 HBM_PEAK_GBPS = 8000.0
+T_START = time.time()
 
 
 def log(*a):
@@ -115,8 +116,28 @@ def calibrate_copy(mib: int, reps: int) -> None:
     for _ in range(reps):
         S.check(S.lib.sc23ad_dev_copy(dst.p, src.p, nbytes))
     S.check(S.lib.sc23ad_dev_sync())
-    print(json.dumps({"copy_kernel": "k_copy_bytes", "bytes_read_per_launch": nbytes,
+    print(json.dumps({"copy_kernel": "k_copy_16", "bytes_read_per_launch": nbytes,
                       "bytes_written_per_launch": nbytes, "launches": reps + 1}), flush=True)
+
+
+def copy_ceiling(gib: int = 2, reps: int = 10):
+    """HBM copy ceiling of this GPU: the library's streaming copy (16-byte
+    loads and stores) of `gib` GiB, `reps` times back to back; GB/s counts
+    the bytes read and written."""
+    import sc23ad as S
+    nbytes = gib << 30
+    src = S.DevArray(nbytes=nbytes)
+    dst = S.DevArray(nbytes=nbytes)
+    S.check(S.lib.sc23ad_dev_copy(dst.p, src.p, nbytes))
+    S.check(S.lib.sc23ad_dev_sync())
+    t = time.perf_counter()
+    for _ in range(reps):
+        S.check(S.lib.sc23ad_dev_copy(dst.p, src.p, nbytes))
+    S.check(S.lib.sc23ad_dev_sync())
+    dt = (time.perf_counter() - t) / reps
+    return {"kernel": "k_copy_16 (16-byte loads and stores, grid-stride)",
+            "bytes_per_copy": 2 * nbytes, "ms_per_copy": dt * 1e3,
+            "gbps": 2 * nbytes / dt / 1e9}
 
 
 def write_system_file(path, a, b_unscaled):
@@ -138,8 +159,10 @@ def cpu_baseline(args, unscaled_arrays, b_unscaled, smoother):
     import oracle.oracle as O  # noqa: F401  (checker; cpu_baseline leg only)
     omp = smoother in (1, 2)   # BlockSOR and JacobiSOR are the OpenMP paths
     exe = O.MULTI_REF if omp else O.SEQ_REF
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
-    threads = min(threads, 16)
+    # every core of this process's CPU share (OMP_NUM_THREADS where the host
+    # sets it, else the cores this process may run on)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or \
+        len(os.sched_getaffinity(0)) or os.cpu_count() or 1
     if os.path.exists(exe):
         sysf = os.path.join("/tmp", f"sc23ad_sys_{os.getpid()}.bin")
         write_system_file(sysf, unscaled_arrays, b_unscaled)
@@ -173,8 +196,14 @@ def cpu_baseline(args, unscaled_arrays, b_unscaled, smoother):
         if rc != 0 or not tm:
             log("cpu baseline failed", rc)
             return None
+        try:
+            import psutil
+            phys = psutil.cpu_count(logical=False)
+        except Exception:  # the count is reported, never required
+            phys = None
         return {"value": float(tm.group(1)), "unit": "s",
                 "cores": threads if omp else 1, "kind": "reference",
+                "host_physical_cores": phys, "host_logical_cpus": os.cpu_count(),
                 "sample": (f"full Restarted::Solve of the same system "
                            f"({'multi.exe' if omp else 'seq.exe'}, {SMOOTHER_NAMES[smoother]},"
                            f" solver 2, m={args.m}, eps={args.rtol}); "
@@ -184,82 +213,121 @@ def cpu_baseline(args, unscaled_arrays, b_unscaled, smoother):
     return None
 
 
+def _status_counts(cvs):
+    names = {0: "converged", 3: "overflow", 4: "stagnation", 6: "not_converged"}
+    out = {}
+    for c in cvs:
+        k = names.get(int(c.status), f"status_{int(c.status)}")
+        out[k] = out.get(k, 0) + 1
+    return out
+
+
+def _conv_stats(cvs):
+    it = [int(c.iter) for c in cvs if c.is_converged]
+    return {"converged": len(it), "solves": len(cvs), "statuses": _status_counts(cvs),
+            "iterations_mean": float(np.mean(it)) if it else None,
+            "iterations_max": int(np.max(it)) if it else None}
+
+
 def run_batch(args):
     """Batched uneven systems: `--systems` 2D convection-diffusion systems
     (5-point first-order upwind; side uniform in [32, 256], Peclet in [1, 100],
     flow angle in [0, 2 pi) drawn from seed 3), 4 right-hand sides each
-    (b = A x*, x* ~ U[-1,1]), int-AMG-FGMRES(20) to rtol.  Reports systems per
-    second for all solves issued at once (each on its own stream) against the
-    same solves one at a time, and checks that both give identical x."""
+    (b = A x*, x* ~ U[-1,1] from seed 1000 (i + 1) + k, row-scaled),
+    int-AMG-FGMRES(20) to rtol with F by the driver's rule.
+
+    Batched: one block solver per system (its 4 right-hand sides share every
+    launch of the step), all systems at once (one stream each, `--threads`
+    host threads).  Plain: the same integer solves one right-hand side at a
+    time, and the FP64 solver 0 one at a time.  Rates count converged solves
+    only (systems/s = converged solves / 4 / seconds); about a tenth of this
+    family's integer solves stop with OVERFLOW, as the reference's do (its
+    process dies at gmres.hpp:244 on the same inputs)."""
     import sc23ad as S
     K, R = args.systems, 4
     ns, pes, angs = S.convdiff_draws(K, 3)
     t0 = time.time()
     prm = S.amg_param(args.smoother)
-    solvers, rhs, nrms, amgs = [], [], [], []
-    rows = 0
+    syss, rows = [], 0
     for i in range(K):
         a = S.CsrMat.generate(S.GEN_CONVDIFF2D, int(ns[i]), float(pes[i]), float(angs[i]))
         rptr, cind, val = a.arrays()
         rmax = np.maximum.reduceat(np.abs(val), rptr[:-1])
-        bs = [a.matvec(S.xstar(a.n, 1000 * (i + 1) + k)) for k in range(R)]
-        a.lmax1_scaling(bs[0].copy())
-        bs = [b / rmax for b in bs]
+        bs = [a.matvec(S.xstar(a.n, 1000 * (i + 1) + k)) / rmax for k in range(R)]
+        a.lmax1_scaling(np.ones(a.n))
         hi, hd = S.Hierarchy.build_multi(a, [S.INT, S.FP64], prm)
         am, ad = S.Amg(hi), S.Amg(hd)
-        for b in bs:
-            nrm = float(np.sqrt(np.add.accumulate(b * b)[-1]))
-            bit = ad.default_frac_bits(b, nrm)
-            solvers.append(S.Solver(S.INT, a, am, m=args.m, eps=args.rtol, bit=bit))
-            rhs.append(b)
-            nrms.append(nrm)
-        amgs.append((am, ad, a))
+        nrms = [float(np.sqrt(np.add.accumulate(b * b)[-1])) for b in bs]
+        bits = [ad.default_frac_bits(b, nb) for b, nb in zip(bs, nrms)]
+        syss.append(dict(a=a, am=am, ad=ad, bs=bs, nrms=nrms, bits=bits))
         rows += a.n
-        if (i + 1) % 32 == 0:
+        if (i + 1) % 64 == 0:
             log(f"batch setup: {i + 1}/{K} systems ({time.time() - t0:.1f} s)")
-    log(f"batch: {K} systems, {rows} rows, {len(solvers)} solves; setup {time.time() - t0:.1f} s")
-    # warm-up (first launches, code objects)
-    S.solve_batch(solvers[:16], rhs[:16], nrms[:16], threads=args.threads)
+    setup_s = time.time() - t0
+    singles = [[S.Solver(S.INT, s["a"], s["am"], m=args.m, eps=args.rtol, bit=bt)
+                for bt in s["bits"]] for s in syss]
+    fp64 = [S.Solver(S.FP64, s["a"], s["ad"], m=args.m, eps=args.rtol) for s in syss]
+    blocks = [S.BlockSolver(s["a"], s["am"], s["bits"], m=args.m, eps=args.rtol) for s in syss]
+    log(f"batch: {K} systems, {rows} rows, {K * R} right-hand sides; setup {setup_s:.1f} s")
+    # warm-up (first launches of every kernel)
+    for q in range(min(4, K)):
+        singles[q][0].solve(syss[q]["bs"][0], syss[q]["nrms"][0])
+        fp64[q].solve(syss[q]["bs"][0], syss[q]["nrms"][0])
+        blocks[q].solve(syss[q]["bs"], syss[q]["nrms"])
     t = time.perf_counter()
-    xs_seq, cvs = [], []
-    for s_, b, nb in zip(solvers, rhs, nrms):
-        x, cv = s_.solve(b, nb)
-        xs_seq.append(x)
-        cvs.append(cv)
+    xs_seq, cv_seq = [], []
+    for s, sol in zip(syss, singles):
+        for k in range(R):
+            x, cv = sol[k].solve(s["bs"][k], s["nrms"][k])
+            xs_seq.append(x)
+            cv_seq.append(cv)
     t_seq = time.perf_counter() - t
-    log(f"one at a time: {t_seq:.2f} s")
+    log(f"integer, one right-hand side at a time: {t_seq:.2f} s")
     t = time.perf_counter()
-    xs_bat, cvb = S.solve_batch(solvers, rhs, nrms, threads=args.threads)
+    xs_blk, cv_blk = S.block_solve_batch(blocks, [s["bs"] for s in syss],
+                                         [s["nrms"] for s in syss], threads=args.threads)
     t_bat = time.perf_counter() - t
-    log(f"all at once: {t_bat:.2f} s")
-    same = all(np.array_equal(p.view(np.uint64), q.view(np.uint64)) for p, q in zip(xs_seq, xs_bat))
-    conv = sum(c.is_converged for c in cvb)
-    iters = [c.iter for c in cvb]
-    rec = {
-        "metric": "batched uneven systems: systems per second (all at once vs one at a time)",
-        "value": K / t_bat, "unit": "systems/s", "n_gpus": 1, "steps": 1, "warmup": 1,
-        "ms_per_step": t_bat * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "int32/int64 (FP64 outer refinement)",
-        "data": "synthetic (2D convection-diffusion family, seed 3)",
-        "config": {"workload": f"{K} 2D convection-diffusion systems (side 32..256), {R} RHS each, "
-                               f"int-AMG-FGMRES({args.m}), rtol {args.rtol}",
+    log(f"integer, batched (block solver per system, all systems at once): {t_bat:.2f} s")
+    cv_bat = [c for cs in cv_blk for c in cs]
+    xb = [x for xs in xs_blk for x in xs]
+    same = all(np.array_equal(p.view(np.uint64), q.view(np.uint64)) for p, q in zip(xs_seq, xb))
+    same = same and [c.status for c in cv_seq] == [c.status for c in cv_bat]
+    t = time.perf_counter()
+    cv_fp = []
+    for s, sol in zip(syss, fp64):
+        for k in range(R):
+            cv_fp.append(sol.solve(s["bs"][k], s["nrms"][k])[1])
+    t_fp = time.perf_counter() - t
+    log(f"FP64 (solver 0), one at a time: {t_fp:.2f} s")
+    st_bat, st_seq, st_fp = _conv_stats(cv_bat), _conv_stats(cv_seq), _conv_stats(cv_fp)
+    rate = lambda st, tt: st["converged"] / R / tt
+    return {
+        "metric": "batched uneven systems: systems per second over converged solves",
+        "value": rate(st_bat, t_bat), "unit": "systems/s", "higher_is_better": True,
+        "config": {"workload": f"{K} 2D convection-diffusion systems (side 32..256, Peclet "
+                               f"1..100), {R} RHS each, int-AMG-FGMRES({args.m}), "
+                               f"rtol {args.rtol}, BlockSOR(40)",
                    "systems": K, "rhs_per_system": R, "rows_total": rows,
-                   "threads": args.threads or 16},
-        "one_at_a_time_systems_per_s": K / t_seq,
-        "speedup_vs_one_at_a_time": t_seq / t_bat,
-        "identical_results": bool(same), "converged": int(conv), "solves": len(cvb),
-        "iterations_mean": float(np.mean(iters)), "iterations_max": int(np.max(iters)),
+                   "threads": args.threads},
+        "batched_s": t_bat, "int_one_at_a_time_s": t_seq, "fp64_one_at_a_time_s": t_fp,
+        "batched_systems_per_s": rate(st_bat, t_bat),
+        "int_one_at_a_time_systems_per_s": rate(st_seq, t_seq),
+        "fp64_one_at_a_time_systems_per_s": rate(st_fp, t_fp),
+        "speedup_vs_int_one_at_a_time": t_seq / t_bat,
+        "speedup_vs_fp64_one_at_a_time": t_fp / t_bat,
+        "batched": st_bat, "int_one_at_a_time": st_seq, "fp64_one_at_a_time": st_fp,
+        "identical_to_one_at_a_time": bool(same), "setup_s": setup_s,
     }
-    print(json.dumps(rec), flush=True)
 
 
 def run_learned(args):
-    """Learned per-level omega / bit caps against the reference's defaults on
-    the 3D 27-point variable-coefficient diffusion system (side --n, default
-    96: coefficient exp(2 g) on 8^3-cell blocks, anisotropy 1:1:1e-3 in z,
-    seed 2), int-AMG-FGMRES(30) to rtol: seconds per solve for both, with b and
-    x resident in HBM, and the iterations; the learned setup falls back to
-    the defaults when its solve does not converge."""
+    """Learned per-level parameters against the reference's defaults on the
+    3D 27-point variable-coefficient diffusion system (side --n, default 96:
+    coefficient exp(2 g) on 8^3-cell blocks, anisotropy 1:1:1e-3 in z, seed 2),
+    int-AMG-FGMRES(30) to rtol: seconds per solve for both, with b and x
+    resident in HBM, and the iterations.  The learned solve stops after
+    STALL_STEPS steps without progress and then the defaults solve it; its
+    time then includes both."""
     import sc23ad as S
     from sc23ad import learned as LR
     n = args.n
@@ -270,15 +338,19 @@ def run_learned(args):
     log(f"diffusion27 {n}^3: N={sy.a.n}, depth {bi.info()[0]} ({time.time() - t0:.1f} s)")
     mlp = LR.load_mlp()
     ls = LR.learned_setup(bi, bd, mlp)
-    log(f"learned omega {np.round(ls.omega, 3).tolist()} bits {ls.bits.tolist()} "
+    log(f"learned omega {np.round(ls.omega, 3).tolist()} vector bits {ls.vbits.tolist()} "
         f"saturated {ls.saturated}")
     b_dev = S.DevArray(sy.b)
     x_dev = S.DevArray(np.zeros(sy.a.n))
     zeros = np.zeros(sy.a.n)
 
-    def timed(hint, hfp):
+    def make(hint, hfp, stall):
         bit = S.Amg(hfp).default_frac_bits(sy.b, sy.nrm_b)
         sol = S.Solver(S.INT, sy.a, S.Amg(hint), m=args.m, eps=args.rtol, bit=bit)
+        sol.set_stall(stall)
+        return sol
+
+    def timed(sol):
         for _ in range(args.warmup):
             x_dev.set(zeros)
             sol.solve_device(b_dev, sy.nrm_b, x_dev)
@@ -291,137 +363,174 @@ def run_learned(args):
             S.check(S.lib.sc23ad_dev_sync())
             ts.append(time.perf_counter() - t)
         rel = sol.relres(sy.b, sy.nrm_b, x_dev.get())
-        return float(np.mean(ts)), cv, rel, bit
+        return float(np.mean(ts)), cv, rel
 
-    t_def, cv_def, rel_def, bit_def = timed(bi, bd)
+    sol_d = make(bi, bd, 0)
+    t_def, cv_def, rel_def = timed(sol_d)
     log(f"default: {t_def * 1e3:.1f} ms, {cv_def.iter} iterations, relres {rel_def:.3e}")
-    t_l, cv_l, rel_l, bit_l = timed(ls.hier_int, ls.hier_fp64)
-    fallback = bool(ls.saturated > 0)
-    if not (cv_l.is_converged and cv_l.status == S.OK):
-        fallback = True
-        t_l, cv_l, rel_l, bit_l = t_def, cv_def, rel_def, bit_def
-    log(f"learned: {t_l * 1e3:.1f} ms, {cv_l.iter} iterations, relres {rel_l:.3e}, "
+    sol_l = make(ls.hier_int, ls.hier_fp64, LR.STALL_STEPS)
+    t_l, cv_l, rel_l = timed(sol_l)
+    fallback = not (cv_l.is_converged and cv_l.status == S.OK and rel_l <= args.rtol)
+    it_l = cv_l.iter
+    if fallback:  # the learned attempt's time plus the default solve
+        t_l, it_l, rel_l = t_l + t_def, cv_def.iter, rel_def
+    log(f"learned: {t_l * 1e3:.1f} ms, {it_l} iterations, relres {rel_l:.3e}, "
         f"fallback {fallback}")
-    rec = {
-        "metric": "learned per-level omega/bits vs reference defaults: solve time to 1e-8",
-        "value": t_l, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": t_l * 1e3, "higher_is_better": False, "scaling": "weak",
-        "vs_baseline": None, "dtype": "int32/int64 (FP64 outer refinement)",
-        "data": "synthetic (27-point variable-coefficient diffusion, seed 2)",
+    return {
+        "metric": "learned per-level omega / vector bits vs reference defaults: solve time "
+                  "to rtol",
+        "value": t_l, "unit": "s", "higher_is_better": False,
         "config": {"workload": f"int-AMG-FGMRES({args.m}), 3D 27-point diffusion {n}^3 "
                                f"(N={sy.a.n}), anisotropy 1:1:1e-3, rtol {args.rtol}"},
         "default_s": t_def, "default_iterations": cv_def.iter, "default_relres": rel_def,
-        "learned_iterations": cv_l.iter, "learned_relres": rel_l,
-        "learned_omega": [float(v) for v in ls.omega], "learned_bits": [int(v) for v in ls.bits],
-        "fallback_used": fallback, "gain_vs_default": t_def / t_l,
+        "learned_s": t_l, "learned_iterations": it_l, "learned_relres": rel_l,
+        "learned_omega": [float(v) for v in ls.omega],
+        "learned_vbits": [int(v) for v in ls.vbits], "saturated": int(ls.saturated),
+        "fallback_used": bool(fallback), "learned_vs_default": t_def / t_l,
     }
-    print(json.dumps(rec), flush=True)
 
 
 def run_scale(args):
     """Learned + batched at scale: the 3D 7-point diffusion system on a 160^3
     grid (4,096,000 rows; k = 1 with 64 spherical inclusions of radius
-    U[4, 12] cells and k = 1e6, seed 4), 16 right-hand sides b = A x*
-    (x* ~ U[-1,1], seeds 400 + k), row-scaled as the reference driver does.
-    Learned + batched: int32 fixed point with the network's per-level omega /
-    bit caps, all 16 solves issued at once (one stream each), any solve that
-    does not converge cleanly repeated with the default hierarchy.  Plain:
-    FP64 (solver 0) and FP32-mixed (solver 1) AMG-FGMRES on the same GPU, one
-    right-hand side after another; also the default integer solver one at a
-    time and batched, which separate the two effects.  Time to solution for
-    all right-hand sides, host arrays in and out for every method; each
-    answer is judged by its explicit FP64 residual."""
+    U[4, 12] cells and k = 1e6, seed 4), --rhs right-hand sides b = A x*
+    (x* ~ U[-1,1], seeds 400 + k), row-scaled as the reference driver does;
+    BlockSOR(40), FGMRES(30), rtol.
+
+    Plain GPU forms: FP64 (solver 0) and FP32-mixed (solver 1) one
+    right-hand side after another, and the integer solver with the default
+    parameters one after another.  Batched: the integer solver on all
+    right-hand sides in one block solver (every V-cycle, SpMV and
+    Gram-Schmidt pass of a step shared).  Learned + batched: the same block
+    solver on the hierarchy with the network's per-level weights and vector
+    bits; a right-hand side whose learned solve does not converge cleanly is
+    solved again with the defaults (counted in its time).  b, x in HBM for
+    every method.  A right-hand side counts as solved only if its explicit
+    FP64 residual is at most rtol (the same accuracy for every method)."""
     import sc23ad as S
     from sc23ad import learned as LR
     n = args.n if "--n" in sys.argv else 160
     R = args.rhs
-    # The host setup of a 4M-row system takes minutes: let it report its stages.
-    os.environ.setdefault("SC23AD_VERBOSE", "1")
     t0 = time.time()
     a = S.CsrMat.generate(S.GEN_INCLUSIONS3D, n, 64, 4.0, 12.0, 1e6, 4)
     rptr, cind, val = a.arrays()
     rmax = np.maximum.reduceat(np.abs(val), rptr[:-1])
-    bs = [a.matvec(S.xstar(a.n, 400 + k)) for k in range(R)]
-    a.lmax1_scaling(bs[0].copy())
-    bs = [b / rmax for b in bs]
+    bs = [a.matvec(S.xstar(a.n, 400 + k)) / rmax for k in range(R)]
+    a.lmax1_scaling(np.ones(a.n))
     nrms = [float(np.sqrt(np.add.accumulate(b * b)[-1])) for b in bs]
     prm = S.amg_param(args.smoother)
     hi, hd, hf = S.Hierarchy.build_multi(a, [S.INT, S.FP64, S.FP32], prm)
     log(f"inclusions {n}^3: N={a.n}, nnz={a.nnz}, depth {hi.info()[0]} "
         f"({time.time() - t0:.1f} s)")
     ls = LR.learned_setup(hi, hd, LR.load_mlp())
-    log(f"learned omega {np.round(ls.omega, 3).tolist()} bits {ls.bits.tolist()} "
+    log(f"learned omega {np.round(ls.omega, 3).tolist()} vector bits {ls.vbits.tolist()} "
         f"saturated {ls.saturated} ({time.time() - t0:.1f} s)")
     amg_i, amg_d, amg_f = S.Amg(hi), S.Amg(hd), S.Amg(hf)
     amg_l, amg_ld = S.Amg(ls.hier_int), S.Amg(ls.hier_fp64)
     bits_d = [amg_d.default_frac_bits(b, nb) for b, nb in zip(bs, nrms)]
     bits_l = [amg_ld.default_frac_bits(b, nb) for b, nb in zip(bs, nrms)]
-    sols_i = [S.Solver(S.INT, a, amg_i, m=args.m, eps=args.rtol, bit=bt) for bt in bits_d]
-    sols_l = [S.Solver(S.INT, a, amg_l, m=args.m, eps=args.rtol, bit=bt) for bt in bits_l]
     sol_d = S.Solver(S.FP64, a, amg_d, m=args.m, eps=args.rtol)
     sol_f = S.Solver(S.FP32, a, amg_f, m=args.m, eps=args.rtol)
-    log(f"setup done ({time.time() - t0:.1f} s); warm-up")
-    for sol in (sols_i[0], sols_l[0], sol_d, sol_f):
-        sol.solve(bs[0], nrms[0])
+    blk_i = S.BlockSolver(a, amg_i, bits_d, m=args.m, eps=args.rtol)
+    blk_l = S.BlockSolver(a, amg_l, bits_l, m=args.m, eps=args.rtol)
+    blk_l.set_stall(LR.STALL_STEPS)
+    setup_s = time.time() - t0
+    log(f"setup done ({setup_s:.1f} s)")
+    B = np.ascontiguousarray(np.stack(bs))
+    b_dev = S.DevArray(B.reshape(-1))
+    x_dev = S.DevArray(np.zeros(R * a.n))
+    zeros = np.zeros(R * a.n)
+    nb = np.asarray(nrms)
+    sol_d.solve(bs[0], nrms[0])  # warm-up
+    blk_i.solve(bs[:1] * R, nrms[:1] * R)
 
-    def one_by_one(sols):
+    def judge(xs):
+        return [sol_d.relres(b, nbv, x) for b, nbv, x in zip(bs, nrms, xs)]
+
+    def one_by_one(sol, label):
         t = time.perf_counter()
-        out = [sol.solve(b, nb) for sol, b, nb in zip(sols, bs, nrms)]
-        return time.perf_counter() - t, [x for x, _ in out], [c for _, c in out]
+        xs, cvs = [], []
+        for k in range(R):
+            x_dev.set(zeros)
+            cv = sol.solve_device(b_dev.view(k * a.n, a.n), nrms[k], x_dev.view(k * a.n, a.n))
+            cvs.append(cv)
+        S.check(S.lib.sc23ad_dev_sync())
+        tt = time.perf_counter() - t
+        xs = list(x_dev.get().reshape(R, a.n))
+        log(f"{label}: {tt:.3f} s, iterations {[c.iter for c in cvs]}")
+        return tt, cvs, judge(xs)
 
-    def judge(sol, xs):
-        return [sol.relres(b, nb, x) for b, nb, x in zip(bs, nrms, xs)]
+    def block(blk, label):
+        x_dev.set(zeros)
+        S.check(S.lib.sc23ad_dev_sync())
+        t = time.perf_counter()
+        cvs = blk.solve_device(b_dev, nb, x_dev)
+        S.check(S.lib.sc23ad_dev_sync())
+        tt = time.perf_counter() - t
+        xs = list(x_dev.get().reshape(R, a.n))
+        log(f"{label}: {tt:.3f} s, iterations {[c.iter for c in cvs]}, "
+            f"statuses {[c.status for c in cvs]}")
+        return tt, cvs, xs
 
     res = {}
-    t, xs, cvs = one_by_one([sol_d] * R)
-    res["fp64"] = (t, cvs, judge(sol_d, xs))
-    log(f"plain FP64, one at a time: {t:.3f} s")
-    t, xs, cvs = one_by_one([sol_f] * R)
-    res["fp32"] = (t, cvs, judge(sol_f, xs))
-    log(f"plain FP32-mixed, one at a time: {t:.3f} s")
-    t, xs, cvs = one_by_one(sols_i)
-    res["int"] = (t, cvs, judge(sols_i[0], xs))
-    log(f"integer defaults, one at a time: {t:.3f} s")
-    t = time.perf_counter()
-    xs, cvs = S.solve_batch(sols_i, bs, nrms, threads=args.threads)
-    res["int_batched"] = (time.perf_counter() - t, cvs, judge(sols_i[0], xs))
-    log(f"integer defaults, batched: {res['int_batched'][0]:.3f} s")
-    t = time.perf_counter()
-    xs, cvs = S.solve_batch(sols_l, bs, nrms, threads=args.threads)
-    redo = [k for k, c in enumerate(cvs) if not (c.is_converged and c.status == S.OK)]
+    res["fp64"] = one_by_one(sol_d, "plain FP64, one at a time")
+    res["fp32"] = one_by_one(sol_f, "plain FP32-mixed, one at a time")
+    if not args.no_int_single:
+        sols_i = [S.Solver(S.INT, a, amg_i, m=args.m, eps=args.rtol, bit=bt) for bt in bits_d]
+        t = time.perf_counter()
+        cvs = []
+        for k in range(R):
+            x_dev.set(zeros)
+            cvs.append(sols_i[k].solve_device(b_dev.view(k * a.n, a.n), nrms[k],
+                                              x_dev.view(k * a.n, a.n)))
+        S.check(S.lib.sc23ad_dev_sync())
+        tt = time.perf_counter() - t
+        res["int"] = (tt, cvs, judge(list(x_dev.get().reshape(R, a.n))))
+        log(f"integer defaults, one at a time: {tt:.3f} s")
+        del sols_i
+    tt, cvs, xs = block(blk_i, "integer defaults, batched")
+    res["int_batched"] = (tt, cvs, judge(xs))
+    tt, cvs, xs = block(blk_l, "learned + batched")
+    rel = judge(xs)
+    redo = [k for k in range(R) if not (cvs[k].is_converged and cvs[k].status == S.OK
+                                        and rel[k] <= args.rtol)]
+    t_redo = 0.0
     if redo:
-        xr, cr = S.solve_batch([sols_i[k] for k in redo], [bs[k] for k in redo],
-                               [nrms[k] for k in redo], threads=args.threads)
-        for k, x_, c_ in zip(redo, xr, cr):
-            xs[k], cvs[k] = x_, c_
-    t_lb = time.perf_counter() - t
-    res["learned_batched"] = (t_lb, cvs, judge(sols_l[0], xs))
-    log(f"learned + batched: {t_lb:.3f} s (fallbacks {len(redo)})")
+        blk_f = S.BlockSolver(a, amg_i, [bits_d[k] for k in redo], m=args.m, eps=args.rtol)
+        t = time.perf_counter()
+        xr, cr = blk_f.solve([bs[k] for k in redo], [nrms[k] for k in redo])
+        t_redo = time.perf_counter() - t
+        for q, k in enumerate(redo):
+            xs[k], cvs[k] = xr[q], cr[q]
+        rel = judge(xs)
+    res["learned_batched"] = (tt + t_redo, cvs, rel)
+    log(f"learned + batched: {tt + t_redo:.3f} s (fallbacks {len(redo)})")
 
     def summary(key):
         t, cvs, rel = res[key]
+        solved = sum(1 for c, r in zip(cvs, rel) if c.is_converged and r <= args.rtol)
         return {"s": t, "iterations": [int(c.iter) for c in cvs],
-                "converged": int(sum(c.is_converged for c in cvs)),
+                "statuses": _status_counts(cvs), "solved_at_rtol": solved,
                 "explicit_relres_max": float(max(rel))}
-    rec = {
-        "metric": "learned + batched vs plain FP64 / FP32-mixed: time to solution for "
-                  f"{R} right-hand sides to rtol {args.rtol}",
-        "value": t_lb, "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 1,
-        "ms_per_step": t_lb * 1e3, "higher_is_better": False, "scaling": "weak",
-        "vs_baseline": None, "dtype": "int32/int64 (FP64 outer refinement)",
-        "data": "synthetic (3D diffusion with 64 spherical inclusions, seed 4)",
+    out = {k: summary(k) for k in res}
+    t_lb = res["learned_batched"][0]
+    return {
+        "metric": "learned + batched vs plain FP64 / FP32-mixed / integer: time to solution "
+                  f"for {R} right-hand sides to rtol {args.rtol}",
+        "value": t_lb, "unit": "s", "higher_is_better": False,
         "config": {"workload": f"{R} RHS, 3D 7-point diffusion {n}^3 (N={a.n}, nnz={a.nnz}) "
                                f"with 64 inclusions k=1e6, FGMRES({args.m}), rtol {args.rtol}",
-                   "rhs": R, "smoother": SMOOTHER_NAMES[args.smoother],
-                   "threads": args.threads or 16},
-        "speedup_vs_fp64": res["fp64"][0] / t_lb,
-        "speedup_vs_fp32": res["fp32"][0] / t_lb,
-        "speedup_vs_int_one_at_a_time": res["int"][0] / t_lb,
+                   "rhs": R, "smoother": SMOOTHER_NAMES[args.smoother]},
+        "speedup_vs_fp64": res["fp64"][0] / t_lb, "speedup_vs_fp32": res["fp32"][0] / t_lb,
+        "speedup_vs_int_one_at_a_time": (res["int"][0] / t_lb) if "int" in res else None,
+        "speedup_batched_vs_int_one_at_a_time":
+            (res["int"][0] / res["int_batched"][0]) if "int" in res else None,
+        "all_solved_at_rtol": {k: v["solved_at_rtol"] == R for k, v in out.items()},
         "fallbacks": len(redo), "learned_saturated_levels": int(ls.saturated),
-        "learned_omega": [float(v) for v in ls.omega], "learned_bits": [int(v) for v in ls.bits],
-        **{k: summary(k) for k in res},
+        "learned_omega": [float(v) for v in ls.omega],
+        "learned_vbits": [int(v) for v in ls.vbits], "setup_s": setup_s,
+        **out,
     }
-    print(json.dumps(rec), flush=True)
 
 
 def main():
@@ -453,6 +562,13 @@ def main():
     ap.add_argument("--rhs", type=int, default=16, help="scale: right-hand sides")
     ap.add_argument("--systems", type=int, default=256, help="batch: number of systems")
     ap.add_argument("--threads", type=int, default=16, help="batch: host threads")
+    ap.add_argument("--no-int-single", action="store_true",
+                    help="scale: skip the integer solves one right-hand side at a time")
+    ap.add_argument("--extras", default="batch,learned,scale",
+                    help="laplace3d: the other workloads run after it and reported as "
+                         "extra keys of the same line (comma list; '' for none)")
+    ap.add_argument("--extras-budget", type=float, default=480.0,
+                    help="seconds since start after which no further extra is started")
     args = ap.parse_args()
     if args.workload == "batch" and args.m == 30 and "--m" not in sys.argv:
         args.m = 20
@@ -470,16 +586,19 @@ def main():
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy, 9)
         return
-    if args.workload == "batch":
-        run_batch(args)
-        return
-    if args.workload == "learned":
-        if "--n" not in sys.argv:
+    if args.workload in ("batch", "learned", "scale"):
+        if args.workload == "learned" and "--n" not in sys.argv:
             args.n = 96
-        run_learned(args)
-        return
-    if args.workload == "scale":
-        run_scale(args)
+        fn = {"batch": run_batch, "learned": run_learned, "scale": run_scale}[args.workload]
+        t = time.perf_counter()
+        rec = fn(args)
+        wall = time.perf_counter() - t
+        rec = {"metric": rec.pop("metric"), "value": rec.pop("value"), "unit": rec.pop("unit"),
+               "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": wall * 1e3,
+               "higher_is_better": rec.pop("higher_is_better"), "scaling": "weak",
+               "vs_baseline": None, "dtype": "int32/int64 (FP64 outer refinement)",
+               "data": "synthetic (generated)", **rec}
+        print(json.dumps(rec), flush=True)
         return
 
     n = args.n
@@ -573,6 +692,8 @@ def main():
         extra = {"solver0_fp64_s": t0s, "solver0_iter": cv0.iter, "solver0_relres": rel0,
                  "solver1_fp32_s": t1s, "solver1_iter": cv1.iter, "solver1_relres": rel1}
 
+    cc = copy_ceiling()
+    log(f"copy ceiling: {cc['gbps']:.0f} GB/s")
     traffic = read_traffic(os.path.join(HERE, "profiles",
                                         f"spmv_int_level0_{tag}_{n}_pmc.txt"))
     trsv_traffic = read_traffic(os.path.join(HERE, "profiles",
@@ -630,6 +751,7 @@ def main():
                             "achieved": vc_achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                             "frac": (vc_achieved / HBM_PEAK_GBPS) if vc_achieved else None,
                             "algorithmic_bytes": vc_bytes, "mean_ms": vc_ms},
+        "copy_ceiling": cc,
         "cpu_baseline": cpu,
         "setup_s": {"host": setup_host, "gpu_interp_galerkin": setup_dev,
                     "identical": bool(same_setup), "precisions": len(precs)},
@@ -637,7 +759,46 @@ def main():
     rec.update(extra)
     if cpu and cpu.get("value"):
         rec["speedup_vs_cpu"] = cpu["value"] / t_int
+    if args.workload == "laplace3d" and args.n == 128:
+        run_extras(args, rec)
     print(json.dumps(rec), flush=True)
+
+
+def run_extras(args, rec):
+    """The metric's third part (learned / batched vs plain) on its own
+    workloads, after the main line's measurement: each as an extra key of the
+    same JSON line, each started only while the time budget lasts."""
+    import copy
+    wanted = [w for w in args.extras.split(",") if w]
+    for w in wanted:
+        elapsed = time.time() - T_START
+        if elapsed > args.extras_budget:
+            rec[f"extra_{w}"] = {"skipped": f"time budget ({elapsed:.0f} s > "
+                                            f"{args.extras_budget:.0f} s)"}
+            log(f"extra {w}: skipped, {elapsed:.0f} s elapsed")
+            continue
+        a = copy.copy(args)
+        a.steps, a.warmup = 3, 1
+        try:
+            if w == "batch":
+                a.m = 20
+                r = run_batch(a)
+                rec["batched_uneven"] = r
+                rec["batched_systems_per_s"] = r["batched_systems_per_s"]
+            elif w == "learned":
+                a.n = 96
+                r = run_learned(a)
+                rec["learned"] = r
+                rec["learned_vs_default"] = r["learned_vs_default"]
+            elif w == "scale":
+                a.n = 160
+                a.no_int_single = True
+                r = run_scale(a)
+                rec["learned_batched_at_scale"] = r
+        except Exception as e:  # reported, never fatal to the main line
+            rec[f"extra_{w}"] = {"error": repr(e)}
+            log(f"extra {w} failed: {e!r}")
+        log(f"extra {w} done at {time.time() - T_START:.0f} s")
 
 
 if __name__ == "__main__":

@@ -283,6 +283,14 @@ int sc23ad_solver_trace(sc23ad_solver* s, uint64_t* buf, int cap, int* len);
 int sc23ad_solver_time_spmv(sc23ad_solver* s, int enable);
 int sc23ad_solver_spmv_stats(sc23ad_solver* s, int* launches, double* mean_ms);
 int sc23ad_solver_trsv_stats(sc23ad_solver* s, int* launches, double* mean_ms);
+/* Stall stop (no counterpart in the reference, whose only early exit is
+   Restarted's stagnation test, irsolver.hpp:35): with steps > 0 a solve whose
+   implicit residual has not fallen below its best value by a relative 1e-3
+   for `steps` consecutive Arnoldi steps stops with SC23AD_STAGNATION and
+   leaves x at its last restart.  0 (the default) keeps the reference's
+   behaviour.  The learned setup uses it to fall back to the defaults early. */
+int sc23ad_solver_set_stall(sc23ad_solver* s, int steps);
+int sc23ad_block_solver_set_stall(sc23ad_block_solver* s, int steps);
 /* |e_{j+1}| / nrm_b after every Arnoldi step of the last solve (the
    reference's "# [j]" convergence history, gmres.hpp:253-255). */
 int sc23ad_solver_history(sc23ad_solver* s, double* buf, int cap, int* len);
@@ -296,13 +304,16 @@ int sc23ad_solver_relres(sc23ad_solver* s, const double* b, double nrm_b,
 int sc23ad_default_frac_bits(sc23ad_amg* m_fp64, const double* b, double nrm_b,
                              int* bits);
 
-/* ---- learned per-level smoother parameters --------------------------------
+/* ---- learned per-level parameters ------------------------------------------
    No counterpart in the reference, which uses one omega for every level
-   (SORParam / BlockSORParam, Work/main.cpp:82-83) and the CSRflex bit rule
-   (matrix/spmat.hpp:108).  A small network (8 -> 32 -> 32 -> 2, tanh,
-   sigmoid outputs) maps per-level matrix features to omega_l in [0.6, 1.9]
-   and a cap bits_l in [8, 30] on the fractional bits of A_l, U_l and LD_l's
-   off-diagonal part; the integer kernels are unchanged. */
+   (SORParam / BlockSORParam, Work/main.cpp:82-83) and one fractional bit
+   length F for every vector (the FGMRESflex ctor, gmres.hpp:194-195; rule
+   main.cpp:153-160).  A small network (8 -> 32 -> 32 -> 2, tanh, sigmoid
+   outputs) maps 8 features of each level's operator to the level's smoother
+   weight omega_l in [0.6, 1.9] and the fractional bits of the next coarser
+   level's vectors relative to F, in [-21, 1]; the integer kernels are
+   unchanged (sc23ad_hier_retune_levels folds the bits into the R / P
+   shifts). */
 typedef struct sc23ad_mlp_s sc23ad_mlp;
 /* Hierarchy with other smoother parameters from base's coarsening (no
    re-coarsening): level_omega / level_bits (nlevels entries, either may be
@@ -340,9 +351,11 @@ int sc23ad_mlp_grad(sc23ad_mlp* m, const float* x, const float* t, int batch, fl
 /* `steps` full-batch Adam steps on the GPU; *loss = loss afterwards. */
 int sc23ad_mlp_train(sc23ad_mlp* m, const float* x, const float* t, int batch, int steps,
                      float lr, float* loss);
-/* The network's omega_l and bits_l for every level of h (cap entries). */
+/* The network's omega_l (levels 0 .. nlevels-1) and vector bits
+   (level_vbits[l] for levels 0 .. nlevels, level_vbits[0] = 0) for h;
+   cap: room in level_omega (level_vbits holds cap + 1).  Either may be NULL. */
 int sc23ad_learned_levels(sc23ad_mlp* m, const sc23ad_hier* h, double* level_omega,
-                          int* level_bits, int cap, int* nlevels);
+                          int* level_vbits, int cap, int* nlevels);
 /* Order-free checksum used by the traces: sum_i mix64(i * 0x9E3779B97F4A7C15 ^
    (uint32)v_i) mod 2^64, mix64 = splitmix64's finaliser.  Host arrays. */
 uint64_t sc23ad_checksum_i32(const int32_t* v, int n);

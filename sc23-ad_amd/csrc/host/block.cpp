@@ -316,6 +316,7 @@ void DevBlockSolver::solve(const double* b_dev, const double* nrm_b, double* x_d
     history[q].clear();
   }
   std::vector<int> active(K, 0), iter(count, 0), none(K, 0);
+  std::vector<StallWatch> stall(count);
   for (int q = 0; q < count; ++q) active[q] = 1;
   char* Vb = V.as<char>();
   char* Zb = Z.as<char>();
@@ -342,7 +343,7 @@ void DevBlockSolver::solve(const double* b_dev, const double* nrm_b, double* x_d
     hip_check(dev::launch_v0_blk(n, groups, r.as<double>(), Vb, kst.as<dev::KState>(),
                                  live.as<int>(), s),
               "v0");
-    std::vector<int> running = cyc, conv(count, 0), ovf(count, 0);
+    std::vector<int> running = cyc, conv(count, 0), ovf(count, 0), stalled(count, 0);
     for (int j = 0; j < m; ++j) {
       void* vj = Vb + vs * j;
       void* zj = Zb + vs * j;
@@ -366,6 +367,7 @@ void DevBlockSolver::solve(const double* b_dev, const double* nrm_b, double* x_d
                 "scal");
       read_flags();
       int still = 0;
+      bool dropped = false;
       for (int q = 0; q < count; ++q) {
         if (!running[q]) continue;
         double rel = 0;
@@ -377,11 +379,17 @@ void DevBlockSolver::solve(const double* b_dev, const double* nrm_b, double* x_d
         } else if (host_flags[6 * q]) {
           conv[q] = j + 1;
           running[q] = 0;
+        } else if (stall[q].step(rel, stall_steps)) {
+          stalled[q] = 1;  // abandoned: leaves the cycle, x not updated
+          running[q] = 0;
+          cyc[q] = 0;
+          dropped = true;
         } else {
           ++still;
         }
       }
       if (still == 0) break;
+      if (dropped) put_masks(cyc, none);
     }
     std::vector<int> up(K, 0);
     for (int q = 0; q < count; ++q) up[q] = cyc[q] && !ovf[q];
@@ -391,6 +399,11 @@ void DevBlockSolver::solve(const double* b_dev, const double* nrm_b, double* x_d
               "update x");
     int left = 0;
     for (int q = 0; q < count; ++q) {
+      if (stalled[q]) {
+        res[q].status = 4;
+        active[q] = 0;
+        continue;
+      }
       if (!cyc[q]) continue;
       if (ovf[q]) {
         res[q].status = 3;

@@ -517,6 +517,18 @@ int sc23ad_solver_set_reduction(sc23ad_solver* s, int sequential) {
   return SC23AD_OK;
 }
 
+int sc23ad_solver_set_stall(sc23ad_solver* s, int steps) {
+  if (!s || steps < 0) return fail(SC23AD_BAD_PARAM, "set_stall: bad arguments");
+  s->s->opts.stall_steps = steps;
+  return SC23AD_OK;
+}
+
+int sc23ad_block_solver_set_stall(sc23ad_block_solver* s, int steps) {
+  if (!s || steps < 0) return fail(SC23AD_BAD_PARAM, "block set_stall: bad arguments");
+  s->s->stall_steps = steps;
+  return SC23AD_OK;
+}
+
 int sc23ad_solver_history(sc23ad_solver* s, double* buf, int cap, int* len) {
   if (!s || !len) return fail(SC23AD_BAD_PARAM, "history: null");
   const auto& h = s->s->history;
@@ -702,21 +714,22 @@ int sc23ad_mlp_train(sc23ad_mlp* m, const float* x, const float* t, int batch, i
 }
 
 int sc23ad_learned_levels(sc23ad_mlp* m, const sc23ad_hier* h, double* level_omega,
-                          int* level_bits, int cap, int* nlevels) {
+                          int* level_vbits, int cap, int* nlevels) {
   if (!m || !h || !nlevels) return fail(SC23AD_BAD_PARAM, "learned_levels: null");
   int nl = 0;
   int rc = sc23ad_hier_features(h, nullptr, 0, &nl);
   if (rc != SC23AD_OK) return rc;
   *nlevels = nl;
-  if (nl == 0 || (!level_omega && !level_bits)) return SC23AD_OK;
+  if (nl == 0 || (!level_omega && !level_vbits)) return SC23AD_OK;
   std::vector<float> f(static_cast<size_t>(nl) * kLevelFeatures), y(static_cast<size_t>(nl) * kMlpOut);
   rc = sc23ad_hier_features(h, f.data(), nl, &nl);
   if (rc != SC23AD_OK) return rc;
   rc = sc23ad_mlp_forward(m, f.data(), nl, y.data());
   if (rc != SC23AD_OK) return rc;
+  if (level_vbits && cap > 0) level_vbits[0] = 0;  // the fine level keeps the solver's F
   for (int l = 0; l < nl && l < cap; ++l) {
     if (level_omega) level_omega[l] = mlp_omega(y[2 * l]);
-    if (level_bits) level_bits[l] = mlp_bits(y[2 * l + 1]);
+    if (level_vbits && l + 1 <= cap) level_vbits[l + 1] = mlp_vbits(y[2 * l + 1]);
   }
   return SC23AD_OK;
 }

@@ -657,6 +657,55 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
         L.rp_lanes = lanes;
         L.rp_waves = rw;
         L.rp_farq = farq;
+        // The block kernel (four right-hand sides per workgroup) keeps four
+        // values per ring slot, so its ring may be shorter than this one:
+        // the shortest that holds every dependency fewer than kEpFarLevels
+        // levels back, the others read from global memory, far entries of a
+        // row first (one-lane layouts).
+        if (lanes == 1 && rows == 1 &&
+            dev::rpb_lds_bytes(L.ring, maxlev_chain) > dev::kTriBlockedMaxLds) {
+          long long near = 0;
+          for (size_t Lv = 0; Lv + 1 < lev_ptr.size(); ++Lv)
+            for (int idx = lev_ptr[Lv]; idx < lev_ptr[Lv + 1]; ++idx)
+              for (int q = lbeg[idx]; q < lbeg[idx + 1]; ++q)
+                if (lev_of[idx] - lev_of[lpos[q]] < dev::kEpFarLevels)
+                  near = std::max<long long>(near, static_cast<long long>(lev_ptr[Lv + 1]) -
+                                                       lpos[q]);
+          int ring_b = 2;
+          while (ring_b < near) ring_b *= 2;
+          if (dev::rpb_lds_bytes(ring_b, maxlev_chain) <= dev::kTriBlockedMaxLds) {
+            std::vector<int> rb(static_cast<size_t>(std::max(n, 1)) * e * 2, 0);
+            int farq_b = 0;
+            bool any_far_b = false;
+            for (size_t Lv = 0; Lv + 1 < lev_ptr.size(); ++Lv)
+              for (int idx = lev_ptr[Lv]; idx < lev_ptr[Lv + 1]; ++idx) {
+                std::vector<int> ord, code;
+                for (int q = lbeg[idx]; q < lbeg[idx + 1]; ++q)
+                  if (static_cast<long long>(lev_ptr[Lv + 1]) - lpos[q] > ring_b) {
+                    ord.push_back(q);
+                    code.push_back(-(lcol[q] + 1));
+                  }
+                farq_b = std::max(farq_b, static_cast<int>(ord.size()));
+                any_far_b = any_far_b || !ord.empty();
+                for (int q = lbeg[idx]; q < lbeg[idx + 1]; ++q)
+                  if (static_cast<long long>(lev_ptr[Lv + 1]) - lpos[q] <= ring_b) {
+                    ord.push_back(q);
+                    code.push_back(lpos[q]);
+                  }
+                for (size_t mm = 0; mm < ord.size(); ++mm) {
+                  const size_t at = (static_cast<size_t>(idx) * e + mm) * 2;
+                  rb[at] = code[mm];
+                  rb[at + 1] = static_cast<int>(lval[ord[mm]]);
+                }
+              }
+            plan.rent_blk.alloc(sizeof(int) * rb.size());
+            plan.rent_blk.upload(rb.data(), sizeof(int) * rb.size());
+            L.rent_blk = plan.rent_blk.get();
+            L.ring_blk = ring_b;
+            L.farq_blk = farq_b;
+            L.far_blk = any_far_b ? 1 : 0;
+          }
+        }
       }
     }
   }
@@ -715,6 +764,59 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
         int rw = 1;
         while (rw < 8 && static_cast<size_t>(rw) * 64 < maxlev_rows) rw *= 2;
         L.rp_waves = rw;
+      }
+    } else if (kind == TriKind::kLD && plan.identity && one_launch_chains && md > 8 &&
+               md <= 16 * 8 && lds <= dev::kTriBlockedMaxLds &&
+               !(env_rp && env_rp[0] == '0')) {
+      // several lanes per row (k_tri_chain_rpg): kG chunks of kE entries, the
+      // fewest waves (at most 8), then the fewest padded slots; every
+      // dependency in the ring
+      int best_g = 0, best_e = 0;
+      size_t best_key = 0;
+      for (int g = 2; g <= 16; g *= 2)
+        for (int ge : {4, 8}) {
+          if (g * ge < md) continue;
+          const size_t w = (maxlev_rows * g + 63) / 64;
+          const size_t key = w * 4096 + static_cast<size_t>(g * ge);
+          if (w <= 8 && (best_g == 0 || key < best_key)) {
+            best_key = key;
+            best_g = g;
+            best_e = ge;
+          }
+        }
+      bool fits = best_g > 0;
+      for (size_t Lv = 0; Lv + 1 < lev_ptr.size() && fits; ++Lv)
+        for (int idx = lev_ptr[Lv]; idx < lev_ptr[Lv + 1] && fits; ++idx)
+          for (int q = lbeg[idx]; q < lbeg[idx + 1]; ++q)
+            if (static_cast<long long>(lev_ptr[Lv + 1]) - lpos[q] > fring) fits = false;
+      if (fits) {
+        const size_t slots = static_cast<size_t>(best_g) * best_e;
+        std::vector<int> rdep(static_cast<size_t>(std::max(n, 1)) * slots, 0),
+            rcnt(static_cast<size_t>(std::max(n, 1)), 0);
+        std::vector<V> rval(rdep.size(), V(0));
+        for (int idx = 0; idx < n; ++idx) {
+          rcnt[idx] = lbeg[idx + 1] - lbeg[idx];
+          for (int q = lbeg[idx]; q < lbeg[idx + 1]; ++q) {
+            const size_t at = static_cast<size_t>(idx) * slots + (q - lbeg[idx]);
+            rdep[at] = lpos[q];  // entry m: lane m / kE, slot m % kE
+            rval[at] = lval[q];
+          }
+        }
+        plan.rpf_dep.alloc(sizeof(int) * rdep.size());
+        plan.rpf_dep.upload(rdep.data(), sizeof(int) * rdep.size());
+        plan.rpf_val.alloc(sizeof(V) * rval.size());
+        plan.rpf_val.upload(rval.data(), sizeof(V) * rval.size());
+        plan.rpf_cnt.alloc(sizeof(int) * rcnt.size());
+        plan.rpf_cnt.upload(rcnt.data(), sizeof(int) * rcnt.size());
+        L.rpf_dep = plan.rpf_dep.as<int>();
+        L.rpf_val = plan.rpf_val.get();
+        L.rpf_cnt = plan.rpf_cnt.as<int>();
+        L.ring = fring;
+        L.ep_far = 0;
+        L.ep_maxlev = maxlev_chain;
+        L.rp_e = best_e;
+        L.rp_lanes = best_g;
+        L.rp_waves = static_cast<int>((maxlev_rows * best_g + 63) / 64);
       }
     }
   }
@@ -1222,14 +1324,25 @@ void DevAmg::vcycle(const void* in, void* out, Work& w) const {
             "permute out");
 }
 
-void DevAmg::level_op(int level, int op, const void* in, void* out, Work& w) const {
+void DevAmg::level_op(int level, int op, const void* in, void* out, Work& w, void* ti_p,
+                      void* to_p) const {
   // Numbering of the operand and of the result (nullptr: original order).
   const int* pin = op == 7 ? nullptr : op == 3 ? perm_of(level + 1) : perm_of(level);
   const int* pout = op == 7 ? nullptr : op == 2 ? perm_of(level + 1) : perm_of(level);
   const int nin = op == 3 ? level_n(level + 1) : level_n(level);
   const int nout = op == 2 ? level_n(level + 1) : level_n(level);
   const size_t es = esize();
-  DevBuf ti(es * std::max(nin, 1)), to(es * std::max(nout, 1));
+  struct Staging {  // caller's buffer, or one allocated for this call
+    DevBuf own;
+    void* p;
+    Staging(void* given, size_t bytes) : p(given) {
+      if (!p) {
+        own.alloc(bytes);
+        p = own.get();
+      }
+    }
+    void* get() const { return p; }
+  } ti(ti_p, es * std::max(nin, 1)), to(to_p, es * std::max(nout, 1));
   auto to_level = [&](const int* p, int n, const void* src, void* dst) {
     if (p) hip_check(dev::launch_permute(elem, n, p, src, dst, true, lib_stream()), "permute");
     else hip_check(hipMemcpyAsync(dst, src, es * n, hipMemcpyDeviceToDevice, lib_stream()), "copy");
@@ -1464,7 +1577,9 @@ DevSolver::Result DevSolver::solve(const double* b_dev, double nrm_b, double* x_
   char* Zb = Z.as<char>();
   int iter = 0;
   res.status = 6;
-  for (int outer = 0; outer < max_outer; ++outer) {
+  StallWatch stall;
+  bool stalled = false;
+  for (int outer = 0; outer < max_outer && !stalled; ++outer) {
     hip_check(dev::launch_outer_residual(a64.d, b_dev, x_dev, r.as<double>(), k, elem, seq, s),
               "outer residual");
     read_flags();
@@ -1493,7 +1608,8 @@ DevSolver::Result DevSolver::solve(const double* b_dev, double nrm_b, double* x_
         hip_check(dev::launch_axpy_dot(elem, n, Vb, w, k, kk, j, seq, s), "axpy_dot");
       hip_check(dev::launch_scal(elem, n, w, k, j, seq, s), "scal");
       read_flags();
-      history.push_back(*reinterpret_cast<double*>(host_flags + 8));
+      const double rel = *reinterpret_cast<double*>(host_flags + 8);
+      history.push_back(rel);
       steps = j + 1;
       if (host_flags[1]) {
         ovf = true;
@@ -1503,10 +1619,18 @@ DevSolver::Result DevSolver::solve(const double* b_dev, double nrm_b, double* x_
         conv = true;
         break;
       }
+      if (stall.step(rel, opts.stall_steps)) {
+        stalled = true;
+        break;
+      }
     }
     if (ovf) {
       res.status = 3;
       res.overflow = 1;
+      break;
+    }
+    if (stalled) {  // abandoned: no update of x, STAGNATION
+      res.status = 4;
       break;
     }
     hip_check(dev::launch_update_x(elem, n, Zb, x_dev, k, steps, s), "update x");

@@ -7,6 +7,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <memory>
 #include <string>
 #include <vector>
@@ -104,6 +105,7 @@ struct TriPlan {
   // level-ordered layout used by the level-scheduled kernels
   DevBuf lrow, lbeg, lcol, lval, ldiag, ecol, eval, lpos, epos, lent, lev_eptr, rent;
   DevBuf rpf_dep, rpf_val, rpf_cnt;  // FP row-parallel layout
+  DevBuf rent_blk;                    // row-parallel entries for the block kernel's ring
   dev::TriLayout layout;
 };
 
@@ -188,8 +190,11 @@ class DevAmg {
   void coarse(const void* in, void* out, Work& w) const;
   // One V-cycle on vectors in the original numbering.
   void vcycle(const void* in, void* out, Work& w) const;
-  // Level operation (sc23ad_level_op) on vectors in the original numbering.
-  void level_op(int level, int op, const void* in, void* out, Work& w) const;
+  // Level operation (sc23ad_level_op) on vectors in the original numbering;
+  // ti / to: staging vectors of the operand and the result in the level's
+  // numbering (at least es * n bytes each; nullptr: allocated here).
+  void level_op(int level, int op, const void* in, void* out, Work& w,
+                void* ti = nullptr, void* to = nullptr) const;
   // Algorithmic HBM bytes of one V-cycle (sum over its kernels).
   double vcycle_bytes() const;
   // Throws if a kernel with a bounded wait gave up (after a stream sync);
@@ -235,6 +240,30 @@ void upload_fp64_operator(DevMat& dm, const HostCsr& a);
 
 struct SolverOpts {
   int seq_reduction = 0;  // 1: FP reductions strictly left to right
+  // > 0: stop with STAGNATION once the implicit residual has not fallen
+  // below its best value by a relative 1e-3 for this many consecutive
+  // Arnoldi steps (the learned setup's fallback trigger); 0: never
+  int stall_steps = 0;
+};
+
+// Stall test of one solve: steps since the implicit residual last fell below
+// its best value by a relative kStallGain.
+struct StallWatch {
+  static constexpr double kStallGain = 1e-3;
+  double best = 0;
+  int since = 0;
+  bool started = false;
+  void reset() { started = false; since = 0; }
+  // true when the solve has stalled for `steps` steps (steps <= 0: never)
+  bool step(double rel, int steps) {
+    if (!started || rel < best * (1.0 - kStallGain)) {
+      best = started ? std::min(best, rel) : rel;
+      started = true;
+      since = 0;
+      return false;
+    }
+    return steps > 0 && ++since >= steps;
+  }
 };
 
 class DevSolver {
@@ -296,6 +325,7 @@ class DevBlockSolver {
   std::vector<dev::KState> ks;
   int* host_flags = nullptr;  // pinned: 6 words per right-hand side
   int* host_mask = nullptr;   // pinned: live then upd masks
+  int stall_steps = 0;        // see SolverOpts::stall_steps
   bool tracing = false;
   int trace_cap = 0;
   std::vector<std::vector<double>> history;

@@ -3,6 +3,8 @@
 // (csrc/kernels/mlp.hip); this header is host-only.
 #pragma once
 
+#include <cmath>
+
 #include <cstdint>
 
 #include "csr.hpp"
@@ -29,5 +31,13 @@ constexpr int kMlpParams = kMlpHidden * kMlpIn + kMlpHidden + kMlpHidden * kMlpH
 void mlp_init(uint64_t seed, float* params);
 inline double mlp_omega(float y0) { return 0.6 + 1.3 * static_cast<double>(y0); }
 inline int mlp_bits(float y1) { return static_cast<int>(8.0 + 22.0 * static_cast<double>(y1) + 0.5); }
+// Second output -> fractional bits of the next coarser level's vectors
+// relative to the fine level's F: [-21, 1] (F_{l+1} in [8, 30] for F = 29).
+constexpr int kVbitsLo = -21, kVbitsHi = 1;
+inline int mlp_vbits(float y1) {
+  const int v = static_cast<int>(std::floor(kVbitsLo + (kVbitsHi - kVbitsLo) *
+                                                           static_cast<double>(y1) + 0.5));
+  return v < kVbitsLo ? kVbitsLo : v > kVbitsHi ? kVbitsHi : v;
+}
 
 }  // namespace sc23ad

@@ -612,7 +612,21 @@ hipError_t launch_spmv_blk(const DCsr& a, int groups, SpmvMode mode, const void*
   return spmv_blk_mode<2>(a, groups, x, z, y, s);
 }
 
-bool trsv_rp_blk_supported(const TriLayout& L) {
+// The layout the block kernel reads: the row-parallel one, with its own
+// shorter ring where the plan built one.
+static TriLayout block_layout(const TriLayout& L) {
+  TriLayout b = L;
+  if (L.rent_blk) {
+    b.rent = L.rent_blk;
+    b.ring = L.ring_blk;
+    b.ep_far = L.far_blk;
+    b.rp_farq = L.farq_blk;
+  }
+  return b;
+}
+
+bool trsv_rp_blk_supported(const TriLayout& Lin) {
+  const TriLayout L = block_layout(Lin);
   if (L.rp_waves <= 0 || !L.rent) return false;
   if (rpb_lds_bytes(L.ring, L.ep_maxlev) > kTriBlockedMaxLds) return false;
   if (L.rp_lanes <= 1)
@@ -623,9 +637,10 @@ bool trsv_rp_blk_supported(const TriLayout& L) {
          (L.rp_lanes == 2 || L.rp_lanes == 4 || L.rp_lanes == 8 || L.rp_lanes == 16);
 }
 
-hipError_t launch_trsv_rp_blk(const TriLayout& L, const TriSegment& seg, int groups, int bit,
+hipError_t launch_trsv_rp_blk(const TriLayout& Lin, const TriSegment& seg, int groups, int bit,
                               int dbit, const void* in, void* out, hipStream_t s) {
-  if (!trsv_rp_blk_supported(L)) return hipErrorInvalidValue;
+  if (!trsv_rp_blk_supported(Lin)) return hipErrorInvalidValue;
+  const TriLayout L = block_layout(Lin);
   const int4* vin = static_cast<const int4*>(in);
   int4* vout = static_cast<int4*>(out);
   if (L.ep_far) return rpb_layout<true>(L, seg, groups, bit, dbit, vin, vout, s);

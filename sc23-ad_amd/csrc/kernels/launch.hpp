@@ -113,6 +113,12 @@ struct TriLayout {
   int rp_rows = 0;
   int rp_lanes = 1;
   int rp_farq = 0;   // most far entries in one row (stored first; one-lane layouts)
+  // the same entries for the block kernel where its ring (four values per
+  // slot) must be shorter: ring_blk slots, far flag and far count per row
+  const void* rent_blk = nullptr;
+  int ring_blk = 0;
+  int far_blk = 0;
+  int farq_blk = 0;
   // FP row-parallel mode (one lane per row, rp_e entries in stored order):
   // dependency codes and values by position, and each row's entry count
   const int* rpf_dep = nullptr;

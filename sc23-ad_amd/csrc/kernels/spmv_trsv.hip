@@ -1556,6 +1556,161 @@ hipError_t tri_rpf_waves(const TriLayout& L, const TriSegment& seg, const T* in,
   return tri_rpf_layout<T, 8>(L, seg, in, out, s);
 }
 
+// FP row-parallel LD solve with several lanes per row (solvers 0 and 1;
+// factors of up to kG * kE dependencies per row, no far entries): a row's
+// entries are split into kG contiguous chunks of kE, lane g holding entries
+// g kE .. g kE + kE - 1.  Every lane forms its products val * x in parallel;
+// the row's first lane then takes them in stored order (lane by lane, slot
+// by slot, through cross-lane moves) and subtracts them one at a time, as
+// the sequential solve does, so the result is the reference's bit for bit.
+// Chains, levels, LDS ring and prefetch as in k_tri_chain_rpf; waves per
+// workgroup are a launch parameter.
+template <typename T, int kG, int kE>
+struct RpgPre {
+  int lo, hi;
+  int dep[kE];
+  T val[kE];
+  T dinv, in;
+  int cnt;
+};
+
+template <typename T, int kG, int kE>
+__device__ __forceinline__ void rpg_load(RpgPre<T, kG, kE>& r, int lo, int hi, int threads,
+                                         const int* __restrict__ rdep, const T* __restrict__ rval,
+                                         const int* __restrict__ rcnt,
+                                         const T* __restrict__ dinv, const T* __restrict__ in) {
+  const int t = threadIdx.x;
+  const int idx = lo + t / kG;
+  const int id = idx < hi ? idx : 0;  // lanes past the level's end load position 0
+  r.lo = lo;
+  r.hi = hi;
+  const size_t base = (static_cast<size_t>(id) * kG + t % kG) * kE;
+#pragma unroll
+  for (int q = 0; q < kE; ++q) {
+    r.dep[q] = rdep[base + q];
+    r.val[q] = rval[base + q];
+  }
+  r.cnt = rcnt[id];
+  r.dinv = dinv[id];
+  r.in = in[id];
+  (void)threads;
+}
+
+template <typename T, TriKind K, int kG, int kE, int kD>
+__device__ __forceinline__ void rpg_level(RpgPre<T, kG, kE>& cur, int L, int nl, int threads,
+                                          const int* lvo, T* ring, int rmask,
+                                          const int* __restrict__ rdep,
+                                          const T* __restrict__ rval,
+                                          const int* __restrict__ rcnt,
+                                          const T* __restrict__ dinv,
+                                          const T* __restrict__ in, T* out) {
+  const int Lp = min(L + kD, nl);
+  const int plo = lvo[Lp], phi = lvo[min(Lp + 1, nl)];
+  const int t = threadIdx.x;
+  const int g = t % kG;
+  const int lane0 = (t & 63) - g;  // the row's first lane inside the wave
+  T prod[kE];
+#pragma unroll
+  for (int q = 0; q < kE; ++q) prod[q] = cur.val[q] * ring[cur.dep[q] & rmask];
+  T tt = cur.in;
+#pragma unroll
+  for (int h = 0; h < kG; ++h)
+#pragma unroll
+    for (int q = 0; q < kE; ++q) {
+      const T v = __shfl(prod[q], lane0 + h, 64);
+      tt = h * kE + q < cur.cnt ? tt - v : tt;
+    }
+  const int k = cur.lo + t / kG;
+  if (k < cur.hi && g == 0) {
+    const T v = acc_final<T, K>(tt, cur.dinv, 0, 0);
+    ring[k & rmask] = v;
+    out[k] = v;
+  }
+  lds_barrier();
+  rpg_load<T, kG, kE>(cur, plo, phi, threads, rdep, rval, rcnt, dinv, in);
+}
+
+template <typename T, TriKind K, int kG, int kE, int kD>
+__global__ __launch_bounds__(512) void k_tri_chain_rpg(
+    const int* __restrict__ rdep, const T* __restrict__ rval, const int* __restrict__ rcnt,
+    const T* __restrict__ dinv, const int* __restrict__ lev_ptr,
+    const int* __restrict__ chain_lev, int rmask, const T* __restrict__ in, T* out) {
+  extern __shared__ __align__(16) unsigned char smem_g[];
+  T* ring = reinterpret_cast<T*>(smem_g);
+  int* lvo = reinterpret_cast<int*>(ring + rmask + 1);
+  const int threads = static_cast<int>(blockDim.x);
+  const int t = threadIdx.x;
+  const int lb = chain_lev[2 * blockIdx.x], le = chain_lev[2 * blockIdx.x + 1];
+  const int nl = le - lb;
+  // the ring starts zeroed: a skipped entry's slot never holds a leftover
+  // bit pattern (its product is formed but not subtracted)
+  for (int k = t; k <= rmask; k += threads) ring[k] = T(0);
+  for (int k = t; k <= nl; k += threads) lvo[k] = lev_ptr[lb + k];
+  __syncthreads();
+  RpgPre<T, kG, kE> buf[kD];
+#pragma unroll
+  for (int d = 0; d < kD; ++d) {
+    const int Lc = min(d, nl);
+    rpg_load<T, kG, kE>(buf[d], lvo[Lc], lvo[min(Lc + 1, nl)], threads, rdep, rval, rcnt, dinv,
+                        in);
+  }
+  int L0 = 0;
+  for (; L0 + kD <= nl; L0 += kD) {
+#pragma unroll
+    for (int d = 0; d < kD; ++d)
+      rpg_level<T, K, kG, kE, kD>(buf[d], L0 + d, nl, threads, lvo, ring, rmask, rdep, rval,
+                                  rcnt, dinv, in, out);
+  }
+#pragma unroll
+  for (int d = 0; d < kD; ++d)
+    if (L0 + d < nl)
+      rpg_level<T, K, kG, kE, kD>(buf[d], L0 + d, nl, threads, lvo, ring, rmask, rdep, rval,
+                                  rcnt, dinv, in, out);
+}
+
+template <typename T, int kG, int kE>
+hipError_t tri_rpg_go(const TriLayout& L, const TriSegment& seg, const T* in, T* out,
+                      hipStream_t s) {
+  constexpr TriKind K = TriKind::kLD;
+  const size_t lds = sizeof(T) * static_cast<size_t>(L.ring) +
+                     sizeof(int) * (static_cast<size_t>(L.ep_maxlev) + 1);
+  if (lds > static_cast<size_t>(kTriBlockedMaxLds) || L.rp_waves < 1 || L.rp_waves > 8)
+    return hipErrorInvalidValue;
+  auto kern = &k_tri_chain_rpg<T, K, kG, kE, 4>;
+  static bool attr = false;
+  if (!attr) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       static_cast<int>(kTriBlockedMaxLds));
+    if (e != hipSuccess) return e;
+    attr = true;
+  }
+  hipLaunchKernelGGL(kern, dim3(seg.nchain), dim3(64 * L.rp_waves), static_cast<uint32_t>(lds),
+                     s, L.rpf_dep, static_cast<const T*>(L.rpf_val), L.rpf_cnt,
+                     static_cast<const T*>(L.ldiag), L.lev_ptr, seg.chain_lev, L.ring - 1, in,
+                     out);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t tri_rpg(const TriLayout& L, const TriSegment& seg, const T* in, T* out,
+                   hipStream_t s) {
+  if (L.ep_far) return hipErrorInvalidValue;
+  auto lanes = [&](auto e_tag) -> hipError_t {
+    constexpr int E = decltype(e_tag)::value;
+    switch (L.rp_lanes) {
+      case 2: return tri_rpg_go<T, 2, E>(L, seg, in, out, s);
+      case 4: return tri_rpg_go<T, 4, E>(L, seg, in, out, s);
+      case 8: return tri_rpg_go<T, 8, E>(L, seg, in, out, s);
+      case 16: return tri_rpg_go<T, 16, E>(L, seg, in, out, s);
+      default: return hipErrorInvalidValue;
+    }
+  };
+  if (L.rp_e == 4) return lanes(std::integral_constant<int, 4>{});
+  if (L.rp_e == 8) return lanes(std::integral_constant<int, 8>{});
+  return hipErrorInvalidValue;
+}
+
 // One level, many workgroups, level-ordered layout.
 template <typename T, TriKind K>
 __global__ __launch_bounds__(kBlock) void k_tri_level_pre(
@@ -1829,6 +1984,14 @@ hipError_t launch_trsv_rp(const TriLayout& L, const TriSegment& seg, int bit, in
 
 hipError_t launch_trsv_rpf(Elem e, const TriLayout& L, const TriSegment& seg, const void* in,
                            void* out, hipStream_t s) {
+  if (L.rp_lanes > 1) {
+    if (e == Elem::kF64)
+      return tri_rpg<double>(L, seg, static_cast<const double*>(in), static_cast<double*>(out),
+                             s);
+    if (e == Elem::kF32)
+      return tri_rpg<float>(L, seg, static_cast<const float*>(in), static_cast<float*>(out), s);
+    return hipErrorInvalidValue;
+  }
   if (e == Elem::kF64)
     return tri_rpf_waves<double>(L, seg, static_cast<const double*>(in),
                                  static_cast<double*>(out), s);

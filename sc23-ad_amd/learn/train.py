@@ -4,8 +4,9 @@ forward/backward kernels, full-batch Adam) from the solver's sweep data:
     python sc23-ad_amd/learn/train.py [--data ...npz] [--steps 3000] [--out ...npy]
 
 Instances are split 80/20 into training and validation sets (by instance);
-the script reports both losses and how often the predicted omega, rounded to
-the sweep grid, is the instance's best, and writes the parameters (float32).
+the script reports both losses and how often the predicted omega and vector
+bits, rounded to the sweep grids, are the level's best, and writes the
+parameters (float32).
 """
 from __future__ import annotations
 
@@ -22,11 +23,11 @@ import sc23ad as S  # noqa: E402
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--data", default=os.path.join(HERE, "data", "diffusion27_sweeps.npz"))
+    ap.add_argument("--data", default=os.path.join(HERE, "data", "levels_sweeps.npz"))
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--out", default=os.path.join(HERE, "weights_diffusion27.npy"))
+    ap.add_argument("--out", default=os.path.join(HERE, "weights_levels.npy"))
     args = ap.parse_args()
     d = np.load(args.data, allow_pickle=False)
     X, T, inst = d["X"], d["T"], d["instance"]
@@ -45,14 +46,17 @@ def main():
         print(f"step {done}: train loss {lt:.5f} validation loss {lv:.5f} (initial {l0:.5f})",
               flush=True)
     y = m.forward(X)
-    om_pred = 0.6 + 1.3 * y[:, 0]
-    om_true = 0.6 + 1.3 * T[:, 0]
-    grid = np.array(sorted(set(np.round(om_true, 6))))
-    snap = grid[np.abs(om_pred[:, None] - grid[None, :]).argmin(1)]
-    for name, sel in (("train", ~va), ("validation", va)):
-        if sel.any():
-            acc = np.mean(np.isclose(snap[sel], om_true[sel]))
-            print(f"{name}: omega on the best grid point for {100 * acc:.1f}% of levels", flush=True)
+    for col, name in ((0, "omega"), (1, "vector bits")):
+        pred, true = y[:, col], T[:, col]
+        grid = np.array(sorted(set(np.round(true, 6))))
+        snap = grid[np.abs(pred[:, None] - grid[None, :]).argmin(1)]
+        default = grid[np.abs(grid - (0.4 / 1.3 if col == 0 else 21 / 22)).argmin()]
+        for part, sel in (("train", ~va), ("validation", va)):
+            if sel.any():
+                acc = np.mean(np.isclose(snap[sel], true[sel]))
+                base = np.mean(np.isclose(default, true[sel]))
+                print(f"{part}: {name} on the best grid point for {100 * acc:.1f}% of levels "
+                      f"(the default for {100 * base:.1f}%)", flush=True)
     np.save(args.out, m.params())
     print("wrote", args.out, flush=True)
 

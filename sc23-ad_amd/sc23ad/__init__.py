@@ -135,6 +135,8 @@ def _load() -> C.CDLL:
                                                C.POINTER(vp)]),
         "sc23ad_block_solver_trace": (C.c_int, [vp, C.c_int, vp, C.c_int, ip]),
         "sc23ad_block_solver_history": (C.c_int, [vp, C.c_int, vp, C.c_int, ip]),
+        "sc23ad_solver_set_stall": (C.c_int, [vp, C.c_int]),
+        "sc23ad_block_solver_set_stall": (C.c_int, [vp, C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -408,10 +410,35 @@ class DevArray:
         assert a.nbytes == self.nbytes
         check(lib.sc23ad_dev_upload(self.p, ptr(a), self.nbytes))
 
+    def view(self, offset: int, count: int) -> "DevView":
+        """Elements [offset, offset + count) of a one-dimensional array, as a
+        device pointer the solvers accept (no copy; the view does not own)."""
+        if offset < 0 or count < 0 or (offset + count) * self.dtype.itemsize > self.nbytes:
+            raise ValueError("view outside the array")
+        return DevView(self, offset, count)
+
     def __del__(self):
-        if getattr(self, "p", None) is not None and self.p.value:
+        if getattr(self, "owner", True) and getattr(self, "p", None) is not None \
+                and self.p.value:
             lib.sc23ad_dev_free(self.p)
             self.p = C.c_void_p()
+
+
+class DevView:
+    """A slice of a DevArray (keeps its parent alive)."""
+
+    def __init__(self, parent: DevArray, offset: int, count: int):
+        self.parent = parent
+        self.dtype = parent.dtype
+        self.shape = (count,)
+        self.nbytes = count * parent.dtype.itemsize
+        self.p = C.c_void_p(parent.p.value + offset * parent.dtype.itemsize)
+
+    def get(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            check(lib.sc23ad_dev_download(ptr(out), self.p, self.nbytes))
+        return out
 
 
 DT = {FP64: np.float64, FP32: np.float32, INT: np.int32}
@@ -482,6 +509,12 @@ class Solver:
 
     def set_sequential_reduction(self, on: bool) -> None:
         check(lib.sc23ad_solver_set_reduction(self.h, 1 if on else 0))
+
+    def set_stall(self, steps: int) -> None:
+        """Stop (status STAGNATION) once the implicit residual has not fallen
+        for `steps` consecutive Arnoldi steps (0: never, the reference's
+        behaviour)."""
+        check(lib.sc23ad_solver_set_stall(self.h, steps))
 
     def trace(self, cap: int) -> None:
         check(lib.sc23ad_solver_trace(self.h, None, cap, None))
@@ -582,13 +615,14 @@ class Mlp:
         return loss.value
 
     def levels(self, hier: "Hierarchy"):
-        """(omega per level, bit cap per level) predicted for hier's levels."""
+        """(omega per level, vector bits per level incl. the coarsest, the
+        first 0) predicted for hier's levels."""
         nl = hier.info()[0]
         om = np.zeros(max(nl, 1))
-        bi = np.zeros(max(nl, 1), np.int32)
+        vb = np.zeros(nl + 1, np.int32)
         n = C.c_int()
-        check(lib.sc23ad_learned_levels(self.h, hier.h, ptr(om), ptr(bi), nl, C.byref(n)))
-        return om[:n.value], bi[:n.value]
+        check(lib.sc23ad_learned_levels(self.h, hier.h, ptr(om), ptr(vb), nl, C.byref(n)))
+        return om[:n.value], vb[:n.value + 1]
 
     def __del__(self):
         if getattr(self, "h", None) and self.h.value:
@@ -690,6 +724,20 @@ class BlockSolver:
         out = (Converge * self.k)()
         check(lib.sc23ad_block_solver_solve(self.h, ptr(b), ptr(nb), ptr(x), 0, out))
         return x, list(out)
+
+    def solve_device(self, b: DevArray, nrm_bs, x: DevArray):
+        """b, x: device arrays of k * n FP64 values (right-hand side r at r * n)."""
+        nb = np.ascontiguousarray(nrm_bs, dtype=np.float64)
+        if len(nb) != self.k or b.nbytes < 8 * self.k * self.n or x.nbytes < 8 * self.k * self.n:
+            raise ValueError("solve_device: sizes")
+        out = (Converge * self.k)()
+        check(lib.sc23ad_block_solver_solve(self.h, b.p, ptr(nb), x.p, 1, out))
+        return list(out)
+
+    def set_stall(self, steps: int) -> None:
+        """Stop a right-hand side (status STAGNATION) whose implicit residual
+        has not fallen for `steps` consecutive Arnoldi steps (0: never)."""
+        check(lib.sc23ad_block_solver_set_stall(self.h, steps))
 
     def trace(self, cap: int) -> None:
         check(lib.sc23ad_block_solver_trace(self.h, 0, None, cap, None))

@@ -1,14 +1,19 @@
-"""Learned per-level smoother parameters (omega_l, bit cap b_l) with the
-reference's defaults as the fallback.
+"""Learned per-level parameters (smoother weight omega_l and the fractional
+bits of every coarse level's vectors) with the reference's defaults as the
+fallback.
 
 The network (`Mlp`, HIP kernels) reads 8 features of every level's operator
-and predicts omega_l in [0.6, 1.9] and a cap b_l in [8, 30] on the fractional
-bits of A_l, U_l and LD_l's off-diagonal entries.  The integer kernels do not
-change.  Fallbacks to the reference's rule (one omega, the CSRflex bits):
-  * a level whose cap exceeds the flex rule's maximum (a saturated
-    prediction) keeps the rule's bits;
-  * a solve that overflows, stagnates or runs out of restarts is repeated
-    with the default hierarchy.
+and predicts omega_l in [0.6, 1.9] and the fractional bits of the next
+coarser level's vectors relative to the solver's F, in [-21, 1]
+(sc23ad_learned_levels).  The integer kernels do not change: the bits are
+folded into the restriction / prolongation shifts
+(sc23ad_hier_retune_levels).  Fallbacks to the reference's rule (one omega,
+one F for every vector):
+  * a level whose bit step leaves the shifts' range is cut to it (a
+    saturated prediction, counted);
+  * a solve that overflows, stops improving for STALL_STEPS Arnoldi steps,
+    stagnates or runs out of restarts is repeated with the default
+    hierarchy; the explicit FP64 residual judges the result.
 """
 from __future__ import annotations
 
@@ -20,7 +25,10 @@ import numpy as np
 from . import (FP64, INT, OK, Amg, Hierarchy, Mlp, Solver, amg_param)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-WEIGHTS = os.path.join(os.path.dirname(HERE), "learn", "weights_diffusion27.npy")
+WEIGHTS = os.path.join(os.path.dirname(HERE), "learn", "weights_levels.npy")
+# Arnoldi steps without a fall of the implicit residual after which a
+# learned solve gives up and the defaults take over.
+STALL_STEPS = 30
 
 
 def load_mlp(path: str = WEIGHTS, seed: int = 0) -> Mlp:
@@ -32,8 +40,8 @@ def load_mlp(path: str = WEIGHTS, seed: int = 0) -> Mlp:
 @dataclass
 class LearnedSetup:
     omega: np.ndarray        # per level
-    bits: np.ndarray         # per-level caps as predicted
-    saturated: int           # levels where the cap exceeded the rule's maximum
+    vbits: np.ndarray        # per level incl. the coarsest (first 0)
+    saturated: int           # bit steps cut to the shifts' range
     hier_int: Hierarchy
     hier_fp64: Hierarchy
 
@@ -41,10 +49,10 @@ class LearnedSetup:
 def learned_setup(base_int: Hierarchy, base_fp64: Hierarchy, mlp: Mlp) -> LearnedSetup:
     """Hierarchies with the network's per-level parameters, derived from the
     default ones' coarsening."""
-    om, bi = mlp.levels(base_fp64)
-    hi = base_int.retune(om, bi, INT)
-    hd = base_fp64.retune(om, None, FP64)
-    return LearnedSetup(om, bi, hi.saturated(), hi, hd)
+    om, vb = mlp.levels(base_fp64)
+    hi = base_int.retune_levels(om, vb, INT)
+    hd = base_fp64.retune_levels(om, None, FP64)
+    return LearnedSetup(om, vb, hi.saturated(), hi, hd)
 
 
 @dataclass
@@ -54,13 +62,16 @@ class LearnedResult:
     fallback_used: bool
     learned: LearnedSetup
     bit: int
+    relres: float
 
 
 def learned_solve(a, b, nrm_b, mlp: Mlp, param=None, m: int = 30, eps: float = 1e-8,
-                  base=None) -> LearnedResult:
+                  base=None, stall_steps: int = STALL_STEPS) -> LearnedResult:
     """Integer AMG-FGMRES(m) with learned per-level parameters; falls back
     to the default hierarchy when the learned solve does not converge
-    cleanly.  `base` = (int, fp64) default hierarchies if already built."""
+    cleanly (overflow, a stall of stall_steps steps, stagnation, budget) or
+    its explicit residual misses eps.  `base` = (int, fp64) default
+    hierarchies if already built."""
     param = param if param is not None else amg_param()
     if base is None:
         base = Hierarchy.build_multi(a, [INT, FP64], param)
@@ -68,10 +79,13 @@ def learned_solve(a, b, nrm_b, mlp: Mlp, param=None, m: int = 30, eps: float = 1
     ls = learned_setup(bi, bd, mlp)
     bit = Amg(ls.hier_fp64).default_frac_bits(b, nrm_b)
     sol = Solver(INT, a, Amg(ls.hier_int), m=m, eps=eps, bit=bit)
+    sol.set_stall(stall_steps)
     x, cv = sol.solve(b, nrm_b)
-    if cv.status == OK and cv.is_converged:
-        return LearnedResult(x, cv, ls.saturated > 0, ls, bit)
+    rel = sol.relres(b, nrm_b, x)
+    if cv.status == OK and cv.is_converged and rel <= eps:
+        return LearnedResult(x, cv, False, ls, bit, rel)
     bit0 = Amg(bd).default_frac_bits(b, nrm_b)
     sol0 = Solver(INT, a, Amg(bi), m=m, eps=eps, bit=bit0)
     x0, cv0 = sol0.solve(b, nrm_b)
-    return LearnedResult(x0, cv0, True, ls, bit0)
+    cv0.fallback_used = 1
+    return LearnedResult(x0, cv0, True, ls, bit0, sol0.relres(b, nrm_b, x0))
