@@ -88,7 +88,7 @@ void DevAmg::trsv_blk(const TriPlan& plan, const DevMat& m, const void* in, void
     hip_check(dev::launch_blk_column(n, r, const_cast<void*>(in), w.col_in.get(), true,
                                      lib_stream()),
               "column");
-    trsv(plan, m, w.col_in.get(), w.col_out.get(), nullptr, nullptr, nullptr);
+    trsv(plan, m, w.col_in.get(), w.col_out.get(), nullptr, nullptr, nullptr, nullptr);
     hip_check(dev::launch_blk_column(n, r, out, w.col_out.get(), false, lib_stream()), "column");
   }
 }

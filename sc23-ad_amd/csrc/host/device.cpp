@@ -657,54 +657,102 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
         L.rp_lanes = lanes;
         L.rp_waves = rw;
         L.rp_farq = farq;
-        // The block kernel (four right-hand sides per workgroup) keeps four
-        // values per ring slot, so its ring may be shorter than this one:
-        // the shortest that holds every dependency fewer than kEpFarLevels
-        // levels back, the others read from global memory, far entries of a
-        // row first (one-lane layouts).
-        if (lanes == 1 && rows == 1 &&
-            dev::rpb_lds_bytes(L.ring, maxlev_chain) > dev::kTriBlockedMaxLds) {
-          long long near = 0;
+      }
+      // The block kernel's layout (four right-hand sides per workgroup,
+      // block.hip), chosen on its own: its ring keeps four values per slot,
+      // so it is at most kTriBlockedMaxLds / 16 slots, and it reads
+      // dependencies at least kRpbFarLevels levels back from global memory;
+      // one or two rows per lane, so that wide levels fit in 8 waves.
+      if (kind == TriKind::kLD) {
+        int md_b = 0;
+        for (int idx = 0; idx < n; ++idx) md_b = std::max(md_b, lbeg[idx + 1] - lbeg[idx]);
+        long long span_near = 0, span_all = 0;
+        for (size_t Lv = 0; Lv + 1 < lev_ptr.size(); ++Lv)
+          for (int idx = lev_ptr[Lv]; idx < lev_ptr[Lv + 1]; ++idx)
+            for (int q = lbeg[idx]; q < lbeg[idx + 1]; ++q) {
+              const long long d = static_cast<long long>(lev_ptr[Lv + 1]) - lpos[q];
+              span_all = std::max(span_all, d);
+              if (lev_of[idx] - lev_of[lpos[q]] < dev::kRpbFarLevels)
+                span_near = std::max(span_near, d);
+            }
+        auto pow2 = [](long long v) {
+          int r = 2;
+          while (r < v) r *= 2;
+          return r;
+        };
+        // far entries per row for a ring of rb slots (one-lane layouts)
+        auto far_count = [&](int rb) {
+          int mx = 0;
           for (size_t Lv = 0; Lv + 1 < lev_ptr.size(); ++Lv)
-            for (int idx = lev_ptr[Lv]; idx < lev_ptr[Lv + 1]; ++idx)
+            for (int idx = lev_ptr[Lv]; idx < lev_ptr[Lv + 1]; ++idx) {
+              int c = 0;
               for (int q = lbeg[idx]; q < lbeg[idx + 1]; ++q)
-                if (lev_of[idx] - lev_of[lpos[q]] < dev::kEpFarLevels)
-                  near = std::max<long long>(near, static_cast<long long>(lev_ptr[Lv + 1]) -
-                                                       lpos[q]);
-          int ring_b = 2;
-          while (ring_b < near) ring_b *= 2;
-          if (dev::rpb_lds_bytes(ring_b, maxlev_chain) <= dev::kTriBlockedMaxLds) {
-            std::vector<int> rb(static_cast<size_t>(std::max(n, 1)) * e * 2, 0);
-            int farq_b = 0;
-            bool any_far_b = false;
-            for (size_t Lv = 0; Lv + 1 < lev_ptr.size(); ++Lv)
-              for (int idx = lev_ptr[Lv]; idx < lev_ptr[Lv + 1]; ++idx) {
-                std::vector<int> ord, code;
-                for (int q = lbeg[idx]; q < lbeg[idx + 1]; ++q)
-                  if (static_cast<long long>(lev_ptr[Lv + 1]) - lpos[q] > ring_b) {
-                    ord.push_back(q);
-                    code.push_back(-(lcol[q] + 1));
-                  }
-                farq_b = std::max(farq_b, static_cast<int>(ord.size()));
-                any_far_b = any_far_b || !ord.empty();
-                for (int q = lbeg[idx]; q < lbeg[idx + 1]; ++q)
-                  if (static_cast<long long>(lev_ptr[Lv + 1]) - lpos[q] <= ring_b) {
-                    ord.push_back(q);
-                    code.push_back(lpos[q]);
-                  }
-                for (size_t mm = 0; mm < ord.size(); ++mm) {
-                  const size_t at = (static_cast<size_t>(idx) * e + mm) * 2;
-                  rb[at] = code[mm];
-                  rb[at + 1] = static_cast<int>(lval[ord[mm]]);
-                }
+                if (static_cast<long long>(lev_ptr[Lv + 1]) - lpos[q] > rb) ++c;
+              mx = std::max(mx, c);
+            }
+          return mx;
+        };
+        int best_e = 0, best_g = 0, best_r = 0, best_w = 0, best_ring = 0, best_fq = 0;
+        size_t best_key = 0;
+        for (int g : {1, 2, 4, 8, 16})
+          for (int ge : {4, 8, 10, 12}) {
+            if ((g > 1 && ge == 4) || g * ge < md_b) continue;
+            const int rb = pow2(g == 1 ? span_near : span_all);
+            if (dev::rpb_lds_bytes(rb, maxlev_chain) > dev::kTriBlockedMaxLds) continue;
+            int fq = 0;
+            if (g == 1 && span_all > rb) {
+              fq = far_count(rb);
+              if (fq > (ge == 4 ? 1 : 2)) continue;
+            }
+            for (int rows_b : {1, 2}) {
+              if (rows_b == 2 && g > 1 && ge > 8) continue;  // no such block kernel
+              const size_t lanes = (maxlev_rows + rows_b - 1) / rows_b * g;
+              const size_t w = (lanes + 63) / 64;
+              if (w > 8) continue;
+              const size_t key = (w * 4096 + static_cast<size_t>(g * ge)) * 4 + rows_b;
+              if (best_e == 0 || key < best_key) {
+                best_key = key;
+                best_e = ge;
+                best_g = g;
+                best_r = rows_b;
+                best_w = static_cast<int>(w);
+                best_ring = rb;
+                best_fq = fq;
               }
-            plan.rent_blk.alloc(sizeof(int) * rb.size());
-            plan.rent_blk.upload(rb.data(), sizeof(int) * rb.size());
-            L.rent_blk = plan.rent_blk.get();
-            L.ring_blk = ring_b;
-            L.farq_blk = farq_b;
-            L.far_blk = any_far_b ? 1 : 0;
+            }
           }
+        if (best_e > 0) {
+          const int e_b = best_e, g_b = best_g;
+          std::vector<int> rb(static_cast<size_t>(std::max(n, 1)) * g_b * e_b * 2, 0);
+          for (size_t Lv = 0; Lv + 1 < lev_ptr.size(); ++Lv)
+            for (int idx = lev_ptr[Lv]; idx < lev_ptr[Lv + 1]; ++idx) {
+              std::vector<int> code, vals;  // far entries first (one-lane layouts)
+              for (int pass = 0; pass < 2; ++pass)
+                for (int q = lbeg[idx]; q < lbeg[idx + 1]; ++q) {
+                  const bool far =
+                      static_cast<long long>(lev_ptr[Lv + 1]) - lpos[q] > best_ring;
+                  if (far != (pass == 0)) continue;
+                  code.push_back(far ? -(lcol[q] + 1) : lpos[q]);
+                  vals.push_back(static_cast<int>(lval[q]));
+                }
+              for (size_t mm = 0; mm < code.size(); ++mm) {
+                const size_t at =
+                    ((static_cast<size_t>(idx) * g_b + mm % g_b) * e_b + mm / g_b) * 2;
+                rb[at] = code[mm];
+                rb[at + 1] = vals[mm];
+              }
+            }
+          plan.rent_blk.alloc(sizeof(int) * rb.size());
+          plan.rent_blk.upload(rb.data(), sizeof(int) * rb.size());
+          L.rent_blk = plan.rent_blk.get();
+          L.ring_blk = best_ring;
+          L.far_blk = best_fq > 0 ? 1 : 0;
+          L.farq_blk = best_fq;
+          L.e_blk = e_b;
+          L.lanes_blk = g_b;
+          L.rows_blk = best_r;
+          L.waves_blk = best_w;
+          L.ep_maxlev = maxlev_chain;
         }
       }
     }
@@ -826,10 +874,12 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
     std::fprintf(stderr,
                  "[sc23ad plan] kind %d n %d nnz %zu chains %d levels(max chain) %d "
                  "max level rows %zu launches %d ring %d span %lld max deps %d ep waves %d far %d "
-                 "narrow %d copies %d rp waves %d entries %d rows %d lanes %d identity %d\n",
+                 "narrow %d copies %d rp waves %d entries %d rows %d lanes %d identity %d "
+                 "| block: waves %d entries %d rows %d lanes %d ring %d far %d\n",
                  static_cast<int>(kind), n, cind.size(), nchain, plan.nlevels, maxlev_rows,
                  plan.nlaunch, L.ring, span, maxdeps, L.ep_waves, L.ep_far, L.ep_narrow,
-                 L.ep_copies, L.rp_waves, L.rp_e, L.rp_rows, L.rp_lanes, plan.identity ? 1 : 0);
+                 L.ep_copies, L.rp_waves, L.rp_e, L.rp_rows, L.rp_lanes, plan.identity ? 1 : 0,
+                 L.waves_blk, L.e_blk, L.rows_blk, L.lanes_blk, L.ring_blk, L.farq_blk);
   }
 }
 
@@ -869,6 +919,12 @@ void build_tiles(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rpt
   const int W = dev::kTriBlockWidth;
   const int nb = (n + W - 1) / W;
   if (n <= 0 || nb > dev::kTileMaxBlocks) return;
+  // the tiles replay the sequential order only for rows stored in ascending
+  // column order (as the factorisation writes them); otherwise keep the
+  // blocked solve, which follows the stored order as it is
+  for (int i = 0; i < n; ++i)
+    for (int p = rptr[i] + 1; p < rptr[i + 1]; ++p)
+      if (cind[p] <= cind[p - 1]) return;
   const bool fwd = kind != TriKind::kDU;
   std::vector<long long> toff(nb + 1, 0);
   for (int b = 0; b < nb; ++b) toff[b + 1] = toff[b] + static_cast<long long>(fwd ? b + 1 : nb - b) * W * W;
@@ -903,6 +959,63 @@ void build_tiles(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rpt
   plan.ddinv.upload(dv.data(), sizeof(int32_t) * dv.size());
   plan.nb = nb;
   plan.dense = true;
+}
+
+// FP version of build_tiles: the same 64 x 64 tiles in processing order with
+// FP values, plus for every tile row a 64-bit mask of the columns the factor
+// stores, so that the tiled FP solve skips absent entries exactly as the
+// sequential CSR solve does.
+template <typename V>
+void build_tiles_fp(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr,
+                    const std::vector<int>& cind, const std::vector<V>& val) {
+  const int W = dev::kTriBlockWidth;
+  const int nb = (n + W - 1) / W;
+  if (n <= 0 || nb > dev::kTileMaxBlocks) return;
+  // the tiles replay the sequential order only for rows stored in ascending
+  // column order (as the factorisation writes them); otherwise keep the
+  // blocked solve, which follows the stored order as it is
+  for (int i = 0; i < n; ++i)
+    for (int p = rptr[i] + 1; p < rptr[i + 1]; ++p)
+      if (cind[p] <= cind[p - 1]) return;
+  const bool fwd = kind != TriKind::kDU;
+  std::vector<long long> toff(nb + 1, 0);
+  for (int b = 0; b < nb; ++b)
+    toff[b + 1] = toff[b] + static_cast<long long>(fwd ? b + 1 : nb - b) * W * W;
+  std::vector<V> tl(static_cast<size_t>(toff[nb]), V(0));
+  std::vector<uint64_t> mk(static_cast<size_t>(toff[nb] / W), 0);
+  std::vector<V> dv(static_cast<size_t>(n), V(0));
+  for (int i = 0; i < n; ++i) {
+    int b0 = rptr[i], e0 = rptr[i + 1];
+    if (kind == TriKind::kLD) {
+      dv[i] = val[e0 - 1];
+      e0 = std::max(b0, e0 - 1);
+    }
+    if (kind == TriKind::kDU) {
+      dv[i] = val[b0];
+      b0 = std::min(e0, b0 + 1);
+    }
+    const int b = i / W;
+    for (int p = b0; p < e0; ++p) {
+      const int c = cind[p];
+      const int bc = c / W;
+      if (fwd ? bc > b : bc < b)
+        throw DevError{1, "coarse factor entry on the wrong side of the diagonal"};
+      const int slot = fwd ? bc : nb - 1 - bc;
+      const size_t tile = static_cast<size_t>(toff[b]) / (W * W) + slot;
+      tl[tile * W * W + static_cast<size_t>(c - bc * W) * W + (i - b * W)] = val[p];
+      mk[tile * W + (i - b * W)] |= uint64_t{1} << (c - bc * W);
+    }
+  }
+  plan.slice.alloc(sizeof(V) * tl.size());
+  plan.slice.upload(tl.data(), sizeof(V) * tl.size());
+  plan.boff.alloc(sizeof(long long) * toff.size());
+  plan.boff.upload(toff.data(), sizeof(long long) * toff.size());
+  plan.tmask.alloc(sizeof(uint64_t) * mk.size());
+  plan.tmask.upload(mk.data(), sizeof(uint64_t) * mk.size());
+  plan.ddinv.alloc(sizeof(V) * dv.size());
+  plan.ddinv.upload(dv.data(), sizeof(V) * dv.size());
+  plan.nb = nb;
+  plan.dense_fp = true;
 }
 
 // Rows and columns of a CSR matrix in a new numbering: new row r is old row
@@ -1140,6 +1253,8 @@ std::unique_ptr<DevAmg::Work> DevAmg::make_work() const {
   }
   w->err.alloc(sizeof(int));
   w->err.upload(zero.data(), sizeof(int));
+  w->tickets.alloc(4 * sizeof(unsigned));
+  w->tickets.upload(zero.data(), 4 * sizeof(unsigned));
   return w;
 }
 
@@ -1198,6 +1313,10 @@ std::unique_ptr<DevAmg> DevAmg::upload_fp(const FpHierarchy<T>& h, const LaunchC
              c.single_wg_max_rows);
   build_blocked(a->lc_plan, TriKind::kL, h.lc.n, h.lc.rptr, h.lc.cind, sizeof(T), sizeof(T));
   build_blocked(a->uc_plan, TriKind::kDU, h.uc.n, h.uc.rptr, h.uc.cind, sizeof(T), sizeof(T));
+  if (!std::getenv("SC23AD_FP_COARSE_BLOCKED")) {
+    build_tiles_fp(a->lc_plan, TriKind::kL, h.lc.n, h.lc.rptr, h.lc.cind, h.lc.val);
+    build_tiles_fp(a->uc_plan, TriKind::kDU, h.uc.n, h.uc.rptr, h.uc.cind, h.uc.val);
+  }
   a->work_ = a->make_work();
   a->tune();
   return a;
@@ -1221,14 +1340,23 @@ bool DevAmg::trsv_timing_aware(const TriPlan& plan) const {
 }
 
 void DevAmg::trsv(const TriPlan& plan, const DevMat& m, const void* in, void* out,
-                  unsigned long long* flags, int* epoch, int* err) const {
+                  unsigned long long* flags, int* epoch, int* err, unsigned* tickets) const {
   if (plan.dense && elem == Elem::kI32 && flags) {
     *epoch = *epoch == INT32_MAX ? 1 : *epoch + 1;
     hip_check(dev::launch_trsv_tiles(plan.kind, m.d.n, plan.nb, plan.slice.as<int32_t>(),
                                      plan.boff.as<long long>(), plan.ddinv.as<int32_t>(),
-                                     m.d.bit, m.d.dbit, in, out, flags, *epoch, err,
+                                     m.d.bit, m.d.dbit, in, out, flags, *epoch, err, tickets,
                                      lib_stream()),
               "trsv tiles");
+    return;
+  }
+  if (plan.dense_fp && elem != Elem::kI32 && flags) {
+    *epoch = *epoch == INT32_MAX ? 1 : *epoch + 1;
+    hip_check(dev::launch_trsv_tiles_fp(elem, plan.kind, m.d.n, plan.nb, plan.slice.get(),
+                                        plan.boff.as<long long>(), plan.tmask.as<uint64_t>(),
+                                        plan.ddinv.get(), in, out, flags, *epoch, err, tickets,
+                                        lib_stream()),
+              "trsv tiles fp");
     return;
   }
   if (plan.blocked) {
@@ -1270,7 +1398,7 @@ void DevAmg::smooth(int l, const void* in, void* out, bool inited, Work& w) cons
   EventTimer* tm = l == 0 ? w.trsv_timer : nullptr;
   if (!inited) {
     if (tm) tm->begin(trsv_timing_aware(L.ld_plan));
-    trsv(L.ld_plan, L.ld, in, out, nullptr, nullptr, nullptr);
+    trsv(L.ld_plan, L.ld, in, out, nullptr, nullptr, nullptr, nullptr);
     if (tm) tm->end();
     k0 = 1;
   }
@@ -1283,13 +1411,13 @@ void DevAmg::smooth(int l, const void* in, void* out, bool inited, Work& w) cons
       continue;
     }
     if (tm) tm->begin(trsv_timing_aware(L.ld_plan));
-    trsv(L.ld_plan, L.ld, t, out, nullptr, nullptr, nullptr);
+    trsv(L.ld_plan, L.ld, t, out, nullptr, nullptr, nullptr, nullptr);
     if (tm) tm->end();
   }
 }
 
 void DevAmg::check_errors(Work& w) const {
-  if (!lc_plan.dense && !uc_plan.dense) return;
+  if (!lc_plan.dense && !uc_plan.dense && !lc_plan.dense_fp && !uc_plan.dense_fp) return;
   int e = 0;
   hip_check(hipMemcpyAsync(&e, w.err.get(), sizeof(int), hipMemcpyDeviceToHost, lib_stream()),
             "error flag");
@@ -1307,10 +1435,11 @@ void DevAmg::check_errors(Work& w) const {
 
 // direct::LUflex / LU<T>::Solve (Library/solver/lu.hpp:91-94, 33-36).
 void DevAmg::coarse(const void* in, void* out, Work& w) const {
+  unsigned* tk = w.tickets.as<unsigned>();
   trsv(lc_plan, lc, in, w.tmpc.get(), w.lflags.as<unsigned long long>(), &w.lepoch,
-       w.err.as<int>());
+       w.err.as<int>(), tk);
   trsv(uc_plan, uc, w.tmpc.get(), out, w.uflags.as<unsigned long long>(), &w.uepoch,
-       w.err.as<int>());
+       w.err.as<int>(), tk + 2);
 }
 
 // intAMGflex::Precondition / dfAMG::Precondition (INTAMG.hpp:269-293,
@@ -1358,7 +1487,7 @@ void DevAmg::level_op(int level, int op, const void* in, void* out, Work& w, voi
       case 1: spmv(L.u, SpmvMode::kPlain, ti.get(), nullptr, to.get()); break;
       case 2: spmv(L.r, SpmvMode::kPlain, ti.get(), nullptr, to.get()); break;
       case 3: spmv(L.p, SpmvMode::kPlain, ti.get(), nullptr, to.get()); break;
-      case 4: trsv(L.ld_plan, L.ld, ti.get(), to.get(), nullptr, nullptr, nullptr); break;
+      case 4: trsv(L.ld_plan, L.ld, ti.get(), to.get(), nullptr, nullptr, nullptr, nullptr); break;
       case 5: smooth(level, ti.get(), to.get(), false, w); break;
       case 6: smooth(level, ti.get(), to.get(), true, w); break;
       default: throw DevError{1, "level_op: bad op"};

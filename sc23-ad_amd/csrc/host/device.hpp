@@ -102,6 +102,8 @@ struct TriPlan {
   // hand-off flags live with the caller's work context
   bool dense = false;
   DevBuf slice, boff, ddinv;
+  bool dense_fp = false;  // FP tiles (slice / boff / ddinv as FP) with present-entry masks
+  DevBuf tmask;
   // level-ordered layout used by the level-scheduled kernels
   DevBuf lrow, lbeg, lcol, lval, ldiag, ecol, eval, lpos, epos, lent, lev_eptr, rent;
   DevBuf rpf_dep, rpf_val, rpf_cnt;  // FP row-parallel layout
@@ -175,6 +177,7 @@ class DevAmg {
     std::vector<DevBuf> ee, xx, rr, tmp;  // index = level (1..depth); tmp[0] fine
     DevBuf r0, tmpc, vin0, vout0;         // fine residual, coarse temp, level-0 order in/out
     DevBuf lflags, uflags, err;           // tiled coarse solves: {epoch, value} words, errors
+    DevBuf tickets;                       // tiled coarse solves: L counters [0,1], U [2,3]
     int lepoch = 0, uepoch = 0;           // last flag values raised
     EventTimer* trsv_timer = nullptr;     // times the level-0 LD solves (owner's hook)
   };
@@ -185,7 +188,7 @@ class DevAmg {
             void* y) const;
   bool trsv_timing_aware(const TriPlan& plan) const;
   void trsv(const TriPlan& plan, const DevMat& m, const void* in, void* out,
-            unsigned long long* flags, int* epoch, int* err) const;
+            unsigned long long* flags, int* epoch, int* err, unsigned* tickets) const;
   void smooth(int l, const void* in, void* out, bool inited, Work& w) const;
   void coarse(const void* in, void* out, Work& w) const;
   // One V-cycle on vectors in the original numbering.

@@ -19,7 +19,7 @@ namespace sc23ad::dev {
 namespace {
 
 constexpr int kBk = 256;
-constexpr int kEpFenceBlk = 32;  // full fence period in levels (kernels with far entries)
+constexpr int kEpFenceBlk = 8;  // full fence period in levels (kernels with far entries)
 
 __device__ __forceinline__ int32_t lane_of(const int4& v, int c) {
   return c == 0 ? v.x : c == 1 ? v.y : c == 2 ? v.z : v.w;
@@ -301,9 +301,8 @@ __global__ __launch_bounds__(512) void k_tri_chain_rpb(const int4* __restrict__ 
                                                        int rmask, int bit, int dbit, int n,
                                                        const int4* __restrict__ in_all,
                                                        int4* out_all) {
-  static_assert(kG == 1 || kR == 1, "several lanes per row serve one row each");
   static_assert(kA >= 1 && kA < kD, "far values need their entries loaded first");
-  static_assert(kEpFarLevels >= kA + kEpFenceBlk,
+  static_assert(kRpbFarLevels >= kA + kEpFenceBlk,
                 "far dependencies must be fenced before their values are loaded");
   extern __shared__ __align__(16) unsigned char smem_b[];
   int4* ring = reinterpret_cast<int4*>(smem_b);
@@ -358,39 +357,66 @@ hipError_t rpb_go(const TriLayout& L, const TriSegment& seg, int groups, int bit
   return hipGetLastError();
 }
 
-// Prefetch distances: 6 levels for the 4-entry layout, 4 for 8 entries, 3
-// for 10 and 12 (the registers of deeper buffers spill).  Far values are
-// loaded for the slots that can hold one (a row's far entries come first):
-// at most 1 (4 entries) or 2 (wider rows); other far layouts are not
-// offered (trsv_rp_blk_supported).
-template <bool kFar>
-hipError_t rpb_layout(const TriLayout& L, const TriSegment& seg, int groups, int bit, int dbit,
-                      const int4* in, int4* out, hipStream_t s) {
+// Prefetch distances by layout (the registers of deeper buffers spill):
+// one row per lane: 6 levels for 4 entries, 4 for 8, 3 for 10 and 12; two
+// rows per lane: 4 for 4 entries, 2 otherwise.  Far values are loaded for
+// the slots that can hold one (a row's far entries come first): 1 with 4
+// entries, 2 with more; layouts with several lanes per row have no far
+// entries.
+template <bool kFar, int kE, int kR>
+hipError_t rpb_one_lane(const TriLayout& L, const TriSegment& seg, int groups, int bit, int dbit,
+                        const int4* in, int4* out, hipStream_t s) {
+  constexpr int kFq = kE == 4 ? 1 : 2;
+  constexpr int kD = kR == 1 ? (kE == 4 ? 6 : kE == 8 ? 4 : 3) : (kE == 4 ? 4 : 2);
+  constexpr int kA = kR == 1 ? (kE == 4 ? 3 : kE == 8 ? 2 : 1) : (kE == 4 ? 2 : 1);
+  return rpb_go<kFar, 1, kE, kR, kD, kA, kFq>(L, seg, groups, bit, dbit, in, out, s);
+}
+
+template <int kG, int kE, int kR>
+hipError_t rpb_lanes(const TriLayout& L, const TriSegment& seg, int groups, int bit, int dbit,
+                     const int4* in, int4* out, hipStream_t s) {
+  constexpr int kD = kR == 1 ? (kE == 8 ? 4 : 3) : 2;
+  constexpr int kA = kR == 1 ? (kE == 8 ? 2 : 1) : 1;
+  return rpb_go<false, kG, kE, kR, kD, kA, 0>(L, seg, groups, bit, dbit, in, out, s);
+}
+
+template <bool kFar, int kR>
+hipError_t rpb_rows(const TriLayout& L, const TriSegment& seg, int groups, int bit, int dbit,
+                    const int4* in, int4* out, hipStream_t s) {
   if (L.rp_lanes <= 1) {
-    if (L.rp_e == 4) return rpb_go<kFar, 1, 4, 1, 6, 3, 1>(L, seg, groups, bit, dbit, in, out, s);
-    if (L.rp_e == 8) return rpb_go<kFar, 1, 8, 1, 4, 2, 2>(L, seg, groups, bit, dbit, in, out, s);
-    if (L.rp_e == 10) return rpb_go<kFar, 1, 10, 1, 3, 1, 2>(L, seg, groups, bit, dbit, in, out, s);
-    if (L.rp_e == 12) return rpb_go<kFar, 1, 12, 1, 3, 1, 2>(L, seg, groups, bit, dbit, in, out, s);
-    return hipErrorInvalidValue;
+    switch (L.rp_e) {
+      case 4: return rpb_one_lane<kFar, 4, kR>(L, seg, groups, bit, dbit, in, out, s);
+      case 8: return rpb_one_lane<kFar, 8, kR>(L, seg, groups, bit, dbit, in, out, s);
+      case 10: return rpb_one_lane<kFar, 10, kR>(L, seg, groups, bit, dbit, in, out, s);
+      case 12: return rpb_one_lane<kFar, 12, kR>(L, seg, groups, bit, dbit, in, out, s);
+      default: return hipErrorInvalidValue;
+    }
   }
   if constexpr (!kFar) {
     auto lanes = [&](auto e_tag) -> hipError_t {
       constexpr int E = decltype(e_tag)::value;
-      constexpr int D = E == 8 ? 4 : 3;
-      constexpr int A = E == 8 ? 2 : 1;
       switch (L.rp_lanes) {
-        case 2: return rpb_go<false, 2, E, 1, D, A, 0>(L, seg, groups, bit, dbit, in, out, s);
-        case 4: return rpb_go<false, 4, E, 1, D, A, 0>(L, seg, groups, bit, dbit, in, out, s);
-        case 8: return rpb_go<false, 8, E, 1, D, A, 0>(L, seg, groups, bit, dbit, in, out, s);
-        case 16: return rpb_go<false, 16, E, 1, D, A, 0>(L, seg, groups, bit, dbit, in, out, s);
+        case 2: return rpb_lanes<2, E, kR>(L, seg, groups, bit, dbit, in, out, s);
+        case 4: return rpb_lanes<4, E, kR>(L, seg, groups, bit, dbit, in, out, s);
+        case 8: return rpb_lanes<8, E, kR>(L, seg, groups, bit, dbit, in, out, s);
+        case 16: return rpb_lanes<16, E, kR>(L, seg, groups, bit, dbit, in, out, s);
         default: return hipErrorInvalidValue;
       }
     };
     if (L.rp_e == 8) return lanes(std::integral_constant<int, 8>{});
-    if (L.rp_e == 10) return lanes(std::integral_constant<int, 10>{});
-    if (L.rp_e == 12) return lanes(std::integral_constant<int, 12>{});
+    if constexpr (kR == 1) {  // two rows of 10 or 12 entries per lane spill
+      if (L.rp_e == 10) return lanes(std::integral_constant<int, 10>{});
+      if (L.rp_e == 12) return lanes(std::integral_constant<int, 12>{});
+    }
   }
   return hipErrorInvalidValue;
+}
+
+template <bool kFar>
+hipError_t rpb_layout(const TriLayout& L, const TriSegment& seg, int groups, int bit, int dbit,
+                      const int4* in, int4* out, hipStream_t s) {
+  if (L.rp_rows == 2) return rpb_rows<kFar, 2>(L, seg, groups, bit, dbit, in, out, s);
+  return rpb_rows<kFar, 1>(L, seg, groups, bit, dbit, in, out, s);
 }
 
 // ---- tiled coarse solve, four right-hand sides per workgroup --------------
@@ -402,8 +428,8 @@ hipError_t rpb_layout(const TriLayout& L, const TriSegment& seg, int groups, int
 // their (block, group) from a ticket counter, in processing order, so a
 // workgroup only waits for tickets handed out before its own, whatever
 // order the hardware starts workgroups in; the last workgroup to finish
-// resets the counters for the next launch.  A wait longer than
-// kTileMaxSpin polls records an error and gives up instead of hanging.
+// resets the counters for the next launch.  A launch out of its wait budget
+// (tile_wait_over) records an error and gives up instead of hanging.
 template <TriKind K>
 __global__ __launch_bounds__(kTileThreads) void k_tri_tiles_blk(
     int n, int nb, int groups, const int32_t* __restrict__ tiles,
@@ -414,10 +440,9 @@ __global__ __launch_bounds__(kTileThreads) void k_tri_tiles_blk(
   __shared__ uint64_t part[kTileThreads / 64][4][64];
   __shared__ int32_t xsh[4][64];
   __shared__ int ticket_sh;
+  const long long t0 = wall_clock64();
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  if (tid == 0) ticket_sh = static_cast<int>(atomicAdd(&tickets[0], 1u));
-  __syncthreads();
-  const int ticket = ticket_sh;
+  const int ticket = tile_ticket(tickets, &ticket_sh);
   // tickets in processing order (forward: blocks 0, 1, ...; backward: nb-1,
   // nb-2, ...), block-major, so every block's dependency blocks (all
   // groups) hold smaller tickets
@@ -460,7 +485,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tri_tiles_blk(
         }
         if (__all(ready)) break;
         __builtin_amdgcn_s_sleep(1);
-        if (++spins > kTileMaxSpin) {
+        if ((++spins & 31) == 0 && tile_wait_over(t0, err)) {
           if (lane == 0) atomicAdd(err, 1);
           break;
         }
@@ -517,15 +542,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tri_tiles_blk(
       }
     }
   }
-  // the last workgroup to finish resets the tickets for the next launch
-  __syncthreads();
-  if (tid == 0) {
-    const unsigned done = atomicAdd(&tickets[1], 1u);
-    if (done == static_cast<unsigned>(nb * groups) - 1) {
-      atomicExch(&tickets[0], 0u);
-      atomicExch(&tickets[1], 0u);
-    }
-  }
+  tile_done(tickets, nb * groups);
 }
 
 }  // namespace
@@ -612,28 +629,29 @@ hipError_t launch_spmv_blk(const DCsr& a, int groups, SpmvMode mode, const void*
   return spmv_blk_mode<2>(a, groups, x, z, y, s);
 }
 
-// The layout the block kernel reads: the row-parallel one, with its own
-// shorter ring where the plan built one.
+// The layout the block kernel reads (the plan's block fields).
 static TriLayout block_layout(const TriLayout& L) {
   TriLayout b = L;
-  if (L.rent_blk) {
-    b.rent = L.rent_blk;
-    b.ring = L.ring_blk;
-    b.ep_far = L.far_blk;
-    b.rp_farq = L.farq_blk;
-  }
+  b.rent = L.rent_blk;
+  b.ring = L.ring_blk;
+  b.ep_far = L.far_blk;
+  b.rp_farq = L.farq_blk;
+  b.rp_e = L.e_blk;
+  b.rp_lanes = L.lanes_blk;
+  b.rp_rows = L.rows_blk;
+  b.rp_waves = L.waves_blk;
   return b;
 }
 
 bool trsv_rp_blk_supported(const TriLayout& Lin) {
   const TriLayout L = block_layout(Lin);
-  if (L.rp_waves <= 0 || !L.rent) return false;
+  if (!L.rent || L.rp_waves < 1 || L.rp_waves > 8) return false;
   if (rpb_lds_bytes(L.ring, L.ep_maxlev) > kTriBlockedMaxLds) return false;
+  if (L.rp_rows != 1 && L.rp_rows != 2) return false;
   if (L.rp_lanes <= 1)
-    return L.rp_rows == 1 &&
-           (L.rp_e == 4 || L.rp_e == 8 || L.rp_e == 10 || L.rp_e == 12) &&
+    return (L.rp_e == 4 || L.rp_e == 8 || L.rp_e == 10 || L.rp_e == 12) &&
            (!L.ep_far || L.rp_farq <= (L.rp_e == 4 ? 1 : 2));
-  return !L.ep_far && (L.rp_e == 8 || L.rp_e == 10 || L.rp_e == 12) &&
+  return !L.ep_far && (L.rp_e == 8 || ((L.rp_e == 10 || L.rp_e == 12) && L.rp_rows == 1)) &&
          (L.rp_lanes == 2 || L.rp_lanes == 4 || L.rp_lanes == 8 || L.rp_lanes == 16);
 }
 

@@ -113,12 +113,18 @@ struct TriLayout {
   int rp_rows = 0;
   int rp_lanes = 1;
   int rp_farq = 0;   // most far entries in one row (stored first; one-lane layouts)
-  // the same entries for the block kernel where its ring (four values per
-  // slot) must be shorter: ring_blk slots, far flag and far count per row
+  // the block kernel's own row-parallel layout (four right-hand sides per
+  // workgroup, four values per ring slot): entries as rent, ring slots, far
+  // flag and most far entries per row, entries per lane, lanes per row,
+  // rows per lane, waves
   const void* rent_blk = nullptr;
   int ring_blk = 0;
   int far_blk = 0;
   int farq_blk = 0;
+  int e_blk = 0;
+  int lanes_blk = 1;
+  int rows_blk = 1;
+  int waves_blk = 0;
   // FP row-parallel mode (one lane per row, rp_e entries in stored order):
   // dependency codes and values by position, and each row's entry count
   const int* rpf_dep = nullptr;
@@ -150,6 +156,10 @@ inline size_t rp_lds_bytes(int ring, int maxlev) {
 // ---- block kernels (block.hip): several right-hand sides of one hierarchy --
 // A block vector of n rows and `groups` groups of four right-hand sides holds
 // element (row i, right-hand side r) at ((r / 4) * n + i) * 4 + r % 4.
+// Least level distance of a dependency the block kernel reads from global
+// memory (it fences every 8 levels; its far values are loaded at most 4
+// levels ahead).
+constexpr int kRpbFarLevels = 12;
 inline size_t rpb_lds_bytes(int ring, int maxlev) {
   return 16 * static_cast<size_t>(ring) + sizeof(int) * (static_cast<size_t>(maxlev) + 1);
 }
@@ -201,7 +211,15 @@ constexpr int kTileMaxBlocks = 64;
 hipError_t launch_trsv_tiles(TriKind kind, int n, int nb, const int32_t* tiles,
                              const long long* toff, const int32_t* dinv, int bit, int dbit,
                              const void* in, void* out, unsigned long long* xp, int epoch,
-                             int* err, hipStream_t s);
+                             int* err, unsigned* tickets, hipStream_t s);
+// Tiled FP coarse solve (one wave per 64-row block, each row's terms in the
+// sequential order): tiles as for launch_trsv_tiles with FP values, tmask
+// one 64-bit mask of present entries per tile row, flags nb words raised to
+// epoch when a block is done (see k_tri_tiles_fp).
+hipError_t launch_trsv_tiles_fp(Elem e, TriKind kind, int n, int nb, const void* tiles,
+                                const long long* toff, const uint64_t* tmask, const void* dinv,
+                                const void* in, void* out, unsigned long long* flags, int epoch,
+                                int* err, unsigned* tickets, hipStream_t s);
 hipError_t launch_trsv_blocked(Elem e, TriKind kind, const DCsr& a, const int* bptr,
                                int nb, const void* in, void* out, hipStream_t s);
 

@@ -631,20 +631,26 @@ __global__ __launch_bounds__(kBlkThreads) void k_tri_blocked_int(
 // current epoch, the workgroup subtracts the tile's product (four waves, 16
 // columns each; integer sums are order-free), then wave 0 sweeps the
 // diagonal tile: at step c lane c finishes its row and the value is broadcast
-// to the other lanes.  Dependencies point to lower (forward) or higher
-// (backward) blocks only, and all blocks of a coarse grid (at most 2000
-// rows: 32 blocks) are resident at once, so every wait ends; a wait longer
-// than kTileMaxSpin polls records an error and gives up instead of hanging.
+// to the other lanes.  Dependencies point to blocks earlier in processing
+// order (forward: lower blocks; backward: higher ones), and workgroups take
+// their block from a ticket counter in that order, so a workgroup only ever
+// waits for workgroups that started before it, whatever order the hardware
+// starts them in and whatever else shares the GPU.  A launch that runs out
+// of its wait budget (tile_wait_over) records an error and gives up instead
+// of hanging.
 
 template <TriKind K>
 __global__ __launch_bounds__(kTileThreads) void k_tri_tiles_int(
     int n, int nb, const int32_t* __restrict__ tiles, const long long* __restrict__ toff,
     const int32_t* __restrict__ dinv_g, int bit, int dbit, const int32_t* __restrict__ in,
-    int32_t* out, unsigned long long* xp, int epoch, int* err) {
+    int32_t* out, unsigned long long* xp, int epoch, int* err, unsigned* tickets) {
   constexpr bool fwd = (K != TriKind::kDU);
   __shared__ uint64_t part[kTileThreads / 64][64];
   __shared__ int32_t xsh[64];
-  const int b = blockIdx.x;
+  __shared__ int ticket_sh;
+  const long long t0 = wall_clock64();
+  const int pb = tile_ticket(tickets, &ticket_sh);
+  const int b = fwd ? pb : nb - 1 - pb;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int r0 = b * kTB;
   const int rows = min(kTB, n - r0);
@@ -667,7 +673,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tri_tiles_int(
                   : static_cast<unsigned long long>(static_cast<unsigned>(epoch)) << 32;
         if (__all(static_cast<int>(v >> 32) == epoch)) break;
         __builtin_amdgcn_s_sleep(1);
-        if (++spins > kTileMaxSpin) {
+        if ((++spins & 31) == 0 && tile_wait_over(t0, err)) {
           if (lane == 0) atomicAdd(err, 1);
           break;
         }
@@ -710,6 +716,96 @@ __global__ __launch_bounds__(kTileThreads) void k_tri_tiles_int(
       __hip_atomic_store(&xp[r0 + lane], word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
+  tile_done(tickets, nb);
+}
+
+// Tiled FP coarse solve (solvers 0 and 1), one wave per 64-row block, with
+// each row's terms subtracted in the sequential solve's order: the
+// dependency blocks in processing order (forward 0, 1, ...; backward nb-1,
+// nb-2, ...) and inside a tile the columns in the same direction, so a row
+// receives its terms in ascending column order (L) or descending (DU),
+// entries absent from the factor (clear bits of the row's 64-bit mask per
+// tile) skipped, exactly as the sequential CSR solve does.  Lane r owns row
+// 64 b + r.  A finished block publishes its values in `out`, then raises its
+// flag to the call's epoch (agent-scope release); a consumer waits for the
+// flag (acquire) and reads the block's values.  Blocks are taken from a
+// ticket counter in processing order, so every wait is on a workgroup that
+// started earlier; a launch out of its wait budget (tile_wait_over) records
+// an error and gives up instead of hanging.
+template <typename T>
+__device__ __forceinline__ T readlane_fp(T v, int src);
+template <>
+__device__ __forceinline__ double readlane_fp<double>(double v, int src) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(static_cast<unsigned long long>(b)), src);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(static_cast<unsigned long long>(b) >> 32), src);
+  return __longlong_as_double(static_cast<long long>(
+      (static_cast<unsigned long long>(static_cast<unsigned>(hi)) << 32) | static_cast<unsigned>(lo)));
+}
+template <>
+__device__ __forceinline__ float readlane_fp<float>(float v, int src) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+
+template <typename T, TriKind K>
+__global__ __launch_bounds__(64) void k_tri_tiles_fp(
+    int n, int nb, const T* __restrict__ tiles, const long long* __restrict__ toff,
+    const uint64_t* __restrict__ tmask, const T* __restrict__ dinv_g,
+    const T* __restrict__ in, T* out, unsigned long long* flags, int epoch, int* err,
+    unsigned* tickets) {
+  constexpr bool fwd = (K != TriKind::kDU);
+  __shared__ int ticket_sh;
+  const long long t0 = wall_clock64();
+  const int pb = tile_ticket(tickets, &ticket_sh);
+  const int b = fwd ? pb : nb - 1 - pb;
+  const int lane = threadIdx.x;
+  const int r0 = b * kTB;
+  const int rows = min(kTB, n - r0);
+  const T* Tb = tiles + toff[b];
+  const uint64_t* Mb = tmask + toff[b] / kTB;  // one mask per tile row
+  const int ndep = fwd ? b : nb - 1 - b;
+  T t = lane < rows ? in[r0 + lane] : T(0);
+  const unsigned long long want = static_cast<unsigned long long>(static_cast<unsigned>(epoch));
+  for (int k = 0; k < ndep; ++k) {
+    const int bp = fwd ? k : nb - 1 - k;
+    int spins = 0;
+    while (__hip_atomic_load(&flags[bp], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != want) {
+      __builtin_amdgcn_s_sleep(1);
+      if ((++spins & 31) == 0 && tile_wait_over(t0, err)) {
+        if (lane == 0) atomicAdd(err, 1);
+        break;
+      }
+    }
+    const T* Tk = Tb + static_cast<size_t>(k) * (kTB * kTB);
+    const uint64_t mk = Mb[static_cast<size_t>(k) * kTB + lane];
+    const int cmax = min(kTB, n - bp * kTB);
+    const T xmine = lane < cmax ? out[bp * kTB + lane] : T(0);
+#pragma unroll 8
+    for (int st = 0; st < kTB; ++st) {
+      const int c = fwd ? st : kTB - 1 - st;
+      const T xc = readlane_fp<T>(xmine, c);
+      const T v = Tk[c * kTB + lane];
+      if ((mk >> c) & 1) t = t - v * xc;
+    }
+  }
+  const T* Td = Tb + static_cast<size_t>(ndep) * (kTB * kTB);
+  const uint64_t md = Mb[static_cast<size_t>(ndep) * kTB + lane];
+  const T dv = (K != TriKind::kL && lane < rows) ? dinv_g[r0 + lane] : T(0);
+  T xv = T(0);
+  for (int st = 0; st < kTB; ++st) {
+    const int c = fwd ? st : kTB - 1 - st;
+    if (c < rows) {
+      const T mine = acc_final<T, K>(t, dv, 0, 0);
+      if (lane == c) xv = mine;
+      const T xc = readlane_fp<T>(mine, c);
+      const T v = Td[c * kTB + lane];
+      if ((md >> c) & 1) t = t - v * xc;
+    }
+  }
+  if (lane < rows) out[r0 + lane] = xv;
+  __threadfence();
+  if (lane == 0) __hip_atomic_store(&flags[b], want, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  tile_done(tickets, nb);
 }
 
 template <typename T, TriKind K>
@@ -1896,17 +1992,43 @@ size_t tri_blocked_lds(int n, size_t elem, size_t acc) {
 hipError_t launch_trsv_tiles(TriKind kind, int n, int nb, const int32_t* tiles,
                              const long long* toff, const int32_t* dinv, int bit, int dbit,
                              const void* in, void* out, unsigned long long* xp, int epoch,
-                             int* err, hipStream_t s) {
+                             int* err, unsigned* tickets, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   auto go = [&](auto kern) -> hipError_t {
     hipLaunchKernelGGL(kern, dim3(nb), dim3(kTileThreads), 0, s, n, nb, tiles, toff, dinv, bit,
                        dbit, static_cast<const int32_t*>(in), static_cast<int32_t*>(out), xp,
-                       epoch, err);
+                       epoch, err, tickets);
     return hipGetLastError();
   };
   if (kind == TriKind::kL) return go(&k_tri_tiles_int<TriKind::kL>);
   if (kind == TriKind::kDU) return go(&k_tri_tiles_int<TriKind::kDU>);
   return go(&k_tri_tiles_int<TriKind::kLD>);
+}
+
+hipError_t launch_trsv_tiles_fp(Elem e, TriKind kind, int n, int nb, const void* tiles,
+                                const long long* toff, const uint64_t* tmask, const void* dinv,
+                                const void* in, void* out, unsigned long long* flags, int epoch,
+                                int* err, unsigned* tickets, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (nb > kTileMaxBlocks) return hipErrorInvalidValue;
+  auto go = [&](auto tag, auto kern) -> hipError_t {
+    using T = decltype(tag);
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(64), 0, s, n, nb, static_cast<const T*>(tiles), toff,
+                       tmask, static_cast<const T*>(dinv), static_cast<const T*>(in),
+                       static_cast<T*>(out), flags, epoch, err, tickets);
+    return hipGetLastError();
+  };
+  if (e == Elem::kF64) {
+    if (kind == TriKind::kL) return go(double{}, &k_tri_tiles_fp<double, TriKind::kL>);
+    if (kind == TriKind::kDU) return go(double{}, &k_tri_tiles_fp<double, TriKind::kDU>);
+    return go(double{}, &k_tri_tiles_fp<double, TriKind::kLD>);
+  }
+  if (e == Elem::kF32) {
+    if (kind == TriKind::kL) return go(float{}, &k_tri_tiles_fp<float, TriKind::kL>);
+    if (kind == TriKind::kDU) return go(float{}, &k_tri_tiles_fp<float, TriKind::kDU>);
+    return go(float{}, &k_tri_tiles_fp<float, TriKind::kLD>);
+  }
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_trsv_blocked(Elem e, TriKind kind, const DCsr& a, const int* bptr,

@@ -62,7 +62,40 @@ __device__ __forceinline__ int32_t acc_final<int32_t, TriKind::kDU>(uint64_t t, 
 }
 
 constexpr int kTileThreads = 256;
-constexpr int kTileMaxSpin = 1 << 18;
+
+// Wait budget of the tiled coarse solves: one launch gives up its waits once
+// 2^27 ticks of the constant 100 MHz clock (1.3 s) have passed since the
+// workgroup started, or as soon as any workgroup has recorded an error.
+// Workgroups take their block from a ticket counter in processing order, so
+// every wait is on a workgroup that is already running and ends within
+// microseconds; the budget only bounds a launch when something else is wrong.
+constexpr long long kTileMaxTicks = 1ll << 27;
+
+__device__ __forceinline__ bool tile_wait_over(long long t0, int* err) {
+  return wall_clock64() - t0 > kTileMaxTicks ||
+         __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+}
+
+// The processing position of this workgroup (0, 1, ... in the order
+// workgroups start), shared through LDS; tickets[0] counts starts.
+__device__ __forceinline__ int tile_ticket(unsigned* tickets, int* sh) {
+  if (threadIdx.x == 0) *sh = static_cast<int>(atomicAdd(&tickets[0], 1u));
+  __syncthreads();
+  return *sh;
+}
+
+// The last of `total` workgroups to finish resets both counters for the
+// next launch on the stream (tickets[1] counts finishes).
+__device__ __forceinline__ void tile_done(unsigned* tickets, int total) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned done = atomicAdd(&tickets[1], 1u);
+    if (done == static_cast<unsigned>(total) - 1) {
+      atomicExch(&tickets[0], 0u);
+      atomicExch(&tickets[1], 0u);
+    }
+  }
+}
 
 // Workgroup barrier that orders LDS only: the solution is exchanged through
 // the LDS ring, so outstanding global loads (the prefetch of the next levels)

@@ -83,30 +83,39 @@ def test_failing_prediction_falls_back_to_defaults():
 
 
 def test_stall_stop_single_and_block():
-    """A solve that stops improving ends with STAGNATION after the set number
-    of steps (single and block solvers alike); with no stall limit the
-    same solve runs as the reference's would."""
+    """A solve that stops improving ends with STAGNATION once its implicit
+    residual has not fallen for the set number of steps (single and block
+    solvers alike, the block's right-hand sides each as their single solve);
+    without the stall limit the same solve runs on until the reference's own
+    tests end it."""
     sy = S.make_system(S.GEN_DIFFUSION27, (20, 5, 1.0, 1.0, 1e-3, 3), 3)
     bi, bd = S.Hierarchy.build_multi(sy.a, [S.INT, S.FP64], S.amg_param(S.BLOCK_SOR))
     bit = S.Amg(bd).default_frac_bits(sy.b, sy.nrm_b)
     amg = S.Amg(bi)
-    # 1e-15 is below what F fractional bits resolve: the implicit residual
-    # levels off
-    sol = S.Solver(S.INT, sy.a, amg, m=30, eps=1e-15, bit=bit)
+    # 1e-30 is out of reach: the residual levels off at the FP64 outer
+    # refinement's floor
+    sol = S.Solver(S.INT, sy.a, amg, m=30, eps=1e-30, bit=bit)
     sol.set_stall(5)
     x, cv = sol.solve(sy.b, sy.nrm_b)
     h = sol.history()
     print(f"[stall] status {cv.status} after {len(h)} steps", flush=True)
     assert cv.status == S.STAGNATION and not cv.is_converged
-    blk = S.BlockSolver(sy.a, amg, [bit, bit], m=30, eps=1e-15)
+    blk = S.BlockSolver(sy.a, amg, [bit, bit, 24], m=30, eps=1e-30)
     blk.set_stall(5)
-    xb, cvb = blk.solve([sy.b, sy.b], [sy.nrm_b, sy.nrm_b])
+    xb, cvb = blk.solve([sy.b, sy.b, sy.b], [sy.nrm_b] * 3)
     for r in range(2):
         assert cvb[r].status == S.STAGNATION
         assert np.array_equal(blk.history(r), h)
         assert np.array_equal(xb[r].view(np.uint64), x.view(np.uint64))
+    one = S.Solver(S.INT, sy.a, amg, m=30, eps=1e-30, bit=24)
+    one.set_stall(5)
+    x3, cv3 = one.solve(sy.b, sy.nrm_b)
+    assert cvb[2].status == cv3.status and np.array_equal(blk.history(2), one.history())
+    assert np.array_equal(xb[2].view(np.uint64), x3.view(np.uint64))
     sol.set_stall(0)
     x2, cv2 = sol.solve(sy.b, sy.nrm_b)
+    print(f"[stall] without the limit: status {cv2.status} after {len(sol.history())} steps",
+          flush=True)
     assert len(sol.history()) > len(h)
 
 
