@@ -21,6 +21,7 @@ import os
 import re
 import subprocess
 import sys
+import threading
 import time
 
 import numpy as np
@@ -390,6 +391,37 @@ def run_learned(args):
     }
 
 
+def run_vcycles(args):
+    """k-RHS block V-cycle against k single V-cycles on device vectors
+    (INTAMG.hpp:269-293 on several right-hand sides per launch), at 128^3
+    (3D 7-point Laplacian) and on the 160^3 inclusions system."""
+    import sc23ad as S
+    out = {}
+    cases = [("laplace3d_128", S.GEN_LAPLACE3D, (128,)),
+             ("inclusions3d_160", S.GEN_INCLUSIONS3D, (160, 64, 4.0, 12.0, 1e6, 4))]
+    for tag, gen, params in cases:
+        a = S.CsrMat.generate(gen, *params)
+        a.lmax1_scaling(np.ones(a.n))
+        hi = S.Hierarchy(a, S.INT, S.amg_param(args.smoother))
+        amg = S.Amg(hi)
+        rng = np.random.default_rng(7)
+        v = rng.integers(-(2**29), 2**29, (16, a.n)).astype(np.int32)
+        single_ms, _ = amg.time_vcycles(v[0], 20)
+        r = {"n": a.n, "native": S.block_native(amg), "single_ms": single_ms}
+        for k in (4, 8, 16):
+            ms = S.time_vcycles_block(amg, v[:k], 20)
+            r[f"block{k}_ms"] = ms
+            r[f"block{k}_per_rhs_vs_single"] = single_ms * k / ms
+        log(f"V-cycles {tag}: single {single_ms:.3f} ms; " +
+            ", ".join(f"k={k} {r[f'block{k}_ms']:.3f} ms ({r[f'block{k}_per_rhs_vs_single']:.2f}x)"
+                      for k in (4, 8, 16)))
+        out[tag] = r
+    return {"metric": "block V-cycle throughput over single V-cycles, 16 right-hand sides "
+                      "(3D 7-point Laplacian 128^3)",
+            "value": out["laplace3d_128"]["block16_per_rhs_vs_single"], "unit": "x",
+            "higher_is_better": True, "config": {"workload": "vcycles"}, "cases": out}
+
+
 def run_scale(args):
     """Learned + batched at scale: the 3D 7-point diffusion system on a 160^3
     grid (4,096,000 rows; k = 1 with 64 spherical inclusions of radius
@@ -551,14 +583,15 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run the copy kernel (MIB MiB per launch) for counter calibration")
     ap.add_argument("--workload", default="laplace3d",
-                    choices=("laplace3d", "poisson2d", "batch", "learned", "scale"),
+                    choices=("laplace3d", "poisson2d", "batch", "learned", "scale", "vcycles"),
                     help="laplace3d: the 3D 7-point solve (default); poisson2d: the 2D "
                          "5-point 256^2 case with SOR (the reference's CPU-sized case); "
                          "batch: uneven 2D "
                          "convection-diffusion systems solved at once; learned: learned "
                          "per-level parameters on the 27-point diffusion system (--n 96); "
                          "scale: learned + batched right-hand sides on the 160^3 "
-                         "inclusions system against the plain FP64 / FP32 solvers")
+                         "inclusions system against the plain FP64 / FP32 solvers; "
+                         "vcycles: block V-cycles of 4-16 right-hand sides against single ones")
     ap.add_argument("--rhs", type=int, default=16, help="scale: right-hand sides")
     ap.add_argument("--systems", type=int, default=256, help="batch: number of systems")
     ap.add_argument("--threads", type=int, default=16, help="batch: host threads")
@@ -567,8 +600,12 @@ def main():
     ap.add_argument("--extras", default="batch,learned,scale",
                     help="laplace3d: the other workloads run after it and reported as "
                          "extra keys of the same line (comma list; '' for none)")
-    ap.add_argument("--extras-budget", type=float, default=480.0,
-                    help="seconds since start after which no further extra is started")
+    ap.add_argument("--extras-budget", type=float, default=500.0,
+                    help="seconds since start by which the extras should have ended: "
+                         "an extra starts only if its expected time fits")
+    ap.add_argument("--extras-deadline", type=float, default=560.0,
+                    help="seconds since start at which the line is printed with the "
+                         "extras done so far and the process ends")
     args = ap.parse_args()
     if args.workload == "batch" and args.m == 30 and "--m" not in sys.argv:
         args.m = 20
@@ -586,10 +623,11 @@ def main():
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy, 9)
         return
-    if args.workload in ("batch", "learned", "scale"):
+    if args.workload in ("batch", "learned", "scale", "vcycles"):
         if args.workload == "learned" and "--n" not in sys.argv:
             args.n = 96
-        fn = {"batch": run_batch, "learned": run_learned, "scale": run_scale}[args.workload]
+        fn = {"batch": run_batch, "learned": run_learned, "scale": run_scale,
+              "vcycles": run_vcycles}[args.workload]
         t = time.perf_counter()
         rec = fn(args)
         wall = time.perf_counter() - t
@@ -761,20 +799,63 @@ def main():
         rec["speedup_vs_cpu"] = cpu["value"] / t_int
     if args.workload == "laplace3d" and args.n == 128:
         run_extras(args, rec)
-    print(json.dumps(rec), flush=True)
+    _emit(rec)
+
+
+_EMIT_LOCK = threading.Lock()
+_EMITTED = [False]
+
+
+def _emit(rec):
+    """Print the one JSON line (once, whichever of the main thread and the
+    deadline watchdog gets here first)."""
+    with _EMIT_LOCK:
+        if _EMITTED[0]:
+            return
+        _EMITTED[0] = True
+        print(json.dumps(rec), flush=True)
+
+
+# Seconds an extra workload takes on one MI355X (setup included), with room:
+# an extra starts only if it ends within the budget by this estimate.
+EXTRA_EST_S = {"batch": 150.0, "learned": 100.0, "scale": 200.0}
 
 
 def run_extras(args, rec):
     """The metric's third part (learned / batched vs plain) on its own
     workloads, after the main line's measurement: each as an extra key of the
-    same JSON line, each started only while the time budget lasts."""
+    same JSON line, each started only if it should end within the time
+    budget.  A watchdog prints the line with what is done by the hard
+    deadline, should an extra run long, and ends the process."""
     import copy
     wanted = [w for w in args.extras.split(",") if w]
+
+    def watchdog():
+        with _EMIT_LOCK:
+            done = _EMITTED[0]
+        if not done:
+            snap = {}
+            for _ in range(10):  # the main thread may be adding a key
+                try:
+                    snap = dict(rec)
+                    break
+                except RuntimeError:
+                    time.sleep(0.01)
+            snap["extras_deadline"] = f"line printed at the {args.extras_deadline:.0f} s deadline"
+            log(f"extras: deadline of {args.extras_deadline:.0f} s reached")
+            _emit(snap)
+            os._exit(0)
+
+    left = args.extras_deadline - (time.time() - T_START)
+    timer = threading.Timer(max(left, 1.0), watchdog)
+    timer.daemon = True
+    timer.start()
     for w in wanted:
         elapsed = time.time() - T_START
-        if elapsed > args.extras_budget:
-            rec[f"extra_{w}"] = {"skipped": f"time budget ({elapsed:.0f} s > "
-                                            f"{args.extras_budget:.0f} s)"}
+        est = EXTRA_EST_S.get(w, 120.0)
+        if elapsed + est > args.extras_budget:
+            rec[f"extra_{w}"] = {"skipped": f"time budget ({elapsed:.0f} s elapsed + "
+                                            f"{est:.0f} s expected > {args.extras_budget:.0f} s)"}
             log(f"extra {w}: skipped, {elapsed:.0f} s elapsed")
             continue
         a = copy.copy(args)

@@ -239,6 +239,13 @@ typedef struct sc23ad_block_solver_s sc23ad_block_solver;
    Integer hierarchies only. */
 int sc23ad_amg_precondition_block(sc23ad_amg* m, int nrhs, const int32_t* in, int32_t* out,
                                   int on_device);
+/* Mean duration of `reps` back-to-back block V-cycles on nrhs device
+   vectors in the block layout (hipEvents on the library stream, after one
+   warm-up cycle); out receives the V-cycles of in.  No reference
+   counterpart (measurement of intAMGflex::Precondition, INTAMG.hpp:269-293,
+   on several right-hand sides at once). */
+int sc23ad_amg_time_vcycles_block(sc23ad_amg* m, int nrhs, int reps, const int32_t* in,
+                                  int32_t* out, double* ms_per_cycle);
 /* 1 when every LD solve of the hierarchy has a block kernel (otherwise those
    levels take the right-hand sides one at a time; results are the same). */
 int sc23ad_amg_block_native(const sc23ad_amg* m, int* native);

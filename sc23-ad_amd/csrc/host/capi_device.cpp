@@ -363,6 +363,33 @@ int sc23ad_amg_precondition_block(sc23ad_amg* m, int nrhs, const int32_t* in, in
   });
 }
 
+int sc23ad_amg_time_vcycles_block(sc23ad_amg* m, int nrhs, int reps, const int32_t* in,
+                                  int32_t* out, double* ms_per_cycle) {
+  if (!m || !in || !out || !ms_per_cycle || nrhs < 1 || reps < 1)
+    return fail(SC23AD_BAD_PARAM, "time_vcycles_block: bad arguments");
+  if (m->precision != SC23AD_INT)
+    return fail(SC23AD_BAD_PARAM, "time_vcycles_block: integer hierarchies only");
+  return guarded([&]() -> int {
+    DevAmg& a = *m->d;
+    auto w = a.make_block_work((nrhs + 3) / 4);
+    a.vcycle_blk(in, out, *w);  // warm-up
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "event");
+    hip_check(hipEventCreate(&e1), "event");
+    hip_check(hipEventRecord(e0, lib_stream()), "record");
+    for (int r = 0; r < reps; ++r) a.vcycle_blk(in, out, *w);
+    hip_check(hipEventRecord(e1, lib_stream()), "record");
+    hip_check(hipEventSynchronize(e1), "sync");
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_per_cycle = ms / reps;
+    a.check_errors(*w);
+    return SC23AD_OK;
+  });
+}
+
 int sc23ad_amg_block_native(const sc23ad_amg* m, int* native) {
   if (!m || !native) return fail(SC23AD_BAD_PARAM, "amg_block_native: null");
   *native = (m->precision == SC23AD_INT && m->d->block_native()) ? 1 : 0;

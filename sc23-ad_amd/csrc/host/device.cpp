@@ -702,10 +702,12 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
             int fq = 0;
             if (g == 1 && span_all > rb) {
               fq = far_count(rb);
-              if (fq > (ge == 4 ? 1 : 2)) continue;
+              if (fq > ge) continue;
             }
             for (int rows_b : {1, 2}) {
               if (rows_b == 2 && g > 1 && ge > 8) continue;  // no such block kernel
+              // two rows per lane: 1 (4 entries) or 2 far slots
+              if (rows_b == 2 && fq > (ge == 4 ? 1 : 2)) continue;
               const size_t lanes = (maxlev_rows + rows_b - 1) / rows_b * g;
               const size_t w = (lanes + 63) / 64;
               if (w > 8) continue;

@@ -361,14 +361,27 @@ hipError_t rpb_go(const TriLayout& L, const TriSegment& seg, int groups, int bit
 // one row per lane: 6 levels for 4 entries, 4 for 8, 3 for 10 and 12; two
 // rows per lane: 4 for 4 entries, 2 otherwise.  Far values are loaded for
 // the slots that can hold one (a row's far entries come first): 1 with 4
-// entries, 2 with more; layouts with several lanes per row have no far
-// entries.
+// entries, 2 with more, and with one row per lane also 4 or every slot for
+// factors whose rows have more far entries; layouts with several lanes per
+// row have no far entries.
 template <bool kFar, int kE, int kR>
 hipError_t rpb_one_lane(const TriLayout& L, const TriSegment& seg, int groups, int bit, int dbit,
                         const int4* in, int4* out, hipStream_t s) {
   constexpr int kFq = kE == 4 ? 1 : 2;
   constexpr int kD = kR == 1 ? (kE == 4 ? 6 : kE == 8 ? 4 : 3) : (kE == 4 ? 4 : 2);
   constexpr int kA = kR == 1 ? (kE == 4 ? 3 : kE == 8 ? 2 : 1) : (kE == 4 ? 2 : 1);
+  if constexpr (kFar && kR == 1) {
+    // rows with more far entries (a ring of four values per slot holds a
+    // quarter of the single kernel's span): 4 far slots, or every slot
+    if (L.rp_farq > kFq) {
+      if (kE > 4 && L.rp_farq <= 4)
+        return rpb_go<kFar, 1, kE, kR, kD, kA, 4>(L, seg, groups, bit, dbit, in, out, s);
+      // every slot far: one level less of prefetch for 8 and 12 entries,
+      // whose registers spill otherwise
+      constexpr int kDa = (kE == 8 || kE == 12) ? kD - 1 : kD;
+      return rpb_go<kFar, 1, kE, kR, kDa, kA, kE>(L, seg, groups, bit, dbit, in, out, s);
+    }
+  }
   return rpb_go<kFar, 1, kE, kR, kD, kA, kFq>(L, seg, groups, bit, dbit, in, out, s);
 }
 
@@ -464,6 +477,16 @@ __global__ __launch_bounds__(kTileThreads) void k_tri_tiles_blk(
     t[2] = static_cast<uint64_t>(static_cast<int64_t>(iv.z)) << bit;
     t[3] = static_cast<uint64_t>(static_cast<int64_t>(iv.w)) << bit;
   }
+  // the diagonal tile and the row scales depend on no other block: wave 0
+  // loads them before its first wait, so their latency stays off the chain
+  int32_t reg[kTB];
+  int32_t dv = 0;
+  if (w == 0) {
+    const int32_t* Td = T + static_cast<size_t>(ndep) * (kTB * kTB);
+#pragma unroll
+    for (int c = 0; c < kTB; ++c) reg[c] = Td[c * kTB + lane];
+    dv = (K != TriKind::kL && lane < rows) ? dinv_g[r0 + lane] : 0;
+  }
   for (int k = 0; k < ndep; ++k) {
     const int bp = fwd ? k : nb - 1 - k;
     const int32_t* Tk = T + static_cast<size_t>(k) * (kTB * kTB);
@@ -511,11 +534,6 @@ __global__ __launch_bounds__(kTileThreads) void k_tri_tiles_blk(
     }
   }
   if (w == 0) {
-    const int32_t* Td = T + static_cast<size_t>(ndep) * (kTB * kTB);
-    int32_t reg[kTB];
-#pragma unroll
-    for (int c = 0; c < kTB; ++c) reg[c] = Td[c * kTB + lane];
-    const int32_t dv = (K != TriKind::kL && lane < rows) ? dinv_g[r0 + lane] : 0;
     int32_t xv[4] = {0, 0, 0, 0};
 #pragma unroll
     for (int st = 0; st < kTB; ++st) {
@@ -650,7 +668,7 @@ bool trsv_rp_blk_supported(const TriLayout& Lin) {
   if (L.rp_rows != 1 && L.rp_rows != 2) return false;
   if (L.rp_lanes <= 1)
     return (L.rp_e == 4 || L.rp_e == 8 || L.rp_e == 10 || L.rp_e == 12) &&
-           (!L.ep_far || L.rp_farq <= (L.rp_e == 4 ? 1 : 2));
+           (!L.ep_far || L.rp_farq <= (L.rp_rows == 1 ? L.rp_e : L.rp_e == 4 ? 1 : 2));
   return !L.ep_far && (L.rp_e == 8 || ((L.rp_e == 10 || L.rp_e == 12) && L.rp_rows == 1)) &&
          (L.rp_lanes == 2 || L.rp_lanes == 4 || L.rp_lanes == 8 || L.rp_lanes == 16);
 }

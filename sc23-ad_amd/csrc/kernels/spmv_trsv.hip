@@ -658,6 +658,16 @@ __global__ __launch_bounds__(kTileThreads) void k_tri_tiles_int(
   const int ndep = fwd ? b : nb - 1 - b;
   uint64_t t = 0;
   if (w == 0 && lane < rows) t = static_cast<uint64_t>(static_cast<int64_t>(in[r0 + lane])) << bit;
+  // the diagonal tile and the row scales depend on no other block: wave 0
+  // loads them before its first wait, so their latency stays off the chain
+  int32_t reg[kTB];
+  int32_t dv = 0;
+  if (w == 0) {
+    const int32_t* Td = T + static_cast<size_t>(ndep) * (kTB * kTB);
+#pragma unroll
+    for (int c = 0; c < kTB; ++c) reg[c] = Td[c * kTB + lane];
+    dv = (K != TriKind::kL && lane < rows) ? dinv_g[r0 + lane] : 0;
+  }
   for (int k = 0; k < ndep; ++k) {
     const int bp = fwd ? k : nb - 1 - k;
     const int32_t* Tk = T + static_cast<size_t>(k) * (kTB * kTB);
@@ -692,19 +702,24 @@ __global__ __launch_bounds__(kTileThreads) void k_tri_tiles_int(
     }
   }
   if (w == 0) {
-    const int32_t* Td = T + static_cast<size_t>(ndep) * (kTB * kTB);
-    int32_t reg[kTB];
-#pragma unroll
-    for (int c = 0; c < kTB; ++c) reg[c] = Td[c * kTB + lane];
-    const int32_t dv = (K != TriKind::kL && lane < rows) ? dinv_g[r0 + lane] : 0;
+    // Diagonal sweep.  The chain from one row's value to the next runs on
+    // the scalar unit: row c's complete accumulator st_c (uniform) gives x_c;
+    // the next row's accumulator is read from its lane before this step's
+    // vector update and gets x_c's term in scalar arithmetic, while the
+    // vector update applies x_c to every row of the tile.  Integer sums are
+    // order-free, so each row gets exactly the sequential solve's value.
     int32_t xv = 0;
+    uint64_t st_c = readlane_u64(t, fwd ? 0 : rows - 1);
 #pragma unroll
     for (int st = 0; st < kTB; ++st) {
       const int c = fwd ? st : kTB - 1 - st;
       if (c < rows) {
-        const int32_t mine = acc_final<int32_t, K>(t, dv, bit, dbit);
-        if (lane == c) xv = mine;
-        const int32_t xc = __builtin_amdgcn_readlane(mine, c);
+        const int32_t xc =
+            acc_final<int32_t, K>(st_c, __builtin_amdgcn_readlane(dv, c), bit, dbit);
+        if (lane == c) xv = xc;
+        const int cn = fwd ? c + 1 : c - 1;
+        if (fwd ? cn < rows : cn >= 0)
+          st_c = readlane_u64(t, cn) - prod64(__builtin_amdgcn_readlane(reg[c], cn), xc);
         t -= prod64(reg[c], xc);
       }
     }
@@ -754,6 +769,7 @@ __global__ __launch_bounds__(64) void k_tri_tiles_fp(
     const T* __restrict__ in, T* out, unsigned long long* flags, int epoch, int* err,
     unsigned* tickets) {
   constexpr bool fwd = (K != TriKind::kDU);
+  __shared__ T dl[kTB][kTB];  // the diagonal tile, [column][row]: lane r reads only its row
   __shared__ int ticket_sh;
   const long long t0 = wall_clock64();
   const int pb = tile_ticket(tickets, &ticket_sh);
@@ -764,10 +780,23 @@ __global__ __launch_bounds__(64) void k_tri_tiles_fp(
   const T* Tb = tiles + toff[b];
   const uint64_t* Mb = tmask + toff[b] / kTB;  // one mask per tile row
   const int ndep = fwd ? b : nb - 1 - b;
+  // the diagonal tile, its masks and the row scales depend on no other
+  // block: loaded before the first wait, so their latency stays off the chain
+  const T* Td = Tb + static_cast<size_t>(ndep) * (kTB * kTB);
+#pragma unroll
+  for (int c = 0; c < kTB; ++c) dl[c][lane] = Td[c * kTB + lane];
+  const uint64_t md = Mb[static_cast<size_t>(ndep) * kTB + lane];
+  const T dv = (K != TriKind::kL && lane < rows) ? dinv_g[r0 + lane] : T(0);
   T t = lane < rows ? in[r0 + lane] : T(0);
   const unsigned long long want = static_cast<unsigned long long>(static_cast<unsigned>(epoch));
   for (int k = 0; k < ndep; ++k) {
     const int bp = fwd ? k : nb - 1 - k;
+    // the dependency tile is loaded before the wait for its block
+    const T* Tk = Tb + static_cast<size_t>(k) * (kTB * kTB);
+    T vk[kTB];
+#pragma unroll
+    for (int c = 0; c < kTB; ++c) vk[c] = Tk[c * kTB + lane];
+    const uint64_t mk = Mb[static_cast<size_t>(k) * kTB + lane];
     int spins = 0;
     while (__hip_atomic_load(&flags[bp], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != want) {
       __builtin_amdgcn_s_sleep(1);
@@ -776,30 +805,24 @@ __global__ __launch_bounds__(64) void k_tri_tiles_fp(
         break;
       }
     }
-    const T* Tk = Tb + static_cast<size_t>(k) * (kTB * kTB);
-    const uint64_t mk = Mb[static_cast<size_t>(k) * kTB + lane];
     const int cmax = min(kTB, n - bp * kTB);
     const T xmine = lane < cmax ? out[bp * kTB + lane] : T(0);
-#pragma unroll 8
+#pragma unroll
     for (int st = 0; st < kTB; ++st) {
       const int c = fwd ? st : kTB - 1 - st;
       const T xc = readlane_fp<T>(xmine, c);
-      const T v = Tk[c * kTB + lane];
-      if ((mk >> c) & 1) t = t - v * xc;
+      if ((mk >> c) & 1) t = t - vk[c] * xc;
     }
   }
-  const T* Td = Tb + static_cast<size_t>(ndep) * (kTB * kTB);
-  const uint64_t md = Mb[static_cast<size_t>(ndep) * kTB + lane];
-  const T dv = (K != TriKind::kL && lane < rows) ? dinv_g[r0 + lane] : T(0);
   T xv = T(0);
+#pragma unroll
   for (int st = 0; st < kTB; ++st) {
     const int c = fwd ? st : kTB - 1 - st;
     if (c < rows) {
       const T mine = acc_final<T, K>(t, dv, 0, 0);
       if (lane == c) xv = mine;
       const T xc = readlane_fp<T>(mine, c);
-      const T v = Td[c * kTB + lane];
-      if ((md >> c) & 1) t = t - v * xc;
+      if ((md >> c) & 1) t = t - dl[c][lane] * xc;
     }
   }
   if (lane < rows) out[r0 + lane] = xv;

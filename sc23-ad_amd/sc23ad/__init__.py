@@ -124,6 +124,8 @@ def _load() -> C.CDLL:
                                        C.POINTER(C.c_float)]),
         "sc23ad_learned_levels": (C.c_int, [vp, vp, vp, vp, C.c_int, ip]),
         "sc23ad_amg_precondition_block": (C.c_int, [vp, C.c_int, vp, vp, C.c_int]),
+        "sc23ad_amg_time_vcycles_block": (C.c_int, [vp, C.c_int, C.c_int, vp, vp,
+                                                    C.POINTER(C.c_double)]),
         "sc23ad_amg_block_native": (C.c_int, [vp, ip]),
         "sc23ad_block_solver_create": (C.c_int, [vp, vp, C.c_int, C.c_int, C.c_double, vp,
                                                  C.c_int, C.POINTER(vp)]),
@@ -689,6 +691,19 @@ def precondition_block(amg: "Amg", vins: np.ndarray) -> np.ndarray:
     out = np.zeros_like(blk)
     check(lib.sc23ad_amg_precondition_block(amg.h, k, ptr(blk), ptr(out), 0))
     return from_block(out, k, n)
+
+
+def time_vcycles_block(amg: "Amg", vins: np.ndarray, reps: int = 20) -> float:
+    """Mean milliseconds of one block V-cycle on the rows of vins ((k, n)
+    int32), on device vectors."""
+    vins = np.asarray(vins, dtype=np.int32)
+    k, n = vins.shape
+    blk = to_block(vins)
+    din = DevArray(blk)
+    dout = DevArray(np.zeros_like(blk))
+    ms = C.c_double()
+    check(lib.sc23ad_amg_time_vcycles_block(amg.h, k, reps, din.p, dout.p, C.byref(ms)))
+    return ms.value
 
 
 def block_native(amg: "Amg") -> bool:

@@ -63,6 +63,10 @@ class LearnedResult:
     learned: LearnedSetup
     bit: int
     relres: float
+    # the learned attempt's own steps (all of them when the fallback ran:
+    # the total cost is then attempt_iter + converge.iter)
+    attempt_iter: int = 0
+    attempt_ms: float = 0.0
 
 
 def learned_solve(a, b, nrm_b, mlp: Mlp, param=None, m: int = 30, eps: float = 1e-8,
@@ -83,9 +87,10 @@ def learned_solve(a, b, nrm_b, mlp: Mlp, param=None, m: int = 30, eps: float = 1
     x, cv = sol.solve(b, nrm_b)
     rel = sol.relres(b, nrm_b, x)
     if cv.status == OK and cv.is_converged and rel <= eps:
-        return LearnedResult(x, cv, False, ls, bit, rel)
+        return LearnedResult(x, cv, False, ls, bit, rel, cv.iter, cv.solve_ms)
     bit0 = Amg(bd).default_frac_bits(b, nrm_b)
     sol0 = Solver(INT, a, Amg(bi), m=m, eps=eps, bit=bit0)
     x0, cv0 = sol0.solve(b, nrm_b)
     cv0.fallback_used = 1
-    return LearnedResult(x0, cv0, True, ls, bit0, sol0.relres(b, nrm_b, x0))
+    return LearnedResult(x0, cv0, True, ls, bit0, sol0.relres(b, nrm_b, x0),
+                         len(sol.history()), cv.solve_ms)

@@ -93,6 +93,23 @@ def test_spmv_and_vcycles(big):
                           big["or_d"].vcycle(d).view(np.uint64))
 
 
+def test_block_vcycle_at_full_size(big):
+    """The block V-cycle at 128^3 (five right-hand sides: a full group of four
+    and a padded one) gives every right-hand side its single V-cycle, which
+    test_spmv_and_vcycles pins to the oracle; the block kernels cover every
+    level (the mid levels read their far dependencies from global memory)."""
+    n = big["sy"].a.n
+    amg = big["amg_i"]
+    rng = np.random.default_rng(11)
+    vins = rng.integers(-(2**29), 2**29, (5, n)).astype(np.int32)
+    native = S.block_native(amg)
+    print(f"[block vcycle] 128^3 native {native}", flush=True)
+    outs = S.precondition_block(amg, vins)
+    for r in range(5):
+        assert np.array_equal(outs[r], amg.precondition(vins[r])), r
+    assert native
+
+
 def test_int_solve_to_1e8(big):
     sy = big["sy"]
     bit = big["amg_d"].default_frac_bits(sy.b, sy.nrm_b)
