@@ -204,6 +204,19 @@ typedef enum {
 int sc23ad_amg_level_op(sc23ad_amg* m, int level, int op, const void* in,
                         void* out);
 
+/* sc23ad_amg_level_op without any allocation (no counterpart in the
+   reference): for callers that cannot allocate device memory (code under
+   HIP graph capture, tight serving loops, harnesses that forbid it).  The
+   operand and the result are staged, in the level's numbering, in a static
+   device array of 16 MiB (two vectors of at most 8 MiB: 2,097,152 int32 or
+   FP32 values, 1,048,576 FP64 values); larger levels are rejected with
+   SC23AD_BAD_PARAM.  Every call on a GPU shares that GPU's array, so calls
+   that overlap on one GPU race; the array keeps the last call's contents,
+   all of which a call overwrites before reading.  in, out: device vectors
+   as for sc23ad_amg_level_op, whose results this returns. */
+int sc23ad_amg_level_op_noalloc(sc23ad_amg* m, int level, int op, const void* in,
+                                void* out);
+
 /* FP64 fine operator for the outer residual (the reference's dA). */
 int sc23ad_solver_create(int precision, const sc23ad_csrmat* a, sc23ad_amg* m,
                          int restart_m, double eps, int frac_bits,

@@ -239,6 +239,39 @@ int sc23ad_amg_level_op(sc23ad_amg* m, int level, int op, const void* in, void* 
   });
 }
 
+int sc23ad_amg_level_op_noalloc(sc23ad_amg* m, int level, int op, const void* in, void* out) {
+  if (!m || !in || !out) return fail(SC23AD_BAD_PARAM, "level_op_noalloc: null");
+  return guarded([&]() -> int {
+    DevAmg& a = *m->d;
+    if (op == SC23AD_OP_COARSE) {
+      if (level != a.depth)
+        return fail(SC23AD_BAD_PARAM, "level_op_noalloc: coarse solve is level depth");
+    } else {
+      if (level < 0 || level >= a.depth)
+        return fail(SC23AD_BAD_PARAM, "level_op_noalloc: bad level");
+      if (op < SC23AD_OP_SPMV_A || op > SC23AD_OP_SMOOTH1)
+        return fail(SC23AD_BAD_PARAM, "level_op_noalloc: bad op");
+    }
+    const int nin = op == SC23AD_OP_SPMV_P ? a.level_n(level + 1) : a.level_n(level);
+    const int nout = op == SC23AD_OP_SPMV_R ? a.level_n(level + 1) : a.level_n(level);
+    const size_t half = dev::kNoAllocScratchBytes / 2;
+    const size_t need = a.esize() * static_cast<size_t>(std::max(nin, nout));
+    if (need > half)
+      return fail(SC23AD_BAD_PARAM,
+                  "level_op_noalloc: a level vector of " + std::to_string(need) +
+                      " bytes exceeds the static scratch's " + std::to_string(half) +
+                      " bytes per vector; use sc23ad_amg_level_op");
+    lib_stream();  // the device is current
+    void* scratch = nullptr;
+    hip_check(dev::noalloc_scratch(&scratch), "static scratch address");
+    char* base = static_cast<char*>(scratch);
+    a.level_op(level, op, in, out, a.own_work(), base, base + half);
+    hip_check(hipStreamSynchronize(lib_stream()), "sync");
+    a.check_errors(a.own_work());
+    return SC23AD_OK;
+  });
+}
+
 int sc23ad_solver_create(int precision, const sc23ad_csrmat* a, sc23ad_amg* m,
                          int restart_m, double eps, int frac_bits, int max_restarts,
                          sc23ad_solver** out) {

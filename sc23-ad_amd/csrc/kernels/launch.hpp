@@ -329,6 +329,13 @@ hipError_t launch_flags_blk(const KState* ks, int count, int* out, hipStream_t s
 hipError_t launch_checksum_blk(int n, int groups, int count, const void* v, const KState* ks,
                                const int* live, hipStream_t s);
 
+// Device scratch of the allocation-free level operation (noalloc.hip): the
+// current device's address of a file-scope array of kNoAllocScratchBytes
+// bytes (two staging vectors of half that size each), looked up once per
+// device.  Every call on a device shares it.
+constexpr size_t kNoAllocScratchBytes = size_t{16} << 20;
+hipError_t noalloc_scratch(void** ptr);
+
 // V-cycle vector helpers
 hipError_t launch_copy(Elem e, int n, const void* src, void* dst, hipStream_t s);
 // gather: dst[k] = src[perm[k]]; otherwise scatter: dst[perm[k]] = src[k].
