@@ -33,6 +33,10 @@ METRIC = ("solve time to 1e-8 rel. residual; int-SpMV HBM GB/s; "
           "learned/batched vs plain")
 HBM_PEAK_GBPS = 8000.0
 T_START = time.time()
+# The batched workloads run one stream per host thread; with HIP's default of
+# 4 hardware queues per process only 4 of them run at once.  Set before the
+# library touches the GPU (the value is read when HIP initialises).
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 
 
 def log(*a):

@@ -702,24 +702,19 @@ __global__ __launch_bounds__(kTileThreads) void k_tri_tiles_int(
     }
   }
   if (w == 0) {
-    // Diagonal sweep.  The chain from one row's value to the next runs on
-    // the scalar unit: row c's complete accumulator st_c (uniform) gives x_c;
-    // the next row's accumulator is read from its lane before this step's
-    // vector update and gets x_c's term in scalar arithmetic, while the
-    // vector update applies x_c to every row of the tile.  Integer sums are
-    // order-free, so each row gets exactly the sequential solve's value.
+    // Diagonal sweep: at step c every lane finishes its row, lane c's value
+    // is broadcast and subtracted from the later rows.  (Chaining the row
+    // finish on the scalar unit instead, the next row's accumulator read
+    // from its lane, measured 45-55 % slower: the dependent scalar
+    // multiplies and the lane reads cost more than the vector finish.)
     int32_t xv = 0;
-    uint64_t st_c = readlane_u64(t, fwd ? 0 : rows - 1);
 #pragma unroll
     for (int st = 0; st < kTB; ++st) {
       const int c = fwd ? st : kTB - 1 - st;
       if (c < rows) {
-        const int32_t xc =
-            acc_final<int32_t, K>(st_c, __builtin_amdgcn_readlane(dv, c), bit, dbit);
-        if (lane == c) xv = xc;
-        const int cn = fwd ? c + 1 : c - 1;
-        if (fwd ? cn < rows : cn >= 0)
-          st_c = readlane_u64(t, cn) - prod64(__builtin_amdgcn_readlane(reg[c], cn), xc);
+        const int32_t mine = acc_final<int32_t, K>(t, dv, bit, dbit);
+        if (lane == c) xv = mine;
+        const int32_t xc = __builtin_amdgcn_readlane(mine, c);
         t -= prod64(reg[c], xc);
       }
     }

@@ -63,13 +63,6 @@ __device__ __forceinline__ int32_t acc_final<int32_t, TriKind::kDU>(uint64_t t, 
 
 constexpr int kTileThreads = 256;
 
-// A 64-bit value of lane l (uniform result).
-__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int l) {
-  const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(v)), l));
-  const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(v >> 32)), l));
-  return (static_cast<uint64_t>(hi) << 32) | lo;
-}
-
 // Wait budget of the tiled coarse solves: one launch gives up its waits once
 // 2^27 ticks of the constant 100 MHz clock (1.3 s) have passed since the
 // workgroup started, or as soon as any workgroup has recorded an error.
