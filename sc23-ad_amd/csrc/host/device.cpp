@@ -697,17 +697,26 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
         for (int g : {1, 2, 4, 8, 16})
           for (int ge : {4, 8, 10, 12}) {
             if ((g > 1 && ge == 4) || g * ge < md_b) continue;
-            const int rb = pow2(g == 1 ? span_near : span_all);
+            // the whole span in the ring when it fits, else the largest ring
+            // that fits (at least the near span), the rest read as far
+            // dependencies
+            int rb = pow2(span_all);
+            if (dev::rpb_lds_bytes(rb, maxlev_chain) > dev::kTriBlockedMaxLds) {
+              rb = pow2(span_near);
+              while (dev::rpb_lds_bytes(2 * rb, maxlev_chain) <= dev::kTriBlockedMaxLds) rb *= 2;
+            }
             if (dev::rpb_lds_bytes(rb, maxlev_chain) > dev::kTriBlockedMaxLds) continue;
             int fq = 0;
-            if (g == 1 && span_all > rb) {
+            if (span_all > rb) {
               fq = far_count(rb);
-              if (fq > ge) continue;
+              // one lane: any number of far slots; g lanes: up to four per
+              // lane (eight with 10 or 12 entries)
+              if (fq > (g == 1 ? ge : (ge >= 10 ? 8 : 4) * g)) continue;
             }
             for (int rows_b : {1, 2}) {
               if (rows_b == 2 && g > 1 && ge > 8) continue;  // no such block kernel
-              // two rows per lane: 1 (4 entries) or 2 far slots
-              if (rows_b == 2 && fq > (ge == 4 ? 1 : 2)) continue;
+              // two rows per lane: 1 (4 entries) or 2 far slots, one lane
+              if (rows_b == 2 && fq > 0 && (g > 1 || fq > (ge == 4 ? 1 : 2))) continue;
               const size_t lanes = (maxlev_rows + rows_b - 1) / rows_b * g;
               const size_t w = (lanes + 63) / 64;
               if (w > 8) continue;
@@ -723,6 +732,16 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
               }
             }
           }
+        if (best_e == 0 && std::getenv("SC23AD_VERBOSE")) {
+          int rb_near = pow2(span_near);
+          while (dev::rpb_lds_bytes(2 * rb_near, maxlev_chain) <= dev::kTriBlockedMaxLds)
+            rb_near *= 2;
+          std::fprintf(stderr,
+                       "[sc23ad plan] no block layout: max deps %d, span %lld (near %lld), "
+                       "max level rows %d, far entries per row %d at ring %d\n",
+                       md_b, span_all, span_near, static_cast<int>(maxlev_rows),
+                       far_count(rb_near), rb_near);
+        }
         if (best_e > 0) {
           const int e_b = best_e, g_b = best_g;
           std::vector<int> rb(static_cast<size_t>(std::max(n, 1)) * g_b * e_b * 2, 0);
@@ -1153,6 +1172,13 @@ void tune_spmv(DevMat& m, Elem e, SpmvMode mode) {
     cands.push_back(c16);
   }
   if (m.has2) cands.push_back(m.d2);
+  // row-sorted SELL copies (rectangular R / P): also with XCD-contiguous rows
+  for (size_t i = 0, n0 = cands.size(); i < n0; ++i)
+    if (cands[i].soff && cands[i].srow && !std::getenv("SC23AD_XCD_REMAP")) {
+      dev::DCsr c = cands[i];
+      c.remap = 1;
+      cands.push_back(c);
+    }
   // SC23AD_SPMV_FORMAT=csr|sell|sell16 keeps only that format's candidates
   // where the matrix has it (tests run every kernel this way).
   if (const char* f = std::getenv("SC23AD_SPMV_FORMAT")) {
@@ -1185,12 +1211,13 @@ void tune_spmv(DevMat& m, Elem e, SpmvMode mode) {
     float ms = 0;
     hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
     if (std::getenv("SC23AD_VERBOSE"))
-      std::fprintf(stderr, "[sc23ad spmv tune] n %d nnz %d %s%s%s lanes %d: %.2f us\n", m.d.n,
-                   m.d.nnz, cands[i].soff ? "SELL" : "CSR",
+      std::fprintf(stderr, "[sc23ad spmv tune] n %d nnz %d %s%s%s%s lanes %d: %.2f us\n",
+                   m.d.n, m.d.nnz, cands[i].soff ? "SELL" : "CSR",
                    (m.has2 && cands[i].soff == m.d2.soff) ? " column-ordered"
                    : cands[i].srow                        ? " sorted"
                                                           : "",
-                   cands[i].scol16 ? " 16-bit" : "", cands[i].group, ms * 1000.0f / 5);
+                   cands[i].scol16 ? " 16-bit" : "", cands[i].remap == 1 ? " XCD rows" : "",
+                   cands[i].group, ms * 1000.0f / 5);
     if (ms < best) {
       best = ms;
       best_i = i;

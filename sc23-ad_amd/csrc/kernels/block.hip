@@ -362,8 +362,10 @@ hipError_t rpb_go(const TriLayout& L, const TriSegment& seg, int groups, int bit
 // rows per lane: 4 for 4 entries, 2 otherwise.  Far values are loaded for
 // the slots that can hold one (a row's far entries come first): 1 with 4
 // entries, 2 with more, and with one row per lane also 4 or every slot for
-// factors whose rows have more far entries; layouts with several lanes per
-// row have no far entries.
+// factors whose rows have more far entries; with several lanes per row (one
+// row per lane) the far entries are dealt over the lanes, slot 0 of each
+// lane, then slot 1, up to four slots per lane (eight with 10 or 12
+// entries).
 template <bool kFar, int kE, int kR>
 hipError_t rpb_one_lane(const TriLayout& L, const TriSegment& seg, int groups, int bit, int dbit,
                         const int4* in, int4* out, hipStream_t s) {
@@ -385,12 +387,30 @@ hipError_t rpb_one_lane(const TriLayout& L, const TriSegment& seg, int groups, i
   return rpb_go<kFar, 1, kE, kR, kD, kA, kFq>(L, seg, groups, bit, dbit, in, out, s);
 }
 
-template <int kG, int kE, int kR>
+template <bool kFar, int kG, int kE, int kR>
 hipError_t rpb_lanes(const TriLayout& L, const TriSegment& seg, int groups, int bit, int dbit,
                      const int4* in, int4* out, hipStream_t s) {
   constexpr int kD = kR == 1 ? (kE == 8 ? 4 : 3) : 2;
   constexpr int kA = kR == 1 ? (kE == 8 ? 2 : 1) : 1;
-  return rpb_go<false, kG, kE, kR, kD, kA, 0>(L, seg, groups, bit, dbit, in, out, s);
+  if constexpr (kFar && kR == 1) {
+    // a row's far entries come first, dealt over its lanes: slot 0 of each
+    // lane, then slot 1
+    if (L.rp_farq <= kG)
+      return rpb_go<true, kG, kE, kR, kD, kA, 1>(L, seg, groups, bit, dbit, in, out, s);
+    // two or four far slots per lane: less prefetch where the registers
+    // would spill otherwise
+    constexpr int kD2 = (kE == 8 || kE == 12) ? kD - 1 : kD;
+    if (L.rp_farq <= 2 * kG)
+      return rpb_go<true, kG, kE, kR, kD2, kA, 2>(L, seg, groups, bit, dbit, in, out, s);
+    constexpr int kD4 = kA + 1;
+    if (kE < 10 || L.rp_farq <= 4 * kG)
+      return rpb_go<true, kG, kE, kR, kD4, kA, 4>(L, seg, groups, bit, dbit, in, out, s);
+    if constexpr (kE >= 10)  // eight far slots per lane (10 or 12 entries)
+      return rpb_go<true, kG, kE, kR, kD4, kA, 8>(L, seg, groups, bit, dbit, in, out, s);
+    return hipErrorInvalidValue;
+  } else {
+    return rpb_go<false, kG, kE, kR, kD, kA, 0>(L, seg, groups, bit, dbit, in, out, s);
+  }
 }
 
 template <bool kFar, int kR>
@@ -405,14 +425,14 @@ hipError_t rpb_rows(const TriLayout& L, const TriSegment& seg, int groups, int b
       default: return hipErrorInvalidValue;
     }
   }
-  if constexpr (!kFar) {
+  if constexpr (!kFar || kR == 1) {  // far entries with several lanes: one row per lane
     auto lanes = [&](auto e_tag) -> hipError_t {
       constexpr int E = decltype(e_tag)::value;
       switch (L.rp_lanes) {
-        case 2: return rpb_lanes<2, E, kR>(L, seg, groups, bit, dbit, in, out, s);
-        case 4: return rpb_lanes<4, E, kR>(L, seg, groups, bit, dbit, in, out, s);
-        case 8: return rpb_lanes<8, E, kR>(L, seg, groups, bit, dbit, in, out, s);
-        case 16: return rpb_lanes<16, E, kR>(L, seg, groups, bit, dbit, in, out, s);
+        case 2: return rpb_lanes<kFar, 2, E, kR>(L, seg, groups, bit, dbit, in, out, s);
+        case 4: return rpb_lanes<kFar, 4, E, kR>(L, seg, groups, bit, dbit, in, out, s);
+        case 8: return rpb_lanes<kFar, 8, E, kR>(L, seg, groups, bit, dbit, in, out, s);
+        case 16: return rpb_lanes<kFar, 16, E, kR>(L, seg, groups, bit, dbit, in, out, s);
         default: return hipErrorInvalidValue;
       }
     };
@@ -669,7 +689,9 @@ bool trsv_rp_blk_supported(const TriLayout& Lin) {
   if (L.rp_lanes <= 1)
     return (L.rp_e == 4 || L.rp_e == 8 || L.rp_e == 10 || L.rp_e == 12) &&
            (!L.ep_far || L.rp_farq <= (L.rp_rows == 1 ? L.rp_e : L.rp_e == 4 ? 1 : 2));
-  return !L.ep_far && (L.rp_e == 8 || ((L.rp_e == 10 || L.rp_e == 12) && L.rp_rows == 1)) &&
+  return (!L.ep_far ||
+          (L.rp_rows == 1 && L.rp_farq <= (L.rp_e >= 10 ? 8 : 4) * L.rp_lanes)) &&
+         (L.rp_e == 8 || ((L.rp_e == 10 || L.rp_e == 12) && L.rp_rows == 1)) &&
          (L.rp_lanes == 2 || L.rp_lanes == 4 || L.rp_lanes == 8 || L.rp_lanes == 16);
 }
 

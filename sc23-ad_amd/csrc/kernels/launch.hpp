@@ -36,6 +36,9 @@ struct DCsr {
   // integer SELL-64 with 16-bit column offsets col - row (used instead of
   // scol when set; square matrices, no srow)
   const int16_t* scol16 = nullptr;
+  // SELL workgroups numbered per XCD (each XCD a contiguous row range):
+  // -1 = the SC23AD_XCD_REMAP default, 0 / 1 = chosen per matrix by the tuner
+  int remap = -1;
 };
 
 // Dispatch-recorded timing.  When armed, the next launch by a timing-aware

@@ -33,9 +33,10 @@ namespace {
 constexpr int kBlock = 256;
 
 // SpMV workgroups renumbered per XCD (xcd_block) when SC23AD_XCD_REMAP=1.
-// Off by default: at 128^3 it cut the HBM fetches of the restriction and
-// prolongation by 30-45 % (their gathers stay under one L2) but not their
-// time, and the fine-level SpMV ran 4 % slower with it.
+// Off by default: the fine-level SpMV ran 4 % slower with it.  The SpMV
+// tuner also tries it per matrix on the row-sorted SELL copies (restriction,
+// prolongation), whose scattered rows of one window then stay under one L2
+// (at 128^3 their HBM fetches fell 30-45 %).
 int xcd_remap() {
   static const int on = [] {
     const char* e = std::getenv("SC23AD_XCD_REMAP");
@@ -1950,22 +1951,23 @@ template <typename T, int MODE>
 hipError_t spmv_sell(const DCsr& a, const void* x, const void* z, void* y, hipStream_t s) {
   const int grid = (a.n + kBlock - 1) / kBlock;
   if (grid == 0) return hipSuccess;
+  const int remap = a.remap >= 0 ? a.remap : xcd_remap();
   if constexpr (std::is_same_v<T, int32_t>) {
     if (a.scol16) {
       launch_timed(&k_spmv_sell16_int<MODE>, dim3(grid), dim3(kBlock), 0, s, a.n, a.soff,
                    a.scol16, static_cast<const int32_t*>(a.sval), a.bit,
                    static_cast<const int32_t*>(x), static_cast<const int32_t*>(z),
-                   static_cast<int32_t*>(y), xcd_remap());
+                   static_cast<int32_t*>(y), remap);
       return hipGetLastError();
     }
     launch_timed(&k_spmv_sell_int<MODE>, dim3(grid), dim3(kBlock), 0, s, a.n, a.soff, a.scol,
                  static_cast<const int32_t*>(a.sval), a.srow, a.bit,
                  static_cast<const int32_t*>(x),
-                 static_cast<const int32_t*>(z), static_cast<int32_t*>(y), xcd_remap());
+                 static_cast<const int32_t*>(z), static_cast<int32_t*>(y), remap);
   } else {
     launch_timed(&k_spmv_sell_fp<T, MODE>, dim3(grid), dim3(kBlock), 0, s, a.n, a.soff, a.scol,
                  static_cast<const T*>(a.sval), a.srow, static_cast<const T*>(x),
-                 static_cast<const T*>(z), static_cast<T*>(y), xcd_remap());
+                 static_cast<const T*>(z), static_cast<T*>(y), remap);
   }
   return hipGetLastError();
 }

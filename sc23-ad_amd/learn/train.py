@@ -6,7 +6,8 @@ forward/backward kernels, full-batch Adam) from the solver's sweep data:
 Instances are split 80/20 into training and validation sets (by instance);
 the script reports both losses and how often the predicted omega and vector
 bits, rounded to the sweep grids, are the level's best, and writes the
-parameters (float32) of the step with the lowest validation loss.
+parameters (float32) of the step with the lowest validation loss, and beside
+them (<out>_range.npy) every feature's [min, max] over the training levels.
 
 Targets come from the sweep table: a level's omega (vector bits) target is
 the sweep value with the fewest iterations only if it beats the defaults by
@@ -97,6 +98,11 @@ def main():
                 print(f"{part}: {name} on the best grid point for {100 * acc:.1f}% of levels "
                       f"(the default for {100 * base:.1f}%)", flush=True)
     np.save(args.out, m.params())
+    # the features' range over the training levels: outside it the learned
+    # setup keeps the defaults
+    rng_path = os.path.splitext(args.out)[0] + "_range.npy"
+    np.save(rng_path, np.stack([X.min(0), X.max(0)]).astype(np.float32))
+    print("wrote", rng_path, flush=True)
     print("wrote", args.out, flush=True)
 
 

@@ -161,19 +161,8 @@ def test_inclusions160_matches_reference(rhs):
     with open(os.path.join(os.path.dirname(__file__), "golden",
                            f"reference_inclusions160_2_{rhs}.json")) as f:
         g = json.load(f)
-    key = "incl160"
-    if key not in _CACHE:
-        print("[setup] inclusions 160^3 BlockSOR", flush=True)
-        a = S.CsrMat.generate(S.GEN_INCLUSIONS3D, 160, 64, 4.0, 12.0, 1e6, 4)
-        rptr, cind, val = a.arrays()
-        rmax = np.maximum.reduceat(np.abs(val), rptr[:-1])
-        a.lmax1_scaling(np.ones(a.n))
-        hi, hd = S.Hierarchy.build_multi(a, [S.INT, S.FP64], S.amg_param(S.BLOCK_SOR))
-        _CACHE.clear()
-        _CACHE[key] = (a, rmax, S.Amg(hi), S.Amg(hd))
-    a, rmax, amg_i, amg_d = _CACHE[key]
+    a, rmax, amg_i, amg_d = _incl160()
     assert _sha(*a.arrays()) == g["a_sha"]
-    b = a.matvec(S.xstar(a.n, 400 + rhs)) if False else None
     # b from the unscaled operator, as the workload builds it
     a0 = S.CsrMat.generate(S.GEN_INCLUSIONS3D, 160, 64, 4.0, 12.0, 1e6, 4)
     b = a0.matvec(S.xstar(a0.n, 400 + rhs)) / rmax
@@ -192,6 +181,38 @@ def test_inclusions160_matches_reference(rhs):
     assert [format(int(v), "016x") for v in trace] == g["solve"]["trace"]
     assert (cv.iter, cv.is_converged) == (g["solve"]["iters"], g["solve"]["converged"])
     assert _sha(x) == g["solve"]["x_sha"]
+
+
+def test_block_vcycle_inclusions160():
+    """The block V-cycle on the 160^3 inclusions hierarchy (five right-hand
+    sides) gives every right-hand side its single V-cycle; every level runs
+    on block kernels, including the ones whose rows span several lanes and
+    read far dependencies from global memory."""
+    a, _, amg_i, _ = _incl160()
+    rng = np.random.default_rng(13)
+    vins = rng.integers(-(2**29), 2**29, (5, a.n)).astype(np.int32)
+    native = S.block_native(amg_i)
+    print(f"[block vcycle] inclusions 160^3 native {native}", flush=True)
+    outs = S.precondition_block(amg_i, vins)
+    for r in range(5):
+        assert np.array_equal(outs[r], amg_i.precondition(vins[r])), r
+    assert native
+
+
+def _incl160():
+    """The 160^3 inclusions system (scaled), the row maxima of the unscaled
+    operator and its integer and FP64 hierarchies, built once per module."""
+    key = "incl160"
+    if key not in _CACHE:
+        print("[setup] inclusions 160^3 BlockSOR", flush=True)
+        a = S.CsrMat.generate(S.GEN_INCLUSIONS3D, 160, 64, 4.0, 12.0, 1e6, 4)
+        rptr, cind, val = a.arrays()
+        rmax = np.maximum.reduceat(np.abs(val), rptr[:-1])
+        a.lmax1_scaling(np.ones(a.n))
+        hi, hd = S.Hierarchy.build_multi(a, [S.INT, S.FP64], S.amg_param(S.BLOCK_SOR))
+        _CACHE.clear()
+        _CACHE[key] = (a, rmax, S.Amg(hi), S.Amg(hd))
+    return _CACHE[key]
 
 
 _CACHE: dict = {}

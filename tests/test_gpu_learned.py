@@ -70,7 +70,7 @@ def test_failing_prediction_falls_back_to_defaults():
     # take over
     m = _forced((1.0 - 0.6) / 1.3, 1e-6)
     res = LR.learned_solve(sy.a, sy.b, sy.nrm_b, m, S.amg_param(S.BLOCK_SOR), base=base,
-                           stall_steps=10)
+                           stall_steps=10, guard=False)
     bit = S.Amg(base[1]).default_frac_bits(sy.b, sy.nrm_b)
     x0, cv0 = S.Solver(S.INT, sy.a, S.Amg(base[0]), m=30, eps=1e-8, bit=bit).solve(sy.b, sy.nrm_b)
     assert np.all(res.learned.vbits[1:] == -21)
