@@ -454,7 +454,9 @@ hipError_t rpb_layout(const TriLayout& L, const TriSegment& seg, int groups, int
 
 // ---- tiled coarse solve, four right-hand sides per workgroup --------------
 // k_tri_tiles_int (the single-vector kernel) for a group of four right-hand
-// sides: workgroup (block b, group g) owns rows 64b .. 64b+63 of group g.  A
+// sides: workgroup (block b, group g) owns rows 64b .. 64b+63 of group g;
+// the four waves share the dependency products, then wave q sweeps the
+// diagonal tile for right-hand side q.  A
 // solved row publishes one {epoch, value} word per right-hand side
 // (xp[((g * n) + row) * 4 + c]); a consumer polls the 4 x 64 words of a
 // dependency block until all carry the current epoch.  Workgroups take
@@ -469,6 +471,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tri_tiles_blk(
     const long long* __restrict__ toff, const int32_t* __restrict__ dinv_g, int bit, int dbit,
     const int4* __restrict__ in_all, int4* out_all, unsigned long long* xp_all, int epoch,
     int* err, unsigned* tickets) {
+  static_assert(kTileThreads == 256, "one wave per right-hand side of a group of four");
   constexpr bool fwd = (K != TriKind::kDU);
   __shared__ uint64_t part[kTileThreads / 64][4][64];
   __shared__ int32_t xsh[4][64];
@@ -489,24 +492,21 @@ __global__ __launch_bounds__(kTileThreads) void k_tri_tiles_blk(
   const int rows = min(kTB, n - r0);
   const int32_t* T = tiles + toff[b];
   const int ndep = fwd ? b : nb - 1 - b;
-  uint64_t t[4] = {0, 0, 0, 0};
-  if (w == 0 && lane < rows) {
-    const int4 iv = in[r0 + lane];
-    t[0] = static_cast<uint64_t>(static_cast<int64_t>(iv.x)) << bit;
-    t[1] = static_cast<uint64_t>(static_cast<int64_t>(iv.y)) << bit;
-    t[2] = static_cast<uint64_t>(static_cast<int64_t>(iv.z)) << bit;
-    t[3] = static_cast<uint64_t>(static_cast<int64_t>(iv.w)) << bit;
-  }
-  // the diagonal tile and the row scales depend on no other block: wave 0
-  // loads them before its first wait, so their latency stays off the chain
+  // wave w owns right-hand side w of the group: its accumulators, its
+  // reduction of the dependency products and its diagonal sweep, so the
+  // four sweeps run side by side on the CU's four SIMDs
+  const int32_t* in32 = reinterpret_cast<const int32_t*>(in);
+  uint64_t t = 0;
+  if (lane < rows)
+    t = static_cast<uint64_t>(static_cast<int64_t>(in32[static_cast<size_t>(r0 + lane) * 4 + w]))
+        << bit;
+  // the diagonal tile and the row scales depend on no other block: loaded
+  // before the first wait, so their latency stays off the chain
   int32_t reg[kTB];
-  int32_t dv = 0;
-  if (w == 0) {
-    const int32_t* Td = T + static_cast<size_t>(ndep) * (kTB * kTB);
+  const int32_t* Td = T + static_cast<size_t>(ndep) * (kTB * kTB);
 #pragma unroll
-    for (int c = 0; c < kTB; ++c) reg[c] = Td[c * kTB + lane];
-    dv = (K != TriKind::kL && lane < rows) ? dinv_g[r0 + lane] : 0;
-  }
+  for (int c = 0; c < kTB; ++c) reg[c] = Td[c * kTB + lane];
+  const int32_t dv = (K != TriKind::kL && lane < rows) ? dinv_g[r0 + lane] : 0;
   for (int k = 0; k < ndep; ++k) {
     const int bp = fwd ? k : nb - 1 - k;
     const int32_t* Tk = T + static_cast<size_t>(k) * (kTB * kTB);
@@ -546,39 +546,27 @@ __global__ __launch_bounds__(kTileThreads) void k_tri_tiles_blk(
       part[w][q][lane] = ps;
     }
     __syncthreads();
-    if (w == 0) {
 #pragma unroll
-      for (int ww = 0; ww < kTileThreads / 64; ++ww)
+    for (int ww = 0; ww < kTileThreads / 64; ++ww) t -= part[ww][w][lane];
+  }
+  int32_t xv = 0;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) t[q] -= part[ww][q][lane];
+  for (int st = 0; st < kTB; ++st) {
+    const int c = fwd ? st : kTB - 1 - st;
+    if (c < rows) {
+      const int32_t mine = acc_final<int32_t, K>(t, dv, bit, dbit);
+      if (lane == c) xv = mine;
+      const int32_t xc = __builtin_amdgcn_readlane(mine, c);
+      t -= prod64(reg[c], xc);
     }
   }
-  if (w == 0) {
-    int32_t xv[4] = {0, 0, 0, 0};
-#pragma unroll
-    for (int st = 0; st < kTB; ++st) {
-      const int c = fwd ? st : kTB - 1 - st;
-      if (c < rows) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int32_t mine = acc_final<int32_t, K>(t[q], dv, bit, dbit);
-          if (lane == c) xv[q] = mine;
-          const int32_t xc = __builtin_amdgcn_readlane(mine, c);
-          t[q] -= prod64(reg[c], xc);
-        }
-      }
-    }
-    if (lane < rows) {
-      out[r0 + lane] = make_int4(xv[0], xv[1], xv[2], xv[3]);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const unsigned long long word =
-            (static_cast<unsigned long long>(static_cast<unsigned>(epoch)) << 32) |
-            static_cast<uint32_t>(xv[q]);
-        __hip_atomic_store(&xp[static_cast<size_t>(r0 + lane) * 4 + q], word, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-      }
-    }
+  if (lane < rows) {
+    reinterpret_cast<int32_t*>(out)[static_cast<size_t>(r0 + lane) * 4 + w] = xv;
+    const unsigned long long word =
+        (static_cast<unsigned long long>(static_cast<unsigned>(epoch)) << 32) |
+        static_cast<uint32_t>(xv);
+    __hip_atomic_store(&xp[static_cast<size_t>(r0 + lane) * 4 + w], word, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
   }
   tile_done(tickets, nb * groups);
 }
