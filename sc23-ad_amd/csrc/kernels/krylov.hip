@@ -21,18 +21,19 @@ namespace {
 
 constexpr int B = kRedThreads;
 
+// lsqrt (blas1.hpp:106-119): floor(sqrt(x)) for x > 0, 0 otherwise.  The
+// reference's Newton iteration from a power-of-two overestimate ends at
+// exactly that floor; here the FP64 square root gives an estimate within a
+// few units and the two loops correct it to the floor (squares in unsigned
+// 64 bits: s + 1 < 2^32 for x < 2^63), without the iteration's chain of
+// 64-bit divisions.
 __device__ __forceinline__ int64_t lsqrt64(int64_t x) {
   if (x <= 0) return 0;
-  int64_t s = 1, t = x;
-  while (s < t) {
-    s <<= 1;
-    t >>= 1;
-  }
-  do {
-    t = s;
-    s = (x / s + s) >> 1;
-  } while (s < t);
-  return t;
+  const uint64_t ux = static_cast<uint64_t>(x);
+  uint64_t s = static_cast<uint64_t>(sqrt(static_cast<double>(x)));
+  while (s * s > ux) --s;
+  while ((s + 1) * (s + 1) <= ux) ++s;
+  return static_cast<int64_t>(s);
 }
 
 // Tree over the kRedBlocks partials in the last block (256 threads, 4 values
@@ -769,19 +770,22 @@ __global__ __launch_bounds__(B) void k_scal_givens_blk(int n, int4* __restrict__
 #pragma unroll
     for (int c = 0; c < 4; ++c) tot[c] += static_cast<uint64_t>(ks[4 * g + c].ipart[pin + b]);
   block_sum4(tot, sh);
-  if (threadIdx.x == 0)
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const int r = 4 * g + c;
-      const KState k = ks[r];
-      const int64_t hn = lsqrt64(as_i64(tot[c]));
-      const int bit = k.bit;
-      hinv_sh[c] = hn == 0 ? 0 : ((static_cast<int64_t>(1) << 62) / hn) >> (62 - bit - bit);
-      if (blockIdx.x == 0 && blk_live(ks, live, r)) {
-        k.iscal[1] = hinv_sh[c];
-        givens_int(k, j, hn);
-      }
+  // lanes 0-3 of the first wave: one right-hand side each, its total from
+  // the per-wave sums block_sum4 leaves in sh
+  if (threadIdx.x < 4) {
+    const int c = threadIdx.x;
+    uint64_t tc = 0;
+    for (int w = 0; w < B / 64; ++w) tc += sh[c][w];
+    const int r = 4 * g + c;
+    const KState k = ks[r];
+    const int64_t hn = lsqrt64(as_i64(tc));
+    const int bit = k.bit;
+    hinv_sh[c] = hn == 0 ? 0 : ((static_cast<int64_t>(1) << 62) / hn) >> (62 - bit - bit);
+    if (blockIdx.x == 0 && blk_live(ks, live, r)) {
+      k.iscal[1] = hinv_sh[c];
+      givens_int(k, j, hn);
     }
+  }
   __syncthreads();
   int64_t hinv[4];
   int bit[4];
