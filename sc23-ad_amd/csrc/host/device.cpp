@@ -839,7 +839,8 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
                !(env_rp && env_rp[0] == '0')) {
       // several lanes per row (k_tri_chain_rpg): kG chunks of kE entries, the
       // fewest waves (at most 8), then the fewest padded slots; every
-      // dependency in the ring
+      // dependency in the ring, or (rows over 2 or 4 lanes, shuffled) the
+      // far rule of the one-lane kernel
       int best_g = 0, best_e = 0;
       size_t best_key = 0;
       for (int g = 2; g <= 16; g *= 2)
@@ -854,10 +855,49 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
           }
         }
       bool fits = best_g > 0;
-      for (size_t Lv = 0; Lv + 1 < lev_ptr.size() && fits; ++Lv)
-        for (int idx = lev_ptr[Lv]; idx < lev_ptr[Lv + 1] && fits; ++idx)
+      // FP64 rows over 16 lanes stage their products in LDS when the ring
+      // covering the span and the staging area fit together; every other
+      // row shuffles them to its lanes, which measured faster for FP32 rows
+      // (one shuffle per product) and for FP64 rows over at most 8 lanes
+      bool stage = false;
+      if (fits && sizeof(V) == 8 && best_g == 16) {
+        const int w = static_cast<int>((maxlev_rows * best_g + 63) / 64);
+        stage = dev::rpg_lds_bytes(sizeof(V), fring, w, best_e, maxlev_chain, true) <=
+                dev::kTriBlockedMaxLds;
+        // FP64 rows of 16 chunks of 8 spill their registers when shuffled
+        if (!stage && sizeof(V) == 8 && best_g == 16 && best_e == 8) fits = false;
+      }
+      const bool far_ok = !stage && best_g <= 4;
+      std::vector<int> lev_of(static_cast<size_t>(std::max(n, 1)), 0);
+      for (size_t Lv = 0; Lv + 1 < lev_ptr.size(); ++Lv)
+        for (int idx = lev_ptr[Lv]; idx < lev_ptr[Lv + 1]; ++idx)
+          lev_of[idx] = static_cast<int>(Lv);
+      // SC23AD_FP_RING_FAR=1 (tests): the smallest ring the far rule admits,
+      // so that smaller factors take the far path too
+      if (far_ok && std::getenv("SC23AD_FP_RING_FAR")) {
+        long long need = 1;
+        for (int idx = 0; idx < n; ++idx)
           for (int q = lbeg[idx]; q < lbeg[idx + 1]; ++q)
-            if (static_cast<long long>(lev_ptr[Lv + 1]) - lpos[q] > fring) fits = false;
+            if (lev_of[idx] - lev_of[lpos[q]] < dev::kEpFarLevels)
+              need = std::max(need, static_cast<long long>(lev_ptr[lev_of[idx] + 1]) - lpos[q]);
+        int r = 2;
+        while (r < need) r *= 2;
+        fring = std::min(fring, r);
+      }
+      // a dependency outside the ring is read from global memory when it is
+      // at least kEpFarLevels levels back
+      auto is_far = [&](int idx, int dpos) {
+        return static_cast<long long>(lev_ptr[lev_of[idx] + 1]) - dpos > fring;
+      };
+      bool any_far = false;
+      for (int idx = 0; idx < n && fits; ++idx)
+        for (int q = lbeg[idx]; q < lbeg[idx + 1]; ++q)
+          if (is_far(idx, lpos[q])) {
+            if (far_ok && lev_of[idx] - lev_of[lpos[q]] >= dev::kEpFarLevels)
+              any_far = true;
+            else
+              fits = false;
+          }
       if (fits) {
         const size_t slots = static_cast<size_t>(best_g) * best_e;
         std::vector<int> rdep(static_cast<size_t>(std::max(n, 1)) * slots, 0),
@@ -867,7 +907,8 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
           rcnt[idx] = lbeg[idx + 1] - lbeg[idx];
           for (int q = lbeg[idx]; q < lbeg[idx + 1]; ++q) {
             const size_t at = static_cast<size_t>(idx) * slots + (q - lbeg[idx]);
-            rdep[at] = lpos[q];  // entry m: lane m / kE, slot m % kE
+            // entry m: lane m / kE, slot m % kE; a far one names -(position + 1)
+            rdep[at] = is_far(idx, lpos[q]) ? -(lpos[q] + 1) : lpos[q];
             rval[at] = lval[q];
           }
         }
@@ -881,11 +922,12 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
         L.rpf_val = plan.rpf_val.get();
         L.rpf_cnt = plan.rpf_cnt.as<int>();
         L.ring = fring;
-        L.ep_far = 0;
+        L.ep_far = any_far ? 1 : 0;
         L.ep_maxlev = maxlev_chain;
         L.rp_e = best_e;
         L.rp_lanes = best_g;
         L.rp_waves = static_cast<int>((maxlev_rows * best_g + 63) / 64);
+        L.rp_stage = stage ? 1 : 0;
       }
     }
   }

@@ -116,6 +116,7 @@ struct TriLayout {
   int rp_rows = 0;
   int rp_lanes = 1;
   int rp_farq = 0;   // most far entries in one row (stored first; one-lane layouts)
+  int rp_stage = 0;  // FP rows over several lanes: products staged in LDS (else shuffled)
   // the block kernel's own row-parallel layout (four right-hand sides per
   // workgroup, four values per ring slot): entries as rent, ring slots, far
   // flag and most far entries per row, entries per lane, lanes per row,
@@ -163,6 +164,12 @@ inline size_t rp_lds_bytes(int ring, int maxlev) {
 // memory (it fences every 8 levels; its far values are loaded at most 4
 // levels ahead).
 constexpr int kRpbFarLevels = 12;
+// LDS of the FP several-lanes-per-row LD solve: the ring, each thread's e
+// products (staged for the row's ordered subtraction), the level offsets.
+inline size_t rpg_lds_bytes(size_t elem, int ring, int waves, int e, int maxlev, bool stage) {
+  return elem * (static_cast<size_t>(ring) + (stage ? static_cast<size_t>(waves) * 64 * e : 0)) +
+         sizeof(int) * (static_cast<size_t>(maxlev) + 1);
+}
 inline size_t rpb_lds_bytes(int ring, int maxlev) {
   return 16 * static_cast<size_t>(ring) + sizeof(int) * (static_cast<size_t>(maxlev) + 1);
 }

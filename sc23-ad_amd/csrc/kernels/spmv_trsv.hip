@@ -1685,9 +1685,16 @@ struct RpgPre {
   int lo, hi;
   int dep[kE];
   T val[kE];
+  T xfar[kE];  // far values (far layouts only)
   T dinv, in;
   int cnt;
 };
+
+// How a row's lanes combine their products: shuffled to every lane of the
+// row; staged in LDS and read back by the row's first lane; shuffled, with
+// far dependencies (dep < 0: position -dep-1, at least kEpFarLevels levels
+// back) read from `out` as in k_tri_chain_rpf.
+constexpr int kRpgShuffle = 0, kRpgStage = 1, kRpgFar = 2;
 
 template <typename T, int kG, int kE>
 __device__ __forceinline__ void rpg_load(RpgPre<T, kG, kE>& r, int lo, int hi, int threads,
@@ -1711,49 +1718,98 @@ __device__ __forceinline__ void rpg_load(RpgPre<T, kG, kE>& r, int lo, int hi, i
   (void)threads;
 }
 
-template <typename T, TriKind K, int kG, int kE, int kD>
-__device__ __forceinline__ void rpg_level(RpgPre<T, kG, kE>& cur, int L, int nl, int threads,
-                                          const int* lvo, T* ring, int rmask,
-                                          const int* __restrict__ rdep,
+template <typename T, TriKind K, int kG, int kE, int kD, int kMode>
+__device__ __forceinline__ void rpg_level(RpgPre<T, kG, kE>& cur, RpgPre<T, kG, kE>& ahead, int L,
+                                          int nl, int threads, const int* lvo, T* ring, T* stage,
+                                          int rmask, const int* __restrict__ rdep,
                                           const T* __restrict__ rval,
                                           const int* __restrict__ rcnt,
                                           const T* __restrict__ dinv,
                                           const T* __restrict__ in, T* out) {
+  constexpr bool kFar = kMode == kRpgFar;
   const int Lp = min(L + kD, nl);
   const int plo = lvo[Lp], phi = lvo[min(Lp + 1, nl)];
   const int t = threadIdx.x;
   const int g = t % kG;
-  const int lane0 = (t & 63) - g;  // the row's first lane inside the wave
-  T prod[kE];
-#pragma unroll
-  for (int q = 0; q < kE; ++q) prod[q] = cur.val[q] * ring[cur.dep[q] & rmask];
-  T tt = cur.in;
-#pragma unroll
-  for (int h = 0; h < kG; ++h)
+  if constexpr (kFar) {
+    // far values of level L + kA (its entries are loaded already)
+    const bool ok = ahead.lo + t / kG < ahead.hi;
 #pragma unroll
     for (int q = 0; q < kE; ++q) {
-      const T v = __shfl(prod[q], lane0 + h, 64);
-      tt = h * kE + q < cur.cnt ? tt - v : tt;
+      const int d = ahead.dep[q];
+      ahead.xfar[q] = out[(ok && d < 0) ? -d - 1 : 0];
     }
-  const int k = cur.lo + t / kG;
-  if (k < cur.hi && g == 0) {
-    const T v = acc_final<T, K>(tt, cur.dinv, 0, 0);
-    ring[k & rmask] = v;
-    out[k] = v;
   }
-  lds_barrier();
+  if constexpr (kMode == kRpgStage) {
+    // each lane forms the products of its chunk and stages them in the
+    // row's LDS slots (entry m of the row at stage[row * kG * kE + m]); the
+    // row's lanes are in one wave, whose LDS accesses stay in order, so its
+    // first lane then reads them back and subtracts them in stored order
+    T* st = stage + static_cast<size_t>(t / kG) * (kG * kE);
+#pragma unroll
+    for (int q = 0; q < kE; ++q) st[g * kE + q] = cur.val[q] * ring[cur.dep[q] & rmask];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+    const int k = cur.lo + t / kG;
+    if (k < cur.hi && g == 0) {
+      T tt = cur.in;
+#pragma unroll 16
+      for (int m = 0; m < kG * kE; ++m)
+        if (m < cur.cnt) tt = tt - st[m];
+      const T v = acc_final<T, K>(tt, cur.dinv, 0, 0);
+      ring[k & rmask] = v;
+      out[k] = v;
+    }
+  } else {
+    // no room for the staging area: the products reach every lane of the
+    // row by shuffles and each subtracts them in stored order
+    const int lane0 = (t & 63) - g;  // the row's first lane inside the wave
+    T prod[kE];
+#pragma unroll
+    for (int q = 0; q < kE; ++q) {
+      T x = ring[cur.dep[q] & rmask];
+      if constexpr (kFar) x = cur.dep[q] < 0 ? cur.xfar[q] : x;
+      prod[q] = cur.val[q] * x;
+    }
+    T tt = cur.in;
+#pragma unroll
+    for (int h = 0; h < kG; ++h)
+#pragma unroll
+      for (int q = 0; q < kE; ++q) {
+        const T v = __shfl(prod[q], lane0 + h, 64);
+        tt = h * kE + q < cur.cnt ? tt - v : tt;
+      }
+    const int k = cur.lo + t / kG;
+    if (k < cur.hi && g == 0) {
+      const T v = acc_final<T, K>(tt, cur.dinv, 0, 0);
+      ring[k & rmask] = v;
+      out[k] = v;
+    }
+  }
+  // far layouts: a full workgroup fence every kEpFence levels makes the
+  // values in `out` visible before any far load of them is issued
+  if (kFar && (L % kEpFence) == kEpFence - 1)
+    __syncthreads();
+  else
+    lds_barrier();
   rpg_load<T, kG, kE>(cur, plo, phi, threads, rdep, rval, rcnt, dinv, in);
 }
 
-template <typename T, TriKind K, int kG, int kE, int kD>
+template <typename T, TriKind K, int kG, int kE, int kD, int kMode>
 __global__ __launch_bounds__(512) void k_tri_chain_rpg(
     const int* __restrict__ rdep, const T* __restrict__ rval, const int* __restrict__ rcnt,
     const T* __restrict__ dinv, const int* __restrict__ lev_ptr,
     const int* __restrict__ chain_lev, int rmask, const T* __restrict__ in, T* out) {
+  constexpr bool kStage = kMode == kRpgStage;
+  constexpr int kA = 2;  // far values are loaded kA levels ahead
+  static_assert(kMode != kRpgFar || (kA < kD && kEpFarLevels >= kA + kEpFence),
+                "far values need their entries loaded and their writes fenced first");
   extern __shared__ __align__(16) unsigned char smem_g[];
   T* ring = reinterpret_cast<T*>(smem_g);
-  int* lvo = reinterpret_cast<int*>(ring + rmask + 1);
+  T* stage = ring + rmask + 1;  // kE products per thread (staged variant)
   const int threads = static_cast<int>(blockDim.x);
+  int* lvo = reinterpret_cast<int*>(stage + (kStage ? static_cast<size_t>(threads) * kE : 0));
   const int t = threadIdx.x;
   const int lb = chain_lev[2 * blockIdx.x], le = chain_lev[2 * blockIdx.x + 1];
   const int nl = le - lb;
@@ -1773,25 +1829,28 @@ __global__ __launch_bounds__(512) void k_tri_chain_rpg(
   for (; L0 + kD <= nl; L0 += kD) {
 #pragma unroll
     for (int d = 0; d < kD; ++d)
-      rpg_level<T, K, kG, kE, kD>(buf[d], L0 + d, nl, threads, lvo, ring, rmask, rdep, rval,
-                                  rcnt, dinv, in, out);
+      rpg_level<T, K, kG, kE, kD, kMode>(buf[d], buf[(d + kA) % kD], L0 + d, nl, threads, lvo,
+                                         ring, stage, rmask, rdep, rval, rcnt, dinv, in, out);
   }
 #pragma unroll
   for (int d = 0; d < kD; ++d)
     if (L0 + d < nl)
-      rpg_level<T, K, kG, kE, kD>(buf[d], L0 + d, nl, threads, lvo, ring, rmask, rdep, rval,
-                                  rcnt, dinv, in, out);
+      rpg_level<T, K, kG, kE, kD, kMode>(buf[d], buf[(d + kA) % kD], L0 + d, nl, threads, lvo,
+                                         ring, stage, rmask, rdep, rval, rcnt, dinv, in, out);
 }
 
-template <typename T, int kG, int kE>
+template <typename T, int kG, int kE, int kMode>
 hipError_t tri_rpg_go(const TriLayout& L, const TriSegment& seg, const T* in, T* out,
                       hipStream_t s) {
   constexpr TriKind K = TriKind::kLD;
-  const size_t lds = sizeof(T) * static_cast<size_t>(L.ring) +
-                     sizeof(int) * (static_cast<size_t>(L.ep_maxlev) + 1);
+  constexpr bool kStage = kMode == kRpgStage;
+  const size_t lds = rpg_lds_bytes(sizeof(T), L.ring, L.rp_waves, kE, L.ep_maxlev, kStage);
   if (lds > static_cast<size_t>(kTriBlockedMaxLds) || L.rp_waves < 1 || L.rp_waves > 8)
     return hipErrorInvalidValue;
-  auto kern = &k_tri_chain_rpg<T, K, kG, kE, 4>;
+  // prefetch 4 levels ahead; 3 for staged FP64 rows of 8-entry chunks,
+  // whose registers spill otherwise
+  constexpr int kD = (kStage && sizeof(T) == 8 && kE == 8) ? 3 : 4;
+  auto kern = &k_tri_chain_rpg<T, K, kG, kE, kD, kMode>;
   static bool attr = false;
   if (!attr) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
@@ -1810,14 +1869,33 @@ hipError_t tri_rpg_go(const TriLayout& L, const TriSegment& seg, const T* in, T*
 template <typename T>
 hipError_t tri_rpg(const TriLayout& L, const TriSegment& seg, const T* in, T* out,
                    hipStream_t s) {
-  if (L.ep_far) return hipErrorInvalidValue;
   auto lanes = [&](auto e_tag) -> hipError_t {
     constexpr int E = decltype(e_tag)::value;
+    // the layout builder makes: staged products for FP64 rows over 16
+    // lanes only, far dependencies for rows over 2 or 4 lanes only, and no
+    // shuffled FP64 rows of 16 chunks of 8 (whose registers spill)
+    if (L.rp_stage && L.ep_far) return hipErrorInvalidValue;
+    if constexpr (sizeof(T) == 8) {
+      if (L.rp_stage)
+        return L.rp_lanes == 16 ? tri_rpg_go<T, 16, E, kRpgStage>(L, seg, in, out, s)
+                                : hipErrorInvalidValue;
+    } else {
+      if (L.rp_stage) return hipErrorInvalidValue;
+    }
+    if (L.ep_far) {
+      switch (L.rp_lanes) {
+        case 2: return tri_rpg_go<T, 2, E, kRpgFar>(L, seg, in, out, s);
+        case 4: return tri_rpg_go<T, 4, E, kRpgFar>(L, seg, in, out, s);
+        default: return hipErrorInvalidValue;
+      }
+    }
     switch (L.rp_lanes) {
-      case 2: return tri_rpg_go<T, 2, E>(L, seg, in, out, s);
-      case 4: return tri_rpg_go<T, 4, E>(L, seg, in, out, s);
-      case 8: return tri_rpg_go<T, 8, E>(L, seg, in, out, s);
-      case 16: return tri_rpg_go<T, 16, E>(L, seg, in, out, s);
+      case 2: return tri_rpg_go<T, 2, E, kRpgShuffle>(L, seg, in, out, s);
+      case 4: return tri_rpg_go<T, 4, E, kRpgShuffle>(L, seg, in, out, s);
+      case 8: return tri_rpg_go<T, 8, E, kRpgShuffle>(L, seg, in, out, s);
+      case 16:
+        if constexpr (sizeof(T) == 8 && E == 8) return hipErrorInvalidValue;
+        else return tri_rpg_go<T, 16, E, kRpgShuffle>(L, seg, in, out, s);
       default: return hipErrorInvalidValue;
     }
   };

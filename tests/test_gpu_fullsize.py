@@ -16,6 +16,7 @@ from __future__ import annotations
 import hashlib
 import json
 import os
+import re
 import tempfile
 
 import numpy as np
@@ -91,6 +92,39 @@ def test_spmv_and_vcycles(big):
     d = rng.standard_normal(n)
     assert np.array_equal(big["amg_d"].precondition(d).view(np.uint64),
                           big["or_d"].vcycle(d).view(np.uint64))
+
+
+def test_fp32_far_dependencies(big, monkeypatch, capfd):
+    """FP32 LD solves over several lanes per row with far dependencies (read
+    from global memory, at least 40 levels back), forced at 128^3 by the
+    smallest LDS ring the far rule admits (the FP64 hierarchy takes them by
+    itself, its ring being too small for level 1's span): every LD solve and
+    one V-cycle are the oracle's bit for bit, and the far layout did occur."""
+    monkeypatch.setenv("SC23AD_FP_RING_FAR", "1")
+    monkeypatch.setenv("SC23AD_VERBOSE", "1")
+    sy = big["sy"]
+    h = S.Hierarchy(sy.a, S.FP32, S.amg_param(S.BLOCK_SOR))
+    capfd.readouterr()
+    amg = S.Amg(h)
+    err = capfd.readouterr().err
+    far = [ln for ln in err.splitlines() if ln.startswith("[sc23ad plan] kind 0")
+           and re.search(r"ep waves \d+ far 1 ", ln)
+           and int(re.search(r"rows \d+ lanes (\d+) identity", ln).group(1)) > 1]
+    print(f"[fp32 far] {len(far)} far multi-lane LD plans", flush=True)
+    assert far
+    fd, path = tempfile.mkstemp(suffix=".bin")
+    os.close(fd)
+    h.export(path)
+    oh = O.OracleHier(path)
+    os.unlink(path)
+    rng = np.random.default_rng(43)
+    for lv in range(amg.depth):
+        n = oh.level_n(lv)
+        x = rng.standard_normal(n).astype(np.float32)
+        assert np.array_equal(amg.level_op(lv, S.OP_TRSV_LD, x, n).view(np.uint32),
+                              oh.level_op(lv, S.OP_TRSV_LD, x, n).view(np.uint32)), lv
+    v = rng.standard_normal(sy.a.n).astype(np.float32)
+    assert np.array_equal(amg.precondition(v).view(np.uint32), oh.vcycle(v).view(np.uint32))
 
 
 def test_block_vcycle_at_full_size(big):
