@@ -3,6 +3,7 @@
 // applied to 4 * groups vectors) and k Restarted(FGMRESflex(m)) solves in
 // lockstep (Library/solver/irsolver.hpp:27-43, gmres.hpp:225-271).
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 #include "device.hpp"
@@ -320,6 +321,10 @@ void DevBlockSolver::solve(const double* b_dev, const double* nrm_b, double* x_d
   for (int q = 0; q < count; ++q) active[q] = 1;
   char* Vb = V.as<char>();
   char* Zb = Z.as<char>();
+  // systems of at most this many rows run each Gram-Schmidt step in one
+  // launch; SC23AD_MGS_FUSED_MAX sets the bound (0: never)
+  int mgs_fused_rows = dev::kMgsFusedRows;
+  if (const char* e = std::getenv("SC23AD_MGS_FUSED_MAX")) mgs_fused_rows = std::atoi(e);
   for (int outer = 0; outer < max_outer; ++outer) {
     put_masks(active, none);
     hip_check(dev::launch_outer_residual_blk(a64.d, K, b_dev, x_dev, r.as<double>(),
@@ -358,13 +363,20 @@ void DevBlockSolver::solve(const double* b_dev, const double* nrm_b, double* x_d
                   "checksum");
       }
       amg->spmv_blk(amg->a0, SpmvMode::kPlain, zj, nullptr, w, groups);
-      hip_check(dev::launch_dot_first_blk(n, groups, Vb, w, kst.as<dev::KState>(), s), "dot");
-      for (int kk = 0; kk <= j; ++kk)
-        hip_check(dev::launch_axpy_dot_blk(n, groups, Vb, w, kst.as<dev::KState>(),
-                                           live.as<int>(), kk, j, s),
-                  "axpy_dot");
-      hip_check(dev::launch_scal_blk(n, groups, w, kst.as<dev::KState>(), live.as<int>(), j, s),
-                "scal");
+      if (n <= mgs_fused_rows) {
+        hip_check(dev::launch_mgs_step_blk(n, groups, Vb, w, kst.as<dev::KState>(),
+                                           live.as<int>(), j, s),
+                  "mgs step");
+      } else {
+        hip_check(dev::launch_dot_first_blk(n, groups, Vb, w, kst.as<dev::KState>(), s), "dot");
+        for (int kk = 0; kk <= j; ++kk)
+          hip_check(dev::launch_axpy_dot_blk(n, groups, Vb, w, kst.as<dev::KState>(),
+                                             live.as<int>(), kk, j, s),
+                    "axpy_dot");
+        hip_check(
+            dev::launch_scal_blk(n, groups, w, kst.as<dev::KState>(), live.as<int>(), j, s),
+            "scal");
+      }
       read_flags();
       int still = 0;
       bool dropped = false;

@@ -805,6 +805,113 @@ __global__ __launch_bounds__(B) void k_scal_givens_blk(int n, int4* __restrict__
   }
 }
 
+// The whole Gram-Schmidt step j of a small system in one launch, one
+// workgroup per group of four solves: <V_0, w>, then for kk = 0..j the
+// coefficient H(kk,j), w += (int32)((-H(kk,j) V_kk) >> F) and the sums for the
+// next coefficient (or <w, w>), then lsqrt, H_inv, the Givens update and
+// w = (int32)((H_inv w) >> F) -- what k_dot_first_blk, k_axpy_dot_blk
+// (kk = 0..j) and k_scal_givens_blk compute over many workgroups and j + 3
+// launches.  Each thread keeps the same rows in every pass, so the updates
+// of w need no barrier; the sums are integer and order-free, so every value
+// is the multi-workgroup one.
+constexpr int kMgsThreads = 1024;
+
+__device__ __forceinline__ void mgs_sum4(uint64_t* acc, uint64_t (*sh)[kMgsThreads / 64],
+                                         uint64_t* tot) {
+#pragma unroll
+  for (int c = 0; c < 4; ++c) acc[c] = wave_sum_u64(acc[c]);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) sh[c][wid] = acc[c];
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    uint64_t t = 0;
+    for (int w = 0; w < kMgsThreads / 64; ++w) t += sh[threadIdx.x][w];
+    tot[threadIdx.x] = t;
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(kMgsThreads) void k_mgs_step_blk(int n, size_t vs,
+                                                              const int4* __restrict__ V,
+                                                              int4* __restrict__ w_all,
+                                                              const KState* ks, const int* live,
+                                                              int j) {
+  __shared__ uint64_t sh[4][kMgsThreads / 64];
+  __shared__ uint64_t tot[4];
+  __shared__ int64_t hsh[4];
+  const int g = blockIdx.x;
+  const size_t off = static_cast<size_t>(g) * n;
+  int4* w = w_all + off;
+  int bit[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) bit[c] = ks[4 * g + c].bit;
+  uint64_t acc[4] = {0, 0, 0, 0};
+  for (int i = threadIdx.x; i < n; i += kMgsThreads) {
+    const int4 a = V[off + i], b = w[i];
+    acc[0] += prod64(a.x, b.x);
+    acc[1] += prod64(a.y, b.y);
+    acc[2] += prod64(a.z, b.z);
+    acc[3] += prod64(a.w, b.w);
+  }
+  mgs_sum4(acc, sh, tot);
+  for (int kk = 0; kk <= j; ++kk) {
+    if (threadIdx.x < 4) {
+      const int c = threadIdx.x, r = 4 * g + c;
+      const int64_t h = as_i64(tot[c]) >> bit[c];
+      hsh[c] = h;
+      if (blk_live(ks, live, r)) static_cast<int64_t*>(ks[r].H)[j * (ks[r].m + 1) + kk] = h;
+    }
+    __syncthreads();
+    int64_t a[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) a[c] = as_i64(0 - static_cast<uint64_t>(hsh[c]));
+    const bool norm = (kk == j);
+    const int4* vk = V + static_cast<size_t>(kk) * vs + off;
+    const int4* vk1 = V + static_cast<size_t>(norm ? kk : kk + 1) * vs + off;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[c] = 0;
+    for (int i = threadIdx.x; i < n; i += kMgsThreads) {
+      const int4 v = vk[i];
+      int4 x = w[i];
+      x.x = add32(x.x, low32(mul_wrap(a[0], v.x) >> bit[0]));
+      x.y = add32(x.y, low32(mul_wrap(a[1], v.y) >> bit[1]));
+      x.z = add32(x.z, low32(mul_wrap(a[2], v.z) >> bit[2]));
+      x.w = add32(x.w, low32(mul_wrap(a[3], v.w) >> bit[3]));
+      w[i] = x;
+      const int4 u = norm ? x : vk1[i];
+      acc[0] += prod64(u.x, x.x);
+      acc[1] += prod64(u.y, x.y);
+      acc[2] += prod64(u.z, x.z);
+      acc[3] += prod64(u.w, x.w);
+    }
+    mgs_sum4(acc, sh, tot);
+  }
+  if (threadIdx.x < 4) {
+    const int c = threadIdx.x, r = 4 * g + c;
+    const KState k = ks[r];
+    const int64_t hn = lsqrt64(as_i64(tot[c]));
+    hsh[c] = hn == 0 ? 0 : ((static_cast<int64_t>(1) << 62) / hn) >> (62 - bit[c] - bit[c]);
+    if (blk_live(ks, live, r)) {
+      k.iscal[1] = hsh[c];
+      givens_int(k, j, hn);
+    }
+  }
+  __syncthreads();
+  int64_t hinv[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) hinv[c] = hsh[c];
+  for (int i = threadIdx.x; i < n; i += kMgsThreads) {
+    int4 x = w[i];
+    x.x = low32(mul_wrap(hinv[0], x.x) >> bit[0]);
+    x.y = low32(mul_wrap(hinv[1], x.y) >> bit[1]);
+    x.z = low32(mul_wrap(hinv[2], x.z) >> bit[2]);
+    x.w = low32(mul_wrap(hinv[3], x.w) >> bit[3]);
+    w[i] = x;
+  }
+}
+
 // y = H^-1 e for every solve whose x is updated (upd[r]), over the steps its
 // last cycle ran (flags[3]).
 __global__ void k_trsv_blk(const KState* ks, const int* upd) {
@@ -1197,6 +1304,15 @@ hipError_t launch_axpy_dot_blk(int n, int groups, const void* V, void* w, const 
   const int4* v = static_cast<const int4*>(V);
   hipLaunchKernelGGL(k_axpy_dot_blk, dim3(red_grid(n), groups), dim3(B), 0, s, n, v + kk * vs,
                      v + (kk + 1) * vs, static_cast<int4*>(w), ks, live, kk, j);
+  return hipGetLastError();
+}
+
+hipError_t launch_mgs_step_blk(int n, int groups, const void* V, void* w, const KState* ks,
+                               const int* live, int j, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const size_t vs = static_cast<size_t>(groups) * n;
+  hipLaunchKernelGGL(k_mgs_step_blk, dim3(groups), dim3(kMgsThreads), 0, s, n, vs,
+                     static_cast<const int4*>(V), static_cast<int4*>(w), ks, live, j);
   return hipGetLastError();
 }
 

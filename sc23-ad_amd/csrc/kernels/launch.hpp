@@ -330,6 +330,12 @@ hipError_t launch_dot_first_blk(int n, int groups, const void* v0, const void* w
                                 const KState* ks, hipStream_t s);
 hipError_t launch_axpy_dot_blk(int n, int groups, const void* V, void* w, const KState* ks,
                                const int* live, int kk, int j, hipStream_t s);
+// The whole Gram-Schmidt step j (dot_first, axpy_dot kk = 0..j, scal) in
+// one launch of one workgroup per group: for systems of at most
+// kMgsFusedRows rows.
+constexpr int kMgsFusedRows = 8192;
+hipError_t launch_mgs_step_blk(int n, int groups, const void* V, void* w, const KState* ks,
+                               const int* live, int j, hipStream_t s);
 hipError_t launch_scal_blk(int n, int groups, void* w, const KState* ks, const int* live, int j,
                            hipStream_t s);
 hipError_t launch_update_x_blk(int n, int groups, int count, const void* Z, double* x,

@@ -163,3 +163,29 @@ def test_block_solve_batch_over_systems():
         assert np.array_equal(xs[q].view(np.uint64), x1.view(np.uint64)), q
         assert [c.iter for c in cvs[q]] == [c.iter for c in c1], q
         assert [c.status for c in cvs[q]] == [c.status for c in c1], q
+
+
+def test_block_gram_schmidt_one_launch_and_many_agree(monkeypatch):
+    """The one-launch Gram-Schmidt step (by default for systems of at most
+    8,192 rows; forced here) and the multi-workgroup one (j + 3 launches)
+    give the same block solves on an 8,100-row system: x, steps, statuses and
+    residual histories, for right-hand sides that converge at different
+    steps."""
+    a = _scaled(S.GEN_CONVDIFF2D, (90, 75.0, 4.0))
+    hi, hd = S.Hierarchy.build_multi(a, [S.INT, S.FP64], S.amg_param(S.BLOCK_SOR))
+    amg_i, amg_d = S.Amg(hi), S.Amg(hd)
+    bs, nrms = _rhs(a, [3, 5, 8, 13, 21])
+    bits = [amg_d.default_frac_bits(b, nb) for b, nb in zip(bs, nrms)]
+    runs = {}
+    for bound in ("0", "100000"):
+        monkeypatch.setenv("SC23AD_MGS_FUSED_MAX", bound)
+        blk = S.BlockSolver(a, amg_i, bits, m=20, eps=1e-10)
+        xb, cvb = blk.solve(bs, nrms)
+        runs[bound] = (xb, [(c.status, c.iter, c.restarts) for c in cvb],
+                       [blk.history(r) for r in range(len(bs))])
+        print(f"[mgs] bound {bound}: {runs[bound][1]}", flush=True)
+    (x0, c0, h0), (x1, c1, h1) = runs["0"], runs["100000"]
+    assert c0 == c1
+    for r in range(len(bs)):
+        assert np.array_equal(x0[r].view(np.uint64), x1[r].view(np.uint64)), r
+        assert np.array_equal(h0[r], h1[r]), r
