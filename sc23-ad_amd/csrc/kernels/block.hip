@@ -247,6 +247,9 @@ __device__ __forceinline__ void rpb_level(RpbPre<kE, kR>& cur, RpbPre<kE, kR>& a
       xv[q][2 * h] = ring[cur.ent[q][h].x & rmask];
       xv[q][2 * h + 1] = ring[cur.ent[q][h].z & rmask];
     }
+  // rows of 8 or more entries: every ring read issued before the first
+  // product (one LDS round trip per level, as in k_tri_chain_rp)
+  if constexpr (kE >= 8) __builtin_amdgcn_sched_barrier(0);
   if (kFar) {
 #pragma unroll
     for (int q = 0; q < kR; ++q)

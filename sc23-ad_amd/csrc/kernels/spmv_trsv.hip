@@ -1341,6 +1341,11 @@ __device__ __forceinline__ void rp_level(RpPre<kE, kR>& cur, RpPre<kE, kR>& ahea
       xv[q][2 * h] = ring[cur.ent[q][h].x & rmask];
       xv[q][2 * h + 1] = ring[cur.ent[q][h].z & rmask];
     }
+  // rows of 8-12 entries: every ring read is issued before the first
+  // product, so the level pays one LDS round trip (left to itself the
+  // scheduler splits them in two: 4-5 % slower; with 4 entries the barrier
+  // measured 2 % slower)
+  if constexpr (kE >= 8) __builtin_amdgcn_sched_barrier(0);
   if (kFar) {
 #pragma unroll
     for (int q = 0; q < kR; ++q)
@@ -1361,6 +1366,10 @@ __device__ __forceinline__ void rp_level(RpPre<kE, kR>& cur, RpPre<kE, kR>& ahea
       acc += prod64(cur.ent[q][h].y, xv[q][2 * h]);
       acc2 += prod64(cur.ent[q][h].w, xv[q][2 * h + 1]);
     }
+    // opaque ends keep the two chains apart (the compiler would otherwise
+    // fold them into one chain of dependent multiply-adds)
+    asm volatile("" : "+v"(acc));
+    asm volatile("" : "+v"(acc2));
     acc = group_sum<kG>(acc + acc2);
     const int k = s + t / kG + q * (kThreads / kG);
     if (k < e && t % kG == 0) {
