@@ -127,6 +127,21 @@ def test_fp32_far_dependencies(big, monkeypatch, capfd):
     assert np.array_equal(amg.precondition(v).view(np.uint32), oh.vcycle(v).view(np.uint32))
 
 
+def test_coarse_tiles_repeated(big):
+    """The tiled coarse LU solve at 128^3 (1,769 rows, 28 blocks of 64)
+    equals the oracle's coarse solve three times in a row (the hand-off
+    epochs advance between calls)."""
+    amg, oh = big["amg_i"], big["or_i"]
+    depth = amg.depth
+    nc = oh.level_n(depth)
+    rng = np.random.default_rng(47)
+    for rep in range(3):
+        xc = rng.integers(-(2**29), 2**29, nc).astype(np.int32)
+        print(f"[coarse tiles] rep {rep}, {nc} rows", flush=True)
+        assert np.array_equal(amg.level_op(depth, S.OP_COARSE, xc, nc),
+                              oh.level_op(depth, S.OP_COARSE, xc, nc))
+
+
 def test_block_vcycle_at_full_size(big):
     """The block V-cycle at 128^3 (five right-hand sides: a full group of four
     and a padded one) gives every right-hand side its single V-cycle, which
