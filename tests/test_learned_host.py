@@ -125,3 +125,57 @@ def test_mlp_oracle_gradient_finite_differences():
         e[k] = 1e-6
         fd = (MO.loss_grad(p + e, x, t)[0] - MO.loss_grad(p - e, x, t)[0]) / 2e-6
         assert abs(fd - g[k]) <= 1e-7 + 1e-5 * abs(fd)
+
+
+def _train_module():
+    import importlib.util
+    here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location(
+        "train", os.path.join(here, "sc23-ad_amd", "learn", "train.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def test_training_range_guard(tmp_path):
+    """A level is in range when every feature lies within the training
+    [min, max] widened by a tenth of its span on either side; without a
+    range file every level is."""
+    from sc23ad import learned as LR
+    path = str(tmp_path / "range.npy")
+    lo, hi = np.array([0.0, 10.0]), np.array([1.0, 20.0])
+    np.save(path, np.stack([lo, hi]))
+    feats = np.array([[0.5, 15.0],     # inside
+                      [1.09, 10.0],    # inside the widened band of feature 0
+                      [-0.09, 21.0],   # both inside their bands
+                      [1.11, 15.0],    # beyond the band of feature 0
+                      [0.5, 8.9]])     # beyond the band of feature 1
+    assert LR.in_range(feats, path).tolist() == [True, True, True, False, False]
+    assert LR.in_range(feats, str(tmp_path / "missing.npy")).all()
+
+
+def test_margin_targets_keep_defaults_unless_the_gain_is_real():
+    """A swept value becomes a level's target only if it beats the defaults
+    by at least max(2 iterations, the margin); unconverged variants (counted
+    as 10^6 iterations) never do; targets are scaled like the network's
+    outputs (omega 0.6..1.9 and vector bits -21..1 onto [0, 1])."""
+    tr = _train_module()
+    # instance 3: defaults 100 iterations, two levels
+    rows = [(3, -1, 0, 0, 100),
+            (3, 0, 1, 1.3, 97),     # gain 3 < 5 % of 100: not a target
+            (3, 0, 1, 0.6, 10**6),  # unconverged
+            (3, 0, 2, -1, 100),
+            (3, 1, 1, 1.6, 90),     # gain 10: target
+            (3, 1, 1, 1.9, 92),
+            (3, 1, 2, -4, 94),      # gain 6: target
+            (3, 1, 2, 1, 95)]
+    # instance 5: defaults 20 iterations, one level; the margin is 2 iterations
+    rows += [(5, -1, 0, 0, 20), (5, 0, 1, 0.8, 18), (5, 0, 2, 1, 19)]
+    sweep = np.array(rows, np.float64)
+    inst = np.array([3, 3, 5])
+    t = tr.margin_targets(sweep, inst, 0.05)
+    omega = 0.6 + 1.3 * t[:, 0]
+    vbits = -21 + 22 * t[:, 1]
+    np.testing.assert_allclose(omega, [1.0, 1.6, 0.8], atol=1e-6)
+    np.testing.assert_allclose(vbits, [0, -4, 0], atol=1e-5)
+    assert t.dtype == np.float32 and t.shape == (3, 2)
