@@ -140,7 +140,7 @@ def copy_ceiling(gib: int = 2, reps: int = 10):
         S.check(S.lib.sc23ad_dev_copy(dst.p, src.p, nbytes))
     S.check(S.lib.sc23ad_dev_sync())
     dt = (time.perf_counter() - t) / reps
-    return {"kernel": "k_copy_16 (16-byte loads and stores, grid-stride)",
+    return {"kernel": "k_copy_16 (16-byte loads and stores, one word per thread)",
             "bytes_per_copy": 2 * nbytes, "ms_per_copy": dt * 1e3,
             "gbps": 2 * nbytes / dt / 1e9}
 

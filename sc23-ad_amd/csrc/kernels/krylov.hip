@@ -189,7 +189,13 @@ __device__ __forceinline__ void int_elems(int n, F&& f) {
   const int stride = gridDim.x * B;
   const int t = blockIdx.x * B + threadIdx.x;
   if ((n & 3) == 0) {
-    for (int q = t; q < (n >> 2); q += stride) f.quad(q);
+    // two steps at a time, both steps' loads issued before either store
+    // (the stores may alias later loads as far as the compiler knows, so
+    // one step at a time leaves a single step's loads in flight)
+    const int nq = n >> 2;
+    int q = t;
+    for (; q + stride < nq; q += 2 * stride) f.quad2(q, q + stride);
+    if (q < nq) f.quad(q);
   } else {
     for (int i = t; i < n; i += stride) f.one(i);
   }
@@ -209,6 +215,14 @@ __global__ __launch_bounds__(B) void k_dot_first_int(int n, const int32_t* __res
       const int4 a = reinterpret_cast<const int4*>(v)[q];
       const int4 b = reinterpret_cast<const int4*>(w)[q];
       *acc += prod64(a.x, b.x) + prod64(a.y, b.y) + prod64(a.z, b.z) + prod64(a.w, b.w);
+    }
+    __device__ void quad2(int q0, int q1) {
+      const int4 a0 = reinterpret_cast<const int4*>(v)[q0];
+      const int4 b0 = reinterpret_cast<const int4*>(w)[q0];
+      const int4 a1 = reinterpret_cast<const int4*>(v)[q1];
+      const int4 b1 = reinterpret_cast<const int4*>(w)[q1];
+      *acc += prod64(a0.x, b0.x) + prod64(a0.y, b0.y) + prod64(a0.z, b0.z) + prod64(a0.w, b0.w);
+      *acc += prod64(a1.x, b1.x) + prod64(a1.y, b1.y) + prod64(a1.z, b1.z) + prod64(a1.w, b1.w);
     }
   } body{v, w, &acc};
   int_elems(n, body);
@@ -270,6 +284,22 @@ __global__ __launch_bounds__(B) void k_axpy_dot_int(int n, const int32_t* __rest
                                                     int64_t* __restrict__ pout) {
   __shared__ uint64_t sh[B / 64];
   __shared__ int64_t hsh;
+  const bool norm = (kk == j);
+  // the operands of the thread's first two steps do not depend on H(kk,j):
+  // their loads are issued before the previous pass's partial sums are
+  // read (indices clamped into the vectors; a step past the end is skipped)
+  const int nq = n >> 2, qstride = gridDim.x * B, qt = blockIdx.x * B + threadIdx.x;
+  const bool vec = (n & 3) == 0 && nq > 0;
+  int4 pv[2], px[2], pu[2];
+  if (vec) {
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int q = min(qt + r * qstride, nq - 1);
+      pv[r] = reinterpret_cast<const int4*>(vk)[q];
+      px[r] = reinterpret_cast<const int4*>(w)[q];
+      pu[r] = norm ? make_int4(0, 0, 0, 0) : reinterpret_cast<const int4*>(vk1)[q];
+    }
+  }
   const uint64_t tot = sum_parts(pin, gridDim.x, sh);
   if (threadIdx.x == 0) {
     const int64_t h = as_i64(tot) >> k.bit;
@@ -279,7 +309,6 @@ __global__ __launch_bounds__(B) void k_axpy_dot_int(int n, const int32_t* __rest
   __syncthreads();
   const int64_t a = as_i64(0 - static_cast<uint64_t>(hsh));
   const int bit = k.bit;
-  const bool norm = (kk == j);
   uint64_t acc = 0;
   struct {
     const int32_t* vk;
@@ -312,8 +341,51 @@ __global__ __launch_bounds__(B) void k_axpy_dot_int(int n, const int32_t* __rest
         *acc += prod64(u.x, x.x) + prod64(u.y, x.y) + prod64(u.z, x.z) + prod64(u.w, x.w);
       }
     }
+    __device__ void quad2(int q0, int q1) {
+      const int4 v0 = reinterpret_cast<const int4*>(vk)[q0];
+      const int4 v1 = reinterpret_cast<const int4*>(vk)[q1];
+      int4 x0 = reinterpret_cast<const int4*>(w)[q0];
+      int4 x1 = reinterpret_cast<const int4*>(w)[q1];
+      int4 u0 = make_int4(0, 0, 0, 0), u1 = u0;
+      if (!norm) {
+        u0 = reinterpret_cast<const int4*>(vk1)[q0];
+        u1 = reinterpret_cast<const int4*>(vk1)[q1];
+      }
+      x0.x = upd(x0.x, v0.x);
+      x0.y = upd(x0.y, v0.y);
+      x0.z = upd(x0.z, v0.z);
+      x0.w = upd(x0.w, v0.w);
+      x1.x = upd(x1.x, v1.x);
+      x1.y = upd(x1.y, v1.y);
+      x1.z = upd(x1.z, v1.z);
+      x1.w = upd(x1.w, v1.w);
+      reinterpret_cast<int4*>(w)[q0] = x0;
+      reinterpret_cast<int4*>(w)[q1] = x1;
+      const int4 p0 = norm ? x0 : u0, p1 = norm ? x1 : u1;
+      *acc += prod64(p0.x, x0.x) + prod64(p0.y, x0.y) + prod64(p0.z, x0.z) + prod64(p0.w, x0.w);
+      *acc += prod64(p1.x, x1.x) + prod64(p1.y, x1.y) + prod64(p1.z, x1.z) + prod64(p1.w, x1.w);
+    }
+    // one step on operands loaded beforehand
+    __device__ void step(int q, int4 v, int4 x, int4 u) {
+      x.x = upd(x.x, v.x);
+      x.y = upd(x.y, v.y);
+      x.z = upd(x.z, v.z);
+      x.w = upd(x.w, v.w);
+      reinterpret_cast<int4*>(w)[q] = x;
+      const int4 p = norm ? x : u;
+      *acc += prod64(p.x, x.x) + prod64(p.y, x.y) + prod64(p.z, x.z) + prod64(p.w, x.w);
+    }
   } body{vk, vk1, w, a, bit, norm, &acc};
-  int_elems(n, body);
+  if (vec) {
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+      if (qt + r * qstride < nq) body.step(qt + r * qstride, pv[r], px[r], pu[r]);
+    int q = qt + 2 * qstride;
+    for (; q + qstride < nq; q += 2 * qstride) body.quad2(q, q + qstride);
+    if (q < nq) body.quad(q);
+  } else {
+    int_elems(n, body);
+  }
   const uint64_t p = block_sum_u64<B>(acc, sh);
   if (threadIdx.x == 0) pout[blockIdx.x] = static_cast<int64_t>(p);
 }
@@ -325,6 +397,15 @@ __global__ __launch_bounds__(B) void k_scal_givens_int(int n, int32_t* __restric
                                                        const int64_t* __restrict__ pin) {
   __shared__ uint64_t sh[B / 64];
   __shared__ int64_t hinv_sh;
+  // the thread's first two words of w are loaded before the partial sums
+  // are read and the Givens step is done (indices clamped into the vector)
+  const int nq = n >> 2, qstride = gridDim.x * B, qt = blockIdx.x * B + threadIdx.x;
+  const bool vec = (n & 3) == 0 && nq > 0;
+  int4 px[2];
+  if (vec) {
+#pragma unroll
+    for (int r = 0; r < 2; ++r) px[r] = reinterpret_cast<const int4*>(w)[min(qt + r * qstride, nq - 1)];
+  }
   const uint64_t tot = sum_parts(pin, gridDim.x, sh);
   if (threadIdx.x == 0) {
     const int64_t hn = lsqrt64(as_i64(tot));
@@ -349,8 +430,38 @@ __global__ __launch_bounds__(B) void k_scal_givens_int(int n, int32_t* __restric
       x.w = low32(mul_wrap(hinv, x.w) >> bit);
       reinterpret_cast<int4*>(w)[q] = x;
     }
+    __device__ void quad2(int q0, int q1) {
+      int4 x0 = reinterpret_cast<int4*>(w)[q0];
+      int4 x1 = reinterpret_cast<int4*>(w)[q1];
+      x0.x = low32(mul_wrap(hinv, x0.x) >> bit);
+      x0.y = low32(mul_wrap(hinv, x0.y) >> bit);
+      x0.z = low32(mul_wrap(hinv, x0.z) >> bit);
+      x0.w = low32(mul_wrap(hinv, x0.w) >> bit);
+      x1.x = low32(mul_wrap(hinv, x1.x) >> bit);
+      x1.y = low32(mul_wrap(hinv, x1.y) >> bit);
+      x1.z = low32(mul_wrap(hinv, x1.z) >> bit);
+      x1.w = low32(mul_wrap(hinv, x1.w) >> bit);
+      reinterpret_cast<int4*>(w)[q0] = x0;
+      reinterpret_cast<int4*>(w)[q1] = x1;
+    }
+    __device__ void step(int q, int4 x) {
+      x.x = low32(mul_wrap(hinv, x.x) >> bit);
+      x.y = low32(mul_wrap(hinv, x.y) >> bit);
+      x.z = low32(mul_wrap(hinv, x.z) >> bit);
+      x.w = low32(mul_wrap(hinv, x.w) >> bit);
+      reinterpret_cast<int4*>(w)[q] = x;
+    }
   } body{w, hinv_sh, k.bit};
-  int_elems(n, body);
+  if (vec) {
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+      if (qt + r * qstride < nq) body.step(qt + r * qstride, px[r]);
+    int q = qt + 2 * qstride;
+    for (; q + qstride < nq; q += 2 * qstride) body.quad2(q, q + qstride);
+    if (q < nq) body.quad(q);
+  } else {
+    int_elems(n, body);
+  }
 }
 
 // ---- FP MGS (fixed blocked order) --------------------------------------------
@@ -1235,13 +1346,14 @@ hipError_t launch_jacobi_update(Elem e, int n, const void* dinv, const void* t, 
   return hipGetLastError();
 }
 
-// Streaming copy with 16-byte loads and stores, each thread a grid-stride
-// run of 16-byte words (the HBM copy ceiling the SpMV is compared with).
+// Streaming copy with 16-byte loads and stores, one word per thread (the
+// HBM copy ceiling the SpMV is compared with).  A grid-stride loop over a
+// capped grid reached 4.8 TB/s on 4 GiB, one word per thread 6.2 TB/s: the
+// loop keeps one load per thread in flight, short-lived waves keep many.
 __global__ __launch_bounds__(B) void k_copy_16(size_t n16, const int4* __restrict__ a,
                                                int4* __restrict__ b) {
-  for (size_t i = blockIdx.x * static_cast<size_t>(B) + threadIdx.x; i < n16;
-       i += static_cast<size_t>(gridDim.x) * B)
-    b[i] = a[i];
+  const size_t i = blockIdx.x * static_cast<size_t>(B) + threadIdx.x;
+  if (i < n16) b[i] = a[i];
 }
 
 hipError_t launch_copy(Elem e, int n, const void* src, void* dst, hipStream_t s) {
@@ -1251,8 +1363,9 @@ hipError_t launch_copy(Elem e, int n, const void* src, void* dst, hipStream_t s)
   if ((bytes & 15) == 0 && (reinterpret_cast<uintptr_t>(src) & 15) == 0 &&
       (reinterpret_cast<uintptr_t>(dst) & 15) == 0) {
     const size_t n16 = bytes / 16;
-    const int grid = static_cast<int>(std::min<size_t>((n16 + B - 1) / B, 16384));
-    hipLaunchKernelGGL(k_copy_16, dim3(grid), dim3(B), 0, s, n16,
+    const size_t blocks = (n16 + B - 1) / B;
+    if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_copy_16, dim3(static_cast<unsigned>(blocks)), dim3(B), 0, s, n16,
                        static_cast<const int4*>(src), static_cast<int4*>(dst));
     return hipGetLastError();
   }
