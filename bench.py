@@ -794,7 +794,14 @@ def main():
                      "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBPS) if achieved else None,
                      "traffic": traffic, "algorithmic_bytes": spmv_bytes,
-                     "launches_timed": launches, "mean_ms": spmv_ms},
+                     "launches_timed": launches, "mean_ms": spmv_ms,
+                     # the measured HBM traffic per launch over the mean
+                     # duration, against this run's copy ceiling
+                     "traffic_gbps": (traffic / (spmv_ms * 1e-3) / 1e9)
+                                     if traffic and spmv_ms else None,
+                     "traffic_frac_of_copy_ceiling":
+                         (traffic / (spmv_ms * 1e-3) / 1e9 / cc["gbps"])
+                         if traffic and spmv_ms and cc.get("gbps") else None},
         "roofline_sptrsv": {"bound": "hbm",
                             "kernel": "integer LD SpTRSV, level 0 (smoother; row-parallel, "
                                       "one workgroup per BlockSOR chain)",
