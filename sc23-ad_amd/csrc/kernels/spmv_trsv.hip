@@ -676,16 +676,42 @@ __global__ __launch_bounds__(kTileThreads) void k_tri_tiles_int(
 #pragma unroll
     for (int j = 0; j < 16; ++j) tv[j] = Tk[(16 * w + j) * kTB + lane];
     if (w == 0) {
+      // two polls in flight: the next load is issued before the previous
+      // one is checked, so a block's words are seen about half a round trip
+      // sooner than with one poll at a time
       const int c = bp * kTB + lane;
-      unsigned long long v = 0;
+      const unsigned long long none =
+          static_cast<unsigned long long>(static_cast<unsigned>(epoch)) << 32;
+      // every lane loads (rows past the end read row 0's word and ignore
+      // it): a load under a divergent branch would make the compiler wait
+      // for both polls where the paths join
+      const int cl = c < n ? c : 0;
+      auto poll = [&]() {
+        const unsigned long long u =
+            __hip_atomic_load(&xp[cl], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return c < n ? u : none;
+      };
+      auto ready = [&](unsigned long long u) {
+        return __all(static_cast<int>(u >> 32) == epoch);
+      };
+      unsigned long long v = 0, va = poll();
       int spins = 0;
       while (true) {
-        v = c < n ? __hip_atomic_load(&xp[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                  : static_cast<unsigned long long>(static_cast<unsigned>(epoch)) << 32;
-        if (__all(static_cast<int>(v >> 32) == epoch)) break;
         __builtin_amdgcn_s_sleep(1);
-        if ((++spins & 31) == 0 && tile_wait_over(t0, err)) {
+        const unsigned long long vb = poll();
+        if (ready(va)) {
+          v = va;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        va = poll();
+        if (ready(vb)) {
+          v = vb;
+          break;
+        }
+        if ((++spins & 15) == 0 && tile_wait_over(t0, err)) {
           if (lane == 0) atomicAdd(err, 1);
+          v = vb;
           break;
         }
       }
