@@ -219,3 +219,24 @@ def test_device_setup_needs_a_gpu():
     with pytest.raises(S.Sc23adError) as e:
         S.Hierarchy.build_multi(a, [S.INT], S.amg_param(S.SOR), device_setup=True)
     assert e.value.code == S.NO_DEVICE
+
+
+def test_bench_traffic_from_committed_counter_summaries():
+    """bench.py's `traffic` comes from the committed rocprofv3 summaries:
+    FETCH_SIZE and WRITE_SIZE per dispatch, each times its copy-kernel
+    calibration factor, in bytes; it must equal the traffic the summary
+    states (the first kernel of each file) and stay below the algorithmic
+    bytes where the summary says so."""
+    import importlib.util
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location("bench", os.path.join(root, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    for name in ("spmv_int_level0_laplace3d_128_pmc.txt",
+                 "sptrsv_int_level0_laplace3d_128_pmc.txt"):
+        path = os.path.join(root, "profiles", name)
+        txt = open(path).read()
+        stated = float(re.search(r"^traffic_bytes_per_dispatch:\s*(\S+)", txt, re.M).group(1))
+        got = bench.read_traffic(path)
+        assert got is not None and abs(got - stated) <= 1e-6 * stated + 512, (name, got, stated)
+    assert bench.read_traffic(os.path.join(root, "profiles", "missing.txt")) is None
