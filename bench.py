@@ -485,7 +485,7 @@ def run_scale(args):
     sol_d.solve(bs[0], nrms[0])  # warm-up
     blk_i.solve(bs[:1] * R, nrms[:1] * R)
 
-    def judge(xs):
+    def explicit_check(xs):
         return [sol_d.relres(b, nbv, x) for b, nbv, x in zip(bs, nrms, xs)]
 
     def one_by_one(sol, label):
@@ -500,7 +500,7 @@ def run_scale(args):
         tt = time.perf_counter() - t
         xs = list(x_dev.get().reshape(R, a.n))
         log(f"{label}: {tt:.3f} s, iterations {[c.iter for c in cvs]}")
-        return tt, cvs, judge(xs)
+        return tt, cvs, explicit_check(xs)
 
     def block(blk, label):
         x_dev.set(zeros)
@@ -528,13 +528,13 @@ def run_scale(args):
                                               x_dev.view(k * a.n, a.n)))
         S.check(S.lib.sc23ad_dev_sync())
         tt = time.perf_counter() - t
-        res["int"] = (tt, cvs, judge(list(x_dev.get().reshape(R, a.n))))
+        res["int"] = (tt, cvs, explicit_check(list(x_dev.get().reshape(R, a.n))))
         log(f"integer defaults, one at a time: {tt:.3f} s")
         del sols_i
     tt, cvs, xs = block(blk_i, "integer defaults, batched")
-    res["int_batched"] = (tt, cvs, judge(xs))
+    res["int_batched"] = (tt, cvs, explicit_check(xs))
     tt, cvs, xs = block(blk_l, "learned + batched")
-    rel = judge(xs)
+    rel = explicit_check(xs)
     # Fallback at scale: the right-hand sides whose learned solve overflowed
     # or stalled are solved again with the defaults.  A learned solve that
     # ran out of restarts, or converged (the reference's implicit test) with
@@ -550,7 +550,7 @@ def run_scale(args):
         t_redo = time.perf_counter() - t
         for q, k in enumerate(redo):
             xs[k], cvs[k] = xr[q], cr[q]
-        rel = judge(xs)
+        rel = explicit_check(xs)
     res["learned_batched"] = (tt + t_redo, cvs, rel)
     log(f"learned + batched: {tt + t_redo:.3f} s (fallbacks {len(redo)})")
 

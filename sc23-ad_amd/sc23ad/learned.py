@@ -16,7 +16,7 @@ one F for every vector):
     saturated prediction, counted);
   * a solve that overflows, stops improving for STALL_STEPS Arnoldi steps,
     stagnates or runs out of restarts is repeated with the default
-    hierarchy; the explicit FP64 residual judges the result.
+    hierarchy; the explicit FP64 residual decides whether the result stands.
 """
 from __future__ import annotations
 
