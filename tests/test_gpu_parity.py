@@ -349,3 +349,29 @@ def test_batch_equals_individual_solves():
                                  mode=O.RED_GPU)
             assert cvs[i].iter == ro.iters and cvs[i].is_converged == ro.converged, (q, k)
             assert _same(xs[i], xo), (q, k)
+
+
+@pytest.mark.parametrize("kind,params", [(S.GEN_POISSON2D, (63,)), (S.GEN_LAPLACE3D, (17,))],
+                         ids=["poisson63", "laplace17"])
+def test_int_solve_odd_sizes_match_oracle(kind, params):
+    """Systems whose size is not a multiple of four (3,969 and 4,913 rows),
+    where the integer Gram-Schmidt kernels take their one-element loops
+    instead of the 16-byte ones: x, the iteration count and every V-cycle
+    checksum equal the oracle's."""
+    sy = S.make_system(kind, params, 3)
+    assert sy.a.n % 4 != 0
+    hi, hd = S.Hierarchy.build_multi(sy.a, [S.INT, S.FP64], S.amg_param(S.BLOCK_SOR))
+    amg = S.Amg(hi)
+    bit = S.Amg(hd).default_frac_bits(sy.b, sy.nrm_b)
+    path = export(hi)
+    oh = O.OracleHier(path)
+    os.unlink(path)
+    sol = S.Solver(S.INT, sy.a, amg, m=30, eps=1e-10, bit=bit)
+    sol.trace(8192)
+    x, cv = sol.solve(sy.b, sy.nrm_b)
+    xo, ro, to = oh.solve(sy.a.arrays(), sy.b, sy.nrm_b, m=30, eps=1e-10, bit=bit,
+                          mode=O.RED_GPU, trace_cap=8192)
+    print(f"[odd size] n {sy.a.n}: {cv.iter} iterations (oracle {ro.iters})", flush=True)
+    assert (cv.iter, cv.is_converged) == (ro.iters, ro.converged)
+    assert np.array_equal(sol.read_trace(8192), to)
+    assert _same(x, xo)
