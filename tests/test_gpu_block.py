@@ -112,7 +112,9 @@ def _compare_block_with_singles(a, hi, hd, bs, nrms, m, eps, bits=None, seq_chec
     (S.GEN_DIFFUSION27, (18, 6, 1.0, 1.0, 1e-3, 9), S.BLOCK_SOR, 3, 10),
     (S.GEN_INCLUSIONS3D, (22, 8, 2.0, 4.0, 1e6, 5), S.BLOCK_SOR, 6, 30),
     (S.GEN_POISSON2D, (70,), S.SOR, 2, 30),
-], ids=["laplace18_k5", "convdiff80_k4_m20", "diff27_k3_m10", "incl22_k6", "poisson70_sor_k2"])
+    (S.GEN_CONVDIFF2D, (45, 30.0, 1.2), S.BLOCK_SOR, 3, 20),
+], ids=["laplace18_k5", "convdiff80_k4_m20", "diff27_k3_m10", "incl22_k6", "poisson70_sor_k2",
+        "convdiff45_odd_k3"])
 def test_block_solve_matches_single_solves(kind, params, smoother, k, m):
     a = _scaled(kind, params)
     hi, hd = S.Hierarchy.build_multi(a, [S.INT, S.FP64], S.amg_param(smoother))
