@@ -790,8 +790,12 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
     while (fring * 2 * sizeof(V) <= 128 * 1024 && fring < span) fring *= 2;
     size_t lds = sizeof(V) * static_cast<size_t>(fring) +
                  sizeof(int) * (static_cast<size_t>(maxlev_chain) + 1);
+    // one row per lane, or two where a level has more than 512 rows
+    // (SC23AD_RP_ROWS=2: two rows per lane wherever the layout applies)
+    const char* env_rows = std::getenv("SC23AD_RP_ROWS");
+    const int fp_rows = (maxlev_rows > 8 * 64 || (env_rows && env_rows[0] == '2')) ? 2 : 1;
     if (kind == TriKind::kLD && plan.identity && one_launch_chains && md <= 8 &&
-        maxlev_rows <= 8 * 64 && lds <= dev::kTriBlockedMaxLds &&
+        maxlev_rows <= static_cast<size_t>(fp_rows) * 8 * 64 && lds <= dev::kTriBlockedMaxLds &&
         !(env_rp && env_rp[0] == '0')) {
       std::vector<int> lev_of(static_cast<size_t>(std::max(n, 1)), 0);
       for (size_t Lv = 0; Lv + 1 < lev_ptr.size(); ++Lv)
@@ -830,8 +834,9 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
         L.ep_far = any_far ? 1 : 0;
         L.ep_maxlev = maxlev_chain;
         L.rp_e = e;
+        L.rp_rows = fp_rows;
         int rw = 1;
-        while (rw < 8 && static_cast<size_t>(rw) * 64 < maxlev_rows) rw *= 2;
+        while (rw < 8 && static_cast<size_t>(rw) * 64 * fp_rows < maxlev_rows) rw *= 2;
         L.rp_waves = rw;
       }
     } else if (kind == TriKind::kLD && plan.identity && one_launch_chains && md > 8 &&
@@ -841,13 +846,19 @@ void build_plan(TriPlan& plan, TriKind kind, int n, const std::vector<int>& rptr
       // fewest waves (at most 8), then the fewest padded slots; every
       // dependency in the ring, or (rows over 2 or 4 lanes, shuffled) the
       // far rule of the one-lane kernel
+      // FP32 chunks of 16 entries only where 4 or 8 do not fit in 8 waves
+      // (wide levels of long rows: at 160^3 the FP32 solve went from 0.55 to
+      // 0.43 s, while FP64 rows of 16 measured slower than the
+      // level-pipelined kernel; SC23AD_RP_E16=1, tests: wherever they fit)
+      const bool e16_first = std::getenv("SC23AD_RP_E16") != nullptr;
       int best_g = 0, best_e = 0;
       size_t best_key = 0;
       for (int g = 2; g <= 16; g *= 2)
-        for (int ge : {4, 8}) {
-          if (g * ge < md) continue;
+        for (int ge : {4, 8, 16}) {
+          if (g * ge < md || (ge == 16 && sizeof(V) == 8)) continue;
           const size_t w = (maxlev_rows * g + 63) / 64;
-          const size_t key = w * 4096 + static_cast<size_t>(g * ge);
+          size_t key = w * 4096 + static_cast<size_t>(g * ge);
+          if ((ge == 16) != e16_first) key += static_cast<size_t>(1) << 30;
           if (w <= 8 && (best_g == 0 || key < best_key)) {
             best_key = key;
             best_g = g;

@@ -1546,49 +1546,56 @@ hipError_t tri_rp_far(const TriLayout& L, const TriSegment& seg, int bit, int db
 // (entries past the row's count are skipped, not added as zeros), so the
 // result is the reference's bit for bit.  Entries are stored by position:
 // kE dependency codes (as in k_tri_chain_ep) and kE values, and the count.
-template <typename T, int kE>
+template <typename T, int kE, int kR>
 struct RpfPre {
   int lo, hi;
-  int dep[kE];
-  T val[kE];
-  T xfar[kE];
-  T dinv, in;
-  int cnt;
+  int dep[kR][kE];
+  T val[kR][kE];
+  T xfar[kR][kE];
+  T dinv[kR], in[kR];
+  int cnt[kR];
 };
 
-template <int kThreads, typename T, int kE>
-__device__ __forceinline__ void rpf_load(RpfPre<T, kE>& r, int lo, int hi,
+// kR rows per lane: lane t serves rows t, t + kThreads, ... of the level.
+template <int kThreads, typename T, int kE, int kR>
+__device__ __forceinline__ void rpf_load(RpfPre<T, kE, kR>& r, int lo, int hi,
                                          const int* __restrict__ rdep, const T* __restrict__ rval,
                                          const int* __restrict__ rcnt,
                                          const T* __restrict__ dinv,
                                          const T* __restrict__ in) {
-  const int idx = lo + static_cast<int>(threadIdx.x);
-  const int id = idx < hi ? idx : 0;  // lanes past the level's end load position 0
   r.lo = lo;
   r.hi = hi;
 #pragma unroll
-  for (int q = 0; q < kE; ++q) {
-    r.dep[q] = rdep[static_cast<size_t>(id) * kE + q];
-    r.val[q] = rval[static_cast<size_t>(id) * kE + q];
-  }
-  r.cnt = rcnt[id];
-  r.dinv = dinv[id];
-  r.in = in[id];
-}
-
-template <int kThreads, typename T, int kE>
-__device__ __forceinline__ void rpf_far(RpfPre<T, kE>& r, const T* out) {
-  const bool ok = r.lo + static_cast<int>(threadIdx.x) < r.hi;
+  for (int u = 0; u < kR; ++u) {
+    const int idx = lo + static_cast<int>(threadIdx.x) + u * kThreads;
+    const int id = idx < hi ? idx : 0;  // lanes past the level's end load position 0
 #pragma unroll
-  for (int q = 0; q < kE; ++q) {
-    const int d = r.dep[q];
-    r.xfar[q] = out[(ok && d < 0) ? -d - 1 : 0];
+    for (int q = 0; q < kE; ++q) {
+      r.dep[u][q] = rdep[static_cast<size_t>(id) * kE + q];
+      r.val[u][q] = rval[static_cast<size_t>(id) * kE + q];
+    }
+    r.cnt[u] = rcnt[id];
+    r.dinv[u] = dinv[id];
+    r.in[u] = in[id];
   }
 }
 
-template <typename T, TriKind K, int kWaves, bool kFar, int kE, int kD, int kA>
-__device__ __forceinline__ void rpf_level(RpfPre<T, kE>& cur, RpfPre<T, kE>& ahead, int L, int nl,
-                                          const int* lvo, T* ring, int rmask,
+template <int kThreads, typename T, int kE, int kR>
+__device__ __forceinline__ void rpf_far(RpfPre<T, kE, kR>& r, const T* out) {
+#pragma unroll
+  for (int u = 0; u < kR; ++u) {
+    const bool ok = r.lo + static_cast<int>(threadIdx.x) + u * kThreads < r.hi;
+#pragma unroll
+    for (int q = 0; q < kE; ++q) {
+      const int d = r.dep[u][q];
+      r.xfar[u][q] = out[(ok && d < 0) ? -d - 1 : 0];
+    }
+  }
+}
+
+template <typename T, TriKind K, int kWaves, bool kFar, int kE, int kR, int kD, int kA>
+__device__ __forceinline__ void rpf_level(RpfPre<T, kE, kR>& cur, RpfPre<T, kE, kR>& ahead,
+                                          int L, int nl, const int* lvo, T* ring, int rmask,
                                           const int* __restrict__ rdep,
                                           const T* __restrict__ rval,
                                           const int* __restrict__ rcnt,
@@ -1598,22 +1605,29 @@ __device__ __forceinline__ void rpf_level(RpfPre<T, kE>& cur, RpfPre<T, kE>& ahe
   const int Lp = min(L + kD, nl);
   const int plo = lvo[Lp], phi = lvo[min(Lp + 1, nl)];
   if (kFar) rpf_far<kThreads>(ahead, out);
-  T xv[kE];
+  T xv[kR][kE];
 #pragma unroll
-  for (int q = 0; q < kE; ++q) xv[q] = ring[cur.dep[q] & rmask];
+  for (int u = 0; u < kR; ++u)
+#pragma unroll
+    for (int q = 0; q < kE; ++q) xv[u][q] = ring[cur.dep[u][q] & rmask];
   if (kFar) {
 #pragma unroll
-    for (int q = 0; q < kE; ++q)
-      if (__any(cur.dep[q] < 0)) xv[q] = cur.dep[q] < 0 ? cur.xfar[q] : xv[q];
-  }
-  T t = cur.in;
+    for (int u = 0; u < kR; ++u)
 #pragma unroll
-  for (int q = 0; q < kE; ++q) t = q < cur.cnt ? acc_sub<T>(t, cur.val[q], xv[q]) : t;
-  const int k = cur.lo + static_cast<int>(threadIdx.x);
-  if (k < cur.hi) {
-    const T v = acc_final<T, K>(t, cur.dinv, 0, 0);
-    ring[k & rmask] = v;
-    out[k] = v;
+      for (int q = 0; q < kE; ++q)
+        if (__any(cur.dep[u][q] < 0)) xv[u][q] = cur.dep[u][q] < 0 ? cur.xfar[u][q] : xv[u][q];
+  }
+#pragma unroll
+  for (int u = 0; u < kR; ++u) {
+    T t = cur.in[u];
+#pragma unroll
+    for (int q = 0; q < kE; ++q) t = q < cur.cnt[u] ? acc_sub<T>(t, cur.val[u][q], xv[u][q]) : t;
+    const int k = cur.lo + static_cast<int>(threadIdx.x) + u * kThreads;
+    if (k < cur.hi) {
+      const T v = acc_final<T, K>(t, cur.dinv[u], 0, 0);
+      ring[k & rmask] = v;
+      out[k] = v;
+    }
   }
   if (kFar && (L % kEpFence) == kEpFence - 1)
     __syncthreads();
@@ -1622,7 +1636,7 @@ __device__ __forceinline__ void rpf_level(RpfPre<T, kE>& cur, RpfPre<T, kE>& ahe
   rpf_load<kThreads>(cur, plo, phi, rdep, rval, rcnt, dinv, in);
 }
 
-template <typename T, TriKind K, int kWaves, bool kFar, int kE, int kD, int kA>
+template <typename T, TriKind K, int kWaves, bool kFar, int kE, int kR, int kD, int kA>
 __global__ __launch_bounds__(64 * kWaves) void k_tri_chain_rpf(
     const int* __restrict__ rdep, const T* __restrict__ rval, const int* __restrict__ rcnt,
     const T* __restrict__ dinv, const int* __restrict__ lev_ptr,
@@ -1642,7 +1656,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_tri_chain_rpf(
   for (int k = t; k <= rmask; k += kThreads) ring[k] = T(0);
   for (int k = t; k <= nl; k += kThreads) lvo[k] = lev_ptr[lb + k];
   __syncthreads();
-  RpfPre<T, kE> buf[kD];
+  RpfPre<T, kE, kR> buf[kD];
 #pragma unroll
   for (int d = 0; d < kD; ++d) {
     const int Lc = min(d, nl);
@@ -1652,24 +1666,30 @@ __global__ __launch_bounds__(64 * kWaves) void k_tri_chain_rpf(
   for (; L0 + kD <= nl; L0 += kD) {
 #pragma unroll
     for (int d = 0; d < kD; ++d)
-      rpf_level<T, K, kWaves, kFar, kE, kD, kA>(buf[d], buf[(d + kA) % kD], L0 + d, nl, lvo,
-                                                ring, rmask, rdep, rval, rcnt, dinv, in, out);
+      rpf_level<T, K, kWaves, kFar, kE, kR, kD, kA>(buf[d], buf[(d + kA) % kD], L0 + d, nl, lvo,
+                                                    ring, rmask, rdep, rval, rcnt, dinv, in,
+                                                    out);
   }
 #pragma unroll
   for (int d = 0; d < kD; ++d)
     if (L0 + d < nl)
-      rpf_level<T, K, kWaves, kFar, kE, kD, kA>(buf[d], buf[(d + kA) % kD], L0 + d, nl, lvo,
-                                                ring, rmask, rdep, rval, rcnt, dinv, in, out);
+      rpf_level<T, K, kWaves, kFar, kE, kR, kD, kA>(buf[d], buf[(d + kA) % kD], L0 + d, nl, lvo,
+                                                    ring, rmask, rdep, rval, rcnt, dinv, in,
+                                                    out);
 }
 
-template <typename T, int kWaves, bool kFar, int kE>
+template <typename T, int kWaves, bool kFar, int kE, int kR>
 hipError_t tri_rpf_go(const TriLayout& L, const TriSegment& seg, const T* in, T* out,
                       hipStream_t s) {
   constexpr TriKind K = TriKind::kLD;
   const size_t lds = sizeof(T) * static_cast<size_t>(L.ring) +
                      sizeof(int) * (static_cast<size_t>(L.ep_maxlev) + 1);
   if (lds > static_cast<size_t>(kTriBlockedMaxLds)) return hipErrorInvalidValue;
-  auto kern = &k_tri_chain_rpf<T, K, kWaves, kFar, kE, 4, 2>;
+  // prefetch 4 levels ahead; 3 with two rows per lane, 2 for two FP64 rows
+  // of 8 entries (whose registers spill otherwise); far values 2 ahead (1)
+  constexpr int kD = kR == 1 ? 4 : (sizeof(T) == 8 && kE == 8) ? 2 : 3;
+  constexpr int kA = kD > 2 ? 2 : 1;
+  auto kern = &k_tri_chain_rpf<T, K, kWaves, kFar, kE, kR, kD, kA>;
   static bool attr = false;
   if (!attr) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
@@ -1685,25 +1705,33 @@ hipError_t tri_rpf_go(const TriLayout& L, const TriSegment& seg, const T* in, T*
   return hipGetLastError();
 }
 
-template <typename T, int kWaves>
+template <typename T, int kWaves, int kR>
 hipError_t tri_rpf_layout(const TriLayout& L, const TriSegment& seg, const T* in, T* out,
                           hipStream_t s) {
   if (L.ep_far) {
-    if (L.rp_e == 4) return tri_rpf_go<T, kWaves, true, 4>(L, seg, in, out, s);
-    return tri_rpf_go<T, kWaves, true, 8>(L, seg, in, out, s);
+    if (L.rp_e == 4) return tri_rpf_go<T, kWaves, true, 4, kR>(L, seg, in, out, s);
+    return tri_rpf_go<T, kWaves, true, 8, kR>(L, seg, in, out, s);
   }
-  if (L.rp_e == 4) return tri_rpf_go<T, kWaves, false, 4>(L, seg, in, out, s);
-  return tri_rpf_go<T, kWaves, false, 8>(L, seg, in, out, s);
+  if (L.rp_e == 4) return tri_rpf_go<T, kWaves, false, 4, kR>(L, seg, in, out, s);
+  return tri_rpf_go<T, kWaves, false, 8, kR>(L, seg, in, out, s);
+}
+
+template <typename T, int kR>
+hipError_t tri_rpf_rows(const TriLayout& L, const TriSegment& seg, const T* in, T* out,
+                        hipStream_t s) {
+  const int w = L.rp_waves;
+  if (w <= 1) return tri_rpf_layout<T, 1, kR>(L, seg, in, out, s);
+  if (w <= 2) return tri_rpf_layout<T, 2, kR>(L, seg, in, out, s);
+  if (w <= 4) return tri_rpf_layout<T, 4, kR>(L, seg, in, out, s);
+  return tri_rpf_layout<T, 8, kR>(L, seg, in, out, s);
 }
 
 template <typename T>
 hipError_t tri_rpf_waves(const TriLayout& L, const TriSegment& seg, const T* in, T* out,
                          hipStream_t s) {
-  const int w = L.rp_waves;
-  if (w <= 1) return tri_rpf_layout<T, 1>(L, seg, in, out, s);
-  if (w <= 2) return tri_rpf_layout<T, 2>(L, seg, in, out, s);
-  if (w <= 4) return tri_rpf_layout<T, 4>(L, seg, in, out, s);
-  return tri_rpf_layout<T, 8>(L, seg, in, out, s);
+  // one or two rows per lane (levels of more than 512 rows)
+  if (L.rp_rows == 2) return tri_rpf_rows<T, 2>(L, seg, in, out, s);
+  return tri_rpf_rows<T, 1>(L, seg, in, out, s);
 }
 
 // FP row-parallel LD solve with several lanes per row (solvers 0 and 1;
@@ -1837,7 +1865,7 @@ __global__ __launch_bounds__(512) void k_tri_chain_rpg(
     const T* __restrict__ dinv, const int* __restrict__ lev_ptr,
     const int* __restrict__ chain_lev, int rmask, const T* __restrict__ in, T* out) {
   constexpr bool kStage = kMode == kRpgStage;
-  constexpr int kA = 2;  // far values are loaded kA levels ahead
+  constexpr int kA = kD > 2 ? 2 : 1;  // far values are loaded kA levels ahead
   static_assert(kMode != kRpgFar || (kA < kD && kEpFarLevels >= kA + kEpFence),
                 "far values need their entries loaded and their writes fenced first");
   extern __shared__ __align__(16) unsigned char smem_g[];
@@ -1882,9 +1910,9 @@ hipError_t tri_rpg_go(const TriLayout& L, const TriSegment& seg, const T* in, T*
   const size_t lds = rpg_lds_bytes(sizeof(T), L.ring, L.rp_waves, kE, L.ep_maxlev, kStage);
   if (lds > static_cast<size_t>(kTriBlockedMaxLds) || L.rp_waves < 1 || L.rp_waves > 8)
     return hipErrorInvalidValue;
-  // prefetch 4 levels ahead; 3 for staged FP64 rows of 8-entry chunks,
-  // whose registers spill otherwise
-  constexpr int kD = (kStage && sizeof(T) == 8 && kE == 8) ? 3 : 4;
+  // prefetch 4 levels ahead; fewer where the registers spill otherwise:
+  // 3 for staged FP64 rows of 8-entry chunks, 2 for 16-entry chunks
+  constexpr int kD = kE == 16 ? 2 : (kStage && sizeof(T) == 8 && kE == 8) ? 3 : 4;
   auto kern = &k_tri_chain_rpg<T, K, kG, kE, kD, kMode>;
   static bool attr = false;
   if (!attr) {
@@ -1907,35 +1935,40 @@ hipError_t tri_rpg(const TriLayout& L, const TriSegment& seg, const T* in, T* ou
   auto lanes = [&](auto e_tag) -> hipError_t {
     constexpr int E = decltype(e_tag)::value;
     // the layout builder makes: staged products for FP64 rows over 16
-    // lanes only, far dependencies for rows over 2 or 4 lanes only, and no
-    // shuffled FP64 rows of 16 chunks of 8 (whose registers spill)
-    if (L.rp_stage && L.ep_far) return hipErrorInvalidValue;
-    if constexpr (sizeof(T) == 8) {
-      if (L.rp_stage)
-        return L.rp_lanes == 16 ? tri_rpg_go<T, 16, E, kRpgStage>(L, seg, in, out, s)
-                                : hipErrorInvalidValue;
-    } else {
-      if (L.rp_stage) return hipErrorInvalidValue;
-    }
-    if (L.ep_far) {
+    // lanes only, far dependencies for rows over 2 or 4 lanes only, no
+    // shuffled FP64 rows of 16 chunks of 8 (whose registers spill), and
+    // chunks of 16 for FP32 rows over at most 8 lanes only
+    if constexpr (sizeof(T) == 8 && E == 16) return hipErrorInvalidValue;
+    else {
+      if (L.rp_stage && L.ep_far) return hipErrorInvalidValue;
+      if constexpr (sizeof(T) == 8) {
+        if (L.rp_stage)
+          return L.rp_lanes == 16 ? tri_rpg_go<T, 16, E, kRpgStage>(L, seg, in, out, s)
+                                  : hipErrorInvalidValue;
+      } else {
+        if (L.rp_stage) return hipErrorInvalidValue;
+      }
+      if (L.ep_far) {
+        switch (L.rp_lanes) {
+          case 2: return tri_rpg_go<T, 2, E, kRpgFar>(L, seg, in, out, s);
+          case 4: return tri_rpg_go<T, 4, E, kRpgFar>(L, seg, in, out, s);
+          default: return hipErrorInvalidValue;
+        }
+      }
       switch (L.rp_lanes) {
-        case 2: return tri_rpg_go<T, 2, E, kRpgFar>(L, seg, in, out, s);
-        case 4: return tri_rpg_go<T, 4, E, kRpgFar>(L, seg, in, out, s);
+        case 2: return tri_rpg_go<T, 2, E, kRpgShuffle>(L, seg, in, out, s);
+        case 4: return tri_rpg_go<T, 4, E, kRpgShuffle>(L, seg, in, out, s);
+        case 8: return tri_rpg_go<T, 8, E, kRpgShuffle>(L, seg, in, out, s);
+        case 16:
+          if constexpr ((sizeof(T) == 8 && E == 8) || E == 16) return hipErrorInvalidValue;
+          else return tri_rpg_go<T, 16, E, kRpgShuffle>(L, seg, in, out, s);
         default: return hipErrorInvalidValue;
       }
-    }
-    switch (L.rp_lanes) {
-      case 2: return tri_rpg_go<T, 2, E, kRpgShuffle>(L, seg, in, out, s);
-      case 4: return tri_rpg_go<T, 4, E, kRpgShuffle>(L, seg, in, out, s);
-      case 8: return tri_rpg_go<T, 8, E, kRpgShuffle>(L, seg, in, out, s);
-      case 16:
-        if constexpr (sizeof(T) == 8 && E == 8) return hipErrorInvalidValue;
-        else return tri_rpg_go<T, 16, E, kRpgShuffle>(L, seg, in, out, s);
-      default: return hipErrorInvalidValue;
     }
   };
   if (L.rp_e == 4) return lanes(std::integral_constant<int, 4>{});
   if (L.rp_e == 8) return lanes(std::integral_constant<int, 8>{});
+  if (L.rp_e == 16) return lanes(std::integral_constant<int, 16>{});
   return hipErrorInvalidValue;
 }
 

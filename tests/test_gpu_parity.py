@@ -143,15 +143,17 @@ def test_spmv_formats_match_oracle(fmt, monkeypatch):
 
 @pytest.mark.parametrize("knobs", [{"SC23AD_TRI_RP": "0"}, {"SC23AD_TRI_RP": "1"},
                                    {"SC23AD_TRI_RP": "2"}, {"SC23AD_RP_ROWS": "2"},
-                                   {"SC23AD_TRI_RP": "0", "SC23AD_EP_COPIES": "1"}],
+                                   {"SC23AD_TRI_RP": "0", "SC23AD_EP_COPIES": "1"},
+                                   {"SC23AD_RP_E16": "1"}],
                          ids=["entry-parallel", "lane-per-row", "lanes-per-row", "two-rows",
-                              "ep-copies"])
+                              "ep-copies", "chunks-of-16"])
 def test_ld_solve_variants_match_oracle(knobs, monkeypatch):
     """Every LD-solve kernel layout, forced in turn (integer: entry-parallel
     with and without accumulator copies, row-parallel with one lane per row,
     several lanes per row, two rows per lane; FP64/FP32: row-parallel in
-    stored order, or level-pipelined), gives the oracle's LD solves,
-    smoothing steps and V-cycles bit for bit."""
+    stored order with one or two rows per lane, several lanes per row with
+    chunks of 4 or 8 entries (FP32 also 16), or level-pipelined), gives the oracle's LD
+    solves, smoothing steps and V-cycles bit for bit."""
     for k, v in knobs.items():
         monkeypatch.setenv(k, v)
     for kind, params in ((S.GEN_LAPLACE3D, (24,)), (S.GEN_INCLUSIONS3D, (24, 8, 2.0, 4.0, 1e4, 3))):
